@@ -1,0 +1,125 @@
+/*
+ * spectrum_amd.h -- C ABI of the MI355X trajectory-integration engine.
+ *
+ * A drop-in for the per-trajectory loop of SPECTRUM's Simulation: plugins are configured
+ * with the reference's own class names and DataContainer bytes, trajectories are integrated
+ * by HIP kernels on one GPU, and the distributions come back as the reference's counts and
+ * sums.  Plain C types only; every call returns an SPX_* status (no exceptions cross the ABI).
+ * Citations name the reference interface (vflorins/SPECTRUM) each entry point replaces.
+ */
+#ifndef SPECTRUM_AMD_H
+#define SPECTRUM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes: the reference's exception set mapped to return values. */
+enum {
+   SPX_OK = 0,
+   SPX_ERR_ARG = -1,          /* bad argument */
+   SPX_ERR_CLASS = -2,        /* unknown plugin class name */
+   SPX_ERR_CONTAINER = -3,    /* DataContainer does not match the class (ExUninitialized analogue) */
+   SPX_ERR_NOT_READY = -4,    /* TrajectoryBase::IsSimmulationReady false (src/trajectory_base.cc:608-632) */
+   SPX_ERR_DEVICE = -5,       /* HIP runtime failure */
+   SPX_ERR_STATE = -6,        /* call out of order (e.g. advance without begin) */
+   SPX_ERR_LIMIT = -7         /* request beyond a documented limit */
+};
+
+/* Plugin families (SimulationWorker::Add*, src/simulation.hh:37-140). */
+enum { SPX_BACKGROUND = 0, SPX_DIFFUSION = 1, SPX_SOURCE = 2, SPX_BOUNDARY = 3, SPX_INITIAL = 4, SPX_DISTRIBUTION = 5 };
+
+/* Transport (configure.ac:237-261 TRAJ_TYPE) and time flow (configure.ac:264-280 TRAJ_TIME_FLOW). */
+enum { SPX_TRAJ_PARKER = 0, SPX_TRAJ_PARKER_SOURCE = 1, SPX_TRAJ_GUIDING_DIFF_SCATT = 2 };
+enum { SPX_FLOW_FORWARD = 0, SPX_FLOW_BACKWARD = 1 };
+
+/* Step precision: FP64 drop-in, or FP32 step arithmetic with FP64 accumulation. */
+enum { SPX_PREC_FP64 = 0, SPX_PREC_MIXED = 1 };
+
+/* Discard reasons (the reference's exceptions, src/trajectory_base.hh:70-152, src/background_base.hh:62). */
+enum {
+   SPX_EX_FIELD = 1, SPX_EX_STEP_TOO_SMALL = 2, SPX_EX_STEP_NAN = 3, SPX_EX_MAX_STEPS = 4,
+   SPX_EX_REPLAY_EXHAUSTED = 5, SPX_EX_BOUNDARY = 6
+};
+
+typedef struct spx_sim spx_sim;
+
+/* Random numbers: counter-based Philox4x32-10 streams keyed by (seed, trajectory id), or replay of
+   supplied N(0,1) / U(0,1] draws, one row per trajectory of the run (host pointers). */
+typedef struct {
+   int32_t mode;               /* 0 counter, 1 replay */
+   int32_t pad_;
+   uint64_t seed;
+   const double* normals;      /* [n_traj][normal_stride] */
+   const double* uniforms;     /* [n_traj][uniform_stride] */
+   int64_t normal_stride;
+   int64_t uniform_stride;
+} spx_rng;
+
+/* Endpoint of one trajectory (what the parity tests compare). */
+typedef struct {
+   double t, pos[3], mom[3], amp, wgt;
+   double t0, pos0[3], mom0[3];
+   int64_t steps;
+   int32_t status;             /* 0 accepted; 1 | (reason << 8) when every attempt was discarded */
+   int32_t attempts;
+   int32_t n_events;
+   int32_t ev_bin[4];          /* linear bin of the last event of each distribution, -1 if none */
+   double ev_weight[4];        /* weight*exp(logwgt)+amp of that event (first component) */
+} spx_traj_record;
+
+typedef struct {
+   uint64_t n_traj, done, failed;
+   uint64_t steps_done;        /* accepted steps of finished trajectories */
+   uint64_t steps_executed;    /* every executed step, the throughput counter */
+   uint64_t discards[8];
+   double t_min, t_max;        /* shortest / longest elapsed trajectory time (src/simulation.cc:266-273) */
+   double wall_s;              /* host wall time of the call */
+   double kernel_ms;           /* HIP-event time of the kernels of the call */
+   int32_t launches, lanes;
+} spx_run_stats;
+
+/* CreateSimulation (src/simulation.cc:899-907) with TRAJ_TYPE / TRAJ_TIME_FLOW chosen at run time. */
+int spx_create(int traj_type, int time_flow, spx_sim** out);
+void spx_destroy(spx_sim* sim);
+
+/* AddBackground / AddDiffusion / AddSource / AddBoundary / AddInitial / AddDistribution
+   (src/simulation.hh:37-140 -> TrajectoryBase::Add*, src/trajectory_base.cc:644-741):
+   `container` holds the DataContainer bytes of common/data_container.cc:133-240. */
+int spx_add_plugin(spx_sim* sim, int role, const char* class_name, const uint8_t* container, size_t len);
+
+/* Run-time replacement of compile-time switches and engine knobs:
+   "DISTRO_KINETIC_ENERGY_POWER_LAW_TYPE" (src/distribution_other.hh:325), "max_trajectory_steps"
+   (src/trajectory_base.hh:38), "max_attempts", "precision" (SPX_PREC_*), "block", "lanes",
+   "steps_per_launch", "graph_launches" (0 = no graph capture). */
+int spx_set_option(spx_sim* sim, const char* name, double value);
+
+/* SetTasks + MainLoop (src/simulation.cc:509-519, :818-835): integrate trajectories
+   first_traj .. first_traj+n_traj-1 to completion.  `records` (host, n_traj entries) may be NULL. */
+int spx_run(spx_sim* sim, uint64_t first_traj, uint64_t n_traj, const spx_rng* rng, spx_traj_record* records, spx_run_stats* stats);
+
+/* The same run split into steps of the persistent engine: begin queues the trajectories, advance
+   executes `n_launches` launches of `steps_per_launch` steps per resident slot, finish drains. */
+int spx_begin(spx_sim* sim, uint64_t first_traj, uint64_t n_traj, const spx_rng* rng, int keep_records);
+int spx_advance(spx_sim* sim, int n_launches, spx_run_stats* stats);
+int spx_finish(spx_sim* sim, spx_traj_record* records, spx_run_stats* stats);
+
+/* DistributionBase::NBins / GetCountsAddress / GetDistroAddress (src/distribution_base.hh,
+   src/distribution_templated.cc).  counts: n_bins entries; sums: n_bins * width doubles. */
+int spx_distro_info(const spx_sim* sim, int d, int32_t* n_bins3, int32_t* width);
+int spx_get_distro(const spx_sim* sim, int d, int64_t* counts, double* sums, int64_t* n_events);
+/* DistributionTemplated::ResetDistribution (src/distribution_templated.cc:212-235) for every distribution. */
+int spx_reset_distros(spx_sim* sim);
+
+const char* spx_last_error(const spx_sim* sim);
+/* Name, compute units and architecture of device 0 ("" and SPX_ERR_DEVICE without a GPU). */
+int spx_device_info(char* buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

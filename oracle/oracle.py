@@ -1,0 +1,160 @@
+"""ctypes wrapper of the CPU oracle (test infrastructure only).
+
+Only the test suite, the smoke check and the CPU-baseline leg of bench.py use
+this module.  It loads ``liboracle.so`` built from ``spx_oracle.c`` in this
+directory (``make -C oracle``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+RNG_COUNTER, RNG_REPLAY = 0, 1
+EX_NAMES = {0: "none", 1: "field", 2: "step_too_small", 3: "step_nan", 4: "max_steps", 5: "replay_exhausted", 6: "boundary"}
+
+
+class OrcRng(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("seed", C.c_uint64), ("normals", C.POINTER(C.c_double)),
+                ("uniforms", C.POINTER(C.c_double)), ("normal_stride", C.c_int64), ("uniform_stride", C.c_int64)]
+
+
+class OrcTraj(C.Structure):
+    _fields_ = [("t", C.c_double), ("pos", C.c_double * 3), ("mom", C.c_double * 3), ("amp", C.c_double), ("wgt", C.c_double),
+                ("t0", C.c_double), ("pos0", C.c_double * 3), ("mom0", C.c_double * 3),
+                ("steps", C.c_int64), ("status", C.c_int32), ("attempts", C.c_int32), ("n_events", C.c_int32),
+                ("ev_bin", C.c_int32 * 4), ("ev_weight", C.c_double * 4)]
+
+
+TRAJ_DTYPE = np.dtype([("t", "f8"), ("pos", "f8", 3), ("mom", "f8", 3), ("amp", "f8"), ("wgt", "f8"),
+                       ("t0", "f8"), ("pos0", "f8", 3), ("mom0", "f8", 3), ("steps", "i8"), ("status", "i4"),
+                       ("attempts", "i4"), ("n_events", "i4"), ("ev_bin", "i4", 4), ("ev_weight", "f8", 4)], align=True)
+assert TRAJ_DTYPE.itemsize == C.sizeof(OrcTraj)
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run `make -C oracle`)")
+        L = C.CDLL(LIB_PATH)
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [C.c_int, C.c_int]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_add.argtypes = [C.c_void_p, C.c_int, C.c_char_p, C.c_char_p, C.c_size_t]
+        L.orc_set_option.argtypes = [C.c_void_p, C.c_char_p, C.c_double]
+        L.orc_run.restype = C.c_int64
+        L.orc_run.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(OrcRng), C.c_int, C.c_void_p]
+        L.orc_distro_nbins.argtypes = [C.c_void_p, C.c_int]
+        L.orc_distro_width.argtypes = [C.c_void_p, C.c_int]
+        L.orc_get_distro.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.POINTER(C.c_int64)]
+        L.orc_reset_distros.argtypes = [C.c_void_p]
+        L.orc_discards.restype = C.c_int64
+        L.orc_discards.argtypes = [C.c_void_p, C.c_int]
+        L.orc_background_fields.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_double), C.c_double,
+                                            C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_kappa.restype = C.c_double
+        L.orc_kappa.argtypes = [C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double]
+        L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        L.orc_counter_normal.restype = C.c_double
+        L.orc_counter_normal.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
+        L.orc_counter_uniform.restype = C.c_double
+        L.orc_counter_uniform.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
+        L.orc_const.restype = C.c_double
+        L.orc_const.argtypes = [C.c_char_p]
+        _lib = L
+    return _lib
+
+
+class Oracle:
+    """One configured oracle simulation (mirrors a configured SimulationWorker)."""
+
+    def __init__(self, setup):
+        L = lib()
+        self.setup = setup
+        self.h = L.orc_create(setup.traj_type, setup.time_flow)
+        for role, cls, blob in setup.plugins:
+            rc = L.orc_add(self.h, role, cls.encode(), blob, len(blob))
+            if rc != 0:
+                raise ValueError(f"oracle rejected {cls} (rc={rc})")
+        for k, v in setup.options.items():
+            if L.orc_set_option(self.h, k.encode(), float(v)) != 0:
+                raise ValueError(f"unknown option {k}")
+
+    def __del__(self):
+        try:
+            lib().orc_destroy(self.h)
+        except Exception:
+            pass
+
+    def set_option(self, name, value):
+        if lib().orc_set_option(self.h, name.encode(), float(value)) != 0:
+            raise ValueError(name)
+
+    def run(self, n_traj, seed=1, first=0, threads=None, normals=None, uniforms=None, records=True):
+        rng = OrcRng()
+        keep = []
+        if normals is not None:
+            normals = np.ascontiguousarray(normals, dtype=np.float64)
+            uniforms = np.ascontiguousarray(uniforms if uniforms is not None else np.zeros((normals.shape[0], 1)), dtype=np.float64)
+            keep += [normals, uniforms]
+            rng.mode = RNG_REPLAY
+            rng.normals = normals.ctypes.data_as(C.POINTER(C.c_double))
+            rng.uniforms = uniforms.ctypes.data_as(C.POINTER(C.c_double))
+            rng.normal_stride = normals.shape[1]
+            rng.uniform_stride = uniforms.shape[1]
+        else:
+            rng.mode = RNG_COUNTER
+        rng.seed = seed
+        out = np.zeros(n_traj, dtype=TRAJ_DTYPE) if records else None
+        nthr = threads or os.cpu_count() or 1
+        steps = lib().orc_run(self.h, first, n_traj, C.byref(rng), nthr, out.ctypes.data if records else None)
+        if steps < 0:
+            raise RuntimeError(f"orc_run failed ({steps})")
+        return int(steps), out
+
+    def distro(self, d):
+        L = lib()
+        nb = L.orc_distro_nbins(self.h, d)
+        w = L.orc_distro_width(self.h, d)
+        counts = np.zeros(nb, dtype=np.int64)
+        sums = np.zeros(nb * w, dtype=np.float64)
+        nev = C.c_int64()
+        L.orc_get_distro(self.h, d, counts.ctypes.data, sums.ctypes.data, C.byref(nev))
+        return counts, sums.reshape(nb, w) if w > 1 else sums, nev.value
+
+    def reset(self):
+        lib().orc_reset_distros(self.h)
+
+    def discards(self):
+        return {EX_NAMES[k]: lib().orc_discards(self.h, k) for k in range(1, 7)}
+
+    def background_fields(self, t, pos, p):
+        P = (C.c_double * 3)(*pos)
+        U = (C.c_double * 3)()
+        B = (C.c_double * 3)()
+        dmax, divU = C.c_double(), C.c_double()
+        lib().orc_background_fields(self.h, t, P, p, U, B, C.byref(dmax), C.byref(divU))
+        return np.array(U), np.array(B), dmax.value, divU.value
+
+    def kappa(self, comp, pos, p, mu=0.0, Bmag=1.0, t=0.0):
+        P = (C.c_double * 3)(*pos)
+        return lib().orc_kappa(self.h, comp, t, P, p, mu, Bmag)
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def const(name):
+    return lib().orc_const(name.encode())

@@ -1,0 +1,65 @@
+// spx_kernels.h -- launch-side types shared by the kernels and the host runtime.
+#pragma once
+
+#include <stdint.h>
+#include <hip/hip_runtime.h>
+
+#include "spx_plan.h"
+#include "../../include/spectrum_amd.h"
+
+namespace spx {
+
+// Device-side run control block (one per run / served batch).
+struct RunCtl {
+   unsigned long long next;             // next trajectory id handed out
+   unsigned long long done;             // trajectories finished (accepted or failed)
+   unsigned long long failed;           // trajectories that exhausted their attempts
+   unsigned long long steps_done;       // accepted steps of finished trajectories
+   unsigned long long steps_executed;   // every executed step (throughput counter)
+   unsigned long long discards[8];      // discarded attempts by reason
+   unsigned long long tmin_bits, tmax_bits;
+};
+
+// Everything a launch needs; passed by value as the kernel argument.
+struct Launch {
+   RunCtl* ctl;
+   spx_rng_desc rng;                    // replay pointers are device pointers here
+   uint64_t first, end;                 // trajectory id range of this run
+   uint64_t n_lanes;
+   int steps_per_launch;
+   int n_dist;
+   int64_t plan_host_hist_off[SPX_MAX_DIST];
+   // lane state (structure of arrays, n_lanes each; boundary arrays SPX_MAX_BND * n_lanes)
+   int32_t* active;
+   double *t, *x, *y, *z, *m0, *m1, *m2, *amp, *wgt, *dt_prev, *src_old, *ds_prev, *dmax, *dt_adapt;
+   double *bdelta, *bdelta_old;
+   int32_t *bcross, *bstat;
+   uint64_t* traj;
+   uint32_t *nn, *nu;
+   int32_t* attempt;
+   int64_t* steps;
+   int32_t* n_events;
+   // start state (read when an event is binned)
+   double *t0, *x0, *y0, *z0, *m00, *m01, *m02;
+   // last event per distribution (SPX_MAX_DIST * n_lanes)
+   int32_t* ev_bin;
+   double* ev_w;
+   // events queued by the step kernel for the service kernel: 3 classes x 9 fields x n_lanes
+   double* evs;
+   int32_t* ev_bnd;                     // 3 * n_lanes
+   int32_t* ev_pending;                 // bit per class
+   int32_t* ex_code;                    // discard reason of the last step kernel
+   // histograms
+   unsigned long long* hist_counts;
+   double* hist_sums;                   // 9 slots per bin
+   unsigned long long* hist_events;     // per distribution
+   // optional per-trajectory records
+   spx_traj_record* records;
+   uint64_t rec_first, rec_n;
+};
+
+// Host-side dispatch to the kernel instantiation for (transport, precision, background, flow).
+hipError_t launch_transport(const spx_plan_desc& host_plan, const spx_plan_desc* dev_plan, const Launch& L, const Launch* dev_launch,
+                            int precision, int block, hipStream_t stream);
+
+}  // namespace spx

@@ -1,0 +1,240 @@
+// spx_physics.h -- device-side physics of the trajectory hot path.
+//
+// Each routine states the reference function it reproduces (file:line in vflorins/SPECTRUM).
+// The step arithmetic type R is double for the drop-in kernel (bit-level agreement with the
+// reference method) and float for the mixed-precision kernel; time, position, momentum,
+// amplitude and weight are always accumulated in double.
+#pragma once
+
+#include "spx_plan.h"
+#include "spx_vec.h"
+
+namespace spx {
+
+// ---- code units (common/specie.hh:30-85, common/physical_constants.hh:20-81) --------------------
+constexpr double C_CGS = 2.99792458e+10;
+constexpr double MP_CGS = 1.67262192596E-24;
+constexpr double E_CGS = 4.8032044E-10;
+constexpr double AU_CGS = 1.495978707E+13;
+constexpr double unit_length = 1.0E10;
+constexpr double unit_velocity = 0.1 * C_CGS;
+constexpr double c_code = C_CGS / unit_velocity;
+constexpr double sp_mass = MP_CGS / MP_CGS;
+constexpr double sp_charge = E_CGS / E_CGS;
+constexpr double hp_rad_sw = 117.0 * AU_CGS / unit_length;   // src/background_solarwind.hh:33
+
+// common/definitions.hh:54-66
+constexpr double sp_large = 1.0E+20;
+constexpr double sp_little = 1.0E-5;
+constexpr double sp_small = 1.0E-8;
+constexpr double sp_tiny = 1.0E-15;
+
+// ---- arithmetic helpers: IEEE in double, hardware approximations in float -------------------------
+SPX_HD double rsqrt_(double x) { return sqrt(x); }
+SPX_HD float rsqrt_(float x) { return __builtin_sqrtf(x); }
+SPX_HD double rdiv(double a, double b) { return a / b; }
+SPX_HD float rdiv(float a, float b) { return a / b; }
+SPX_HD double rpow(double a, double b) { return pow(a, b); }
+SPX_HD float rpow(float a, float b) { return __powf(a, b); }
+SPX_HD double rtanh(double a) { return tanh(a); }
+SPX_HD float rtanh(float a) { return tanhf(a); }
+
+// ---- relativistic kinematics (common/physics.hh:246-456) ---------------------------------------
+template <typename R> SPX_HD R EnrTot(R p) { return rsqrt_(sqr(p) + sqr(R(sp_mass * c_code))) * R(c_code); }
+template <typename R> SPX_HD R EnrKin(R p) { return EnrTot(p) - R(sp_mass * (c_code * c_code)); }
+template <typename R> SPX_HD R VelP(R p) { return p * R(c_code * c_code) / EnrTot(p); }
+template <typename R> SPX_HD R Rigidity(R p) { return p * R(c_code) / R(fabs(sp_charge)); }
+template <typename R> SPX_HD R LarmorRadius(R p, R B) { return p * R(c_code) / (R(fabs(sp_charge)) * B); }
+template <typename R> SPX_HD R RelFactor(R v) { return R(1) / rsqrt_(R(1) - sqr(v / R(c_code))); }
+template <typename R> SPX_HD R CyclotronFrequency(R v, R B) { return R(sp_charge) * B / (RelFactor(v) * R(sp_mass) * R(c_code)); }
+// Particle::Mom (common/physics.hh:262-266)
+SPX_HD double MomT(double T) { return sqrt(T * (T + 2.0 * sp_mass * (c_code * c_code))) / c_code; }
+
+// ---- fields at one point: the SpatialData members the path reads --------------------------------
+template <typename R>
+struct Fields {
+   v3<R> U, B, E, bhat;
+   R Bmag, dmax, divU;
+   m3<R> gradB;        // guiding centre only
+   v3<R> gradBmag;     // guiding centre only
+   v3<R> dBdt;         // guiding centre only
+   R dBmagdt;
+};
+
+// Plan access: the descriptors of the step type R.
+template <typename R> struct PlanView;
+template <> struct PlanView<double> {
+   static SPX_HD const spx_bg_t<double>& bg(const spx_plan_desc* P) { return P->bg; }
+   static SPX_HD const spx_df_t<double>& df(const spx_plan_desc* P) { return P->df; }
+};
+template <> struct PlanView<float> {
+   static SPX_HD const spx_bg_t<float>& bg(const spx_plan_desc* P) { return P->bgf; }
+   static SPX_HD const spx_df_t<float>& df(const spx_plan_desc* P) { return P->dff; }
+};
+
+template <typename R> SPX_HD v3<R> v3of(const R* a) { return mk<R>(a[0], a[1], a[2]); }
+
+// Relative position x - r0 (double subtraction, then the step type).
+template <typename R>
+SPX_HD v3<R> rel(const spx_bg_t<R>& b, double x, double y, double z) { return mk<R>(R(x - b.r0[0]), R(y - b.r0[1]), R(z - b.r0[2])); }
+
+// BackgroundSolarWind::EvaluateBackground (src/background_solarwind.cc:111-255), every switch at 0.
+// Flags select what is evaluated: U (bit 0), B (bit 1), E (bit 2).
+template <typename R>
+SPX_HD void solarwind_eval(const spx_bg_t<R>& b, v3<R> d, int want, v3<R>& U, v3<R>& B)
+{
+   v3<R> ep[3] = {v3of(b.eprime[0]), v3of(b.eprime[1]), v3of(b.eprime[2])};
+   v3<R> pp = to_basis(d, ep);
+   R r = norm(pp);
+   if (want & 1) U = from_basis(b.ur0 * unit(pp), ep);
+   if (want & 2) {
+      R r_mns = r - b.r_ref;
+      R costheta = pp.z / r;
+      R sintheta = rsqrt_(fmax(R(1) - sqr(costheta), R(0)));
+      R s = rsqrt_(sqr(pp.x) + sqr(pp.y));
+      R cosphi, sinphi;
+      if (s < r * R(sp_tiny)) { cosphi = R(0); sinphi = R(0); }
+      else { cosphi = pp.x / s; sinphi = pp.y / s; }
+      R Br = b.B0[0] * sqr(b.r_ref / r);
+      R Bp = -Br * sintheta * r_mns * b.w0 / b.ur0;
+      v3<R> Bl = mk<R>(Br * sintheta * cosphi - Bp * sinphi, Br * sintheta * sinphi + Bp * cosphi, Br * costheta);
+      B = from_basis(Bl, ep);
+   }
+}
+
+// src/background_smooth_shock.cc:46-75 (SMOOTH_SHOCK_ORDER 4)
+template <typename R> SPX_HD R shock_transition(R x) { return R(0.5) * (R(1) + rtanh(R(8) * x)); }
+template <typename R> SPX_HD R shock_transition_deriv(R x) { return R(0.5) * R(8) * (R(1) - sqr(rtanh(R(8) * x))); }
+
+// Generic EvaluateBackground for U and B (want bits), used at the finite-difference points.
+// `ds` receives BackgroundSmoothShock's transition coordinate of this evaluation.
+template <typename R, int BG>
+SPX_HD void bg_eval(const spx_bg_t<R>& b, double t, double x, double y, double z, int want, v3<R>& U, v3<R>& B, R& ds)
+{
+   if (BG == SPX_BG_SOLARWIND) {
+      solarwind_eval<R>(b, rel(b, x, y, z), want, U, B);
+   }
+   else if (BG == SPX_BG_UNIFORM) {   // src/background_uniform.cc:70-77
+      U = v3of(b.u0); B = v3of(b.B0);
+   }
+   else if (BG == SPX_BG_SHOCK) {     // src/background_shock.cc:91-111
+      v3<R> d = rel(b, x, y, z);
+      if (dot(d, v3of(b.n_shock)) - b.v_shock * R(t) > R(0)) { U = v3of(b.u0); B = v3of(b.B0); }
+      else { U = v3of(b.u1); B = v3of(b.B1); }
+   }
+   else if (BG == SPX_BG_SMOOTHSHOCK) {   // src/background_smooth_shock.cc:139-154
+      v3<R> d = rel(b, x, y, z);
+      ds = (dot(d, v3of(b.n_shock)) - b.v_shock * R(t)) / b.width_shock;
+      R a1 = shock_transition(ds), a2 = R(1) - a1;
+      U = a1 * v3of(b.u0) + a2 * v3of(b.u1);
+      B = a1 * v3of(b.B0) + a2 * v3of(b.B1);
+   }
+}
+
+// EvaluateDmax per class (src/background_solarwind.cc:310-314, src/background_smooth_shock.cc:192-196,
+// src/background_base.cc:378-382).  `ds_prev` is the smooth shock coordinate of the previous evaluation.
+template <typename R, int BG>
+SPX_HD R bg_dmax(const spx_bg_t<R>& b, double x, double y, double z, R ds_prev)
+{
+   if (BG == SPX_BG_SOLARWIND) return fmin(b.dmax_fraction * norm(rel(b, x, y, z)), b.dmax0);
+   if (BG == SPX_BG_SMOOTHSHOCK) return fmin(b.dmax_fraction * b.width_shock * fmax(R(1), fabs(ds_prev)), b.dmax0);
+   return b.dmax0;
+}
+
+// ---- diffusion: DiffusionBase::GetComponent (src/diffusion_base.cc:105-121) + models ----------------
+// Kappa components 0 (perp) and 1 (para) at one point; r is |x| (KINRAD), Umag |U|.
+template <typename R>
+SPX_HD void kappa01(const spx_df_t<R>& d, R p, R Bmag, R r, R Umag, R& K0, R& K1)
+{
+   switch (d.kind) {
+   case SPX_DF_PARA: K1 = d.D0; break;                                      // :381-385
+   case SPX_DF_PERP: K0 = d.D0; break;                                      // :459-463
+   case SPX_DF_FULL: K0 = d.Dperp; K1 = d.Dpara; break;                     // :538-543
+   case SPX_DF_FLOWMOM:                                                      // :622-627
+      K1 = d.kap0 * rpow(Umag / d.U0, d.pow_law_U) * rpow(p / d.p0, d.pow_law_p);
+      K0 = d.kap_rat * K1;
+      break;
+   case SPX_DF_KINRAD: {                                                     // :713-723
+      R kap_mom = d.kap0 * rpow(EnrKin(p) / d.T0, d.pow_law_T);
+      if (d.stream_dep_idx == 0 || r < d.r0) K1 = kap_mom * rpow(r / d.r0, d.pow_law_r);
+      else if (r < d.r0 + d.w_sh) K1 = kap_mom * sqr(Umag / d.u_up);
+      else K1 = kap_mom * d.dn_up_rat;
+      K0 = d.kap_rat * K1;
+      break;
+   }
+   case SPX_DF_RIGBMAG: {                                                    // :804-809
+      R vmag = VelP(p);
+      K1 = (d.lam0 * vmag / R(3)) * rpow(Rigidity(p) / d.R0, d.pow_law_R) * rpow(Bmag / d.B0, d.pow_law_B);
+      K0 = d.kap_rat * K1;
+      break;
+   }
+   case SPX_DF_PERPISO: K0 = d.Dperp; break;
+   default: break;
+   }
+}
+
+// ---- counter-based RNG: Philox4x32-10 (Salmon et al., SC'11) ------------------------------------
+SPX_HD void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1, uint32_t out[4])
+{
+#pragma unroll
+   for (int r = 0; r < 10; r++) {
+      uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+      uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+      uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+      uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+      c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+      k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+   }
+   out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+SPX_HD void philox_block(uint64_t seed, uint64_t traj, uint64_t draw, uint32_t kind, uint32_t o[4])
+{
+   philox4x32_10((uint32_t)draw, ((uint32_t)(draw >> 32) & 0x0FFFFFFFu) | (kind << 28), (uint32_t)traj, (uint32_t)(traj >> 32),
+                 (uint32_t)seed, (uint32_t)(seed >> 32), o);
+}
+
+// uniform in (0,1]
+SPX_HD double u53(uint32_t lo, uint32_t hi)
+{
+   uint64_t x = ((uint64_t)hi << 32) | lo;
+   return (double)((x >> 11) + 1) * 0x1.0p-53;
+}
+SPX_HD float u24(uint32_t x) { return (float)((x >> 8) + 1) * 0x1.0p-24f; }
+
+SPX_HD double counter_uniform(uint64_t seed, uint64_t traj, uint64_t draw)
+{
+   uint32_t o[4];
+   philox_block(seed, traj, draw, 1u, o);
+   return u53(o[0], o[1]);
+}
+
+// Box-Muller pairs: normal draw k comes from Philox block k/2 of the normal stream; even k takes the
+// cosine branch, odd k the sine branch of the same radius and angle.
+__device__ __forceinline__ void normal_pair_f64(uint64_t seed, uint64_t traj, uint64_t block, double& c, double& s)
+{
+   uint32_t o[4];
+   philox_block(seed, traj, block, 2u, o);
+   const double u1 = u53(o[0], o[1]), u2 = u53(o[2], o[3]);
+   const double r = sqrt(-2.0 * log(u1));
+   double sn, cs;
+   sincospi(2.0 * u2, &sn, &cs);
+   c = r * cs;
+   s = r * sn;
+}
+__device__ __forceinline__ void normal_pair_f32(uint64_t seed, uint64_t traj, uint64_t block, float& c, float& s)
+{
+   uint32_t o[4];
+   philox_block(seed, traj, block, 2u, o);
+   const float u1 = u24(o[0]), u2 = u24(o[2]);
+   const float r = __builtin_sqrtf(-2.0f * __logf(u1));
+   float sn, cs;
+   __sincosf(6.2831853071795864f * u2, &sn, &cs);
+   c = r * cs;
+   s = r * sn;
+}
+template <typename R> __device__ __forceinline__ void normal_pair(uint64_t seed, uint64_t traj, uint64_t block, R& c, R& s);
+template <> __device__ __forceinline__ void normal_pair<double>(uint64_t seed, uint64_t traj, uint64_t block, double& c, double& s) { normal_pair_f64(seed, traj, block, c, s); }
+template <> __device__ __forceinline__ void normal_pair<float>(uint64_t seed, uint64_t traj, uint64_t block, float& c, float& s) { normal_pair_f32(seed, traj, block, c, s); }
+
+}  // namespace spx

@@ -1,0 +1,102 @@
+// spx_plan.h -- plain-data descriptors of a configured trajectory simulation.
+//
+// The host lowers each plugin (class name + DataContainer bytes, the reference's
+// configuration format) into these POD blocks; the HIP kernels read them from device
+// memory with uniform (scalar) loads.  Shared by host and device code.
+#pragma once
+
+#include <stdint.h>
+
+// Host and device C++ (hipcc); the C ABI of include/spectrum_amd.h does not expose these types.
+
+#define SPX_MAX_BND 4     // boundary slots per simulation (time + space + momentum)
+#define SPX_MAX_DIST 4    // distributions per simulation
+#define SPX_MAX_ACT 4     // one action per distribution
+
+enum spx_bg_kind { SPX_BG_NONE = 0, SPX_BG_UNIFORM = 1, SPX_BG_SOLARWIND = 2, SPX_BG_SHOCK = 3, SPX_BG_SMOOTHSHOCK = 4 };
+enum spx_df_kind {
+   SPX_DF_NONE = 0, SPX_DF_ISO, SPX_DF_PARA, SPX_DF_PERP, SPX_DF_FULL, SPX_DF_FLOWMOM, SPX_DF_KINRAD, SPX_DF_RIGBMAG, SPX_DF_PERPISO
+};
+enum spx_bnd_class { SPX_BC_TIME = 0, SPX_BC_SPACE = 1, SPX_BC_MOMENTUM = 2 };
+enum spx_bnd_kind { SPX_BK_TIME = 0, SPX_BK_SPHERE = 1, SPX_BK_PLANE = 2, SPX_BK_MOMENTUM = 3 };
+enum spx_im_kind { SPX_IM_FIXED = 0, SPX_IM_SHELL = 1, SPX_IM_THICKSHELL = 2, SPX_IM_SHELLGRID = 3 };
+enum spx_ds_kind { SPX_DS_SPECTRUM = 0, SPX_DS_TIME, SPX_DS_MOMENTUM, SPX_DS_POSMOM, SPX_DS_CUMO1, SPX_DS_CUMO2, SPX_DS_POSPITCH };
+enum spx_src_kind { SPX_SRC_NONE = 0, SPX_SRC_MOMINJ = 1 };
+
+// Status bits of a boundary slot (src/boundary_base.hh:22-40 of the reference).
+#define SPX_BND_REFLECT 0x0080
+#define SPX_BND_TERMINAL 0x0100
+#define SPX_BND_CROSSED 0x0200
+#define SPX_BND_RECURRENT 0x0400
+
+// Background parameters in the step type T (double for the drop-in kernel, float for the mixed one).
+template <typename T>
+struct spx_bg_t {
+   int32_t kind, pad_;
+   double t0, r0[3];                   // always double: positions are accumulated in double
+   T u0[3], B0[3], dmax0;
+   // BackgroundSolarWind (src/background_solarwind.cc:58-84)
+   T r_ref, dmax_fraction, eprime[3][3], ur0, w0, hp_rad;
+   // BackgroundShock / BackgroundSmoothShock (src/background_shock.cc:58-86, src/background_smooth_shock.cc:139-146)
+   T n_shock[3], v_shock, compression, u1[3], B1[3], E0[3], E1[3], width_shock;
+   // BackgroundUniform (src/background_uniform.cc:58-64)
+   T Eu[3];
+};
+typedef spx_bg_t<double> spx_bg_desc;
+
+template <typename T>
+struct spx_df_t {
+   int32_t kind, stream_dep_idx, pos_dep, pad_;
+   T D0, Dperp, Dpara;
+   T kap0, U0, pow_law_U, p0, pow_law_p, kap_rat;
+   T T0, r0, pow_law_T, pow_law_r, u_up, w_sh, s_sh, dn_up_rat;
+   T lam0, R0, B0, pow_law_R, pow_law_B;
+};
+typedef spx_df_t<double> spx_df_desc;
+
+typedef struct {
+   int32_t cls, kind;           // spx_bnd_class, spx_bnd_kind
+   int32_t status0;             // REFLECT / RECURRENT bits
+   int32_t max_crossings;
+   int32_t n_act, pad_;
+   int32_t actions[SPX_MAX_ACT];
+   double timemark, origin[3], radius, norm[3], momentum;
+} spx_bnd_desc;
+
+typedef struct {
+   int32_t kind, n_bins[3], log_bins[3], bin_outside[3], dims, width, nb;
+   int32_t val_time, val_coord, pos_idx, mom_idx, pad_;
+   double limits0[3], bin_size[3];
+   double val_hot, val_cold, J0, T0, pow_law;
+} spx_ds_desc;
+
+typedef struct {
+   int32_t traj_type, time_flow;        // 0 Parker, 1 ParkerSource, 2 GuidingDiffScatt; 0 forward, 1 backward
+   int32_t n_bnd, n_dist;
+   int32_t spectrum_type;               // DISTRO_KINETIC_ENERGY_POWER_LAW_TYPE
+   int32_t max_attempts;
+   int64_t max_steps;                   // max_trajectory_steps
+   spx_bg_desc bg;
+   spx_df_desc df;
+   spx_bg_t<float> bgf;                 // the same parameters rounded to float (mixed-precision kernel)
+   spx_df_t<float> dff;
+   int32_t src_kind, pad_;
+   double src_p_inj, src_rate;
+   // initial conditions (src/initial_{time,space,momentum}.cc)
+   int32_t im_kind, im_log, im_n, pad2_;
+   double it_t, is_pos[3], im_mom[3], im_p0, im_p1, im_p2, im_T1, im_T2;
+   spx_bnd_desc bnd[SPX_MAX_BND];       // ordered: all time, then space, then momentum (ComputeAllBoundaries order)
+   int32_t n_bt, n_bs, n_bm, pad3_;
+   spx_ds_desc dist[SPX_MAX_DIST];
+   int64_t hist_off[SPX_MAX_DIST];      // offsets of each distribution in the count / sum arrays
+} spx_plan_desc;
+
+// RNG description passed to a run (counter-based streams or replayed draws).
+typedef struct {
+   int32_t mode;                       // 0 counter, 1 replay
+   int32_t pad_;
+   uint64_t seed;
+   const double* normals;              // device pointers in a launch; host pointers at the C ABI
+   const double* uniforms;
+   int64_t normal_stride, uniform_stride;
+} spx_rng_desc;
