@@ -80,6 +80,8 @@ typedef struct {
    double wall_s;              /* host wall time of the call */
    double kernel_ms;           /* HIP-event time of the kernels of the call */
    int32_t launches, lanes;
+   double step_kernel_ms;      /* HIP-event time of the step kernels alone (0 under graph replay) */
+   double service_kernel_ms;   /* HIP-event time of the service kernels alone (0 under graph replay) */
 } spx_run_stats;
 
 /* CreateSimulation (src/simulation.cc:899-907) with TRAJ_TYPE / TRAJ_TIME_FLOW chosen at run time. */

@@ -13,6 +13,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
+FLOPS_LIB_PATH = os.path.join(HERE, "liboracle_flops.so")   # same source, operation-counting scalar
 
 RNG_COUNTER, RNG_REPLAY = 0, 1
 EX_NAMES = {0: "none", 1: "field", 2: "step_too_small", 3: "step_nan", 4: "max_steps", 5: "replay_exhausted", 6: "boundary"}
@@ -35,15 +36,14 @@ TRAJ_DTYPE = np.dtype([("t", "f8"), ("pos", "f8", 3), ("mom", "f8", 3), ("amp", 
                        ("attempts", "i4"), ("n_events", "i4"), ("ev_bin", "i4", 4), ("ev_weight", "f8", 4)], align=True)
 assert TRAJ_DTYPE.itemsize == C.sizeof(OrcTraj)
 
-_lib = None
+_libs: dict = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run `make -C oracle`)")
-        L = C.CDLL(LIB_PATH)
+def lib(path: str = LIB_PATH):
+    if path not in _libs:
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library missing: {path} (run `make -C oracle`)")
+        L = C.CDLL(path)
         L.orc_create.restype = C.c_void_p
         L.orc_create.argtypes = [C.c_int, C.c_int]
         L.orc_destroy.argtypes = [C.c_void_p]
@@ -68,15 +68,19 @@ def lib():
         L.orc_counter_uniform.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
         L.orc_const.restype = C.c_double
         L.orc_const.argtypes = [C.c_char_p]
-        _lib = L
-    return _lib
+        if hasattr(L, "orc_flops_read"):
+            L.orc_flops_read.argtypes = [C.POINTER(C.c_ulonglong), C.c_int]
+            L.orc_flops_reset.argtypes = []
+            L.orc_flops_reset.restype = None
+        _libs[path] = L
+    return _libs[path]
 
 
 class Oracle:
     """One configured oracle simulation (mirrors a configured SimulationWorker)."""
 
-    def __init__(self, setup):
-        L = lib()
+    def __init__(self, setup, path: str = LIB_PATH):
+        self.L = L = lib(path)
         self.setup = setup
         self.h = L.orc_create(setup.traj_type, setup.time_flow)
         for role, cls, blob in setup.plugins:
@@ -89,12 +93,12 @@ class Oracle:
 
     def __del__(self):
         try:
-            lib().orc_destroy(self.h)
+            self.L.orc_destroy(self.h)
         except Exception:
             pass
 
     def set_option(self, name, value):
-        if lib().orc_set_option(self.h, name.encode(), float(value)) != 0:
+        if self.L.orc_set_option(self.h, name.encode(), float(value)) != 0:
             raise ValueError(name)
 
     def run(self, n_traj, seed=1, first=0, threads=None, normals=None, uniforms=None, records=True):
@@ -114,13 +118,13 @@ class Oracle:
         rng.seed = seed
         out = np.zeros(n_traj, dtype=TRAJ_DTYPE) if records else None
         nthr = threads or os.cpu_count() or 1
-        steps = lib().orc_run(self.h, first, n_traj, C.byref(rng), nthr, out.ctypes.data if records else None)
+        steps = self.L.orc_run(self.h, first, n_traj, C.byref(rng), nthr, out.ctypes.data if records else None)
         if steps < 0:
             raise RuntimeError(f"orc_run failed ({steps})")
         return int(steps), out
 
     def distro(self, d):
-        L = lib()
+        L = self.L
         nb = L.orc_distro_nbins(self.h, d)
         w = L.orc_distro_width(self.h, d)
         counts = np.zeros(nb, dtype=np.int64)
@@ -130,22 +134,22 @@ class Oracle:
         return counts, sums.reshape(nb, w) if w > 1 else sums, nev.value
 
     def reset(self):
-        lib().orc_reset_distros(self.h)
+        self.L.orc_reset_distros(self.h)
 
     def discards(self):
-        return {EX_NAMES[k]: lib().orc_discards(self.h, k) for k in range(1, 7)}
+        return {EX_NAMES[k]: self.L.orc_discards(self.h, k) for k in range(1, 7)}
 
     def background_fields(self, t, pos, p):
         P = (C.c_double * 3)(*pos)
         U = (C.c_double * 3)()
         B = (C.c_double * 3)()
         dmax, divU = C.c_double(), C.c_double()
-        lib().orc_background_fields(self.h, t, P, p, U, B, C.byref(dmax), C.byref(divU))
+        self.L.orc_background_fields(self.h, t, P, p, U, B, C.byref(dmax), C.byref(divU))
         return np.array(U), np.array(B), dmax.value, divU.value
 
     def kappa(self, comp, pos, p, mu=0.0, Bmag=1.0, t=0.0):
         P = (C.c_double * 3)(*pos)
-        return lib().orc_kappa(self.h, comp, t, P, p, mu, Bmag)
+        return self.L.orc_kappa(self.h, comp, t, P, p, mu, Bmag)
 
 
 def philox(ctr, key):
@@ -158,3 +162,16 @@ def philox(ctr, key):
 
 def const(name):
     return lib().orc_const(name.encode())
+
+
+FLOP_KINDS = ["add", "mul", "div", "sqrt", "cmp", "neg", "exp", "log", "pow", "tanh", "trig", "absminmax"]
+
+
+def flops_reset():
+    lib(FLOPS_LIB_PATH).orc_flops_reset()
+
+
+def flops_read() -> dict:
+    buf = (C.c_ulonglong * 16)()
+    n = lib(FLOPS_LIB_PATH).orc_flops_read(buf, 16)
+    return {FLOP_KINDS[k]: int(buf[k]) for k in range(n)}

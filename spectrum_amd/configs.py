@@ -132,14 +132,16 @@ def spectrum_bins(T1: float, T2: float, n: int):
 # --- workloads --------------------------------------------------------------------------
 
 def parker_modulation(n_energies=16, T1=0.1 * GeV, T2=10.0 * GeV, r_start_au=1.0, r_outer_au=90.0,
-                      t_limit_years=50.0, spectrum_type=1, inner_au=0.01, name="parker_modulation") -> Setup:
+                      t_limit_years=50.0, spectrum_type=1, inner_au=0.01, time_action=-1,
+                      name="parker_modulation") -> Setup:
     """Backward Parker modulation in the Parker spiral (the CPU reference problem and its GPU scale-up).
 
     SolarWind V = 400 km/s, |B| = 5 nT at 1 au, Omega = 2.87e-6 rad/s (r_ref = 3 R_sun as in
     runs/gcr_modulation_example.cc:52-57); kappa_par = 1.5e22 cm^2/s * beta * (R / 1 GV)^0.5,
     kappa_perp = 0.02 kappa_par; start at r = 1 au on the ecliptic at n log-spaced kinetic
     energies; absorbing sphere at 90 au weighted by a LIS j ~ T^-2.7 (intensity weighting).
-    An inner absorbing sphere and a time limit without binning follow the reference example.
+    An inner absorbing sphere and a time limit without binning follow the reference example
+    (``inner_au=None`` drops the sphere; ``time_action=0`` bins trajectories at the time limit too).
     """
     s = Setup(name, TRAJ_PARKER, BACKWARD, options={"DISTRO_KINETIC_ENERGY_POWER_LAW_TYPE": spectrum_type})
     s.add(BACKGROUND, "BackgroundSolarWind", solarwind())
@@ -150,9 +152,10 @@ def parker_modulation(n_energies=16, T1=0.1 * GeV, T2=10.0 * GeV, r_start_au=1.0
     s.add(INITIAL, "InitialTimeFixed", DataContainer().insert_double(0.0))
     s.add(INITIAL, "InitialSpaceFixed", DataContainer().insert_vec((r_start_au * one_au, 0.0, 0.0)))
     s.add(INITIAL, "InitialMomentumShellGrid", DataContainer().insert_double(T1).insert_double(T2).insert_int(n_energies))
-    s.add(BOUNDARY, "BoundarySphereAbsorb", boundary(1, [-1], ("vec", (0.0, 0.0, 0.0)), ("double", inner_au * one_au)))
+    if inner_au is not None:
+        s.add(BOUNDARY, "BoundarySphereAbsorb", boundary(1, [-1], ("vec", (0.0, 0.0, 0.0)), ("double", inner_au * one_au)))
     s.add(BOUNDARY, "BoundarySphereAbsorb", boundary(1, [0], ("vec", (0.0, 0.0, 0.0)), ("double", r_outer_au * one_au)))
-    s.add(BOUNDARY, "BoundaryTimeExpire", boundary(1, [-1], ("double", -t_limit_years * one_year)))
+    s.add(BOUNDARY, "BoundaryTimeExpire", boundary(1, [time_action], ("double", -t_limit_years * one_year)))
     lo, hi = spectrum_bins(T1, T2, n_energies)
     dst = distribution_base(DataContainer(), (n_energies, 0, 0), (lo, 0.0, 0.0), (hi, 0.0, 0.0), (1, 0, 0), (0, 0, 0),
                             1.0, (unit_energy, 1.0, 1.0), False)

@@ -117,6 +117,8 @@ struct spx_sim {
    bool dev_ready = false;
    hipStream_t stream = nullptr;
    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+   std::vector<hipEvent_t> tev;        // per-launch events: [2i] before launch i, [2i+1] between its two kernels
+   double step_ms = 0.0, service_ms = 0.0;
    spx_plan_desc* d_plan = nullptr;
    RunCtl* d_ctl = nullptr;
    Launch* d_launch = nullptr;
@@ -255,7 +257,7 @@ static int add_boundary(spx_sim* s, const std::string& cls, Reader& r)
    else if (cls == "BoundaryMomentumInject") { b.cls = SPX_BC_MOMENTUM; b.kind = SPX_BK_MOMENTUM; b.momentum = r.d(); b.max_crossings = 1; }
    else if (cls == "BoundaryMomentumPass") { b.cls = SPX_BC_MOMENTUM; b.kind = SPX_BK_MOMENTUM; b.momentum = r.d(); }
    else return fail(s, SPX_ERR_CLASS, "unknown boundary class " + cls);
-   if (s->plan.n_bnd >= SPX_MAX_BND) return fail(s, SPX_ERR_LIMIT, "more than 6 boundaries");
+   if (s->plan.n_bnd >= SPX_MAX_BND) return fail(s, SPX_ERR_LIMIT, "more than 4 boundaries");
    // keep slots ordered time, space, momentum (the order of ComputeAllBoundaries, src/trajectory_base.cc:87-93)
    int pos = s->plan.n_bnd;
    while (pos > 0 && s->plan.bnd[pos - 1].cls > b.cls) { s->plan.bnd[pos] = s->plan.bnd[pos - 1]; pos--; }
@@ -482,6 +484,7 @@ void spx_destroy(spx_sim* s)
    if (s->d_plan) (void)hipFree(s->d_plan);
    if (s->d_ctl) (void)hipFree(s->d_ctl);
    if (s->d_launch) (void)hipFree(s->d_launch);
+   for (hipEvent_t e : s->tev) (void)hipEventDestroy(e);
    if (s->ev0) (void)hipEventDestroy(s->ev0);
    if (s->ev1) (void)hipEventDestroy(s->ev1);
    if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -614,6 +617,8 @@ static int read_stats(spx_sim* s, spx_run_stats* st, double wall, double kms, in
       st->kernel_ms = kms;
       st->launches = launches;
       st->lanes = (int32_t)s->n_lanes;
+      st->step_kernel_ms = s->step_ms;
+      st->service_kernel_ms = s->service_ms;
    }
    return SPX_OK;
 }
@@ -626,7 +631,7 @@ static int launch_many(spx_sim* s, int n, float* ms_out)
       if (!s->graph_exec) {
          HIPCHK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
          for (int i = 0; i < s->graph_launches; i++) {
-            hipError_t e = launch_transport(s->plan, s->d_plan, L, s->d_launch, s->precision, s->block, s->stream);
+            hipError_t e = launch_transport(s->plan, s->d_plan, L, s->d_launch, s->precision, s->block, s->stream, nullptr);
             if (e != hipSuccess) {
                hipGraph_t g;
                (void)hipStreamEndCapture(s->stream, &g);
@@ -641,8 +646,14 @@ static int launch_many(spx_sim* s, int n, float* ms_out)
       n = reps * s->graph_launches;
    }
    else {
+      while (s->tev.size() < (size_t)2 * n) {
+         hipEvent_t e;
+         HIPCHK(s, hipEventCreate(&e));
+         s->tev.push_back(e);
+      }
       for (int i = 0; i < n; i++) {
-         hipError_t e = launch_transport(s->plan, s->d_plan, L, s->d_launch, s->precision, s->block, s->stream);
+         HIPCHK(s, hipEventRecord(s->tev[2 * i], s->stream));
+         hipError_t e = launch_transport(s->plan, s->d_plan, L, s->d_launch, s->precision, s->block, s->stream, s->tev[2 * i + 1]);
          if (e != hipSuccess) return fail(s, SPX_ERR_DEVICE, std::string("launch: ") + hipGetErrorString(e));
       }
    }
@@ -650,6 +661,15 @@ static int launch_many(spx_sim* s, int n, float* ms_out)
    HIPCHK(s, hipEventSynchronize(s->ev1));
    float ms = 0.f;
    HIPCHK(s, hipEventElapsedTime(&ms, s->ev0, s->ev1));
+   if (s->graph_launches == 0) {
+      for (int i = 0; i < n; i++) {
+         float a = 0.f, b = 0.f;
+         HIPCHK(s, hipEventElapsedTime(&a, s->tev[2 * i], s->tev[2 * i + 1]));
+         HIPCHK(s, hipEventElapsedTime(&b, s->tev[2 * i + 1], i + 1 < n ? s->tev[2 * i + 2] : s->ev1));
+         s->service_ms += a;
+         s->step_ms += b;
+      }
+   }
    if (ms_out) *ms_out = ms;
    s->launches_total += n;
    return SPX_OK;
@@ -660,6 +680,7 @@ int spx_advance(spx_sim* s, int n_launches, spx_run_stats* st)
    if (!s || n_launches < 0) return SPX_ERR_ARG;
    if (!s->running) return fail(s, SPX_ERR_STATE, "advance without begin");
    auto w0 = std::chrono::steady_clock::now();
+   s->step_ms = s->service_ms = 0.0;
    float ms = 0.f;
    int rc = launch_many(s, n_launches, &ms);
    if (rc) return rc;
@@ -672,6 +693,7 @@ int spx_finish(spx_sim* s, spx_traj_record* records, spx_run_stats* st)
    if (!s) return SPX_ERR_ARG;
    if (!s->running) return fail(s, SPX_ERR_STATE, "finish without begin");
    auto w0 = std::chrono::steady_clock::now();
+   s->step_ms = s->service_ms = 0.0;
    double kms = 0.0;
    int launches = 0;
    // Every launch advances each live slot by steps_per_launch steps; a trajectory needs at most

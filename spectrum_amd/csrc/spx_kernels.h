@@ -59,7 +59,9 @@ struct Launch {
 };
 
 // Host-side dispatch to the kernel instantiation for (transport, precision, background, flow).
+// One engine launch is the service kernel followed by the step kernel on `stream`; when `mid` is
+// not null it is recorded between the two, so the caller can time each kernel with events.
 hipError_t launch_transport(const spx_plan_desc& host_plan, const spx_plan_desc* dev_plan, const Launch& L, const Launch* dev_launch,
-                            int precision, int block, hipStream_t stream);
+                            int precision, int block, hipStream_t stream, hipEvent_t mid);
 
 }  // namespace spx

@@ -783,51 +783,55 @@ __global__ __launch_bounds__(BLOCK) void parker_step_kernel(const spx_plan_desc*
 // Dispatch: one engine launch = service kernel + step kernel on the same stream
 // ---------------------------------------------------------------------------------------------
 template <typename R, int BG, int FWD, int TT, int DF>
-static hipError_t launch_parker(const spx_plan_desc* d_plan, const Launch& L, const Launch* d_launch, hipStream_t st)
+static hipError_t launch_parker(const spx_plan_desc* d_plan, const Launch& L, const Launch* d_launch, hipStream_t st, hipEvent_t mid)
 {
    const unsigned grid = (unsigned)((L.n_lanes + BLOCK - 1) / BLOCK);
    hipLaunchKernelGGL((parker_service<R, BG, TT>), dim3(grid), dim3(BLOCK), 0, st, d_plan, d_launch);
+   if (mid) {
+      hipError_t e = hipEventRecord(mid, st);
+      if (e != hipSuccess) return e;
+   }
    hipLaunchKernelGGL((parker_step_kernel<R, BG, FWD, TT, DF>), dim3(grid), dim3(BLOCK), 0, st, d_plan, d_launch);
    return hipGetLastError();
 }
 
 template <typename R, int BG, int FWD, int TT>
-static hipError_t dispatch_df(const spx_plan_desc& hp, const spx_plan_desc* d_plan, const Launch& L, const Launch* d_launch, hipStream_t st)
+static hipError_t dispatch_df(const spx_plan_desc& hp, const spx_plan_desc* d_plan, const Launch& L, const Launch* d_launch, hipStream_t st, hipEvent_t mid)
 {
    switch (hp.df.kind) {
-   case SPX_DF_RIGBMAG: return launch_parker<R, BG, FWD, TT, SPX_DF_RIGBMAG>(d_plan, L, d_launch, st);
-   case SPX_DF_FULL: return launch_parker<R, BG, FWD, TT, SPX_DF_FULL>(d_plan, L, d_launch, st);
-   default: return launch_parker<R, BG, FWD, TT, 0>(d_plan, L, d_launch, st);
+   case SPX_DF_RIGBMAG: return launch_parker<R, BG, FWD, TT, SPX_DF_RIGBMAG>(d_plan, L, d_launch, st, mid);
+   case SPX_DF_FULL: return launch_parker<R, BG, FWD, TT, SPX_DF_FULL>(d_plan, L, d_launch, st, mid);
+   default: return launch_parker<R, BG, FWD, TT, 0>(d_plan, L, d_launch, st, mid);
    }
 }
 
 template <typename R, int BG>
-static hipError_t dispatch_flow(const spx_plan_desc& hp, const spx_plan_desc* d_plan, const Launch& L, const Launch* d_launch, hipStream_t st)
+static hipError_t dispatch_flow(const spx_plan_desc& hp, const spx_plan_desc* d_plan, const Launch& L, const Launch* d_launch, hipStream_t st, hipEvent_t mid)
 {
    const bool fwd = hp.time_flow == 0;
-   if (hp.traj_type == 1) return fwd ? dispatch_df<R, BG, 1, 1>(hp, d_plan, L, d_launch, st) : dispatch_df<R, BG, 0, 1>(hp, d_plan, L, d_launch, st);
-   return fwd ? dispatch_df<R, BG, 1, 0>(hp, d_plan, L, d_launch, st) : dispatch_df<R, BG, 0, 0>(hp, d_plan, L, d_launch, st);
+   if (hp.traj_type == 1) return fwd ? dispatch_df<R, BG, 1, 1>(hp, d_plan, L, d_launch, st, mid) : dispatch_df<R, BG, 0, 1>(hp, d_plan, L, d_launch, st, mid);
+   return fwd ? dispatch_df<R, BG, 1, 0>(hp, d_plan, L, d_launch, st, mid) : dispatch_df<R, BG, 0, 0>(hp, d_plan, L, d_launch, st, mid);
 }
 
 template <typename R>
-static hipError_t dispatch_parker(const spx_plan_desc& hp, const spx_plan_desc* d_plan, const Launch& L, const Launch* d_launch, hipStream_t st)
+static hipError_t dispatch_parker(const spx_plan_desc& hp, const spx_plan_desc* d_plan, const Launch& L, const Launch* d_launch, hipStream_t st, hipEvent_t mid)
 {
    switch (hp.bg.kind) {
-   case SPX_BG_SOLARWIND: return dispatch_flow<R, SPX_BG_SOLARWIND>(hp, d_plan, L, d_launch, st);
-   case SPX_BG_SMOOTHSHOCK: return dispatch_flow<R, SPX_BG_SMOOTHSHOCK>(hp, d_plan, L, d_launch, st);
-   case SPX_BG_SHOCK: return dispatch_flow<R, SPX_BG_SHOCK>(hp, d_plan, L, d_launch, st);
-   case SPX_BG_UNIFORM: return dispatch_flow<R, SPX_BG_UNIFORM>(hp, d_plan, L, d_launch, st);
+   case SPX_BG_SOLARWIND: return dispatch_flow<R, SPX_BG_SOLARWIND>(hp, d_plan, L, d_launch, st, mid);
+   case SPX_BG_SMOOTHSHOCK: return dispatch_flow<R, SPX_BG_SMOOTHSHOCK>(hp, d_plan, L, d_launch, st, mid);
+   case SPX_BG_SHOCK: return dispatch_flow<R, SPX_BG_SHOCK>(hp, d_plan, L, d_launch, st, mid);
+   case SPX_BG_UNIFORM: return dispatch_flow<R, SPX_BG_UNIFORM>(hp, d_plan, L, d_launch, st, mid);
    default: return hipErrorInvalidValue;
    }
 }
 
 hipError_t launch_transport(const spx_plan_desc& hp, const spx_plan_desc* d_plan, const Launch& L, const Launch* d_launch, int precision, int block,
-                            hipStream_t st)
+                            hipStream_t st, hipEvent_t mid)
 {
    (void)block;
    if (hp.traj_type == 0 || hp.traj_type == 1) {
-      if (precision == 1) return dispatch_parker<float>(hp, d_plan, L, d_launch, st);
-      return dispatch_parker<double>(hp, d_plan, L, d_launch, st);
+      if (precision == 1) return dispatch_parker<float>(hp, d_plan, L, d_launch, st, mid);
+      return dispatch_parker<double>(hp, d_plan, L, d_launch, st, mid);
    }
    return hipErrorInvalidValue;
 }

@@ -65,6 +65,10 @@ class DataContainer:
         vals = [float(x) for x in v]
         return self._record(struct.pack(f"<{len(vals)}d", *vals))
 
+    def insert_string(self, s: str) -> "DataContainer":
+        """std::string: raw characters, no terminator (common/data_container.cc:159-175)."""
+        return self._record(s.encode())
+
     # -- access ----------------------------------------------------------------------------
     def bytes(self) -> bytes:
         return bytes(self._buf)
@@ -87,3 +91,48 @@ class DataContainer:
             out.append(blob[i + 1:i + 1 + n])
             i += 1 + n
         return out
+
+
+class DataReader:
+    """Sequential reader over a blob; ``read_*`` mirror ``DataContainer::Read<T>``
+    (common/data_container.cc:203-240): each call consumes one record."""
+
+    def __init__(self, blob: bytes) -> None:
+        self._recs = DataContainer.records(blob)
+        self._i = 0
+
+    def _next(self) -> bytes:
+        if self._i >= len(self._recs):
+            raise ValueError("read past the last record")
+        r = self._recs[self._i]
+        self._i += 1
+        return r
+
+    def reset(self) -> None:
+        self._i = 0
+
+    def read_int(self) -> int:
+        return struct.unpack("<i", self._next())[0]
+
+    def read_double(self) -> float:
+        return struct.unpack("<d", self._next())[0]
+
+    def read_bool(self) -> bool:
+        return self._next()[0] != 0
+
+    def read_vec(self) -> tuple:
+        return struct.unpack("<3d", self._next())
+
+    def read_multiindex(self) -> tuple:
+        return struct.unpack("<3i", self._next())
+
+    def read_string(self) -> str:
+        return self._next().decode()
+
+    def read_int_vector(self) -> list:
+        r = self._next()
+        return list(struct.unpack(f"<{len(r) // 4}i", r))
+
+    def read_double_vector(self) -> list:
+        r = self._next()
+        return list(struct.unpack(f"<{len(r) // 8}d", r))

@@ -37,7 +37,8 @@ class SpxRng(C.Structure):
 class SpxRunStats(C.Structure):
     _fields_ = [("n_traj", C.c_uint64), ("done", C.c_uint64), ("failed", C.c_uint64), ("steps_done", C.c_uint64),
                 ("steps_executed", C.c_uint64), ("discards", C.c_uint64 * 8), ("t_min", C.c_double), ("t_max", C.c_double),
-                ("wall_s", C.c_double), ("kernel_ms", C.c_double), ("launches", C.c_int32), ("lanes", C.c_int32)]
+                ("wall_s", C.c_double), ("kernel_ms", C.c_double), ("launches", C.c_int32), ("lanes", C.c_int32),
+                ("step_kernel_ms", C.c_double), ("service_kernel_ms", C.c_double)]
 
 
 RECORD_DTYPE = np.dtype([("t", "f8"), ("pos", "f8", 3), ("mom", "f8", 3), ("amp", "f8"), ("wgt", "f8"),
@@ -93,12 +94,14 @@ class RunStats:
     kernel_ms: float
     launches: int
     lanes: int
+    step_kernel_ms: float
+    service_kernel_ms: float
 
     @staticmethod
     def of(s: SpxRunStats) -> "RunStats":
         return RunStats(s.n_traj, s.done, s.failed, s.steps_done, s.steps_executed,
                         {EX_NAMES[k]: int(s.discards[k]) for k in range(1, 7)}, s.t_min, s.t_max, s.wall_s,
-                        s.kernel_ms, s.launches, s.lanes)
+                        s.kernel_ms, s.launches, s.lanes, s.step_kernel_ms, s.service_kernel_ms)
 
 
 class SpxError(RuntimeError):
