@@ -35,6 +35,7 @@ def workloads():
     return [
         ("parker_modulation_64x156250", cf.parker_modulation(n_energies=64), 64, 1),
         ("dsa_smooth_shock", cf.dsa_shock(), 32, 1),
+        ("gc_diff_scatt", cf.gc_diff_scatt(), 16, 1),
     ]
 
 

@@ -301,7 +301,7 @@ enum { DF_NONE = 0, DF_ISO, DF_PARA, DF_PERP, DF_FULL, DF_FLOWMOM, DF_KINRAD, DF
 enum { BK_TIME = 0, BK_SPHERE, BK_PLANE, BK_MOMENTUM };
 enum { IT_FIXED = 0 };
 enum { IS_FIXED = 0 };
-enum { IM_FIXED = 0, IM_SHELL, IM_THICKSHELL, IM_SHELLGRID };
+enum { IM_FIXED = 0, IM_SHELL, IM_THICKSHELL, IM_SHELLGRID, IM_RING };
 enum { DS_SPECTRUM = 0, DS_TIME, DS_MOMENTUM, DS_POSMOM, DS_CUMO1, DS_CUMO2, DS_POSPITCH };
 enum { SRC_NONE = 0, SRC_MOMINJ };
 
@@ -349,7 +349,7 @@ struct orc_sim {
    int src_kind; double src_p_inj, src_rate;
    int it_kind; double it_t;
    int is_kind; vec3 is_pos;
-   int im_kind; double im_p0, im_p1, im_p2; int im_log; vec3 im_mom; double im_T1, im_T2; int im_n;
+   int im_kind; double im_p0, im_p1, im_p2; int im_log; vec3 im_mom; double im_T1, im_T2; int im_n; double im_mu0, im_st0;
    int n_t, n_s, n_m; bnd_t bt[MAX_BND], bs[MAX_BND], bm[MAX_BND];
    int n_dist; dist_t dist[MAX_DIST];
    int spectrum_type;          /* DISTRO_KINETIC_ENERGY_POWER_LAW_TYPE, src/distribution_other.hh:325 */
@@ -1147,6 +1147,9 @@ static vec3 mom_sample(const orc_sim* s, lane_t* L)
    double p;
    switch (s->im_kind) {
    case IM_FIXED: return s->im_mom;
+   case IM_RING:  /* src/initial_momentum.cc:203-220: fixed (p sin theta0, 0, p cos theta0), no draw */
+      if (gc) return V(s->im_p0 * s->im_st0, 0.0, s->im_p0 * s->im_mu0);
+      return V(s->im_p0, 0.0, 0.0);
    case IM_SHELL: p = s->im_p0; break;
    case IM_THICKSHELL:
       if (s->im_log) p = pow(10.0, s->im_p1 + (s->im_p2 - s->im_p1) * draw_uniform(L));
@@ -1334,6 +1337,11 @@ static int setup_initial(orc_sim* s, const char* cls, dcread* r)
    else if (!strcmp(cls, "InitialSpaceFixed")) { s->is_kind = IS_FIXED; s->is_pos = rd_v(r); }
    else if (!strcmp(cls, "InitialMomentumFixed")) { s->im_kind = IM_FIXED; s->im_mom = rd_v(r); }
    else if (!strcmp(cls, "InitialMomentumShell")) { s->im_kind = IM_SHELL; s->im_p0 = rd_d(r); }
+   else if (!strcmp(cls, "InitialMomentumRing")) {
+      double th;
+      s->im_kind = IM_RING; s->im_p0 = rd_d(r); th = rd_d(r);
+      s->im_mu0 = cos(th); s->im_st0 = sin(th);
+   }
    else if (!strcmp(cls, "InitialMomentumThickShell")) {
       s->im_kind = IM_THICKSHELL; s->im_p1 = rd_d(r); s->im_p2 = rd_d(r); s->im_log = rd_b(r);
       if (s->im_log) { s->im_p1 = log10(s->im_p1); s->im_p2 = log10(s->im_p2); }

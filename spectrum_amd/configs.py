@@ -216,3 +216,46 @@ def gc_diff_scatt(n_r=32, n_mu=64, T_MeV=100.0, nu=1.0e-4, kperp_cgs=8.6e19, t_l
     s.add(DISTRIBUTION, "DistributionPositionPitchUniform", dst)
     s.meta.update(D0=D0, p=mom_from_ekin(T_MeV * MeV))
     return s
+
+
+def gc_uniform(D0=1.0e-3, kperp=0.5, T_MeV=100.0, theta0=None, t_limit=500.0, B=(0.0, 0.0, 10.0), name="gc_uniform") -> Setup:
+    """Guiding centre in a uniform field with U = 0: free streaming along B, pitch-angle scattering
+    D_mumu = D0 (1 - mu^2) and constant perpendicular diffusion; started at the origin either as a
+    ring at pitch angle theta0 (InitialMomentumRing) or isotropic (InitialMomentumShell), stopped and
+    binned at t_limit.  Closed forms: <P_l(mu)> decays as exp(-l(l+1) D0 t), <x^2 + y^2> = 4 kperp t,
+    and for an isotropic start <z^2> = (2 v^2 / 3) [t / (2 D0) - (1 - exp(-2 D0 t)) / (4 D0^2)]."""
+    s = Setup(name, TRAJ_GUIDING_DIFF_SCATT, FORWARD)
+    s.add(BACKGROUND, "BackgroundUniform", background_base(DataContainer(), 0.0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), B, 1.0e6))
+    s.add(DIFFUSION, "DiffusionPerpConstantPitchIsotropic", DataContainer().insert_double(kperp).insert_double(D0))
+    s.add(INITIAL, "InitialTimeFixed", DataContainer().insert_double(0.0))
+    s.add(INITIAL, "InitialSpaceFixed", DataContainer().insert_vec((0.0, 0.0, 0.0)))
+    p = mom_from_ekin(T_MeV * MeV)
+    if theta0 is None:
+        s.add(INITIAL, "InitialMomentumShell", DataContainer().insert_double(p))
+    else:
+        s.add(INITIAL, "InitialMomentumRing", DataContainer().insert_double(p).insert_double(theta0))
+    s.add(BOUNDARY, "BoundaryTimeExpire", boundary(1, [0], ("double", t_limit)))
+    dst = distribution_base(DataContainer(), (10, 0, 0), (0.0, 0, 0), (1.1 * t_limit, 0, 0), (0, 0, 0), (0, 0, 0))
+    dst.insert_double(1.0).insert_double(0.0).insert_int(1)
+    s.add(DISTRIBUTION, "DistributionTimeUniform", dst)
+    s.meta.update(D0=D0, kperp=kperp, p=p, v=vel_from_mom(p), theta0=theta0, t_limit=t_limit)
+    return s
+
+
+def uniform_walk(D=2.0, U=(0.0, 0.0, 0.0), B=(3.0, 4.0, 0.0), t_limit=1000.0, dmax=5.0, flow=BACKWARD, name="uniform_walk") -> Setup:
+    """Parker transport in a uniform background with constant isotropic diffusion (DiffusionFullConstant
+    with kappa_perp = kappa_par = D): x(t) - x0 - (-/+) U t is Gaussian with variance 2 D t per axis."""
+    s = Setup(name, TRAJ_PARKER, flow)
+    s.add(BACKGROUND, "BackgroundUniform", background_base(DataContainer(), 0.0, (0.0, 0.0, 0.0), U, B, dmax))
+    s.add(DIFFUSION, "DiffusionFullConstant", DataContainer().insert_double(D).insert_double(D))
+    s.add(INITIAL, "InitialTimeFixed", DataContainer().insert_double(0.0))
+    s.add(INITIAL, "InitialSpaceFixed", DataContainer().insert_vec((1.0, 2.0, 3.0)))
+    s.add(INITIAL, "InitialMomentumShell", DataContainer().insert_double(mom_from_ekin(MeV)))
+    tmark = -t_limit if flow == BACKWARD else t_limit
+    s.add(BOUNDARY, "BoundaryTimeExpire", boundary(1, [0], ("double", tmark)))
+    lo, hi = (tmark * 1.1, 0.0) if flow == BACKWARD else (0.0, tmark * 1.1)
+    dst = distribution_base(DataContainer(), (10, 0, 0), (lo, 0, 0), (hi, 0, 0), (0, 0, 0), (0, 0, 0))
+    dst.insert_double(1.0).insert_double(0.0).insert_int(1)
+    s.add(DISTRIBUTION, "DistributionTimeUniform", dst)
+    s.meta.update(D=D, U=U, t_limit=t_limit)
+    return s

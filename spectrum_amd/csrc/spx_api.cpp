@@ -124,7 +124,7 @@ struct spx_sim {
    Launch* d_launch = nullptr;
    uint64_t n_lanes = 0;
    DevBuf<int32_t> active, bcross, bstat, attempt, n_events, ev_bin, ev_bnd, ev_pending, ex_code;
-   DevBuf<double> t, x, y, z, m0, m1, m2, amp, wgt, dt_prev, src_old, ds_prev, dmax, dt_adapt, bdelta, bdelta_old;
+   DevBuf<double> t, x, y, z, m0, m1, m2, amp, wgt, dt_prev, src_old, ds_prev, dmax, dt_adapt, pf, bdelta, bdelta_old;
    DevBuf<double> t0, x0, y0, z0, m00, m01, m02, ev_w, evs;
    DevBuf<uint64_t> traj;
    DevBuf<uint32_t> nn, nu;
@@ -276,6 +276,12 @@ static int add_initial(spx_sim* s, const std::string& cls, Reader& r)
    else if (cls == "InitialSpaceFixed") { r.v(P.is_pos); s->have_is = true; }                    // src/initial_space.cc:49-59
    else if (cls == "InitialMomentumFixed") { P.im_kind = SPX_IM_FIXED; r.v(P.im_mom); s->have_im = true; }
    else if (cls == "InitialMomentumShell") { P.im_kind = SPX_IM_SHELL; P.im_p0 = r.d(); s->have_im = true; }   // src/initial_momentum.cc:283-295
+   else if (cls == "InitialMomentumRing") {                                                       // src/initial_momentum.cc:203-220
+      P.im_kind = SPX_IM_RING; P.im_p0 = r.d();
+      const double th = r.d();
+      P.im_mu0 = std::cos(th); P.im_st0 = std::sin(th);
+      s->have_im = true;
+   }
    else if (cls == "InitialMomentumThickShell") {                                                 // src/initial_momentum.cc:363-377
       P.im_kind = SPX_IM_THICKSHELL; P.im_p1 = r.d(); P.im_p2 = r.d(); P.im_log = r.b() ? 1 : 0;
       if (P.im_log) { P.im_p1 = std::log10(P.im_p1); P.im_p2 = std::log10(P.im_p2); }
@@ -373,7 +379,7 @@ static int alloc_lanes(spx_sim* s, uint64_t n)
    HIPCHK(s, s->ev_bin.alloc(n * SPX_MAX_DIST)); HIPCHK(s, s->ev_w.alloc(n * SPX_MAX_DIST));
    HIPCHK(s, s->evs.alloc(n * 3 * 9)); HIPCHK(s, s->ev_bnd.alloc(n * 3)); HIPCHK(s, s->ev_pending.alloc(n)); HIPCHK(s, s->ex_code.alloc(n));
    DevBuf<double>* dv[] = {&s->t, &s->x, &s->y, &s->z, &s->m0, &s->m1, &s->m2, &s->amp, &s->wgt, &s->dt_prev, &s->src_old,
-                           &s->ds_prev, &s->dmax, &s->dt_adapt, &s->t0, &s->x0, &s->y0, &s->z0, &s->m00, &s->m01, &s->m02};
+                           &s->ds_prev, &s->dmax, &s->dt_adapt, &s->pf, &s->t0, &s->x0, &s->y0, &s->z0, &s->m00, &s->m01, &s->m02};
    for (auto* b : dv) HIPCHK(s, b->alloc(n));
    HIPCHK(s, s->bdelta.alloc(n * SPX_MAX_BND)); HIPCHK(s, s->bdelta_old.alloc(n * SPX_MAX_BND));
    HIPCHK(s, s->traj.alloc(n)); HIPCHK(s, s->nn.alloc(n)); HIPCHK(s, s->nu.alloc(n)); HIPCHK(s, s->steps.alloc(n));
@@ -395,7 +401,7 @@ static Launch make_launch(spx_sim* s)
    L.active = s->active.p;
    L.t = s->t.p; L.x = s->x.p; L.y = s->y.p; L.z = s->z.p; L.m0 = s->m0.p; L.m1 = s->m1.p; L.m2 = s->m2.p;
    L.amp = s->amp.p; L.wgt = s->wgt.p; L.dt_prev = s->dt_prev.p; L.src_old = s->src_old.p; L.ds_prev = s->ds_prev.p;
-   L.dmax = s->dmax.p; L.dt_adapt = s->dt_adapt.p;
+   L.dmax = s->dmax.p; L.dt_adapt = s->dt_adapt.p; L.pf = s->pf.p;
    L.bdelta = s->bdelta.p; L.bdelta_old = s->bdelta_old.p; L.bcross = s->bcross.p; L.bstat = s->bstat.p;
    L.traj = s->traj.p; L.nn = s->nn.p; L.nu = s->nu.p; L.attempt = s->attempt.p; L.steps = s->steps.p; L.n_events = s->n_events.p;
    L.t0 = s->t0.p; L.x0 = s->x0.p; L.y0 = s->y0.p; L.z0 = s->z0.p; L.m00 = s->m00.p; L.m01 = s->m01.p; L.m02 = s->m02.p;
@@ -476,7 +482,7 @@ void spx_destroy(spx_sim* s)
    DevBuf<int32_t>* di[] = {&s->active, &s->bcross, &s->bstat, &s->attempt, &s->n_events, &s->ev_bin, &s->ev_bnd, &s->ev_pending, &s->ex_code};
    for (auto* b : di) b->release();
    DevBuf<double>* dv[] = {&s->t, &s->x, &s->y, &s->z, &s->m0, &s->m1, &s->m2, &s->amp, &s->wgt, &s->dt_prev, &s->src_old, &s->ds_prev,
-                           &s->dmax, &s->dt_adapt, &s->bdelta, &s->bdelta_old, &s->t0, &s->x0, &s->y0, &s->z0, &s->m00, &s->m01,
+                           &s->dmax, &s->dt_adapt, &s->pf, &s->bdelta, &s->bdelta_old, &s->t0, &s->x0, &s->y0, &s->z0, &s->m00, &s->m01,
                            &s->m02, &s->ev_w, &s->evs, &s->hist_sums, &s->rep_normals, &s->rep_uniforms};
    for (auto* b : dv) b->release();
    s->traj.release(); s->nn.release(); s->nu.release(); s->steps.release();
@@ -538,8 +544,13 @@ int spx_begin(spx_sim* s, uint64_t first, uint64_t n_traj, const spx_rng* rng, i
    finalize_plan(s);
    rc = ensure_device(s);
    if (rc) return rc;
-   if (s->plan.traj_type == SPX_TRAJ_GUIDING_DIFF_SCATT && s->plan.time_flow != SPX_FLOW_FORWARD)
-      return fail(s, SPX_ERR_ARG, "guiding-centre transport runs forward in time");
+   if (s->plan.traj_type == SPX_TRAJ_GUIDING_DIFF_SCATT) {
+      if (s->plan.time_flow != SPX_FLOW_FORWARD) return fail(s, SPX_ERR_ARG, "guiding-centre transport runs forward in time");
+      if (s->plan.bg.kind != SPX_BG_SOLARWIND && s->plan.bg.kind != SPX_BG_UNIFORM)
+         return fail(s, SPX_ERR_ARG, "guiding-centre transport supports the static SolarWind and Uniform backgrounds");
+      if (s->plan.df.kind == SPX_DF_FLOWMOM || s->plan.df.kind == SPX_DF_KINRAD)
+         return fail(s, SPX_ERR_ARG, "guiding-centre transport: diffusion model needs grad U, which its field mask does not compute");
+   }
    uint64_t lanes = s->lanes_opt > 0 ? (uint64_t)s->lanes_opt : auto_lanes(s);
    lanes = (lanes + s->block - 1) / s->block * s->block;
    rc = alloc_lanes(s, lanes);

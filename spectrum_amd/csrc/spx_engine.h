@@ -1,0 +1,452 @@
+// spx_engine.h -- device pieces shared by the transports' service and step kernels.
+//
+// Slot states, wave-level work claiming, random streams, boundary evaluation and handling,
+// distribution binning and trajectory records.  Included by the kernel translation units only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "spx_physics.h"
+#include "spx_kernels.h"
+
+namespace spx {
+
+// Slot states
+enum { SLOT_IDLE = 0, SLOT_RUN = 1, SLOT_START = 2, SLOT_EVENT = 3, SLOT_DONE = 4, SLOT_DISCARD = 5 };
+
+constexpr int BLOCK = 256;
+
+// ---------------------------------------------------------------------------------------------
+// Wave helpers (64 lanes)
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+__device__ __forceinline__ uint64_t lanes_below() { return (1ull << lane_id()) - 1ull; }
+
+// Hand out trajectory ids to the requesting lanes of this wave with a single atomic.
+__device__ __forceinline__ uint64_t wave_claim(RunCtl* ctl, bool need)
+{
+   const uint64_t mask = __ballot(need);
+   if (mask == 0) return ~0ull;
+   const int leader = __ffsll((unsigned long long)mask) - 1;
+   unsigned long long base = 0;
+   if (lane_id() == leader) base = atomicAdd(&ctl->next, (unsigned long long)__popcll(mask));
+   base = __shfl(base, leader);
+   return need ? base + (uint64_t)__popcll(mask & lanes_below()) : ~0ull;
+}
+
+__device__ __forceinline__ void atomic_min_pos(unsigned long long* a, double v)
+{
+   if (v >= 0.0) atomicMin(a, (unsigned long long)__double_as_longlong(v));
+}
+__device__ __forceinline__ void atomic_max_pos(unsigned long long* a, double v)
+{
+   if (v >= 0.0) atomicMax(a, (unsigned long long)__double_as_longlong(v));
+}
+
+// ---------------------------------------------------------------------------------------------
+// Random draws: counter streams (Philox4x32-10 + Box-Muller pairs) or replayed buffers
+// ---------------------------------------------------------------------------------------------
+template <typename R>
+struct NormalStream {
+   uint64_t traj;
+   uint32_t nn;          // normal draws consumed
+   R spare;              // sine branch of the last even draw
+   bool spare_ok;
+   bool exhausted;
+};
+
+template <typename R>
+__device__ __forceinline__ R next_normal(const spx_rng_desc& rng, uint64_t first, NormalStream<R>& S)
+{
+   const uint32_t k = S.nn++;
+   if (rng.mode == 1) {
+      if ((int64_t)k >= rng.normal_stride) { S.exhausted = true; return R(0); }
+      return R(rng.normals[(S.traj - first) * (uint64_t)rng.normal_stride + k]);
+   }
+   if ((k & 1u) && S.spare_ok) { S.spare_ok = false; return S.spare; }
+   R c, s;
+   normal_pair<R>(rng.seed, S.traj, k >> 1, c, s);
+   if (k & 1u) return s;
+   S.spare = s;
+   S.spare_ok = true;
+   return c;
+}
+
+__device__ __forceinline__ double next_uniform(const spx_rng_desc& rng, uint64_t first, uint64_t traj, uint32_t& nu, bool& exhausted)
+{
+   const uint32_t k = nu++;
+   if (rng.mode == 1) {
+      if ((int64_t)k >= rng.uniform_stride) { exhausted = true; return 0.5; }
+      return rng.uniforms[(traj - first) * (uint64_t)rng.uniform_stride + k];
+   }
+   return counter_uniform(rng.seed, traj, k);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Boundaries (src/boundary_base.cc:105-191, src/boundary_{time,space,momentum}.cc)
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ double bnd_eval(const spx_bnd_desc& b, int cross_left, bool gc, double t, double x, double y, double z,
+                                           double m0, double m1, double m2, double* nrm)
+{
+   switch (b.kind) {
+   case SPX_BK_TIME:
+      if (b.status0 & SPX_BND_RECURRENT) return t - (double)(b.max_crossings - cross_left + 1) * b.timemark;  // src/boundary_time.cc:205-208
+      return t - b.timemark;                                                                                   // :70-73
+   case SPX_BK_SPHERE: {                                                                                        // src/boundary_space.cc:395-411
+      const v3<double> d = mk<double>(x - b.origin[0], y - b.origin[1], z - b.origin[2]);
+      if (nrm) { const v3<double> u = unit(d); nrm[0] = u.x; nrm[1] = u.y; nrm[2] = u.z; }
+      return norm(d) - b.radius;
+   }
+   case SPX_BK_PLANE: {                                                                                         // src/boundary_space.cc:71-84
+      if (nrm) { nrm[0] = b.norm[0]; nrm[1] = b.norm[1]; nrm[2] = b.norm[2]; }
+      const v3<double> d = mk<double>(x - b.origin[0], y - b.origin[1], z - b.origin[2]);
+      return dot(d, mk<double>(b.norm[0], b.norm[1], b.norm[2]));
+   }
+   default:                                                                                                     // src/boundary_momentum.cc:69-76
+      if (gc) return norm(mk<double>(m0, m1, m2)) - b.momentum;
+      return m0 - b.momentum;
+   }
+}
+
+// Does boundary b carry an action >= 0 for any distribution?
+__device__ __forceinline__ bool bnd_has_action(const spx_plan_desc* __restrict__ P, int b)
+{
+   const spx_bnd_desc& B = P->bnd[b];
+   bool any = false;
+   for (int d = 0; d < P->n_dist; d++) any = any || (d < B.n_act && B.actions[d] >= 0);
+   return any;
+}
+
+// Pending-event buffer: 3 entries per slot (momentum, space, time classes), 9 doubles each.
+enum { EV_T = 0, EV_X, EV_Y, EV_Z, EV_M0, EV_M1, EV_M2, EV_AMP, EV_WGT, EV_N };
+__device__ __forceinline__ double& ev_field(const Launch* __restrict__ L, int cls, int f, uint64_t slot)
+{
+   return L->evs[((uint64_t)cls * EV_N + f) * L->n_lanes + slot];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Distribution events (src/distribution_base.cc:174-194, src/distribution_templated.cc:145-180)
+// Bins the events of every distribution for a crossing of boundary `bnd`: value and weight per
+// distribution, exact integer counts and FP64 sums, and the per-slot last-event record.
+// ---------------------------------------------------------------------------------------------
+static __device__ int bin_events(const spx_plan_desc* __restrict__ P, const Launch* __restrict__ L, uint64_t slot, int bnd,
+                          double t2, double x2, double y2, double z2, double m20, double m21, double m22, double amp, double logwgt)
+{
+   const bool parker = (P->traj_type != 2);
+   const double t1 = L->t0[slot], x1 = L->x0[slot], y1 = L->y0[slot], z1 = L->z0[slot];
+   const double m10 = L->m00[slot], m11 = L->m01[slot], m12 = L->m02[slot];
+   const spx_bnd_desc& B = P->bnd[bnd];
+   int n_ev = 0;
+   for (int d = 0; d < P->n_dist; d++) {
+      const int action = d < B.n_act ? B.actions[d] : -1;
+      if (action < 0) continue;
+      n_ev++;
+      const spx_ds_desc& D = P->dist[d];
+      const bool first = D.val_time == 0;
+      const double px = first ? x1 : x2, py = first ? y1 : y2, pz = first ? z1 : z2;
+      const double ma = first ? m10 : m20, mb = first ? m11 : m21, mc = first ? m12 : m22;
+      double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+      switch (D.kind) {
+      case SPX_DS_SPECTRUM: v0 = EnrKin<double>(parker ? m10 : norm(mk<double>(m10, m11, m12))); break;   // src/distribution_other.cc:561-570
+      case SPX_DS_TIME: v0 = first ? t1 : t2; break;                                                      // :143-148
+      case SPX_DS_MOMENTUM:                                                                                // :262-290
+         if (D.val_coord == 0 || parker) { v0 = ma; v1 = mb; v2 = mc; }
+         else { v0 = norm(mk<double>(ma, mb, mc)); v1 = mc / v0; v2 = 0.0; }
+         break;
+      case SPX_DS_POSMOM:                                                                                  // :342-353
+         v0 = D.pos_idx == 3 ? norm(mk<double>(px, py, pz)) : (D.pos_idx == 0 ? px : (D.pos_idx == 1 ? py : pz));
+         v1 = D.mom_idx == 0 ? ma : (D.mom_idx == 1 ? mb : mc);
+         break;
+      case SPX_DS_POSPITCH:                                                                                // (|x|, p_par/|p|)
+         v0 = norm(mk<double>(px, py, pz));
+         v1 = mc / norm(mk<double>(ma, mb, mc));
+         break;
+      default: v0 = t2; break;                                                                             // :730-733, :777-780
+      }
+      // EvaluateWeight through the action table
+      double w[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      bool keep = true;
+      switch (D.kind) {
+      case SPX_DS_SPECTRUM:                                                                                // :575-594
+         if (action == 0) {
+            const double m2 = parker ? m20 : norm(mk<double>(m20, m21, m22));
+            const double T = EnrKin<double>(m2);
+            if (P->spectrum_type == 0) w[0] = D.J0 * VelP<double>(m2) * pow(T / D.T0, D.pow_law) / sqr(m2);
+            else if (P->spectrum_type == 1) w[0] = D.J0 * pow(T / D.T0, D.pow_law) / sqr(m2);
+            else w[0] = D.J0 * pow(T / D.T0, D.pow_law);
+         }
+         else if (action == 1) w[0] = D.val_cold;
+         else keep = false;
+         break;
+      case SPX_DS_CUMO1:                                                                                   // :738-741
+         if (action != 0) { keep = false; break; }
+         w[0] = x2 - x1; w[1] = y2 - y1; w[2] = z2 - z1;
+         break;
+      case SPX_DS_CUMO2: {                                                                                 // :785-788
+         if (action != 0) { keep = false; break; }
+         const double dx = x2 - x1, dy = y2 - y1, dz = z2 - z1;
+         w[0] = dx * dx; w[1] = dx * dy; w[2] = dx * dz; w[4] = dy * dy; w[5] = dy * dz; w[8] = dz * dz;
+         w[3] = w[1]; w[6] = w[2]; w[7] = w[5];
+         break;
+      }
+      default:                                                                                             // :79-93
+         if (action == 0) w[0] = D.val_hot;
+         else if (action == 1) w[0] = D.val_cold;
+         else keep = false;
+      }
+      // AddEvent: bin index per active dimension
+      int bin[3] = {0, 0, 0};
+#pragma unroll
+      for (int ijk = 0; ijk < 3; ijk++) {
+         if (!keep || !(D.dims & (1 << ijk))) continue;
+         const double val = ijk == 0 ? v0 : (ijk == 1 ? v1 : v2);
+         const double v = (D.log_bins[ijk] ? log10(val) : val) - D.limits0[ijk];
+         if (v < 0.0 && !D.bin_outside[ijk]) { keep = false; continue; }
+         int b = (int)(v / D.bin_size[ijk]);
+         if (b < 0) b = 0;   // the reference can index below zero here (src/distribution_templated.cc:162-167)
+         if (b >= D.n_bins[ijk]) {
+            if (D.bin_outside[ijk]) b = D.n_bins[ijk] - 1;
+            else keep = false;
+         }
+         bin[ijk] = b;
+      }
+      if (!keep) { L->ev_bin[(uint64_t)d * L->n_lanes + slot] = -1; continue; }
+      const int lin = D.n_bins[2] * (D.n_bins[1] * bin[0] + bin[1]) + bin[2];
+      const double ew = exp(logwgt);
+#pragma unroll
+      for (int k = 0; k < 9; k++) w[k] = w[k] * ew + amp;
+      const int64_t off = L->plan_host_hist_off[d] + lin;
+      atomicAdd(&L->hist_counts[off], 1ull);
+#pragma unroll
+      for (int k = 0; k < 9; k++)
+         if (k < D.width) atomicAdd(&L->hist_sums[off * 9 + k], w[k]);
+      atomicAdd(&L->hist_events[d], 1ull);
+      L->ev_bin[(uint64_t)d * L->n_lanes + slot] = lin;
+      L->ev_w[(uint64_t)d * L->n_lanes + slot] = w[0];
+   }
+   return n_ev;
+}
+
+// Record a finished (or failed) trajectory: run statistics and, when asked, its endpoint.
+static __device__ void finish_record(const Launch* __restrict__ Lp, uint64_t slot, uint64_t traj, int ex, int attempts, int64_t steps, int n_events,
+                              double t, double x, double y, double z, double m0, double m1, double m2, double amp, double wgt)
+{
+   const Launch& L = *Lp;
+   if (ex == 0) {
+      const double el = fabs(t - L.t0[slot]);
+      atomic_min_pos(&L.ctl->tmin_bits, el);
+      atomic_max_pos(&L.ctl->tmax_bits, el);
+      atomicAdd(&L.ctl->steps_done, (unsigned long long)steps);
+   }
+   else atomicAdd(&L.ctl->failed, 1ull);
+   atomicAdd(&L.ctl->done, 1ull);
+   if (L.records && traj >= L.rec_first && traj < L.rec_first + L.rec_n) {
+      spx_traj_record& r = L.records[traj - L.rec_first];
+      r.t = t; r.pos[0] = x; r.pos[1] = y; r.pos[2] = z; r.mom[0] = m0; r.mom[1] = m1; r.mom[2] = m2;
+      r.amp = amp; r.wgt = wgt;
+      r.t0 = L.t0[slot]; r.pos0[0] = L.x0[slot]; r.pos0[1] = L.y0[slot]; r.pos0[2] = L.z0[slot];
+      r.mom0[0] = L.m00[slot]; r.mom0[1] = L.m01[slot]; r.mom0[2] = L.m02[slot];
+      r.steps = steps;
+      r.status = ex == 0 ? 0 : (1 | (ex << 8));
+      r.attempts = attempts;
+      r.n_events = n_events;
+      for (int d = 0; d < 4; d++) {
+         const bool has = ex == 0 && d < L.n_dist && L.ev_bin[(uint64_t)d * L.n_lanes + slot] >= 0;
+         r.ev_bin[d] = has ? L.ev_bin[(uint64_t)d * L.n_lanes + slot] : -1;
+         r.ev_weight[d] = has ? L.ev_w[(uint64_t)d * L.n_lanes + slot] : 0.0;
+      }
+   }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Step-kernel helpers
+// ---------------------------------------------------------------------------------------------
+// Per-lane state that is rarely touched lives in LDS for the duration of a launch.
+struct ColdLDS {
+   double amp[BLOCK], wgt[BLOCK], dt_adapt[BLOCK], dt_prev[BLOCK], src_old[BLOCK];
+   int32_t cross[SPX_MAX_BND][BLOCK];
+};
+
+
+// TrajectoryBase::TimeBoundaryProximityCheck (src/trajectory_base.cc:151-179)
+template <int FWD, typename HOT>
+__device__ __forceinline__ double time_proximity(const spx_plan_desc* __restrict__ P, const HOT& H, double dt, double dmax)
+{
+   double dn = FWD ? -sp_large * dmax / c_code : sp_large * dmax / c_code;
+#pragma unroll
+   for (int b = 0; b < SPX_MAX_BND; b++) {
+      if (b < P->n_bt) {
+         const double d = H.bd[b];
+         if (FWD) { if ((d <= 0.0) && (d > dn)) dn = d; }
+         else { if ((d >= 0.0) && (d < dn)) dn = d; }
+      }
+   }
+   if (FWD) { if (dt >= -dn) dt = fmax(-(1.0 + sp_little) * dn, sp_small * dmax / c_code); }
+   else { if (dt >= dn) dt = fmax((1.0 + sp_little) * dn, sp_small * dmax / c_code); }
+   return dt;
+}
+
+// TrajectoryBase::HandleBoundaries (src/trajectory_base.cc:413-535) + UpdateAllBoundaries.  Events are
+// queued in the slot's event buffer for the service kernel.  Returns a bit set: 1 = TRAJ_FINISH,
+// 2 = an event was queued.  HOT is the transport's register state: t, x, y, z, the boundary arrays
+// bd / bdo / bst and the momentum accessors m0() / m2() (Parker keeps (p, 0, 0), the guiding centre
+// (p_perp, 0, p_par)) and reflect_mom() (a reflecting wall reverses p_par of a guiding centre,
+// src/trajectory_guiding.hh:121-124).
+template <bool GC, typename HOT>
+__device__ __forceinline__ int handle_boundaries(const spx_plan_desc* __restrict__ P, const Launch* __restrict__ L, uint64_t slot,
+                                                 HOT& H, ColdLDS& C, int tid)
+{
+   bool any = false;
+#pragma unroll
+   for (int b = 0; b < SPX_MAX_BND; b++) {
+      if (b < P->n_bnd) {
+         // ComputeBoundary (src/boundary_base.cc:105-127)
+         const int left = (H.bst[b] & SPX_BND_RECURRENT) ? C.cross[b][tid] : 0;
+         const double d = bnd_eval(P->bnd[b], left, GC, H.t, H.x, H.y, H.z, H.m0(), 0.0, H.m2(), nullptr);
+         const double prod = d * H.bdo[b];
+         H.bd[b] = (prod == 0.0) ? -H.bdo[b] : d;
+         H.bst[b] = (prod <= 0.0) ? (H.bst[b] | SPX_BND_CROSSED) : (H.bst[b] & ~SPX_BND_CROSSED);
+         any = any || (prod <= 0.0);
+      }
+   }
+   if (!any) {
+      // RecordBoundary without crossings: _delta_old = _delta
+#pragma unroll
+      for (int b = 0; b < SPX_MAX_BND; b++) H.bdo[b] = H.bd[b];
+      return 0;
+   }
+   int out = 0;
+   // momentum, then space, then time
+   for (int cls = 2; cls >= 0; cls--) {
+      int active = -1;
+#pragma unroll
+      for (int b = 0; b < SPX_MAX_BND; b++) {
+         if (b >= P->n_bnd || P->bnd[b].cls != cls || !(H.bst[b] & SPX_BND_CROSSED)) continue;
+         active = b;
+         if (cls == SPX_BC_SPACE && (H.bst[b] & SPX_BND_REFLECT)) {
+            double nrm[3];
+            (void)bnd_eval(P->bnd[b], 0, GC, H.t, H.x, H.y, H.z, H.m0(), 0.0, H.m2(), nrm);
+            H.x -= 2.0 * H.bd[b] * nrm[0];
+            H.y -= 2.0 * H.bd[b] * nrm[1];
+            H.z -= 2.0 * H.bd[b] * nrm[2];
+            H.reflect_mom();
+            const double d = bnd_eval(P->bnd[b], 0, GC, H.t, H.x, H.y, H.z, H.m0(), 0.0, H.m2(), nullptr);
+            const double prod = d * H.bdo[b];
+            H.bd[b] = (prod == 0.0) ? -H.bdo[b] : d;
+            H.bst[b] = (prod <= 0.0) ? (H.bst[b] | SPX_BND_CROSSED) : (H.bst[b] & ~SPX_BND_CROSSED);
+            C.cross[b][tid] -= 1;
+         }
+         else if (cls != SPX_BC_MOMENTUM || !(H.bst[b] & SPX_BND_REFLECT)) {
+            if (H.bst[b] & SPX_BND_TERMINAL) out |= 1;
+         }
+      }
+      if (active >= 0 && bnd_has_action(P, active)) {
+         ev_field(L, cls, EV_T, slot) = H.t; ev_field(L, cls, EV_X, slot) = H.x; ev_field(L, cls, EV_Y, slot) = H.y;
+         ev_field(L, cls, EV_Z, slot) = H.z; ev_field(L, cls, EV_M0, slot) = H.m0(); ev_field(L, cls, EV_M1, slot) = 0.0;
+         ev_field(L, cls, EV_M2, slot) = H.m2(); ev_field(L, cls, EV_AMP, slot) = C.amp[tid]; ev_field(L, cls, EV_WGT, slot) = C.wgt[tid];
+         L->ev_bnd[cls * L->n_lanes + slot] = active;
+         L->ev_pending[slot] |= (1 << cls);
+         out |= 2;
+      }
+   }
+   // RecordBoundary (src/boundary_base.cc:168-191)
+#pragma unroll
+   for (int b = 0; b < SPX_MAX_BND; b++) {
+      if (b < P->n_bnd) {
+         const bool crossed = (H.bst[b] & SPX_BND_CROSSED) != 0;
+         const bool recur = crossed && (H.bst[b] & SPX_BND_RECURRENT) != 0;
+         const int left = C.cross[b][tid] - (crossed ? 1 : 0);
+         C.cross[b][tid] = left;
+         const double d_now = H.bd[b];
+         const double d_rec = recur ? bnd_eval(P->bnd[b], left, GC, H.t, H.x, H.y, H.z, H.m0(), 0.0, H.m2(), nullptr) : d_now;
+         H.bdo[b] = recur ? H.bdo[b] : d_now;
+         H.bd[b] = d_rec;
+         const int s = H.bst[b] & ~(SPX_BND_CROSSED | SPX_BND_TERMINAL);
+         H.bst[b] = (left == 1) ? (s | SPX_BND_TERMINAL) : s;
+      }
+   }
+   return out;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Service pass, one thread per slot: bin the events the step kernel queued, finish or retry
+// trajectories, refill idle slots from the work counter and run SetStart through `set_start`
+// (a functor (rng, slot, traj, nu&) -> exception code of the transport).
+// ---------------------------------------------------------------------------------------------
+template <typename StartFn>
+__device__ __forceinline__ void service_slot(const spx_plan_desc* __restrict__ P, const Launch* __restrict__ L, StartFn&& set_start)
+{
+   const uint64_t slot = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+   const bool in_range = slot < L->n_lanes;
+   int st = in_range ? L->active[slot] : SLOT_IDLE;
+   const spx_rng_desc rng = L->rng;
+   // 1. pending distribution events of the last step kernel
+   if (st == SLOT_EVENT || st == SLOT_DONE) {
+      const int pend = L->ev_pending[slot];
+      int n_ev = L->n_events[slot];
+      for (int cls = 2; cls >= 0; cls--) {
+         if (!(pend & (1 << cls))) continue;
+         const int b = L->ev_bnd[cls * L->n_lanes + slot];
+         n_ev += bin_events(P, L, slot, b, ev_field(L, cls, EV_T, slot), ev_field(L, cls, EV_X, slot), ev_field(L, cls, EV_Y, slot),
+                            ev_field(L, cls, EV_Z, slot), ev_field(L, cls, EV_M0, slot), ev_field(L, cls, EV_M1, slot),
+                            ev_field(L, cls, EV_M2, slot), ev_field(L, cls, EV_AMP, slot), ev_field(L, cls, EV_WGT, slot));
+      }
+      L->ev_pending[slot] = 0;
+      L->n_events[slot] = n_ev;
+      if (st == SLOT_DONE) {
+         finish_record(L, slot, L->traj[slot], 0, L->attempt[slot] + 1, L->steps[slot], n_ev, L->t[slot], L->x[slot], L->y[slot], L->z[slot],
+                       L->m0[slot], L->m1[slot], L->m2[slot], L->amp[slot], L->wgt[slot]);
+         st = SLOT_IDLE;
+      }
+      else st = SLOT_RUN;
+   }
+   // 2. discarded attempts: retry with continuing draws, or give up
+   if (st == SLOT_DISCARD) {
+      const int exc = L->ex_code[slot];
+      atomicAdd(&L->ctl->discards[exc], 1ull);
+      L->ev_pending[slot] = 0;
+      const int att = L->attempt[slot] + 1;
+      L->attempt[slot] = att;
+      if (att < P->max_attempts && exc != 5) st = SLOT_START;
+      else {
+         finish_record(L, slot, L->traj[slot], exc, att, L->steps[slot], L->n_events[slot], L->t[slot], L->x[slot], L->y[slot], L->z[slot],
+                       L->m0[slot], L->m1[slot], L->m2[slot], L->amp[slot], L->wgt[slot]);
+         st = SLOT_IDLE;
+      }
+   }
+   // 3. refill idle slots from the work counter
+   const bool need = in_range && st == SLOT_IDLE;
+   if (__ballot(need) != 0) {
+      const uint64_t id = wave_claim(L->ctl, need);
+      if (need && id < L->end) {
+         L->traj[slot] = id;
+         L->nu[slot] = 0;
+         L->attempt[slot] = 0;
+         st = SLOT_START;
+      }
+   }
+   // 4. SetStart, repeated with continuing draws when an attempt is discarded
+   if (st == SLOT_START) {
+      const uint64_t traj = L->traj[slot];
+      uint32_t nu = L->nu[slot];
+      int att = L->attempt[slot];
+      int exc = 1;
+      while (att < P->max_attempts) {
+         exc = set_start(rng, slot, traj, nu);
+         if (exc == 0) break;
+         atomicAdd(&L->ctl->discards[exc], 1ull);
+         att++;
+         if (exc == 5) break;
+      }
+      L->nu[slot] = nu;
+      L->attempt[slot] = att;
+      if (exc == 0) st = SLOT_RUN;
+      else {
+         finish_record(L, slot, traj, exc, att, 0, 0, P->it_t, P->is_pos[0], P->is_pos[1], P->is_pos[2], 0.0, 0.0, 0.0, 0.0, 0.0);
+         st = SLOT_IDLE;
+      }
+   }
+   if (in_range) L->active[slot] = st;
+}
+
+}  // namespace spx

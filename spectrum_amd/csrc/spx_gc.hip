@@ -1,0 +1,410 @@
+// spx_gc.hip -- guiding-centre transport with perpendicular diffusion and pitch-angle scattering.
+//
+// TrajectoryGuidingDiffScatt::Advance (src/trajectory_guiding_diff_scatt.cc:76-174), forward in
+// time, on the engine of spx_engine.h: one lane per trajectory slot, service kernel + step kernel
+// per engine launch.  The momentum is kept in the reference's guiding-centre layout
+// (p_perp, 0, p_par).  Static backgrounds only (SolarWind, Uniform): their time derivatives
+// vanish exactly, so the dB/dt terms of the modified fields and of dp_perp/dt are left out,
+// which changes no result bit beyond the sign of a zero.
+#include <hip/hip_runtime.h>
+
+#include "spx_engine.h"
+
+namespace spx {
+
+// The SpatialData members the guiding-centre step reads, at one point.
+template <typename R>
+struct GcFields {
+   v3<R> E, B, bhat, gradBmag, curlbhat;
+   R Bmag, dmax;
+};
+
+// CommonFields with mask ALL|gradB|dBdt (src/trajectory_guiding.cc:51): BackgroundBase::GetFields
+// (src/background_base.cc:425-456) with NumericalDerivatives (:228-312) for SolarWind in the
+// field-aligned frame at r_g = min(r_L(pm), dmax).  gradB enters only through curl(bhat) and
+// grad|B| (common/spatial_data.hh:304-341), which are accumulated here row by row in the order
+// of transpose(fa) * G.  Returns false on ExFieldError.
+template <typename R, int BG>
+__device__ __forceinline__ bool gc_fields(const spx_bg_t<R>& bg, double t, double x, double y, double z, R pm, GcFields<R>& F)
+{
+   F.dmax = bg_dmax<R, BG>(bg, x, y, z, R(0));
+   v3<R> U, B;
+   R ds = R(0);
+   bg_eval<R, BG>(bg, t, x, y, z, 3, U, B, ds);
+   F.B = B;
+   F.Bmag = norm(B);
+   if (F.Bmag < R(sp_tiny)) return false;
+   F.bhat = B / F.Bmag;
+   if (BG == SPX_BG_UNIFORM) F.E = v3of(bg.Eu);                    // src/background_uniform.cc:58-77
+   else F.E = (-cross(U, B)) / R(c_code);                          // src/background_solarwind.cc:250-253
+   if (BG == SPX_BG_SOLARWIND) {
+      const R r_g = fmin(LarmorRadius(pm, F.Bmag), F.dmax);
+      v3<R> fa[3];
+      fa[2] = F.bhat;
+      fa[0] = second_unit(F.bhat);
+      fa[1] = cross(fa[2], fa[0]);
+      R g01 = R(0), g02 = R(0), g10 = R(0), g12 = R(0), g20 = R(0), g21 = R(0);
+      v3<R> gbm = mk<R>(R(0), R(0), R(0));
+#pragma unroll 1
+      for (int k = 0; k < 3; k++) {
+         const v3<R> e = k == 0 ? fa[0] : (k == 1 ? fa[1] : fa[2]);
+         R dr = r_g;
+         const double xf = x + (double)(dr * e.x), yf = y + (double)(dr * e.y), zf = z + (double)(dr * e.z);
+         v3<R> Ud, Bf, Bb;
+         R dsd;
+         bg_eval<R, BG>(bg, t, xf, yf, zf, 2, Ud, Bf, dsd);
+         dr *= R(2);
+         bg_eval<R, BG>(bg, t, xf - (double)(dr * e.x), yf - (double)(dr * e.y), zf - (double)(dr * e.z), 2, Ud, Bb, dsd);
+         const v3<R> gB = (Bf - Bb) / dr;
+         const R gm = (norm(Bf) - norm(Bb)) / dr;
+         if (k == 0) {
+            g01 = e.x * gB.y; g02 = e.x * gB.z; g10 = e.y * gB.x; g12 = e.y * gB.z; g20 = e.z * gB.x; g21 = e.z * gB.y;
+            gbm = mk<R>(e.x * gm, e.y * gm, e.z * gm);
+         }
+         else {
+            g01 += e.x * gB.y; g02 += e.x * gB.z; g10 += e.y * gB.x; g12 += e.y * gB.z; g20 += e.z * gB.x; g21 += e.z * gB.y;
+            gbm.x += e.x * gm; gbm.y += e.y * gm; gbm.z += e.z * gm;
+         }
+      }
+      F.gradBmag = gbm;
+      const v3<R> curlB = mk<R>(g12 - g21, g20 - g02, g01 - g10);
+      F.curlbhat = (curlB - cross(gbm, F.bhat)) / F.Bmag;
+   }
+   else {
+      F.gradBmag = mk<R>(R(0), R(0), R(0));
+      F.curlbhat = mk<R>(R(0), R(0), R(0));
+   }
+   return true;
+}
+
+// DiffusionBase::GetComponent(0) (src/diffusion_base.cc:105-121) of the models usable with the
+// guiding centre; |p| = pmag.
+template <typename R>
+__device__ __forceinline__ R gc_kperp(const spx_df_t<R>& df, R pmag, R Bmag)
+{
+   switch (df.kind) {
+   case SPX_DF_PERP: return df.D0;                                  // src/diffusion_other.cc:459-463
+   case SPX_DF_FULL: return df.Dperp;                               // :538-543
+   case SPX_DF_PERPISO: return df.Dperp;
+   case SPX_DF_RIGBMAG: {                                           // :804-809
+      R K1 = (df.lam0 * VelP(pmag) / R(3)) * rpow(Rigidity(pmag) / df.R0, df.pow_law_R);
+      K1 = K1 * ((df.pow_law_B != R(0)) ? rpow(Bmag / df.B0, df.pow_law_B) : R(1));
+      return df.kap_rat * K1;
+   }
+   default: return R(0);                                            // IsotropicConstant, ParaConstant
+   }
+}
+
+// TrajectoryGuidingDiff::DiffusionCoeff (src/trajectory_guiding_diff.cc:72-98): D_perp and the
+// drift V_perp = grad_perp D_perp in the field-aligned frame.  Only the rigidity power law has a
+// spatial derivative (src/diffusion_other.cc:818-823), and only through |B|.
+template <typename R>
+__device__ __forceinline__ void gc_perp(const spx_df_t<R>& df, const GcFields<R>& F, const v3<R>* fa, R pmag, R& Dperp, v3<R>& Vperp)
+{
+   Dperp = gc_kperp(df, pmag, F.Bmag);
+   if (df.kind == SPX_DF_RIGBMAG && df.pow_law_B != R(0)) {
+      const R c = Dperp * df.pow_law_B;
+      const v3<R> g = mk<R>(c * F.gradBmag.x / F.Bmag, c * F.gradBmag.y / F.Bmag, c * F.gradBmag.z / F.Bmag);
+      Vperp = from_basis(mk<R>(dot(g, fa[0]), dot(g, fa[1]), R(0)), fa);
+   }
+   else Vperp = mk<R>(R(0), R(0), R(0));
+}
+
+// TrajectoryGuiding::DriftCoeff / ModifiedFields (src/trajectory_guiding.cc:65-146): E*, B* and the
+// guiding-centre velocity for velocity components (v_perp, 0, v_par).
+template <typename R>
+__device__ __forceinline__ void gc_drift(const GcFields<R>& F, v3<R> vel, R pp, R pl, v3<R>& drift, v3<R>& Es, v3<R>& Bs)
+{
+   const R rL = LarmorRadius(pp, F.Bmag), rR = LarmorRadius(pl, F.Bmag);
+   Es = F.E - ((rL * vel.x) * F.gradBmag) / (R(2) * R(c_code));
+   Bs = F.B + (rR * F.Bmag) * F.curlbhat;
+   drift = (vel.z * Bs + R(c_code) * cross(Es, F.bhat)) / dot(Bs, F.bhat);
+}
+
+// Particle velocity for a guiding-centre momentum (p_perp, 0, p_par) (common/physics.hh:290-297).
+template <typename R>
+__device__ __forceinline__ v3<R> gc_vel(R pp, R pl)
+{
+   const R mm = norm(mk<R>(pp, R(0), pl));
+   const R a = VelP(mm) / mm;
+   return mk<R>(pp * a, R(0), pl * a);
+}
+
+// Initial momentum (src/initial_momentum.cc:283-410) in the guiding-centre layout: the magnitude
+// of the configured class and mu = 2u - 1 from one uniform draw.
+__device__ __forceinline__ void gc_mom_sample(const spx_plan_desc* __restrict__ P, const spx_rng_desc& rng, uint64_t first, uint64_t traj,
+                                              uint32_t& nu, bool& ex, double& pp, double& pl)
+{
+   double p;
+   switch (P->im_kind) {
+   case SPX_IM_FIXED: pp = P->im_mom[0]; pl = P->im_mom[2]; return;
+   case SPX_IM_RING: pp = P->im_p0 * P->im_st0; pl = P->im_p0 * P->im_mu0; return;   // no draw
+   case SPX_IM_SHELL: p = P->im_p0; break;
+   case SPX_IM_THICKSHELL: {
+      const double u = next_uniform(rng, first, traj, nu, ex);
+      p = P->im_log ? pow(10.0, P->im_p1 + (P->im_p2 - P->im_p1) * u) : P->im_p1 + (P->im_p2 - P->im_p1) * u;
+      break;
+   }
+   default: {
+      const int k = (int)(traj % (uint64_t)P->im_n);
+      const double lT1 = log10(P->im_T1), lT2 = log10(P->im_T2);
+      const double Tk = (P->im_n > 1) ? pow(10.0, lT1 + (lT2 - lT1) * k / (double)(P->im_n - 1)) : P->im_T1;
+      p = MomT(Tk);
+   }
+   }
+   const double mu = -1.0 + 2.0 * next_uniform(rng, first, traj, nu, ex);
+   const double st = sqrt(1.0 - mu * mu);
+   pp = p * st;
+   pl = p * mu;
+}
+
+// TrajectoryBase::SetStart (src/trajectory_base.cc:867-929) for the guiding centre: the momentum is
+// sampled twice (before and after the first field evaluation, :875 and :891), each time drawing mu.
+template <typename R, int BG>
+__device__ int gc_set_start(const spx_plan_desc* __restrict__ P, const Launch* __restrict__ L, const spx_rng_desc& rng, uint64_t slot,
+                            uint64_t traj, uint32_t& nu)
+{
+   const spx_bg_t<R>& bg = PlanView<R>::bg(P);
+   const uint64_t first = L->first;
+   const double t = P->it_t, x = P->is_pos[0], y = P->is_pos[1], z = P->is_pos[2];
+   bool ex = false;
+   double pp1, pl1, pp, pl;
+   gc_mom_sample(P, rng, first, traj, nu, ex, pp1, pl1);
+   const double pm1 = sqrt(pp1 * pp1 + pl1 * pl1);
+   GcFields<R> F;
+   if (!gc_fields<R, BG>(bg, t, x, y, z, R(pm1), F)) return 1;
+   gc_mom_sample(P, rng, first, traj, nu, ex, pp, pl);
+   if (ex) return 5;
+   const uint64_t n = L->n_lanes;
+   L->t[slot] = t; L->x[slot] = x; L->y[slot] = y; L->z[slot] = z; L->m0[slot] = pp; L->m1[slot] = 0.0; L->m2[slot] = pl;
+   L->amp[slot] = 0.0; L->wgt[slot] = 0.0; L->dt_prev[slot] = 0.0; L->src_old[slot] = 0.0; L->ds_prev[slot] = 0.0;
+   L->dmax[slot] = (double)F.dmax;
+   L->pf[slot] = pm1;
+   L->steps[slot] = 0; L->nn[slot] = 0; L->n_events[slot] = 0;
+   L->dt_adapt[slot] = sp_large * (double)F.dmax / c_code;
+   L->t0[slot] = t; L->x0[slot] = x; L->y0[slot] = y; L->z0[slot] = z; L->m00[slot] = pp; L->m01[slot] = 0.0; L->m02[slot] = pl;
+   for (int d = 0; d < SPX_MAX_DIST; d++) L->ev_bin[(uint64_t)d * n + slot] = -1;
+   const double dmax = (double)F.dmax;
+   const double pmag = norm(mk<double>(pp, 0.0, pl));
+   for (int b = 0; b < P->n_bnd; b++) {
+      const spx_bnd_desc& B = P->bnd[b];
+      const double scale = (B.cls == SPX_BC_TIME) ? fabs(dmax / c_code) : (B.cls == SPX_BC_SPACE ? fabs(dmax) : fabs(pmag));
+      const int left = B.max_crossings;
+      const double d = bnd_eval(B, left, true, t, x, y, z, pp, 0.0, pl, nullptr);
+      L->bcross[b * n + slot] = left;
+      L->bdelta[b * n + slot] = d;
+      L->bdelta_old[b * n + slot] = (d == 0.0) ? -sp_tiny * scale : d;
+      L->bstat[b * n + slot] = (B.status0 & ~(SPX_BND_TERMINAL | SPX_BND_CROSSED)) | (left == 1 ? SPX_BND_TERMINAL : 0);
+   }
+   return 0;
+}
+
+template <typename R, int BG>
+__global__ __launch_bounds__(BLOCK) void gc_service(const spx_plan_desc* __restrict__ P, const Launch* __restrict__ L)
+{
+   service_slot(P, L, [&](const spx_rng_desc& rng, uint64_t slot, uint64_t traj, uint32_t& nu) {
+      return gc_set_start<R, BG>(P, L, rng, slot, traj, nu);
+   });
+}
+
+// ---- step kernel -----------------------------------------------------------------------------
+template <typename R>
+struct GcHot {
+   double t, x, y, z, pp, pl;                   // time, position, (p_perp, p_par)
+   double pf;                                   // |p| of the last field evaluation
+   GcFields<R> F;
+   double bd[SPX_MAX_BND], bdo[SPX_MAX_BND];
+   int bst[SPX_MAX_BND];
+   NormalStream<R> rs;
+   int32_t steps;
+   __device__ __forceinline__ double m0() const { return pp; }
+   __device__ __forceinline__ double m2() const { return pl; }
+   __device__ __forceinline__ void reflect_mom() { pl = -pl; }
+};
+
+// One step of TrajectoryGuidingDiffScatt::Advance (src/trajectory_guiding_diff_scatt.cc:76-174):
+// DiffusionCoeff, DriftCoeff, PhysicalStep (src/trajectory_guiding_diff.cc:324-332 then
+// src/trajectory_guiding_scatt.cc:255-262), EulerPitchAngleScatt (:81-106), Slopes, EulerPerpDiffSlopes
+// (src/trajectory_guiding_diff.cc:123-140), the Euler RKStep, boundaries and the end-of-step fields.
+template <typename R, int BG>
+__device__ __forceinline__ int gc_step(const spx_plan_desc* __restrict__ P, const Launch* __restrict__ L, const spx_rng_desc& rng,
+                                       uint64_t first, uint64_t slot, GcHot<R>& H, ColdLDS& C, int tid, int& bflags)
+{
+   const spx_bg_t<R>& bg = PlanView<R>::bg(P);
+   const spx_df_t<R>& df = PlanView<R>::df(P);
+   const bool scatt = (df.kind == SPX_DF_ISO || df.kind == SPX_DF_PERPISO);
+   R pp = R(H.pp), pl = R(H.pl);
+   const double pmag_d = sqrt(H.pp * H.pp + H.pl * H.pl);
+   v3<R> vel = gc_vel(pp, pl);
+   v3<R> fa[3];
+   fa[2] = H.F.bhat;
+   fa[0] = second_unit(H.F.bhat);
+   fa[1] = cross(fa[2], fa[0]);
+   // DiffusionCoeff: D_perp, V_perp, D_mumu and its mu derivative (src/diffusion_other.cc:64-79)
+   R pmag = R(pmag_d);
+   const R mu = pl / norm(mk<R>(pp, R(0), pl));
+   R Dperp;
+   v3<R> Vperp;
+   gc_perp(df, H.F, fa, pmag, Dperp, Vperp);
+   const R Dmumu = scatt ? df.D0 * (R(1) - sqr(mu)) : R(0);
+   const R Vmu = scatt ? R(-2) * df.D0 * mu : R(0);
+   v3<R> drift, Es, Bs;
+   gc_drift(H.F, vel, pp, pl, drift, Es, Bs);
+   // PhysicalStep
+   R dtp = R(0.5) * H.F.dmax / (norm(drift + Vperp) + R(0.5) * norm(vel));
+   dtp = fmin(dtp, R(0.5) * sqr(H.F.dmax) / Dperp);
+   {
+      const R dthetamax = R(2.0 * M_PI / 180.0);
+      const R amu = fabs(pl / norm(mk<R>(pp, R(0), pl)));
+      const R dmumax = rsqrt_(R(1) - amu) * dthetamax + R(0.5) * amu * sqr(dthetamax);
+      dtp = fmin(dtp, R(0.5) * sqr(dmumax) / Dmumu);
+      dtp = fmin(dtp, R(0.5) * dmumax / fabs(Vmu));
+   }
+   double dt = fmin((double)dtp, C.dt_adapt[tid]);
+   dt = time_proximity<1>(P, H, dt, (double)H.F.dmax);
+   const R dtr = R(dt);
+   // EulerPitchAngleScatt: redraw until |mu'| <= 1
+   double ppd, pld;
+   {
+      const R sd = rsqrt_(R(2) * Dmumu * dtr);
+      R mu_new;
+      int guard = 0;
+      do {
+         mu_new = mu + Vmu * dtr + sd * next_normal<R>(rng, first, H.rs);
+         if (H.rs.exhausted) return 5;
+         if (++guard > 100000) return 3;
+      } while (fabs(mu_new) > R(1));
+      // the new momentum components are formed in double from |p| (an FP32 step keeps |p| exact here)
+      ppd = pmag_d * sqrt(1.0 - (double)mu_new * (double)mu_new);
+      pld = pmag_d * (double)mu_new;
+      pp = R(ppd);
+      pl = R(pld);
+      vel = gc_vel(pp, pl);
+   }
+   // Slopes with the scattered momentum
+   gc_drift(H.F, vel, pp, pl, drift, Es, Bs);
+   const R slope_pp = R(0.5) * pp / H.F.Bmag * dot(drift, H.F.gradBmag);
+   const R slope_pl = R(sp_charge) * dot(Es, Bs) / dot(Bs, H.F.bhat);
+   pmag = rsqrt_(sqr(pp) + sqr(pl));
+   gc_perp(df, H.F, fa, pmag, Dperp, Vperp);
+   const v3<R> slope_pos = drift + Vperp;
+   // EulerPerpDiffSlopes
+   const R sdt = rsqrt_(dtr);
+   const R dWx = sdt * next_normal<R>(rng, first, H.rs);
+   const R dWy = sdt * next_normal<R>(rng, first, H.rs);
+   if (H.rs.exhausted) return 5;
+   Dperp = gc_kperp(df, pmag, H.F.Bmag);
+   const v3<R> dr = from_basis(mk<R>(rsqrt_(R(2) * Dperp) * dWx, rsqrt_(R(2) * Dperp) * dWy, R(0)), fa);
+   // RKStep (Euler), then the perpendicular displacement
+   const R dtv = dtr * R(1);
+   H.t += dt;
+   H.x += (double)(dtv * slope_pos.x); H.y += (double)(dtv * slope_pos.y); H.z += (double)(dtv * slope_pos.z);
+   H.pp = ppd + (double)(dtv * slope_pp);
+   H.pl = pld + (double)(dtv * slope_pl);
+   H.x += (double)dr.x; H.y += (double)dr.y; H.z += (double)dr.z;
+   bflags = handle_boundaries<true>(P, L, slot, H, C, tid);
+   if (!(bflags & 1)) {
+      H.pf = sqrt(H.pp * H.pp + H.pl * H.pl);
+      if (!gc_fields<R, BG>(bg, H.t, H.x, H.y, H.z, R(H.pf), H.F)) return 1;
+   }
+   H.steps++;
+   if (H.steps > P->max_steps) return 4;
+   if (dt < sp_tiny * (double)H.F.dmax / c_code) return 2;
+   if (!__builtin_isnormal(dt)) return 3;
+   return 0;
+}
+
+template <typename R, int BG>
+__global__ __launch_bounds__(BLOCK) void gc_step_kernel(const spx_plan_desc* __restrict__ P, const Launch* __restrict__ Lp)
+{
+   __shared__ ColdLDS C;
+   const int tid = threadIdx.x;
+   const uint64_t slot = (uint64_t)blockIdx.x * BLOCK + tid;
+   const bool in_range = slot < Lp->n_lanes;
+   const bool run = in_range && Lp->active[slot] == SLOT_RUN;
+   if (__ballot(run) == 0) return;
+   const spx_rng_desc rng = Lp->rng;
+   const uint64_t first = Lp->first;
+   GcHot<R> H;
+   if (run) {
+      const uint64_t n = Lp->n_lanes;
+      H.t = Lp->t[slot]; H.x = Lp->x[slot]; H.y = Lp->y[slot]; H.z = Lp->z[slot]; H.pp = Lp->m0[slot]; H.pl = Lp->m2[slot];
+      H.pf = Lp->pf[slot];
+      H.steps = (int32_t)Lp->steps[slot];
+      H.rs.traj = Lp->traj[slot]; H.rs.nn = Lp->nn[slot]; H.rs.spare_ok = false; H.rs.exhausted = false; H.rs.spare = R(0);
+      C.amp[tid] = Lp->amp[slot]; C.wgt[tid] = Lp->wgt[slot]; C.dt_adapt[tid] = Lp->dt_adapt[slot];
+#pragma unroll
+      for (int b = 0; b < SPX_MAX_BND; b++) {
+         H.bd[b] = Lp->bdelta[b * n + slot]; H.bdo[b] = Lp->bdelta_old[b * n + slot];
+         C.cross[b][tid] = Lp->bcross[b * n + slot]; H.bst[b] = Lp->bstat[b * n + slot];
+      }
+      // the fields are a function of (t, x, |p| of their evaluation) and are recomputed bit for bit
+      (void)gc_fields<R, BG>(PlanView<R>::bg(P), H.t, H.x, H.y, H.z, R(H.pf), H.F);
+   }
+   int st = run ? SLOT_RUN : SLOT_IDLE;
+   int exc = 0;
+   unsigned long long steps_here = 0;
+   for (int it = 0; it < Lp->steps_per_launch; it++) {
+      if (__ballot(st == SLOT_RUN) == 0) break;
+      if (st != SLOT_RUN) continue;
+      const spx_plan_desc* Pi = P;
+      asm volatile("" : "+s"(Pi));
+      const Launch* Li = Lp;
+      asm volatile("" : "+s"(Li));
+      int bflags = 0;
+      const int e = gc_step<R, BG>(Pi, Li, rng, first, slot, H, C, tid, bflags);
+      steps_here++;
+      if (e != 0) { exc = e; st = SLOT_DISCARD; }
+      else if (bflags & 1) st = SLOT_DONE;
+      else if (bflags & 2) st = SLOT_EVENT;
+   }
+   const Launch* Ls = Lp;
+   asm volatile("" : "+s"(Ls));
+   unsigned long long s = steps_here;
+   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+   if (lane_id() == 0 && s) atomicAdd(&Ls->ctl->steps_executed, s);
+   if (!run) return;
+   const uint64_t n = Ls->n_lanes;
+   Ls->active[slot] = st;
+   Ls->ex_code[slot] = exc;
+   Ls->t[slot] = H.t; Ls->x[slot] = H.x; Ls->y[slot] = H.y; Ls->z[slot] = H.z; Ls->m0[slot] = H.pp; Ls->m2[slot] = H.pl;
+   Ls->pf[slot] = H.pf;
+   Ls->steps[slot] = H.steps; Ls->nn[slot] = H.rs.nn;
+   Ls->dmax[slot] = (double)H.F.dmax;
+   Ls->amp[slot] = C.amp[tid]; Ls->wgt[slot] = C.wgt[tid];
+#pragma unroll
+   for (int b = 0; b < SPX_MAX_BND; b++) {
+      Ls->bdelta[b * n + slot] = H.bd[b]; Ls->bdelta_old[b * n + slot] = H.bdo[b];
+      Ls->bcross[b * n + slot] = C.cross[b][tid]; Ls->bstat[b * n + slot] = H.bst[b];
+   }
+}
+
+template <typename R, int BG>
+static hipError_t launch_gc_bg(const spx_plan_desc* d_plan, const Launch& L, const Launch* d_launch, hipStream_t st, hipEvent_t mid)
+{
+   const unsigned grid = (unsigned)((L.n_lanes + BLOCK - 1) / BLOCK);
+   hipLaunchKernelGGL((gc_service<R, BG>), dim3(grid), dim3(BLOCK), 0, st, d_plan, d_launch);
+   if (mid) {
+      hipError_t e = hipEventRecord(mid, st);
+      if (e != hipSuccess) return e;
+   }
+   hipLaunchKernelGGL((gc_step_kernel<R, BG>), dim3(grid), dim3(BLOCK), 0, st, d_plan, d_launch);
+   return hipGetLastError();
+}
+
+hipError_t launch_gc(const spx_plan_desc& hp, const spx_plan_desc* d_plan, const Launch& L, const Launch* d_launch, int precision,
+                     hipStream_t st, hipEvent_t mid)
+{
+   if (hp.time_flow != 0) return hipErrorInvalidValue;
+   switch (hp.bg.kind) {
+   case SPX_BG_SOLARWIND:
+      return precision == 1 ? launch_gc_bg<float, SPX_BG_SOLARWIND>(d_plan, L, d_launch, st, mid)
+                            : launch_gc_bg<double, SPX_BG_SOLARWIND>(d_plan, L, d_launch, st, mid);
+   case SPX_BG_UNIFORM:
+      return precision == 1 ? launch_gc_bg<float, SPX_BG_UNIFORM>(d_plan, L, d_launch, st, mid)
+                            : launch_gc_bg<double, SPX_BG_UNIFORM>(d_plan, L, d_launch, st, mid);
+   default: return hipErrorInvalidValue;
+   }
+}
+
+}  // namespace spx

@@ -32,6 +32,7 @@ struct Launch {
    // lane state (structure of arrays, n_lanes each; boundary arrays SPX_MAX_BND * n_lanes)
    int32_t* active;
    double *t, *x, *y, *z, *m0, *m1, *m2, *amp, *wgt, *dt_prev, *src_old, *ds_prev, *dmax, *dt_adapt;
+   double* pf;                          // momentum magnitude of the slot's last field evaluation (guiding centre)
    double *bdelta, *bdelta_old;
    int32_t *bcross, *bstat;
    uint64_t* traj;
@@ -61,6 +62,8 @@ struct Launch {
 // Host-side dispatch to the kernel instantiation for (transport, precision, background, flow).
 // One engine launch is the service kernel followed by the step kernel on `stream`; when `mid` is
 // not null it is recorded between the two, so the caller can time each kernel with events.
+hipError_t launch_gc(const spx_plan_desc& host_plan, const spx_plan_desc* dev_plan, const Launch& L, const Launch* dev_launch,
+                     int precision, hipStream_t stream, hipEvent_t mid);
 hipError_t launch_transport(const spx_plan_desc& host_plan, const spx_plan_desc* dev_plan, const Launch& L, const Launch* dev_launch,
                             int precision, int block, hipStream_t stream, hipEvent_t mid);
 

@@ -19,7 +19,7 @@ enum spx_df_kind {
 };
 enum spx_bnd_class { SPX_BC_TIME = 0, SPX_BC_SPACE = 1, SPX_BC_MOMENTUM = 2 };
 enum spx_bnd_kind { SPX_BK_TIME = 0, SPX_BK_SPHERE = 1, SPX_BK_PLANE = 2, SPX_BK_MOMENTUM = 3 };
-enum spx_im_kind { SPX_IM_FIXED = 0, SPX_IM_SHELL = 1, SPX_IM_THICKSHELL = 2, SPX_IM_SHELLGRID = 3 };
+enum spx_im_kind { SPX_IM_FIXED = 0, SPX_IM_SHELL = 1, SPX_IM_THICKSHELL = 2, SPX_IM_SHELLGRID = 3, SPX_IM_RING = 4 };
 enum spx_ds_kind { SPX_DS_SPECTRUM = 0, SPX_DS_TIME, SPX_DS_MOMENTUM, SPX_DS_POSMOM, SPX_DS_CUMO1, SPX_DS_CUMO2, SPX_DS_POSPITCH };
 enum spx_src_kind { SPX_SRC_NONE = 0, SPX_SRC_MOMINJ = 1 };
 
@@ -84,7 +84,7 @@ typedef struct {
    double src_p_inj, src_rate;
    // initial conditions (src/initial_{time,space,momentum}.cc)
    int32_t im_kind, im_log, im_n, pad2_;
-   double it_t, is_pos[3], im_mom[3], im_p0, im_p1, im_p2, im_T1, im_T2;
+   double it_t, is_pos[3], im_mom[3], im_p0, im_p1, im_p2, im_T1, im_T2, im_mu0, im_st0;
    spx_bnd_desc bnd[SPX_MAX_BND];       // ordered: all time, then space, then momentum (ComputeAllBoundaries order)
    int32_t n_bt, n_bs, n_bm, pad3_;
    spx_ds_desc dist[SPX_MAX_DIST];

@@ -90,10 +90,10 @@ def test_parker_modulation_fp64_full_trajectories(device, oracle_mod):
 
 def test_parker_modulation_fp64_replay(device, oracle_mod):
     """Identical pre-generated N(0,1) increments and uniforms fed to both paths."""
-    s = cf.parker_modulation(n_energies=8, t_limit_years=3.0 / 365.0, time_action=0)
+    s = cf.parker_modulation(n_energies=8, t_limit_years=1.0 / 365.0, time_action=0)
     n = 256
     rng = np.random.default_rng(2024)
-    normals = rng.standard_normal((n, 9000))
+    normals = rng.standard_normal((n, 15000))      # 3 draws per step; a 1-day trajectory takes < 4000 steps
     uniforms = rng.random((n, 8))
     sim = E.Simulation.from_setup(s, precision=E.PREC_FP64, lanes=512, steps_per_launch=250)
     st, rec = sim.run(n, normals=normals, uniforms=uniforms)
@@ -155,3 +155,25 @@ def test_full_size_modulation_binning(device):
     np.testing.assert_array_equal(c, np.full(n_e, per))
     assert st.t_min == pytest.approx(st.t_max, rel=1e-6)
     assert np.all(sm > 0)
+
+
+def test_gc_diff_scatt_fp64_counter(device, oracle_mod):
+    """Guiding centre with perpendicular diffusion and pitch-angle scattering in the Parker spiral
+    (numerical grad B, curl b, E x B and gradient drifts, rejection-sampled pitch-angle steps)."""
+    s = cf.gc_diff_scatt(t_limit_days=0.3)
+    n = 256
+    sim = E.Simulation.from_setup(s, precision=E.PREC_FP64, lanes=512, steps_per_launch=1500)
+    st, rec = sim.run(n, seed=13)
+    o = oracle_mod.Oracle(s)
+    _, orec = o.run(n, seed=13, threads=THREADS)
+    check_parity(st, rec, orec, sim, o, n)
+
+
+def test_gc_uniform_ring_fp64_counter(device, oracle_mod):
+    s = cf.gc_uniform(theta0=0.6435011087932844, t_limit=300.0)
+    n = 512
+    sim = E.Simulation.from_setup(s, precision=E.PREC_FP64, lanes=512, steps_per_launch=700)
+    st, rec = sim.run(n, seed=17)
+    o = oracle_mod.Oracle(s)
+    _, orec = o.run(n, seed=17, threads=THREADS)
+    check_parity(st, rec, orec, sim, o, n)

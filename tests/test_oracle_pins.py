@@ -221,3 +221,49 @@ def test_spectrum_binning_exact(oracle_mod):
     np.testing.assert_array_equal(counts, np.bincount(rec["ev_bin"][binned, 0], minlength=8))
     np.testing.assert_array_equal(rec["ev_bin"][binned, 0], (np.arange(n) % 8)[binned])
     np.testing.assert_allclose(sums, np.bincount(rec["ev_bin"][binned, 0], weights=rec["ev_weight"][binned, 0], minlength=8), rtol=1e-14)
+
+
+def legendre_mean(mu, l):
+    P = [np.ones_like(mu), mu, 0.5 * (3 * mu**2 - 1), 0.5 * (5 * mu**3 - 3 * mu)]
+    return P[l].mean(), P[l].std() / math.sqrt(mu.size)
+
+
+def test_gc_pitch_angle_isotropization(oracle_mod):
+    """Guiding centre in a uniform field, isotropic D_mumu = D0 (1 - mu^2): the Legendre moments of a ring
+    relax as <P_l(mu)>(t) = P_l(mu0) exp(-l (l + 1) D0 t); |p| is conserved exactly; the perpendicular
+    spread is <x^2 + y^2> = 4 kappa_perp t; the mean parallel displacement is v mu0 (1 - exp(-2 D0 t)) / (2 D0)."""
+    mu0 = 0.8
+    s = cf.gc_uniform(theta0=math.acos(mu0), t_limit=500.0)
+    o = oracle_mod.Oracle(s)
+    n = 20000
+    _, rec = o.run(n, seed=3, threads=8)
+    assert (rec["status"] == 0).all()
+    p = np.hypot(rec["mom"][:, 0], rec["mom"][:, 2])
+    np.testing.assert_allclose(p, s.meta["p"], rtol=1e-13)
+    mu = rec["mom"][:, 2] / p
+    D0, T, v = s.meta["D0"], s.meta["t_limit"], s.meta["v"]
+    for l, Pl0 in ((1, mu0), (2, 0.5 * (3 * mu0**2 - 1)), (3, 0.5 * (5 * mu0**3 - 3 * mu0))):
+        m, e = legendre_mean(mu, l)
+        assert abs(m - Pl0 * math.exp(-l * (l + 1) * D0 * T)) < 3.5 * e, (l, m, e)
+    r2 = rec["pos"][:, 0] ** 2 + rec["pos"][:, 1] ** 2
+    assert abs(r2.mean() - 4 * s.meta["kperp"] * T) < 3.5 * r2.std() / math.sqrt(n)
+    z = rec["pos"][:, 2]
+    assert abs(z.mean() - v * mu0 * (1 - math.exp(-2 * D0 * T)) / (2 * D0)) < 3.5 * z.std() / math.sqrt(n)
+
+
+def test_gc_isotropic_parallel_spread(oracle_mod):
+    """Isotropic start: <z^2>(t) = (2 v^2 / 3) [t / (2 D0) - (1 - exp(-2 D0 t)) / (4 D0^2)] (the l = 1 mode
+    correlation <mu(s) mu(u)> = exp(-2 D0 |s - u|) / 3); for t >> 1 / D0 this is 2 kappa_par t with
+    kappa_par = v^2 / (6 D0), the reference's pitch-angle benchmark (benchmarks/main_test_pa_scatt.cc:252)."""
+    s = cf.gc_uniform(theta0=None, t_limit=2000.0)
+    o = oracle_mod.Oracle(s)
+    n = 8000
+    _, rec = o.run(n, seed=4, threads=8)
+    D0, T, v = s.meta["D0"], s.meta["t_limit"], s.meta["v"]
+    z2 = rec["pos"][:, 2] ** 2
+    expect = (2 * v * v / 3) * (T / (2 * D0) - (1 - math.exp(-2 * D0 * T)) / (4 * D0 * D0))
+    assert abs(z2.mean() - expect) < 3.5 * z2.std() / math.sqrt(n), (z2.mean(), expect)
+    mu = rec["mom"][:, 2] / np.hypot(rec["mom"][:, 0], rec["mom"][:, 2])
+    for l in (1, 2):
+        m, e = legendre_mean(mu, l)
+        assert abs(m) < 3.5 * e
