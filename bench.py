@@ -97,6 +97,17 @@ def cpu_leg(cf, seconds, threads):
                       f"OpenMP over trajectories, {wall:.1f} s"}
 
 
+def copy_probe(mib):
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copy probe: 3 x {mib} MiB device-to-device copies done")
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,7 +118,13 @@ def main():
     ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a plain device-to-device copy of this many MiB (3 times) and exit: the reference "
+                         "kernel of known traffic that scales the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.copy_probe_mib > 0:
+        copy_probe(args.copy_probe_mib)
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: trajectories are independent and each GPU would take its own id range "

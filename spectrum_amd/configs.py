@@ -165,12 +165,15 @@ def parker_modulation(n_energies=16, T1=0.1 * GeV, T2=10.0 * GeV, r_start_au=1.0
     return s
 
 
-def dsa_shock(n_bins=128, p_ratio=1.0e4, downstream_kappa_lengths=4.0, t_limit_days=3000.0, name="dsa_smooth_shock") -> Setup:
+def dsa_shock(n_bins=128, p_ratio=1.0e4, downstream_kappa_lengths=4.0, t_limit_days=3000.0, src_ratio=1.0,
+              name="dsa_smooth_shock") -> Setup:
     """Diffusive shock acceleration at a planar smooth shock, forward ParkerSource.
 
     u1 = 400 km/s, r = 4, width = 0.01 kappa / u1, kappa = 1e20 cm^2/s, 10 keV protons injected at the
     shock; a downstream absorbing plane at a few kappa/u2 and a long time limit; momentum at escape
     binned on n_bins log bins over [p_inj, p_ratio p_inj].  Expected counts per log bin ~ p^(3-q), q = 3r/(r-1).
+    The source term (SourceMomentumInjection) fires when the momentum crosses src_ratio * p_inj; with
+    the default ratio 1 the particles start on it and it never fires.
     """
     s = Setup(name, TRAJ_PARKER_SOURCE, FORWARD)
     u1 = 4.0e7 / unit_velocity
@@ -182,7 +185,7 @@ def dsa_shock(n_bins=128, p_ratio=1.0e4, downstream_kappa_lengths=4.0, t_limit_d
     s.add(BACKGROUND, "BackgroundSmoothShock", dc)
     s.add(DIFFUSION, "DiffusionFullConstant", DataContainer().insert_double(kappa).insert_double(kappa))
     p0 = mom_from_ekin(10.0 * keV)
-    s.add(SOURCE, "SourceMomentumInjection", DataContainer().insert_double(p0).insert_double(1.0))
+    s.add(SOURCE, "SourceMomentumInjection", DataContainer().insert_double(src_ratio * p0).insert_double(1.0))
     s.add(INITIAL, "InitialTimeFixed", DataContainer().insert_double(0.0))
     s.add(INITIAL, "InitialSpaceFixed", DataContainer().insert_vec((0.0, 0.0, 0.0)))
     s.add(INITIAL, "InitialMomentumShell", DataContainer().insert_double(p0))

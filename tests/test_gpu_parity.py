@@ -104,7 +104,7 @@ def test_parker_modulation_fp64_replay(device, oracle_mod):
 
 def test_dsa_smooth_shock_fp64_counter(device, oracle_mod):
     """Forward ParkerSource at the smooth shock: injection source term, plane absorber, momentum bins."""
-    s = cf.dsa_shock(t_limit_days=1.0)
+    s = cf.dsa_shock(t_limit_days=1.0, src_ratio=1.2)
     n = 192
     sim = E.Simulation.from_setup(s, precision=E.PREC_FP64, lanes=512, steps_per_launch=2000)
     st, rec = sim.run(n, seed=11)
