@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libspectrum_amd.so")
-SOURCES = ["spx_kernels.hip", "spx_gc.hip", "spx_api.cpp"]
+SOURCES = ["spx_kernels_f64.hip", "spx_kernels_f32.hip", "spx_api.cpp"]
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # FP64 kernels keep the reference's rounding: no contraction into multiply-adds
@@ -24,6 +24,15 @@ HIP_FLAGS = [
     # no packed FP32 instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
+# The FP32-step kernels: multiply-adds, reciprocals and approximate library functions are allowed
+# (their error is far below the Monte Carlo error they are compared at); NaN checks stay valid.
+F32_FLAGS = ["-ffp-contract=fast", "-freciprocal-math", "-fno-signed-zeros", "-fapprox-func"]
+
+
+def flags_for(src: str) -> list:
+    if src.endswith("_f32.hip"):
+        return [f for f in HIP_FLAGS if f != "-ffp-contract=off"] + F32_FLAGS
+    return HIP_FLAGS
 
 
 def hipcc() -> str:
@@ -42,7 +51,7 @@ def build(verbose: bool = False) -> str:
     for src in SOURCES:
         obj = os.path.join(objdir, src + ".o")
         objs.append(obj)
-        cmd = [hipcc(), *HIP_FLAGS, "-c", "-I", os.path.join(ROOT, "include"), "-o", obj, os.path.join(CSRC, src)]
+        cmd = [hipcc(), *flags_for(src), "-c", "-I", os.path.join(ROOT, "include"), "-o", obj, os.path.join(CSRC, src)]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))

@@ -180,6 +180,11 @@ static int add_background(spx_sim* s, const std::string& cls, Reader& r)
       second_unit(e2, e0);
       vcross(e2, e0, e1);
       for (int k = 0; k < 3; k++) { g.eprime[0][k] = e0[k]; g.eprime[1][k] = e1[k]; g.eprime[2][k] = e2[k]; }
+      // a diagonal +-1 basis turns the rotations into sign flips with the same results
+      g.ep_diag = 1;
+      for (int k = 0; k < 3; k++)
+         for (int j = 0; j < 3; j++)
+            if ((j != k && g.eprime[k][j] != 0.0) || (j == k && std::fabs(g.eprime[k][j]) != 1.0)) g.ep_diag = 0;
       g.ur0 = std::fabs(g.u0[0]);
       g.hp_rad = 117.0 * 1.495978707E+13 / 1.0E10;
    }
@@ -422,7 +427,7 @@ static void finalize_plan(spx_sim* s)
    const spx_bg_t<double>& g = P.bg;
    spx_bg_t<float>& f = P.bgf;
    std::memset(&f, 0, sizeof(f));
-   f.kind = g.kind; f.t0 = g.t0;
+   f.kind = g.kind; f.ep_diag = g.ep_diag; f.t0 = g.t0;
    for (int k = 0; k < 3; k++) {
       f.r0[k] = g.r0[k]; f.u0[k] = (float)g.u0[k]; f.B0[k] = (float)g.B0[k]; f.n_shock[k] = (float)g.n_shock[k];
       f.u1[k] = (float)g.u1[k]; f.B1[k] = (float)g.B1[k]; f.E0[k] = (float)g.E0[k]; f.E1[k] = (float)g.E1[k]; f.Eu[k] = (float)g.Eu[k];
@@ -642,7 +647,7 @@ static int launch_many(spx_sim* s, int n, float* ms_out)
       if (!s->graph_exec) {
          HIPCHK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
          for (int i = 0; i < s->graph_launches; i++) {
-            hipError_t e = launch_transport(s->plan, s->d_plan, L, s->d_launch, s->precision, s->block, s->stream, nullptr);
+            hipError_t e = launch_transport(s->plan, s->d_plan, L, s->d_launch, s->precision, s->stream, nullptr);
             if (e != hipSuccess) {
                hipGraph_t g;
                (void)hipStreamEndCapture(s->stream, &g);
@@ -664,7 +669,7 @@ static int launch_many(spx_sim* s, int n, float* ms_out)
       }
       for (int i = 0; i < n; i++) {
          HIPCHK(s, hipEventRecord(s->tev[2 * i], s->stream));
-         hipError_t e = launch_transport(s->plan, s->d_plan, L, s->d_launch, s->precision, s->block, s->stream, s->tev[2 * i + 1]);
+         hipError_t e = launch_transport(s->plan, s->d_plan, L, s->d_launch, s->precision, s->stream, s->tev[2 * i + 1]);
          if (e != hipSuccess) return fail(s, SPX_ERR_DEVICE, std::string("launch: ") + hipGetErrorString(e));
       }
    }

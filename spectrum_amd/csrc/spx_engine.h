@@ -11,6 +11,21 @@
 
 namespace spx {
 
+typedef const SPX_CP Launch* LaunchPtr;
+
+// Member-wise copy of the run's random-stream description out of the constant launch block.
+__device__ __forceinline__ spx_rng_desc load_rng(LaunchPtr L)
+{
+   spx_rng_desc r;
+   r.mode = L->rng.mode;
+   r.seed = L->rng.seed;
+   r.normals = L->rng.normals;
+   r.uniforms = L->rng.uniforms;
+   r.normal_stride = L->rng.normal_stride;
+   r.uniform_stride = L->rng.uniform_stride;
+   return r;
+}
+
 // Slot states
 enum { SLOT_IDLE = 0, SLOT_RUN = 1, SLOT_START = 2, SLOT_EVENT = 3, SLOT_DONE = 4, SLOT_DISCARD = 5 };
 
@@ -85,7 +100,7 @@ __device__ __forceinline__ double next_uniform(const spx_rng_desc& rng, uint64_t
 // ---------------------------------------------------------------------------------------------
 // Boundaries (src/boundary_base.cc:105-191, src/boundary_{time,space,momentum}.cc)
 // ---------------------------------------------------------------------------------------------
-__device__ __forceinline__ double bnd_eval(const spx_bnd_desc& b, int cross_left, bool gc, double t, double x, double y, double z,
+__device__ __forceinline__ double bnd_eval(const SPX_CP spx_bnd_desc& b, int cross_left, bool gc, double t, double x, double y, double z,
                                            double m0, double m1, double m2, double* nrm)
 {
    switch (b.kind) {
@@ -109,9 +124,9 @@ __device__ __forceinline__ double bnd_eval(const spx_bnd_desc& b, int cross_left
 }
 
 // Does boundary b carry an action >= 0 for any distribution?
-__device__ __forceinline__ bool bnd_has_action(const spx_plan_desc* __restrict__ P, int b)
+__device__ __forceinline__ bool bnd_has_action(PlanPtr P, int b)
 {
-   const spx_bnd_desc& B = P->bnd[b];
+   const SPX_CP spx_bnd_desc& B = P->bnd[b];
    bool any = false;
    for (int d = 0; d < P->n_dist; d++) any = any || (d < B.n_act && B.actions[d] >= 0);
    return any;
@@ -119,7 +134,7 @@ __device__ __forceinline__ bool bnd_has_action(const spx_plan_desc* __restrict__
 
 // Pending-event buffer: 3 entries per slot (momentum, space, time classes), 9 doubles each.
 enum { EV_T = 0, EV_X, EV_Y, EV_Z, EV_M0, EV_M1, EV_M2, EV_AMP, EV_WGT, EV_N };
-__device__ __forceinline__ double& ev_field(const Launch* __restrict__ L, int cls, int f, uint64_t slot)
+__device__ __forceinline__ double& ev_field(LaunchPtr L, int cls, int f, uint64_t slot)
 {
    return L->evs[((uint64_t)cls * EV_N + f) * L->n_lanes + slot];
 }
@@ -129,19 +144,19 @@ __device__ __forceinline__ double& ev_field(const Launch* __restrict__ L, int cl
 // Bins the events of every distribution for a crossing of boundary `bnd`: value and weight per
 // distribution, exact integer counts and FP64 sums, and the per-slot last-event record.
 // ---------------------------------------------------------------------------------------------
-static __device__ int bin_events(const spx_plan_desc* __restrict__ P, const Launch* __restrict__ L, uint64_t slot, int bnd,
+static __device__ int bin_events(PlanPtr P, LaunchPtr L, uint64_t slot, int bnd,
                           double t2, double x2, double y2, double z2, double m20, double m21, double m22, double amp, double logwgt)
 {
    const bool parker = (P->traj_type != 2);
    const double t1 = L->t0[slot], x1 = L->x0[slot], y1 = L->y0[slot], z1 = L->z0[slot];
    const double m10 = L->m00[slot], m11 = L->m01[slot], m12 = L->m02[slot];
-   const spx_bnd_desc& B = P->bnd[bnd];
+   const SPX_CP spx_bnd_desc& B = P->bnd[bnd];
    int n_ev = 0;
    for (int d = 0; d < P->n_dist; d++) {
       const int action = d < B.n_act ? B.actions[d] : -1;
       if (action < 0) continue;
       n_ev++;
-      const spx_ds_desc& D = P->dist[d];
+      const SPX_CP spx_ds_desc& D = P->dist[d];
       const bool first = D.val_time == 0;
       const double px = first ? x1 : x2, py = first ? y1 : y2, pz = first ? z1 : z2;
       const double ma = first ? m10 : m20, mb = first ? m11 : m21, mc = first ? m12 : m22;
@@ -228,10 +243,10 @@ static __device__ int bin_events(const spx_plan_desc* __restrict__ P, const Laun
 }
 
 // Record a finished (or failed) trajectory: run statistics and, when asked, its endpoint.
-static __device__ void finish_record(const Launch* __restrict__ Lp, uint64_t slot, uint64_t traj, int ex, int attempts, int64_t steps, int n_events,
+static __device__ void finish_record(LaunchPtr Lp, uint64_t slot, uint64_t traj, int ex, int attempts, int64_t steps, int n_events,
                               double t, double x, double y, double z, double m0, double m1, double m2, double amp, double wgt)
 {
-   const Launch& L = *Lp;
+   const SPX_CP Launch& L = *Lp;
    if (ex == 0) {
       const double el = fabs(t - L.t0[slot]);
       atomic_min_pos(&L.ctl->tmin_bits, el);
@@ -270,7 +285,7 @@ struct ColdLDS {
 
 // TrajectoryBase::TimeBoundaryProximityCheck (src/trajectory_base.cc:151-179)
 template <int FWD, typename HOT>
-__device__ __forceinline__ double time_proximity(const spx_plan_desc* __restrict__ P, const HOT& H, double dt, double dmax)
+__device__ __forceinline__ double time_proximity(PlanPtr P, const HOT& H, double dt, double dmax)
 {
    double dn = FWD ? -sp_large * dmax / c_code : sp_large * dmax / c_code;
 #pragma unroll
@@ -293,7 +308,7 @@ __device__ __forceinline__ double time_proximity(const spx_plan_desc* __restrict
 // (p_perp, 0, p_par)) and reflect_mom() (a reflecting wall reverses p_par of a guiding centre,
 // src/trajectory_guiding.hh:121-124).
 template <bool GC, typename HOT>
-__device__ __forceinline__ int handle_boundaries(const spx_plan_desc* __restrict__ P, const Launch* __restrict__ L, uint64_t slot,
+__device__ __forceinline__ int handle_boundaries(PlanPtr P, LaunchPtr L, uint64_t slot,
                                                  HOT& H, ColdLDS& C, int tid)
 {
    bool any = false;
@@ -374,12 +389,12 @@ __device__ __forceinline__ int handle_boundaries(const spx_plan_desc* __restrict
 // (a functor (rng, slot, traj, nu&) -> exception code of the transport).
 // ---------------------------------------------------------------------------------------------
 template <typename StartFn>
-__device__ __forceinline__ void service_slot(const spx_plan_desc* __restrict__ P, const Launch* __restrict__ L, StartFn&& set_start)
+__device__ __forceinline__ void service_slot(PlanPtr P, LaunchPtr L, StartFn&& set_start)
 {
    const uint64_t slot = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
    const bool in_range = slot < L->n_lanes;
    int st = in_range ? L->active[slot] : SLOT_IDLE;
-   const spx_rng_desc rng = L->rng;
+   const spx_rng_desc rng = load_rng(L);
    // 1. pending distribution events of the last step kernel
    if (st == SLOT_EVENT || st == SLOT_DONE) {
       const int pend = L->ev_pending[slot];

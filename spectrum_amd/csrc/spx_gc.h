@@ -1,4 +1,4 @@
-// spx_gc.hip -- guiding-centre transport with perpendicular diffusion and pitch-angle scattering.
+// spx_gc.h -- guiding-centre transport with perpendicular diffusion and pitch-angle scattering.
 //
 // TrajectoryGuidingDiffScatt::Advance (src/trajectory_guiding_diff_scatt.cc:76-174), forward in
 // time, on the engine of spx_engine.h: one lane per trajectory slot, service kernel + step kernel
@@ -6,6 +6,8 @@
 // (p_perp, 0, p_par).  Static backgrounds only (SolarWind, Uniform): their time derivatives
 // vanish exactly, so the dB/dt terms of the modified fields and of dp_perp/dt are left out,
 // which changes no result bit beyond the sign of a zero.
+#pragma once
+
 #include <hip/hip_runtime.h>
 
 #include "spx_engine.h"
@@ -25,7 +27,7 @@ struct GcFields {
 // grad|B| (common/spatial_data.hh:304-341), which are accumulated here row by row in the order
 // of transpose(fa) * G.  Returns false on ExFieldError.
 template <typename R, int BG>
-__device__ __forceinline__ bool gc_fields(const spx_bg_t<R>& bg, double t, double x, double y, double z, R pm, GcFields<R>& F)
+__device__ __forceinline__ bool gc_fields(const SPX_CP spx_bg_t<R>& bg, double t, double x, double y, double z, R pm, GcFields<R>& F)
 {
    F.dmax = bg_dmax<R, BG>(bg, x, y, z, R(0));
    v3<R> U, B;
@@ -80,7 +82,7 @@ __device__ __forceinline__ bool gc_fields(const spx_bg_t<R>& bg, double t, doubl
 // DiffusionBase::GetComponent(0) (src/diffusion_base.cc:105-121) of the models usable with the
 // guiding centre; |p| = pmag.
 template <typename R>
-__device__ __forceinline__ R gc_kperp(const spx_df_t<R>& df, R pmag, R Bmag)
+__device__ __forceinline__ R gc_kperp(const SPX_CP spx_df_t<R>& df, R pmag, R Bmag)
 {
    switch (df.kind) {
    case SPX_DF_PERP: return df.D0;                                  // src/diffusion_other.cc:459-463
@@ -99,7 +101,7 @@ __device__ __forceinline__ R gc_kperp(const spx_df_t<R>& df, R pmag, R Bmag)
 // drift V_perp = grad_perp D_perp in the field-aligned frame.  Only the rigidity power law has a
 // spatial derivative (src/diffusion_other.cc:818-823), and only through |B|.
 template <typename R>
-__device__ __forceinline__ void gc_perp(const spx_df_t<R>& df, const GcFields<R>& F, const v3<R>* fa, R pmag, R& Dperp, v3<R>& Vperp)
+__device__ __forceinline__ void gc_perp(const SPX_CP spx_df_t<R>& df, const GcFields<R>& F, const v3<R>* fa, R pmag, R& Dperp, v3<R>& Vperp)
 {
    Dperp = gc_kperp(df, pmag, F.Bmag);
    if (df.kind == SPX_DF_RIGBMAG && df.pow_law_B != R(0)) {
@@ -132,7 +134,7 @@ __device__ __forceinline__ v3<R> gc_vel(R pp, R pl)
 
 // Initial momentum (src/initial_momentum.cc:283-410) in the guiding-centre layout: the magnitude
 // of the configured class and mu = 2u - 1 from one uniform draw.
-__device__ __forceinline__ void gc_mom_sample(const spx_plan_desc* __restrict__ P, const spx_rng_desc& rng, uint64_t first, uint64_t traj,
+__device__ __forceinline__ void gc_mom_sample(PlanPtr P, const spx_rng_desc& rng, uint64_t first, uint64_t traj,
                                               uint32_t& nu, bool& ex, double& pp, double& pl)
 {
    double p;
@@ -161,10 +163,10 @@ __device__ __forceinline__ void gc_mom_sample(const spx_plan_desc* __restrict__ 
 // TrajectoryBase::SetStart (src/trajectory_base.cc:867-929) for the guiding centre: the momentum is
 // sampled twice (before and after the first field evaluation, :875 and :891), each time drawing mu.
 template <typename R, int BG>
-__device__ int gc_set_start(const spx_plan_desc* __restrict__ P, const Launch* __restrict__ L, const spx_rng_desc& rng, uint64_t slot,
+__device__ int gc_set_start(PlanPtr P, LaunchPtr L, const spx_rng_desc& rng, uint64_t slot,
                             uint64_t traj, uint32_t& nu)
 {
-   const spx_bg_t<R>& bg = PlanView<R>::bg(P);
+   const SPX_CP spx_bg_t<R>& bg = PlanView<R>::bg(P);
    const uint64_t first = L->first;
    const double t = P->it_t, x = P->is_pos[0], y = P->is_pos[1], z = P->is_pos[2];
    bool ex = false;
@@ -187,7 +189,7 @@ __device__ int gc_set_start(const spx_plan_desc* __restrict__ P, const Launch* _
    const double dmax = (double)F.dmax;
    const double pmag = norm(mk<double>(pp, 0.0, pl));
    for (int b = 0; b < P->n_bnd; b++) {
-      const spx_bnd_desc& B = P->bnd[b];
+      const SPX_CP spx_bnd_desc& B = P->bnd[b];
       const double scale = (B.cls == SPX_BC_TIME) ? fabs(dmax / c_code) : (B.cls == SPX_BC_SPACE ? fabs(dmax) : fabs(pmag));
       const int left = B.max_crossings;
       const double d = bnd_eval(B, left, true, t, x, y, z, pp, 0.0, pl, nullptr);
@@ -200,8 +202,10 @@ __device__ int gc_set_start(const spx_plan_desc* __restrict__ P, const Launch* _
 }
 
 template <typename R, int BG>
-__global__ __launch_bounds__(BLOCK) void gc_service(const spx_plan_desc* __restrict__ P, const Launch* __restrict__ L)
+__global__ __launch_bounds__(BLOCK) void gc_service(const spx_plan_desc* __restrict__ Pg, const Launch* __restrict__ Lg)
 {
+   const PlanPtr P = (PlanPtr)Pg;
+   const LaunchPtr L = (LaunchPtr)Lg;
    service_slot(P, L, [&](const spx_rng_desc& rng, uint64_t slot, uint64_t traj, uint32_t& nu) {
       return gc_set_start<R, BG>(P, L, rng, slot, traj, nu);
    });
@@ -227,11 +231,11 @@ struct GcHot {
 // src/trajectory_guiding_scatt.cc:255-262), EulerPitchAngleScatt (:81-106), Slopes, EulerPerpDiffSlopes
 // (src/trajectory_guiding_diff.cc:123-140), the Euler RKStep, boundaries and the end-of-step fields.
 template <typename R, int BG>
-__device__ __forceinline__ int gc_step(const spx_plan_desc* __restrict__ P, const Launch* __restrict__ L, const spx_rng_desc& rng,
+__device__ __forceinline__ int gc_step(PlanPtr P, LaunchPtr L, const spx_rng_desc& rng,
                                        uint64_t first, uint64_t slot, GcHot<R>& H, ColdLDS& C, int tid, int& bflags)
 {
-   const spx_bg_t<R>& bg = PlanView<R>::bg(P);
-   const spx_df_t<R>& df = PlanView<R>::df(P);
+   const SPX_CP spx_bg_t<R>& bg = PlanView<R>::bg(P);
+   const SPX_CP spx_df_t<R>& df = PlanView<R>::df(P);
    const bool scatt = (df.kind == SPX_DF_ISO || df.kind == SPX_DF_PERPISO);
    R pp = R(H.pp), pl = R(H.pl);
    const double pmag_d = sqrt(H.pp * H.pp + H.pl * H.pl);
@@ -315,15 +319,17 @@ __device__ __forceinline__ int gc_step(const spx_plan_desc* __restrict__ P, cons
 }
 
 template <typename R, int BG>
-__global__ __launch_bounds__(BLOCK) void gc_step_kernel(const spx_plan_desc* __restrict__ P, const Launch* __restrict__ Lp)
+__global__ __launch_bounds__(BLOCK) void gc_step_kernel(const spx_plan_desc* __restrict__ Pg, const Launch* __restrict__ Lg)
 {
+   const PlanPtr P = (PlanPtr)Pg;
+   const LaunchPtr Lp = (LaunchPtr)Lg;
    __shared__ ColdLDS C;
    const int tid = threadIdx.x;
    const uint64_t slot = (uint64_t)blockIdx.x * BLOCK + tid;
    const bool in_range = slot < Lp->n_lanes;
    const bool run = in_range && Lp->active[slot] == SLOT_RUN;
    if (__ballot(run) == 0) return;
-   const spx_rng_desc rng = Lp->rng;
+   const spx_rng_desc rng = load_rng(Lp);
    const uint64_t first = Lp->first;
    GcHot<R> H;
    if (run) {
@@ -347,9 +353,9 @@ __global__ __launch_bounds__(BLOCK) void gc_step_kernel(const spx_plan_desc* __r
    for (int it = 0; it < Lp->steps_per_launch; it++) {
       if (__ballot(st == SLOT_RUN) == 0) break;
       if (st != SLOT_RUN) continue;
-      const spx_plan_desc* Pi = P;
+      PlanPtr Pi = P;
       asm volatile("" : "+s"(Pi));
-      const Launch* Li = Lp;
+      LaunchPtr Li = Lp;
       asm volatile("" : "+s"(Li));
       int bflags = 0;
       const int e = gc_step<R, BG>(Pi, Li, rng, first, slot, H, C, tid, bflags);
@@ -358,7 +364,7 @@ __global__ __launch_bounds__(BLOCK) void gc_step_kernel(const spx_plan_desc* __r
       else if (bflags & 1) st = SLOT_DONE;
       else if (bflags & 2) st = SLOT_EVENT;
    }
-   const Launch* Ls = Lp;
+   LaunchPtr Ls = Lp;
    asm volatile("" : "+s"(Ls));
    unsigned long long s = steps_here;
    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
@@ -392,17 +398,13 @@ static hipError_t launch_gc_bg(const spx_plan_desc* d_plan, const Launch& L, con
    return hipGetLastError();
 }
 
-hipError_t launch_gc(const spx_plan_desc& hp, const spx_plan_desc* d_plan, const Launch& L, const Launch* d_launch, int precision,
-                     hipStream_t st, hipEvent_t mid)
+template <typename R>
+hipError_t dispatch_gc(const spx_plan_desc& hp, const spx_plan_desc* d_plan, const Launch& L, const Launch* d_launch, hipStream_t st, hipEvent_t mid)
 {
    if (hp.time_flow != 0) return hipErrorInvalidValue;
    switch (hp.bg.kind) {
-   case SPX_BG_SOLARWIND:
-      return precision == 1 ? launch_gc_bg<float, SPX_BG_SOLARWIND>(d_plan, L, d_launch, st, mid)
-                            : launch_gc_bg<double, SPX_BG_SOLARWIND>(d_plan, L, d_launch, st, mid);
-   case SPX_BG_UNIFORM:
-      return precision == 1 ? launch_gc_bg<float, SPX_BG_UNIFORM>(d_plan, L, d_launch, st, mid)
-                            : launch_gc_bg<double, SPX_BG_UNIFORM>(d_plan, L, d_launch, st, mid);
+   case SPX_BG_SOLARWIND: return launch_gc_bg<R, SPX_BG_SOLARWIND>(d_plan, L, d_launch, st, mid);
+   case SPX_BG_UNIFORM: return launch_gc_bg<R, SPX_BG_UNIFORM>(d_plan, L, d_launch, st, mid);
    default: return hipErrorInvalidValue;
    }
 }

@@ -59,12 +59,20 @@ struct Launch {
    uint64_t rec_first, rec_n;
 };
 
-// Host-side dispatch to the kernel instantiation for (transport, precision, background, flow).
-// One engine launch is the service kernel followed by the step kernel on `stream`; when `mid` is
-// not null it is recorded between the two, so the caller can time each kernel with events.
-hipError_t launch_gc(const spx_plan_desc& host_plan, const spx_plan_desc* dev_plan, const Launch& L, const Launch* dev_launch,
-                     int precision, hipStream_t stream, hipEvent_t mid);
-hipError_t launch_transport(const spx_plan_desc& host_plan, const spx_plan_desc* dev_plan, const Launch& L, const Launch* dev_launch,
-                            int precision, int block, hipStream_t stream, hipEvent_t mid);
+// Host-side dispatch.  One engine launch is the service kernel followed by the step kernel on
+// `stream`; when `mid` is not null it is recorded between the two, so the caller can time each
+// kernel with events.  launch_f64 / launch_f32 pick the instantiation for (transport, background,
+// flow, diffusion) of each precision; they live in separately compiled units.
+hipError_t launch_f64(const spx_plan_desc& host_plan, const spx_plan_desc* dev_plan, const Launch& L, const Launch* dev_launch,
+                      hipStream_t stream, hipEvent_t mid);
+hipError_t launch_f32(const spx_plan_desc& host_plan, const spx_plan_desc* dev_plan, const Launch& L, const Launch* dev_launch,
+                      hipStream_t stream, hipEvent_t mid);
+
+inline hipError_t launch_transport(const spx_plan_desc& host_plan, const spx_plan_desc* dev_plan, const Launch& L, const Launch* dev_launch,
+                                   int precision, hipStream_t stream, hipEvent_t mid)
+{
+   return precision == SPX_PREC_MIXED ? launch_f32(host_plan, dev_plan, L, dev_launch, stream, mid)
+                                      : launch_f64(host_plan, dev_plan, L, dev_launch, stream, mid);
+}
 
 }  // namespace spx

@@ -1,0 +1,19 @@
+// spx_kernels_f32.hip -- float instantiations of the transport kernels.
+//
+// The FP32-step / FP64-accumulate kernels: built with reciprocal and approximate-function math
+// and multiply-add contraction (the step arithmetic is FP32; positions, time, momentum and the
+// weights accumulate in FP64).
+#include "spx_parker.h"
+#include "spx_gc.h"
+
+namespace spx {
+
+hipError_t launch_f32(const spx_plan_desc& hp, const spx_plan_desc* d_plan, const Launch& L, const Launch* d_launch, hipStream_t st,
+                      hipEvent_t mid)
+{
+   if (hp.traj_type == SPX_TRAJ_PARKER || hp.traj_type == SPX_TRAJ_PARKER_SOURCE) return dispatch_parker<float>(hp, d_plan, L, d_launch, st, mid);
+   if (hp.traj_type == SPX_TRAJ_GUIDING_DIFF_SCATT) return dispatch_gc<float>(hp, d_plan, L, d_launch, st, mid);
+   return hipErrorInvalidValue;
+}
+
+}  // namespace spx

@@ -1,4 +1,4 @@
-// spx_kernels.hip -- trajectory engine kernels for gfx950 (MI355X).
+// spx_parker.h -- Parker and Parker-with-source transport kernels for gfx950 (MI355X).
 //
 // One lane per trajectory slot; the slots of a run are resident in HBM as structure-of-arrays state.
 // Each engine launch is a pair of kernels, both captured together in a HIP graph for steady loops:
@@ -10,6 +10,8 @@
 //     for the next service pass.
 // The step reproduces TrajectoryParker::Advance (src/trajectory_parker.cc:255-306) with the Euler
 // table and TrajectoryBase::Integrate's checks (src/trajectory_base.cc:946-995).
+#pragma once
+
 #include <hip/hip_runtime.h>
 
 #include "spx_engine.h"
@@ -30,7 +32,7 @@ struct PkFields {
 // derivatives of src/background_smooth_shock.cc:159-187).  The transport consumes only div(U) of the
 // gradients, so only U is evaluated at the derivative points.  Returns false on ExFieldError.
 template <typename R, int BG>
-__device__ __forceinline__ bool parker_fields(const spx_bg_t<R>& bg, double t, double x, double y, double z, R p, R& ds_prev, PkFields<R>& F)
+__device__ __forceinline__ bool parker_fields(const SPX_CP spx_bg_t<R>& bg, double t, double x, double y, double z, R p, R& ds_prev, PkFields<R>& F)
 {
    F.dmax = bg_dmax<R, BG>(bg, x, y, z, ds_prev);
    v3<R> B;
@@ -76,7 +78,7 @@ __device__ __forceinline__ bool parker_fields(const spx_bg_t<R>& bg, double t, d
 // DiffusionBase::GetComponent for components 0 and 1 at a position.  DF is the diffusion class when it
 // is known at compile time (0 = any, dispatched at run time).
 template <typename R, int DF>
-__device__ __forceinline__ void parker_kappa(const spx_df_t<R>& df, R p, R Bmag, double x, double y, double z, v3<R> U, R& K0, R& K1)
+__device__ __forceinline__ void parker_kappa(const SPX_CP spx_df_t<R>& df, R p, R Bmag, double x, double y, double z, v3<R> U, R& K0, R& K1)
 {
    if (DF == SPX_DF_RIGBMAG) {   // src/diffusion_other.cc:804-809; pow(x, 0) == 1 exactly
       const R vmag = VelP(p);
@@ -93,7 +95,7 @@ __device__ __forceinline__ void parker_kappa(const spx_df_t<R>& df, R p, R Bmag,
 }
 
 // Momentum sample of the initial condition (src/initial_momentum.cc:283-410), Parker layout (p, 0, 0).
-__device__ __forceinline__ double parker_mom_sample(const spx_plan_desc* __restrict__ P, const spx_rng_desc& rng, uint64_t first,
+__device__ __forceinline__ double parker_mom_sample(PlanPtr P, const spx_rng_desc& rng, uint64_t first,
                                                     uint64_t traj, uint32_t& nu, bool& ex)
 {
    switch (P->im_kind) {
@@ -117,10 +119,10 @@ __device__ __forceinline__ double parker_mom_sample(const spx_plan_desc* __restr
 // + TrajectoryParkerSource::SetStart (src/trajectory_parker_source.cc:43-52).  Writes the slot's full
 // state to HBM.  Returns an exception code.
 template <typename R, int BG, int TT>
-__device__ int parker_set_start(const spx_plan_desc* __restrict__ P, const Launch* __restrict__ L, const spx_rng_desc& rng, uint64_t slot,
+__device__ int parker_set_start(PlanPtr P, LaunchPtr L, const spx_rng_desc& rng, uint64_t slot,
                                 uint64_t traj, uint32_t& nu)
 {
-   const spx_bg_t<R>& bg = PlanView<R>::bg(P);
+   const SPX_CP spx_bg_t<R>& bg = PlanView<R>::bg(P);
    const uint64_t first = L->first;
    const double t = P->it_t, x = P->is_pos[0], y = P->is_pos[1], z = P->is_pos[2];
    bool ex = false;
@@ -147,7 +149,7 @@ __device__ int parker_set_start(const spx_plan_desc* __restrict__ P, const Launc
    // ResetAllBoundaries (src/trajectory_base.cc:67-81, src/boundary_base.cc:138-162)
    const double dmax = (double)F.dmax;
    for (int b = 0; b < P->n_bnd; b++) {
-      const spx_bnd_desc& B = P->bnd[b];
+      const SPX_CP spx_bnd_desc& B = P->bnd[b];
       const double scale = (B.cls == SPX_BC_TIME) ? fabs(dmax / c_code) : (B.cls == SPX_BC_SPACE ? fabs(dmax) : fabs(p));
       const int left = B.max_crossings;
       const double d = bnd_eval(B, left, false, t, x, y, z, p, 0.0, 0.0, nullptr);
@@ -162,8 +164,10 @@ __device__ int parker_set_start(const spx_plan_desc* __restrict__ P, const Launc
 }
 
 template <typename R, int BG, int TT>
-__global__ __launch_bounds__(BLOCK) void parker_service(const spx_plan_desc* __restrict__ P, const Launch* __restrict__ L)
+__global__ __launch_bounds__(BLOCK) void parker_service(const spx_plan_desc* __restrict__ Pg, const Launch* __restrict__ Lg)
 {
+   const PlanPtr P = (PlanPtr)Pg;
+   const LaunchPtr L = (LaunchPtr)Lg;
    service_slot(P, L, [&](const spx_rng_desc& rng, uint64_t slot, uint64_t traj, uint32_t& nu) {
       return parker_set_start<R, BG, TT>(P, L, rng, slot, traj, nu);
    });
@@ -188,11 +192,11 @@ struct PkHot {
 // (common/rk_config.hh:72-80) and TrajectoryBase::Integrate's checks (src/trajectory_base.cc:964-992).
 // Returns an exception code; `bflags` gets handle_boundaries' result.
 template <typename R, int BG, int FWD, int TT, int DF>
-__device__ __forceinline__ int parker_step(const spx_plan_desc* __restrict__ P, const Launch* __restrict__ L, const spx_rng_desc& rng,
+__device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng_desc& rng,
                                            uint64_t first, uint64_t slot, PkHot<R>& H, ColdLDS& C, int tid, int& bflags)
 {
-   const spx_bg_t<R>& bg = PlanView<R>::bg(P);
-   const spx_df_t<R>& df = PlanView<R>::df(P);
+   const SPX_CP spx_bg_t<R>& bg = PlanView<R>::bg(P);
+   const SPX_CP spx_df_t<R>& df = PlanView<R>::df(P);
    const R p = R(H.p);
    // DriftCoeff + DiffusionCoeff (src/trajectory_parker.cc:189-206, 83-153)
    R K0 = R(0), K1 = R(0);
@@ -294,15 +298,17 @@ __device__ __forceinline__ int parker_step(const spx_plan_desc* __restrict__ P, 
 }
 
 template <typename R, int BG, int FWD, int TT, int DF>
-__global__ __launch_bounds__(BLOCK) void parker_step_kernel(const spx_plan_desc* __restrict__ P, const Launch* __restrict__ Lp)
+__global__ __launch_bounds__(BLOCK) void parker_step_kernel(const spx_plan_desc* __restrict__ Pg, const Launch* __restrict__ Lg)
 {
+   const PlanPtr P = (PlanPtr)Pg;
+   const LaunchPtr Lp = (LaunchPtr)Lg;
    __shared__ ColdLDS C;
    const int tid = threadIdx.x;
    const uint64_t slot = (uint64_t)blockIdx.x * BLOCK + tid;
    const bool in_range = slot < Lp->n_lanes;
    const bool run = in_range && Lp->active[slot] == SLOT_RUN;
    if (__ballot(run) == 0) return;
-   const spx_rng_desc rng = Lp->rng;
+   const spx_rng_desc rng = load_rng(Lp);
    const uint64_t first = Lp->first;
    PkHot<R> H;
    if (run) {
@@ -332,9 +338,9 @@ __global__ __launch_bounds__(BLOCK) void parker_step_kernel(const spx_plan_desc*
       if (st != SLOT_RUN) continue;
       // Re-read the plan and launch block each step instead of pinning their fields in registers: they are
       // uniform scalar loads, cheap to repeat, while keeping them live would exhaust the register file.
-      const spx_plan_desc* Pi = P;
+      PlanPtr Pi = P;
       asm volatile("" : "+s"(Pi));
-      const Launch* Li = Lp;
+      LaunchPtr Li = Lp;
       asm volatile("" : "+s"(Li));
       int bflags = 0;
       const int e = parker_step<R, BG, FWD, TT, DF>(Pi, Li, rng, first, slot, H, C, tid, bflags);
@@ -343,7 +349,7 @@ __global__ __launch_bounds__(BLOCK) void parker_step_kernel(const spx_plan_desc*
       else if (bflags & 1) st = SLOT_DONE;
       else if (bflags & 2) st = SLOT_EVENT;
    }
-   const Launch* Ls = Lp;
+   LaunchPtr Ls = Lp;
    asm volatile("" : "+s"(Ls));
    unsigned long long s = steps_here;
    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
@@ -399,7 +405,7 @@ static hipError_t dispatch_flow(const spx_plan_desc& hp, const spx_plan_desc* d_
 }
 
 template <typename R>
-static hipError_t dispatch_parker(const spx_plan_desc& hp, const spx_plan_desc* d_plan, const Launch& L, const Launch* d_launch, hipStream_t st, hipEvent_t mid)
+hipError_t dispatch_parker(const spx_plan_desc& hp, const spx_plan_desc* d_plan, const Launch& L, const Launch* d_launch, hipStream_t st, hipEvent_t mid)
 {
    switch (hp.bg.kind) {
    case SPX_BG_SOLARWIND: return dispatch_flow<R, SPX_BG_SOLARWIND>(hp, d_plan, L, d_launch, st, mid);
@@ -408,18 +414,6 @@ static hipError_t dispatch_parker(const spx_plan_desc& hp, const spx_plan_desc* 
    case SPX_BG_UNIFORM: return dispatch_flow<R, SPX_BG_UNIFORM>(hp, d_plan, L, d_launch, st, mid);
    default: return hipErrorInvalidValue;
    }
-}
-
-hipError_t launch_transport(const spx_plan_desc& hp, const spx_plan_desc* d_plan, const Launch& L, const Launch* d_launch, int precision, int block,
-                            hipStream_t st, hipEvent_t mid)
-{
-   (void)block;
-   if (hp.traj_type == 0 || hp.traj_type == 1) {
-      if (precision == 1) return dispatch_parker<float>(hp, d_plan, L, d_launch, st, mid);
-      return dispatch_parker<double>(hp, d_plan, L, d_launch, st, mid);
-   }
-   if (hp.traj_type == 2) return launch_gc(hp, d_plan, L, d_launch, precision, st, mid);
-   return hipErrorInvalidValue;
 }
 
 }  // namespace spx

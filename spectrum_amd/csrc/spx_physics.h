@@ -61,32 +61,44 @@ struct Fields {
    R dBmagdt;
 };
 
-// Plan access: the descriptors of the step type R.
+// Plan access.  The kernels read the plan through constant-address-space pointers: the plan never
+// changes while a kernel runs, so every read is a scalar load through the scalar cache instead of a
+// per-lane vector load.
+#define SPX_CP __attribute__((address_space(4)))
+typedef const SPX_CP spx_plan_desc* PlanPtr;
+
+// The descriptors of the step type R.
 template <typename R> struct PlanView;
 template <> struct PlanView<double> {
-   static SPX_HD const spx_bg_t<double>& bg(const spx_plan_desc* P) { return P->bg; }
-   static SPX_HD const spx_df_t<double>& df(const spx_plan_desc* P) { return P->df; }
+   static __device__ const SPX_CP spx_bg_t<double>& bg(PlanPtr P) { return P->bg; }
+   static __device__ const SPX_CP spx_df_t<double>& df(PlanPtr P) { return P->df; }
 };
 template <> struct PlanView<float> {
-   static SPX_HD const spx_bg_t<float>& bg(const spx_plan_desc* P) { return P->bgf; }
-   static SPX_HD const spx_df_t<float>& df(const spx_plan_desc* P) { return P->dff; }
+   static __device__ const SPX_CP spx_bg_t<float>& bg(PlanPtr P) { return P->bgf; }
+   static __device__ const SPX_CP spx_df_t<float>& df(PlanPtr P) { return P->dff; }
 };
 
-template <typename R> SPX_HD v3<R> v3of(const R* a) { return mk<R>(a[0], a[1], a[2]); }
+template <typename R> __device__ v3<R> v3of(const SPX_CP R* a) { return mk<R>(a[0], a[1], a[2]); }
 
 // Relative position x - r0 (double subtraction, then the step type).
 template <typename R>
-SPX_HD v3<R> rel(const spx_bg_t<R>& b, double x, double y, double z) { return mk<R>(R(x - b.r0[0]), R(y - b.r0[1]), R(z - b.r0[2])); }
+SPX_HD v3<R> rel(const SPX_CP spx_bg_t<R>& b, double x, double y, double z) { return mk<R>(R(x - b.r0[0]), R(y - b.r0[1]), R(z - b.r0[2])); }
 
 // BackgroundSolarWind::EvaluateBackground (src/background_solarwind.cc:111-255), every switch at 0.
 // Flags select what is evaluated: U (bit 0), B (bit 1), E (bit 2).
 template <typename R>
-SPX_HD void solarwind_eval(const spx_bg_t<R>& b, v3<R> d, int want, v3<R>& U, v3<R>& B)
+SPX_HD void solarwind_eval(const SPX_CP spx_bg_t<R>& b, v3<R> d, int want, v3<R>& U, v3<R>& B)
 {
+   // With a diagonal +-1 basis, to_basis / from_basis reduce to sign flips: every product with an
+   // off-diagonal zero adds a signed zero, so the results agree bit for bit up to the sign of a zero.
    v3<R> ep[3] = {v3of(b.eprime[0]), v3of(b.eprime[1]), v3of(b.eprime[2])};
-   v3<R> pp = to_basis(d, ep);
+   const bool dg = b.ep_diag != 0;
+   const v3<R> pp = dg ? mk<R>(d.x * b.eprime[0][0], d.y * b.eprime[1][1], d.z * b.eprime[2][2]) : to_basis(d, ep);
    R r = norm(pp);
-   if (want & 1) U = from_basis(b.ur0 * unit(pp), ep);
+   if (want & 1) {
+      const v3<R> u = b.ur0 * unit(pp);
+      U = dg ? mk<R>(u.x * b.eprime[0][0], u.y * b.eprime[1][1], u.z * b.eprime[2][2]) : from_basis(u, ep);
+   }
    if (want & 2) {
       R r_mns = r - b.r_ref;
       R costheta = pp.z / r;
@@ -98,7 +110,7 @@ SPX_HD void solarwind_eval(const spx_bg_t<R>& b, v3<R> d, int want, v3<R>& U, v3
       R Br = b.B0[0] * sqr(b.r_ref / r);
       R Bp = -Br * sintheta * r_mns * b.w0 / b.ur0;
       v3<R> Bl = mk<R>(Br * sintheta * cosphi - Bp * sinphi, Br * sintheta * sinphi + Bp * cosphi, Br * costheta);
-      B = from_basis(Bl, ep);
+      B = dg ? mk<R>(Bl.x * b.eprime[0][0], Bl.y * b.eprime[1][1], Bl.z * b.eprime[2][2]) : from_basis(Bl, ep);
    }
 }
 
@@ -109,7 +121,7 @@ template <typename R> SPX_HD R shock_transition_deriv(R x) { return R(0.5) * R(8
 // Generic EvaluateBackground for U and B (want bits), used at the finite-difference points.
 // `ds` receives BackgroundSmoothShock's transition coordinate of this evaluation.
 template <typename R, int BG>
-SPX_HD void bg_eval(const spx_bg_t<R>& b, double t, double x, double y, double z, int want, v3<R>& U, v3<R>& B, R& ds)
+SPX_HD void bg_eval(const SPX_CP spx_bg_t<R>& b, double t, double x, double y, double z, int want, v3<R>& U, v3<R>& B, R& ds)
 {
    if (BG == SPX_BG_SOLARWIND) {
       solarwind_eval<R>(b, rel(b, x, y, z), want, U, B);
@@ -134,7 +146,7 @@ SPX_HD void bg_eval(const spx_bg_t<R>& b, double t, double x, double y, double z
 // EvaluateDmax per class (src/background_solarwind.cc:310-314, src/background_smooth_shock.cc:192-196,
 // src/background_base.cc:378-382).  `ds_prev` is the smooth shock coordinate of the previous evaluation.
 template <typename R, int BG>
-SPX_HD R bg_dmax(const spx_bg_t<R>& b, double x, double y, double z, R ds_prev)
+SPX_HD R bg_dmax(const SPX_CP spx_bg_t<R>& b, double x, double y, double z, R ds_prev)
 {
    if (BG == SPX_BG_SOLARWIND) return fmin(b.dmax_fraction * norm(rel(b, x, y, z)), b.dmax0);
    if (BG == SPX_BG_SMOOTHSHOCK) return fmin(b.dmax_fraction * b.width_shock * fmax(R(1), fabs(ds_prev)), b.dmax0);
@@ -144,7 +156,7 @@ SPX_HD R bg_dmax(const spx_bg_t<R>& b, double x, double y, double z, R ds_prev)
 // ---- diffusion: DiffusionBase::GetComponent (src/diffusion_base.cc:105-121) + models ----------------
 // Kappa components 0 (perp) and 1 (para) at one point; r is |x| (KINRAD), Umag |U|.
 template <typename R>
-SPX_HD void kappa01(const spx_df_t<R>& d, R p, R Bmag, R r, R Umag, R& K0, R& K1)
+SPX_HD void kappa01(const SPX_CP spx_df_t<R>& d, R p, R Bmag, R r, R Umag, R& K0, R& K1)
 {
    switch (d.kind) {
    case SPX_DF_PARA: K1 = d.D0; break;                                      // :381-385

@@ -32,7 +32,8 @@ enum spx_src_kind { SPX_SRC_NONE = 0, SPX_SRC_MOMINJ = 1 };
 // Background parameters in the step type T (double for the drop-in kernel, float for the mixed one).
 template <typename T>
 struct spx_bg_t {
-   int32_t kind, pad_;
+   int32_t kind;
+   int32_t ep_diag;                    // SolarWind: eprime is diagonal with entries +-1 (rotation axis along +-z)
    double t0, r0[3];                   // always double: positions are accumulated in double
    T u0[3], B0[3], dmax0;
    // BackgroundSolarWind (src/background_solarwind.cc:58-84)
