@@ -97,6 +97,53 @@ def cpu_leg(cf, seconds, threads):
                       f"OpenMP over trajectories, {wall:.1f} s"}
 
 
+def serve_leg(E, cf, args):
+    """A served stream: Poisson arrivals of modulation-spectrum queries (radius U[1, 80] au, 8-64
+    energies, log-uniform trajectories per energy) with 20% diffusive-shock-acceleration queries,
+    submitted at their arrival times to a query server running the mixed kernel."""
+    import numpy as np
+    from spectrum_amd.server import Server
+    rng = np.random.default_rng(args.serve_seed)
+    plan = []
+    t = 0.0
+    for i in range(args.serve_queries):
+        t += rng.exponential(1.0 / args.serve_rate)
+        if rng.random() < 0.2:
+            n = max(64, int(10.0 ** rng.uniform(3.0, 5.0) * args.serve_scale * 8))
+            setup = cf.dsa_shock(name=f"dsa_{i}")
+            kind = "dsa"
+        else:
+            ne = int(rng.integers(8, 65))
+            per = 10.0 ** rng.uniform(3.0, 5.0) * args.serve_scale
+            n = max(ne, int(ne * per))
+            setup = cf.parker_modulation(n_energies=ne, r_start_au=float(rng.uniform(1.0, 80.0)), name=f"mod_{i}")
+            kind = "modulation"
+        plan.append((t, E.Simulation.from_setup(setup), n, i + 1, kind))
+    with Server(precision=E.PREC_MIXED, steps_per_launch=args.serve_steps_per_launch) as srv:
+        t0 = time.perf_counter()
+        ids = []
+        for ta, proto, n, seed, kind in plan:
+            delay = t0 + ta - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            ids.append(srv.submit(proto, n, seed=seed))
+        for q in ids:
+            srv.wait(q)
+        makespan = time.perf_counter() - t0
+        info = [srv.info(q) for q in ids]
+        st = srv.stats()
+    lat = np.array([i["latency_s"] for i in info])
+    steps = sum(i["steps_executed"] for i in info)
+    n_traj = sum(p[2] for p in plan)
+    log(f"serve: {len(ids)} queries ({sum(p[4] == 'dsa' for p in plan)} DSA), {n_traj} trajectories, {steps} steps in {makespan:.2f}s; "
+        f"latency p50 {np.percentile(lat, 50):.3f}s p99 {np.percentile(lat, 99):.3f}s; engine busy {st['busy_s']:.2f}s")
+    return {"queries": len(ids), "dsa_queries": int(sum(p[4] == "dsa" for p in plan)), "arrival_rate_hz": args.serve_rate,
+            "trajectories": int(n_traj), "trajectories_per_energy": [round(1e3 * args.serve_scale, 1), round(1e5 * args.serve_scale, 1)],
+            "p50_latency_s": round(float(np.percentile(lat, 50)), 4), "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
+            "mean_latency_s": round(float(lat.mean()), 4), "makespan_s": round(makespan, 3),
+            "trajectory_steps_per_s": steps / makespan, "precision": "mixed", "steps_per_launch": args.serve_steps_per_launch}
+
+
 def copy_probe(mib):
     import torch
     n = mib * (1 << 20) // 4
@@ -118,6 +165,12 @@ def main():
     ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-queries", type=int, default=40, help="served-stream leg: number of queries (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="mean query arrival rate (1/s)")
+    ap.add_argument("--serve-scale", type=float, default=0.01,
+                    help="trajectories per energy are log-uniform in [1e3, 1e5] times this factor")
+    ap.add_argument("--serve-steps-per-launch", type=int, default=500)
+    ap.add_argument("--serve-seed", type=int, default=7)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB (3 times) and exit: the reference "
                          "kernel of known traffic that scales the FETCH_SIZE / WRITE_SIZE counters")
@@ -171,6 +224,9 @@ def main():
                        "roofline": roofline(f, "fp64", flops, traffic)}
         if main_prec == "mixed":
             out["speedup_mixed_vs_fp64"] = round(value / out["fp64"]["value"], 3)
+    if args.serve_queries > 0:
+        out["serve"] = serve_leg(E, cf, args)
+        out["p99_query_latency_s"] = out["serve"]["p99_latency_s"]
     if not args.no_cpu:
         threads = max(1, min(16, len(os.sched_getaffinity(0))))
         out["cpu_baseline"] = cpu_leg(cf, args.cpu_seconds, threads)

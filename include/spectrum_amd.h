@@ -120,6 +120,56 @@ const char* spx_last_error(const spx_sim* sim);
 /* Name, compute units and architecture of device 0 ("" and SPX_ERR_DEVICE without a GPU). */
 int spx_device_info(char* buf, size_t len);
 
+/* ---- Query server ---------------------------------------------------------------------------
+   Replaces the MPI master/worker request loop (SimulationMaster::MainLoop / WorkerDuties,
+   src/simulation.cc:253-305, 743-835) for many concurrent simulations: each submitted query is a
+   prototype simulation (its plugins and options) plus a trajectory range and a random-stream key.
+   Queries wait in arrival order until their device memory fits `hbm_budget_bytes`, then run beside
+   the others on shared resident slots (continuous batching); each keeps its own counters and
+   histograms on the device, so its result is the same alone and in a mix. */
+typedef struct spx_server spx_server;
+
+enum { SPX_Q_QUEUED = 0, SPX_Q_RUNNING = 1, SPX_Q_DONE = 2, SPX_Q_FAILED = 3 };
+
+typedef struct {
+   int32_t precision;          /* SPX_PREC_* of every query */
+   int32_t steps_per_launch;   /* steps per slot per engine launch (0: 500) */
+   int64_t lanes;              /* resident slots per kernel pool (0: 4 waves per SIMD on every CU) */
+   double hbm_budget_bytes;    /* device memory the admitted queries may hold (0: 4 GiB) */
+   int32_t max_queries;        /* queries admitted at once per pool (0: 64) */
+   int32_t launches_per_poll;  /* engine launches between completion checks (0: 1) */
+} spx_server_opts;
+
+typedef struct {
+   int32_t state;              /* SPX_Q_* */
+   int32_t pad_;
+   uint64_t n_traj, done, failed, steps_done, steps_executed;
+   double t_submit, t_admit, t_done;   /* seconds since the server started (-1: not yet) */
+   double latency_s;           /* t_done - t_submit (-1 until done) */
+   int64_t footprint_bytes;    /* device memory charged against the budget */
+} spx_query_info;
+
+typedef struct {
+   uint64_t submitted, completed, launches, steps_executed;
+   double busy_s;              /* engine time with admitted work */
+   double wall_s;              /* since the server started */
+   int32_t pools, running, queued, pad_;
+} spx_server_stats;
+
+int spx_server_create(const spx_server_opts* opts, spx_server** out);
+/* Snapshot `proto`'s configuration and queue trajectories first_traj .. first_traj+n_traj-1 with
+   counter-based streams keyed by `seed`. */
+int spx_server_submit(spx_server* srv, const spx_sim* proto, uint64_t first_traj, uint64_t n_traj, uint64_t seed, int64_t* query_id);
+int spx_server_info(spx_server* srv, int64_t query_id, spx_query_info* info);
+/* Block until the query is done (SPX_OK), failed (SPX_ERR_DEVICE) or `timeout_s` passed (SPX_ERR_LIMIT; < 0 waits forever). */
+int spx_server_wait(spx_server* srv, int64_t query_id, double timeout_s);
+/* Histograms of a finished query, laid out as spx_get_distro's. */
+int spx_server_result(spx_server* srv, int64_t query_id, int d, int64_t* counts, double* sums, int64_t* n_events);
+int spx_server_release(spx_server* srv, int64_t query_id);
+int spx_server_stats_get(spx_server* srv, spx_server_stats* stats);
+const char* spx_server_last_error(spx_server* srv);
+void spx_server_destroy(spx_server* srv);
+
 #ifdef __cplusplus
 }
 #endif

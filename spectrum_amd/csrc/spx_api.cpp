@@ -5,14 +5,21 @@
 // by the persistent kernels, and distributions accumulate in device histograms.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "spx_kernels.h"
+#include "spx_runtime.h"
 
 using namespace spx;
 
@@ -83,25 +90,6 @@ inline void induced_e(const double* u, const double* B, double* E)
    E[0] = -c[0] / c_code_h; E[1] = -c[1] / c_code_h; E[2] = -c[2] / c_code_h;
 }
 
-template <typename T>
-struct DevBuf {
-   T* p = nullptr;
-   size_t n = 0;
-   hipError_t alloc(size_t count)
-   {
-      if (count <= n && p) return hipSuccess;
-      release();
-      n = count;
-      return hipMalloc((void**)&p, sizeof(T) * (count ? count : 1));
-   }
-   void release()
-   {
-      if (p) (void)hipFree(p);
-      p = nullptr;
-      n = 0;
-   }
-};
-
 }  // namespace
 
 struct spx_sim {
@@ -113,35 +101,18 @@ struct spx_sim {
    int steps_per_launch = 2000;
    int graph_launches = 0;
    std::string err;
-   // device resources
+   int64_t hist_bins = 0;
+   // device resources: one slot pool with a single query, and histograms that accumulate over runs
+   // like the reference's distributions until they are reset
    bool dev_ready = false;
-   hipStream_t stream = nullptr;
-   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-   std::vector<hipEvent_t> tev;        // per-launch events: [2i] before launch i, [2i+1] between its two kernels
-   double step_ms = 0.0, service_ms = 0.0;
-   spx_plan_desc* d_plan = nullptr;
-   RunCtl* d_ctl = nullptr;
-   Launch* d_launch = nullptr;
-   uint64_t n_lanes = 0;
-   DevBuf<int32_t> active, bcross, bstat, attempt, n_events, ev_bin, ev_bnd, ev_pending, ex_code;
-   DevBuf<double> t, x, y, z, m0, m1, m2, amp, wgt, dt_prev, src_old, ds_prev, dmax, dt_adapt, pf, bdelta, bdelta_old;
-   DevBuf<double> t0, x0, y0, z0, m00, m01, m02, ev_w, evs;
-   DevBuf<uint64_t> traj;
-   DevBuf<uint32_t> nn, nu;
-   DevBuf<int64_t> steps;
+   Pool pool;
+   QueryRes q;
    DevBuf<unsigned long long> hist_counts, hist_events;
    DevBuf<double> hist_sums;
-   DevBuf<double> rep_normals, rep_uniforms;
-   DevBuf<spx_traj_record> records;
-   int64_t hist_bins = 0;
    // run state
    bool running = false;
    uint64_t run_first = 0, run_n = 0;
-   spx_rng_desc run_rng{};
    bool keep_records = false;
-   int64_t launches_total = 0;
-   hipGraphExec_t graph_exec = nullptr;
-   hipGraph_t graph = nullptr;
 };
 
 static int fail(spx_sim* s, int code, const std::string& msg)
@@ -356,67 +327,18 @@ static int ensure_device(spx_sim* s)
    if (s->dev_ready) return SPX_OK;
    int n = 0;
    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return fail(s, SPX_ERR_DEVICE, "no HIP device");
-   HIPCHK(s, hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
-   HIPCHK(s, hipEventCreate(&s->ev0));
-   HIPCHK(s, hipEventCreate(&s->ev1));
-   HIPCHK(s, hipMalloc((void**)&s->d_plan, sizeof(spx_plan_desc)));
-   HIPCHK(s, hipMalloc((void**)&s->d_ctl, sizeof(RunCtl)));
-   HIPCHK(s, hipMalloc((void**)&s->d_launch, sizeof(Launch)));
+   HIPCHK(s, hipSetDevice(0));
    s->dev_ready = true;
    return SPX_OK;
 }
 
-static uint64_t auto_lanes(spx_sim* s)
+// Resident slots per device: 4 waves per SIMD on every CU.  The state is register resident, so a few
+// waves per SIMD hide latency; more would only lengthen the tail of a run.
+static uint64_t auto_lanes()
 {
    hipDeviceProp_t prop;
    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return 65536;
-   // a few waves per SIMD keep the VALU busy; the state is register resident
-   uint64_t per_cu = 4 * 4 * 64;   // 4 SIMDs x 4 waves x 64 lanes
-   return (uint64_t)prop.multiProcessorCount * per_cu;
-}
-
-static int alloc_lanes(spx_sim* s, uint64_t n)
-{
-   if (s->n_lanes == n && s->active.p) return SPX_OK;
-   s->n_lanes = n;
-   HIPCHK(s, s->active.alloc(n)); HIPCHK(s, s->attempt.alloc(n)); HIPCHK(s, s->n_events.alloc(n));
-   HIPCHK(s, s->bcross.alloc(n * SPX_MAX_BND)); HIPCHK(s, s->bstat.alloc(n * SPX_MAX_BND));
-   HIPCHK(s, s->ev_bin.alloc(n * SPX_MAX_DIST)); HIPCHK(s, s->ev_w.alloc(n * SPX_MAX_DIST));
-   HIPCHK(s, s->evs.alloc(n * 3 * 9)); HIPCHK(s, s->ev_bnd.alloc(n * 3)); HIPCHK(s, s->ev_pending.alloc(n)); HIPCHK(s, s->ex_code.alloc(n));
-   DevBuf<double>* dv[] = {&s->t, &s->x, &s->y, &s->z, &s->m0, &s->m1, &s->m2, &s->amp, &s->wgt, &s->dt_prev, &s->src_old,
-                           &s->ds_prev, &s->dmax, &s->dt_adapt, &s->pf, &s->t0, &s->x0, &s->y0, &s->z0, &s->m00, &s->m01, &s->m02};
-   for (auto* b : dv) HIPCHK(s, b->alloc(n));
-   HIPCHK(s, s->bdelta.alloc(n * SPX_MAX_BND)); HIPCHK(s, s->bdelta_old.alloc(n * SPX_MAX_BND));
-   HIPCHK(s, s->traj.alloc(n)); HIPCHK(s, s->nn.alloc(n)); HIPCHK(s, s->nu.alloc(n)); HIPCHK(s, s->steps.alloc(n));
-   return SPX_OK;
-}
-
-static Launch make_launch(spx_sim* s)
-{
-   Launch L;
-   std::memset(&L, 0, sizeof(L));
-   L.ctl = s->d_ctl;
-   L.rng = s->run_rng;
-   L.first = s->run_first;
-   L.end = s->run_first + s->run_n;
-   L.n_lanes = s->n_lanes;
-   L.steps_per_launch = s->steps_per_launch;
-   L.n_dist = s->plan.n_dist;
-   for (int d = 0; d < SPX_MAX_DIST; d++) L.plan_host_hist_off[d] = s->plan.hist_off[d];
-   L.active = s->active.p;
-   L.t = s->t.p; L.x = s->x.p; L.y = s->y.p; L.z = s->z.p; L.m0 = s->m0.p; L.m1 = s->m1.p; L.m2 = s->m2.p;
-   L.amp = s->amp.p; L.wgt = s->wgt.p; L.dt_prev = s->dt_prev.p; L.src_old = s->src_old.p; L.ds_prev = s->ds_prev.p;
-   L.dmax = s->dmax.p; L.dt_adapt = s->dt_adapt.p; L.pf = s->pf.p;
-   L.bdelta = s->bdelta.p; L.bdelta_old = s->bdelta_old.p; L.bcross = s->bcross.p; L.bstat = s->bstat.p;
-   L.traj = s->traj.p; L.nn = s->nn.p; L.nu = s->nu.p; L.attempt = s->attempt.p; L.steps = s->steps.p; L.n_events = s->n_events.p;
-   L.t0 = s->t0.p; L.x0 = s->x0.p; L.y0 = s->y0.p; L.z0 = s->z0.p; L.m00 = s->m00.p; L.m01 = s->m01.p; L.m02 = s->m02.p;
-   L.ev_bin = s->ev_bin.p; L.ev_w = s->ev_w.p;
-   L.evs = s->evs.p; L.ev_bnd = s->ev_bnd.p; L.ev_pending = s->ev_pending.p; L.ex_code = s->ex_code.p;
-   L.hist_counts = s->hist_counts.p; L.hist_sums = s->hist_sums.p; L.hist_events = s->hist_events.p;
-   L.records = s->keep_records ? s->records.p : nullptr;
-   L.rec_first = s->run_first;
-   L.rec_n = s->keep_records ? s->run_n : 0;
-   return L;
+   return (uint64_t)prop.multiProcessorCount * 4 * 4 * 64;
 }
 
 // Float copies of the background / diffusion parameters for the mixed-precision kernel, and the
@@ -461,6 +383,19 @@ static int check_ready(spx_sim* s)
    return SPX_OK;
 }
 
+// Combinations the kernels implement (the reference rejects the others at compile time).
+static int check_transport(spx_sim* s)
+{
+   if (s->plan.traj_type == SPX_TRAJ_GUIDING_DIFF_SCATT) {
+      if (s->plan.time_flow != SPX_FLOW_FORWARD) return fail(s, SPX_ERR_ARG, "guiding-centre transport runs forward in time");
+      if (s->plan.bg.kind != SPX_BG_SOLARWIND && s->plan.bg.kind != SPX_BG_UNIFORM)
+         return fail(s, SPX_ERR_ARG, "guiding-centre transport supports the static SolarWind and Uniform backgrounds");
+      if (s->plan.df.kind == SPX_DF_FLOWMOM || s->plan.df.kind == SPX_DF_KINRAD)
+         return fail(s, SPX_ERR_ARG, "guiding-centre transport: diffusion model needs grad U, which its field mask does not compute");
+   }
+   return SPX_OK;
+}
+
 // ---------------------------------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------------------------------
@@ -482,23 +417,9 @@ int spx_create(int traj_type, int time_flow, spx_sim** out)
 void spx_destroy(spx_sim* s)
 {
    if (!s) return;
-   if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
-   if (s->graph) (void)hipGraphDestroy(s->graph);
-   DevBuf<int32_t>* di[] = {&s->active, &s->bcross, &s->bstat, &s->attempt, &s->n_events, &s->ev_bin, &s->ev_bnd, &s->ev_pending, &s->ex_code};
-   for (auto* b : di) b->release();
-   DevBuf<double>* dv[] = {&s->t, &s->x, &s->y, &s->z, &s->m0, &s->m1, &s->m2, &s->amp, &s->wgt, &s->dt_prev, &s->src_old, &s->ds_prev,
-                           &s->dmax, &s->dt_adapt, &s->pf, &s->bdelta, &s->bdelta_old, &s->t0, &s->x0, &s->y0, &s->z0, &s->m00, &s->m01,
-                           &s->m02, &s->ev_w, &s->evs, &s->hist_sums, &s->rep_normals, &s->rep_uniforms};
-   for (auto* b : dv) b->release();
-   s->traj.release(); s->nn.release(); s->nu.release(); s->steps.release();
-   s->hist_counts.release(); s->hist_events.release(); s->records.release();
-   if (s->d_plan) (void)hipFree(s->d_plan);
-   if (s->d_ctl) (void)hipFree(s->d_ctl);
-   if (s->d_launch) (void)hipFree(s->d_launch);
-   for (hipEvent_t e : s->tev) (void)hipEventDestroy(e);
-   if (s->ev0) (void)hipEventDestroy(s->ev0);
-   if (s->ev1) (void)hipEventDestroy(s->ev1);
-   if (s->stream) (void)hipStreamDestroy(s->stream);
+   s->q.release();
+   s->pool.release();
+   s->hist_counts.release(); s->hist_events.release(); s->hist_sums.release();
    delete s;
 }
 
@@ -544,79 +465,52 @@ int spx_set_option(spx_sim* s, const char* name, double v)
 int spx_begin(spx_sim* s, uint64_t first, uint64_t n_traj, const spx_rng* rng, int keep_records)
 {
    if (!s || !rng) return SPX_ERR_ARG;
+   if (rng->mode != 0 && rng->mode != 1) return fail(s, SPX_ERR_ARG, "rng mode");
+   if (rng->mode == 1 && (!rng->normals || rng->normal_stride < 1 || rng->uniform_stride < 0)) return fail(s, SPX_ERR_ARG, "replay needs draws");
    int rc = check_ready(s);
    if (rc) return rc;
    finalize_plan(s);
+   rc = check_transport(s);
+   if (rc) return rc;
    rc = ensure_device(s);
    if (rc) return rc;
-   if (s->plan.traj_type == SPX_TRAJ_GUIDING_DIFF_SCATT) {
-      if (s->plan.time_flow != SPX_FLOW_FORWARD) return fail(s, SPX_ERR_ARG, "guiding-centre transport runs forward in time");
-      if (s->plan.bg.kind != SPX_BG_SOLARWIND && s->plan.bg.kind != SPX_BG_UNIFORM)
-         return fail(s, SPX_ERR_ARG, "guiding-centre transport supports the static SolarWind and Uniform backgrounds");
-      if (s->plan.df.kind == SPX_DF_FLOWMOM || s->plan.df.kind == SPX_DF_KINRAD)
-         return fail(s, SPX_ERR_ARG, "guiding-centre transport: diffusion model needs grad U, which its field mask does not compute");
-   }
-   uint64_t lanes = s->lanes_opt > 0 ? (uint64_t)s->lanes_opt : auto_lanes(s);
+   uint64_t lanes = s->lanes_opt > 0 ? (uint64_t)s->lanes_opt : auto_lanes();
    lanes = (lanes + s->block - 1) / s->block * s->block;
-   rc = alloc_lanes(s, lanes);
-   if (rc) return rc;
-   if (s->hist_counts.n < (size_t)std::max<int64_t>(s->hist_bins, 1)) {
-      HIPCHK(s, s->hist_counts.alloc(std::max<int64_t>(s->hist_bins, 1)));
-      HIPCHK(s, s->hist_sums.alloc(std::max<int64_t>(s->hist_bins, 1) * 9));
-      HIPCHK(s, s->hist_events.alloc(SPX_MAX_DIST));
-      HIPCHK(s, hipMemsetAsync(s->hist_counts.p, 0, sizeof(unsigned long long) * s->hist_counts.n, s->stream));
-      HIPCHK(s, hipMemsetAsync(s->hist_sums.p, 0, sizeof(double) * s->hist_sums.n, s->stream));
-      HIPCHK(s, hipMemsetAsync(s->hist_events.p, 0, sizeof(unsigned long long) * SPX_MAX_DIST, s->stream));
+   lanes = (lanes + 255) / 256 * 256;   // whole blocks of the kernels' block size
+   Pool& P = s->pool;
+   if (!P.matches(s->plan, s->precision) || P.n_lanes != lanes || P.steps_per_launch != s->steps_per_launch ||
+       P.graph_launches != s->graph_launches) {
+      HIPCHK(s, P.init(s->plan, s->precision, lanes, s->steps_per_launch, s->graph_launches, 1));
    }
-   // run description
+   else HIPCHK(s, P.reset_slots());
+   P.sig = s->plan;
+   if (s->hist_counts.n < (size_t)std::max<int64_t>(s->hist_bins, 1)) {
+      const size_t nb = (size_t)std::max<int64_t>(s->hist_bins, 1);
+      HIPCHK(s, s->hist_counts.alloc(nb));
+      HIPCHK(s, s->hist_sums.alloc(nb * 9));
+      HIPCHK(s, s->hist_events.alloc(SPX_MAX_DIST));
+      HIPCHK(s, hipMemsetAsync(s->hist_counts.p, 0, sizeof(unsigned long long) * nb, P.stream));
+      HIPCHK(s, hipMemsetAsync(s->hist_sums.p, 0, sizeof(double) * nb * 9, P.stream));
+      HIPCHK(s, hipMemsetAsync(s->hist_events.p, 0, sizeof(unsigned long long) * SPX_MAX_DIST, P.stream));
+   }
+   s->q.release();
+   s->keep_records = keep_records != 0;
+   HIPCHK(s, s->q.create(s->plan, s->hist_bins, first, n_traj, rng, s->keep_records, s->hist_counts.p, s->hist_sums.p,
+                         s->hist_events.p, P.stream));
+   const int qi = P.alloc_index();
+   if (qi < 0) return fail(s, SPX_ERR_STATE, "query table full");
+   HIPCHK(s, P.set_query(qi, s->q.desc));
+   HIPCHK(s, P.push_queue(qi));
    s->run_first = first;
    s->run_n = n_traj;
-   std::memset(&s->run_rng, 0, sizeof(s->run_rng));
-   s->run_rng.mode = rng->mode;
-   s->run_rng.seed = rng->seed;
-   if (rng->mode == 1) {
-      if (!rng->normals || rng->normal_stride < 1 || rng->uniform_stride < 0) return fail(s, SPX_ERR_ARG, "replay needs draws");
-      size_t nn = (size_t)n_traj * (size_t)rng->normal_stride, nu = (size_t)n_traj * (size_t)std::max<int64_t>(rng->uniform_stride, 1);
-      HIPCHK(s, s->rep_normals.alloc(nn));
-      HIPCHK(s, s->rep_uniforms.alloc(nu));
-      HIPCHK(s, hipMemcpyAsync(s->rep_normals.p, rng->normals, sizeof(double) * nn, hipMemcpyHostToDevice, s->stream));
-      if (rng->uniforms && rng->uniform_stride > 0)
-         HIPCHK(s, hipMemcpyAsync(s->rep_uniforms.p, rng->uniforms, sizeof(double) * n_traj * rng->uniform_stride, hipMemcpyHostToDevice, s->stream));
-      s->run_rng.normals = s->rep_normals.p;
-      s->run_rng.uniforms = s->rep_uniforms.p;
-      s->run_rng.normal_stride = rng->normal_stride;
-      s->run_rng.uniform_stride = (rng->uniforms && rng->uniform_stride > 0) ? rng->uniform_stride : 0;
-   }
-   else if (rng->mode != 0) return fail(s, SPX_ERR_ARG, "rng mode");
-   s->keep_records = keep_records != 0;
-   if (s->keep_records) {
-      HIPCHK(s, s->records.alloc(n_traj ? n_traj : 1));
-      HIPCHK(s, hipMemsetAsync(s->records.p, 0, sizeof(spx_traj_record) * (n_traj ? n_traj : 1), s->stream));
-   }
-   RunCtl ctl;
-   std::memset(&ctl, 0, sizeof(ctl));
-   ctl.next = first;
-   ctl.tmin_bits = 0x7FF0000000000000ull;
-   HIPCHK(s, hipMemcpyAsync(s->d_plan, &s->plan, sizeof(spx_plan_desc), hipMemcpyHostToDevice, s->stream));
-   HIPCHK(s, hipMemcpyAsync(s->d_ctl, &ctl, sizeof(ctl), hipMemcpyHostToDevice, s->stream));
-   HIPCHK(s, hipMemsetAsync(s->active.p, 0, sizeof(int32_t) * s->n_lanes, s->stream));
-   HIPCHK(s, hipMemsetAsync(s->ev_pending.p, 0, sizeof(int32_t) * s->n_lanes, s->stream));
    s->running = true;
-   Launch Lh = make_launch(s);
-   HIPCHK(s, hipMemcpyAsync(s->d_launch, &Lh, sizeof(Launch), hipMemcpyHostToDevice, s->stream));
-   HIPCHK(s, hipStreamSynchronize(s->stream));
-   if (s->graph_exec) { (void)hipGraphExecDestroy(s->graph_exec); s->graph_exec = nullptr; }
-   if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
-   s->running = true;
-   s->launches_total = 0;
    return SPX_OK;
 }
 
 static int read_stats(spx_sim* s, spx_run_stats* st, double wall, double kms, int launches)
 {
    RunCtl ctl;
-   HIPCHK(s, hipMemcpyAsync(&ctl, s->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost, s->stream));
-   HIPCHK(s, hipStreamSynchronize(s->stream));
+   HIPCHK(s, s->q.read_ctl(ctl, s->pool.stream));
    if (st) {
       std::memset(st, 0, sizeof(*st));
       st->n_traj = s->run_n;
@@ -632,62 +526,10 @@ static int read_stats(spx_sim* s, spx_run_stats* st, double wall, double kms, in
       st->wall_s = wall;
       st->kernel_ms = kms;
       st->launches = launches;
-      st->lanes = (int32_t)s->n_lanes;
-      st->step_kernel_ms = s->step_ms;
-      st->service_kernel_ms = s->service_ms;
+      st->lanes = (int32_t)s->pool.n_lanes;
+      st->step_kernel_ms = s->pool.step_ms;
+      st->service_kernel_ms = s->pool.service_ms;
    }
-   return SPX_OK;
-}
-
-static int launch_many(spx_sim* s, int n, float* ms_out)
-{
-   Launch L = make_launch(s);
-   HIPCHK(s, hipEventRecord(s->ev0, s->stream));
-   if (s->graph_launches > 0) {
-      if (!s->graph_exec) {
-         HIPCHK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-         for (int i = 0; i < s->graph_launches; i++) {
-            hipError_t e = launch_transport(s->plan, s->d_plan, L, s->d_launch, s->precision, s->stream, nullptr);
-            if (e != hipSuccess) {
-               hipGraph_t g;
-               (void)hipStreamEndCapture(s->stream, &g);
-               return fail(s, SPX_ERR_DEVICE, std::string("launch: ") + hipGetErrorString(e));
-            }
-         }
-         HIPCHK(s, hipStreamEndCapture(s->stream, &s->graph));
-         HIPCHK(s, hipGraphInstantiate(&s->graph_exec, s->graph, nullptr, nullptr, 0));
-      }
-      int reps = (n + s->graph_launches - 1) / s->graph_launches;
-      for (int i = 0; i < reps; i++) HIPCHK(s, hipGraphLaunch(s->graph_exec, s->stream));
-      n = reps * s->graph_launches;
-   }
-   else {
-      while (s->tev.size() < (size_t)2 * n) {
-         hipEvent_t e;
-         HIPCHK(s, hipEventCreate(&e));
-         s->tev.push_back(e);
-      }
-      for (int i = 0; i < n; i++) {
-         HIPCHK(s, hipEventRecord(s->tev[2 * i], s->stream));
-         hipError_t e = launch_transport(s->plan, s->d_plan, L, s->d_launch, s->precision, s->stream, s->tev[2 * i + 1]);
-         if (e != hipSuccess) return fail(s, SPX_ERR_DEVICE, std::string("launch: ") + hipGetErrorString(e));
-      }
-   }
-   HIPCHK(s, hipEventRecord(s->ev1, s->stream));
-   HIPCHK(s, hipEventSynchronize(s->ev1));
-   float ms = 0.f;
-   HIPCHK(s, hipEventElapsedTime(&ms, s->ev0, s->ev1));
-   if (s->graph_launches == 0) {
-      for (int i = 0; i < n; i++) {
-         float a = 0.f, b = 0.f;
-         HIPCHK(s, hipEventElapsedTime(&a, s->tev[2 * i], s->tev[2 * i + 1]));
-         HIPCHK(s, hipEventElapsedTime(&b, s->tev[2 * i + 1], i + 1 < n ? s->tev[2 * i + 2] : s->ev1));
-         s->service_ms += a;
-         s->step_ms += b;
-      }
-   }
-   if (ms_out) *ms_out = ms;
-   s->launches_total += n;
    return SPX_OK;
 }
 
@@ -696,10 +538,9 @@ int spx_advance(spx_sim* s, int n_launches, spx_run_stats* st)
    if (!s || n_launches < 0) return SPX_ERR_ARG;
    if (!s->running) return fail(s, SPX_ERR_STATE, "advance without begin");
    auto w0 = std::chrono::steady_clock::now();
-   s->step_ms = s->service_ms = 0.0;
+   s->pool.step_ms = s->pool.service_ms = 0.0;
    float ms = 0.f;
-   int rc = launch_many(s, n_launches, &ms);
-   if (rc) return rc;
+   HIPCHK(s, s->pool.launch(n_launches, &ms));
    double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();
    return read_stats(s, st, wall, ms, n_launches);
 }
@@ -709,23 +550,21 @@ int spx_finish(spx_sim* s, spx_traj_record* records, spx_run_stats* st)
    if (!s) return SPX_ERR_ARG;
    if (!s->running) return fail(s, SPX_ERR_STATE, "finish without begin");
    auto w0 = std::chrono::steady_clock::now();
-   s->step_ms = s->service_ms = 0.0;
+   s->pool.step_ms = s->pool.service_ms = 0.0;
    double kms = 0.0;
    int launches = 0;
    // Every launch advances each live slot by steps_per_launch steps; a trajectory needs at most
    // max_attempts * (max_steps + 1) steps, so this bound is never reached by a healthy run.
    const double guard = 64.0 + (double)s->plan.max_attempts * ((double)s->plan.max_steps + 1.0) / s->steps_per_launch *
-                                   std::max(1.0, (double)s->run_n / (double)s->n_lanes + 1.0);
+                                   std::max(1.0, (double)s->run_n / (double)s->pool.n_lanes + 1.0);
    int batch = 1;
    while (true) {
       RunCtl ctl;
-      HIPCHK(s, hipMemcpyAsync(&ctl, s->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost, s->stream));
-      HIPCHK(s, hipStreamSynchronize(s->stream));
+      HIPCHK(s, s->q.read_ctl(ctl, s->pool.stream));
       if (ctl.done >= s->run_n) break;
       if (launches > guard) return fail(s, SPX_ERR_STATE, "run did not drain");
       float ms = 0.f;
-      int rc = launch_many(s, batch, &ms);
-      if (rc) return rc;
+      HIPCHK(s, s->pool.launch(batch, &ms));
       kms += ms;
       launches += batch;
       if (batch < 8) batch *= 2;
@@ -734,9 +573,11 @@ int spx_finish(spx_sim* s, spx_traj_record* records, spx_run_stats* st)
    int rc = read_stats(s, st, wall, kms, launches);
    if (rc) return rc;
    if (records && s->keep_records) {
-      HIPCHK(s, hipMemcpyAsync(records, s->records.p, sizeof(spx_traj_record) * s->run_n, hipMemcpyDeviceToHost, s->stream));
-      HIPCHK(s, hipStreamSynchronize(s->stream));
+      HIPCHK(s, hipMemcpyAsync(records, s->q.records.p, sizeof(spx_traj_record) * s->run_n, hipMemcpyDeviceToHost, s->pool.stream));
+      HIPCHK(s, hipStreamSynchronize(s->pool.stream));
    }
+   HIPCHK(s, s->pool.remove_queue(0));
+   s->pool.free_index(0);
    s->running = false;
    return SPX_OK;
 }
@@ -761,23 +602,20 @@ int spx_get_distro(const spx_sim* sc, int d, int64_t* counts, double* sums, int6
    spx_sim* s = const_cast<spx_sim*>(sc);
    if (!s || d < 0 || d >= s->plan.n_dist) return SPX_ERR_ARG;
    const spx_ds_desc& D = s->plan.dist[d];
-   const int64_t off = s->plan.hist_off[d];
    if (!s->hist_counts.p) {
       if (counts) std::memset(counts, 0, sizeof(int64_t) * D.nb);
       if (sums) std::memset(sums, 0, sizeof(double) * D.nb * D.width);
       if (n_events) *n_events = 0;
       return SPX_OK;
    }
-   std::vector<unsigned long long> c(D.nb);
-   std::vector<double> sm((size_t)D.nb * 9);
-   unsigned long long ev[SPX_MAX_DIST];
-   HIPCHK(s, hipMemcpyAsync(c.data(), s->hist_counts.p + off, sizeof(unsigned long long) * D.nb, hipMemcpyDeviceToHost, s->stream));
-   HIPCHK(s, hipMemcpyAsync(sm.data(), s->hist_sums.p + off * 9, sizeof(double) * D.nb * 9, hipMemcpyDeviceToHost, s->stream));
-   HIPCHK(s, hipMemcpyAsync(ev, s->hist_events.p, sizeof(ev), hipMemcpyDeviceToHost, s->stream));
-   HIPCHK(s, hipStreamSynchronize(s->stream));
-   if (counts) for (int b = 0; b < D.nb; b++) counts[b] = (int64_t)c[b];
-   if (sums) for (int b = 0; b < D.nb; b++) for (int k = 0; k < D.width; k++) sums[(size_t)b * D.width + k] = sm[(size_t)b * 9 + k];
-   if (n_events) *n_events = (int64_t)ev[d];
+   QueryDesc qd = s->q.desc;
+   qd.hist_counts = s->hist_counts.p;
+   qd.hist_sums = s->hist_sums.p;
+   qd.hist_events = s->hist_events.p;
+   QueryRes view;
+   view.desc = qd;
+   HIPCHK(s, view.read_distro(s->plan, d, counts, sums, n_events, s->pool.stream));
+   view.desc = QueryDesc{};
    return SPX_OK;
 }
 
@@ -785,10 +623,10 @@ int spx_reset_distros(spx_sim* s)
 {
    if (!s) return SPX_ERR_ARG;
    if (!s->hist_counts.p) return SPX_OK;
-   HIPCHK(s, hipMemsetAsync(s->hist_counts.p, 0, sizeof(unsigned long long) * s->hist_counts.n, s->stream));
-   HIPCHK(s, hipMemsetAsync(s->hist_sums.p, 0, sizeof(double) * s->hist_sums.n, s->stream));
-   HIPCHK(s, hipMemsetAsync(s->hist_events.p, 0, sizeof(unsigned long long) * SPX_MAX_DIST, s->stream));
-   HIPCHK(s, hipStreamSynchronize(s->stream));
+   HIPCHK(s, hipMemsetAsync(s->hist_counts.p, 0, sizeof(unsigned long long) * s->hist_counts.n, s->pool.stream));
+   HIPCHK(s, hipMemsetAsync(s->hist_sums.p, 0, sizeof(double) * s->hist_sums.n, s->pool.stream));
+   HIPCHK(s, hipMemsetAsync(s->hist_events.p, 0, sizeof(unsigned long long) * SPX_MAX_DIST, s->pool.stream));
+   HIPCHK(s, hipStreamSynchronize(s->pool.stream));
    return SPX_OK;
 }
 
@@ -804,6 +642,336 @@ int spx_device_info(char* buf, size_t len)
    if (hipGetDeviceProperties(&p, 0) != hipSuccess) return SPX_ERR_DEVICE;
    std::snprintf(buf, len, "%s|%s|cu=%d|clock_khz=%d|mem=%zu", p.name, p.gcnArchName, p.multiProcessorCount, p.clockRate, (size_t)p.totalGlobalMem);
    return SPX_OK;
+}
+
+}  // extern "C"
+
+// ===================================================================================================
+// Query server: a persistent engine that runs many queries at once.
+//
+// Queries are submitted from any thread with a configured simulation as their prototype; they wait
+// in a FIFO until their device footprint fits the memory budget and their pool has a free query-table
+// entry, then join their pool's admission queue.  A worker thread keeps every pool with admitted
+// queries launching; inside the kernels, each wave draws trajectories from its query and moves to the
+// oldest admitted query with work left when it runs dry, so queries that arrive while others run
+// start on the slots those free (continuous batching).  Each query has its own random-stream key,
+// control block and histograms, so its result does not depend on what else runs beside it.
+// ===================================================================================================
+namespace {
+
+struct SQuery {
+   int64_t id = 0;
+   int state = SPX_Q_QUEUED;
+   spx_plan_desc plan;
+   int64_t hist_bins = 0;
+   uint64_t first = 0, n_traj = 0;
+   spx_rng rng{};
+   size_t footprint = 0;
+   Pool* pool = nullptr;
+   int qidx = -1;
+   QueryRes res;
+   double t_submit = 0.0, t_admit = -1.0, t_done = -1.0;
+   RunCtl ctl{};
+   std::vector<std::vector<int64_t>> counts;
+   std::vector<std::vector<double>> sums;
+   std::vector<int64_t> nev;
+};
+
+}  // namespace
+
+struct spx_server {
+   spx_server_opts o{};
+   std::string err;
+   std::mutex mu;
+   std::condition_variable cv_work, cv_done;
+   std::thread worker;
+   bool stop = false;
+   bool failed = false;
+   std::deque<SQuery*> pending;
+   std::vector<SQuery*> running;
+   std::map<int64_t, SQuery*> all;
+   std::vector<Pool*> pools;
+   int64_t next_id = 1;
+   size_t hbm_used = 0;
+   std::chrono::steady_clock::time_point t0;
+   uint64_t submitted = 0, completed = 0;
+   double busy_s = 0.0;
+
+   double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+};
+
+static Pool* server_pool(spx_server* S, const spx_plan_desc& plan, hipError_t& e)
+{
+   for (Pool* p : S->pools)
+      if (p->matches(plan, S->o.precision)) return p;
+   Pool* p = new Pool();
+   const uint64_t lanes = S->o.lanes > 0 ? (uint64_t)((S->o.lanes + 255) / 256 * 256) : auto_lanes();
+   e = p->init(plan, S->o.precision, lanes, S->o.steps_per_launch, 0, S->o.max_queries);
+   if (e != hipSuccess) {
+      delete p;
+      return nullptr;
+   }
+   S->pools.push_back(p);
+   return p;
+}
+
+static void server_fail(spx_server* S, const std::string& msg)
+{
+   std::lock_guard<std::mutex> lk(S->mu);
+   S->err = msg;
+   S->failed = true;
+   for (SQuery* q : S->running) { q->state = SPX_Q_FAILED; q->t_done = S->now(); }
+   for (SQuery* q : S->pending) { q->state = SPX_Q_FAILED; q->t_done = S->now(); }
+   S->running.clear();
+   S->pending.clear();
+   S->cv_done.notify_all();
+}
+
+static void server_loop(spx_server* S)
+{
+   if (hipSetDevice(0) != hipSuccess) { server_fail(S, "hipSetDevice failed"); return; }
+   hipEvent_t e0 = nullptr, e1 = nullptr;
+   (void)hipEventCreate(&e0);
+   (void)hipEventCreate(&e1);
+   while (true) {
+      std::vector<SQuery*> admitted;
+      {
+         std::unique_lock<std::mutex> lk(S->mu);
+         S->cv_work.wait(lk, [&] { return S->stop || !S->pending.empty() || !S->running.empty(); });
+         if (S->stop) break;
+         // admission in arrival order, by device-memory footprint and free query-table entries
+         while (!S->pending.empty()) {
+            SQuery* q = S->pending.front();
+            if (!S->running.empty() && S->hbm_used + q->footprint > (size_t)S->o.hbm_budget_bytes) break;
+            hipError_t e = hipSuccess;
+            Pool* pl = server_pool(S, q->plan, e);
+            if (!pl) { lk.unlock(); server_fail(S, std::string("pool: ") + hipGetErrorString(e)); goto out; }
+            const int qi = pl->alloc_index();
+            if (qi < 0) break;
+            S->pending.pop_front();
+            q->pool = pl;
+            q->qidx = qi;
+            q->state = SPX_Q_RUNNING;
+            q->t_admit = S->now();
+            S->hbm_used += q->footprint;
+            S->running.push_back(q);
+            admitted.push_back(q);
+         }
+      }
+      for (SQuery* q : admitted) {
+         hipError_t e = q->res.create(q->plan, q->hist_bins, q->first, q->n_traj, &q->rng, false, nullptr, nullptr, nullptr, q->pool->stream);
+         if (e == hipSuccess) e = q->pool->set_query(q->qidx, q->res.desc);
+         if (e == hipSuccess) e = q->pool->push_queue(q->qidx);
+         if (e != hipSuccess) { server_fail(S, std::string("admit: ") + hipGetErrorString(e)); goto out; }
+      }
+      // one round of launches on every pool that has admitted queries; pools run on their own streams
+      {
+         (void)hipEventRecord(e0, nullptr);
+         bool any = false;
+         for (Pool* pl : S->pools) {
+            if (pl->host_queue.empty()) continue;
+            hipError_t e = pl->enqueue(S->o.launches_per_poll);
+            if (e != hipSuccess) { server_fail(S, std::string("launch: ") + hipGetErrorString(e)); goto out; }
+            any = true;
+         }
+         for (Pool* pl : S->pools)
+            if (!pl->host_queue.empty()) {
+               hipError_t e = hipStreamSynchronize(pl->stream);
+               if (e != hipSuccess) { server_fail(S, std::string("sync: ") + hipGetErrorString(e)); goto out; }
+            }
+         (void)hipEventRecord(e1, nullptr);
+         (void)hipEventSynchronize(e1);
+         float ms = 0.f;
+         if (any && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) S->busy_s += 1e-3 * ms;
+      }
+      // completions
+      {
+         std::vector<SQuery*> snapshot;
+         {
+            std::lock_guard<std::mutex> lk(S->mu);
+            snapshot = S->running;
+         }
+         for (SQuery* q : snapshot) {
+            RunCtl c;
+            hipError_t e = q->res.read_ctl(c, q->pool->stream);
+            if (e != hipSuccess) { server_fail(S, std::string("poll: ") + hipGetErrorString(e)); goto out; }
+            bool finished = c.done >= q->n_traj;
+            if (finished) {
+               q->counts.resize((size_t)q->plan.n_dist);
+               q->sums.resize((size_t)q->plan.n_dist);
+               q->nev.resize((size_t)q->plan.n_dist);
+               for (int d = 0; d < q->plan.n_dist; d++) {
+                  q->counts[(size_t)d].resize((size_t)q->plan.dist[d].nb);
+                  q->sums[(size_t)d].resize((size_t)q->plan.dist[d].nb * q->plan.dist[d].width);
+                  e = q->res.read_distro(q->plan, d, q->counts[(size_t)d].data(), q->sums[(size_t)d].data(), &q->nev[(size_t)d], q->pool->stream);
+                  if (e != hipSuccess) { server_fail(S, std::string("result: ") + hipGetErrorString(e)); goto out; }
+               }
+               e = q->pool->remove_queue(q->qidx);
+               if (e != hipSuccess) { server_fail(S, std::string("retire: ") + hipGetErrorString(e)); goto out; }
+               q->pool->free_index(q->qidx);
+               q->res.release();
+            }
+            std::lock_guard<std::mutex> lk(S->mu);
+            q->ctl = c;
+            if (finished) {
+               q->state = SPX_Q_DONE;
+               q->t_done = S->now();
+               S->hbm_used -= q->footprint;
+               S->running.erase(std::remove(S->running.begin(), S->running.end(), q), S->running.end());
+               S->completed++;
+               S->cv_done.notify_all();
+            }
+         }
+      }
+   }
+out:
+   if (e0) (void)hipEventDestroy(e0);
+   if (e1) (void)hipEventDestroy(e1);
+}
+
+extern "C" {
+
+int spx_server_create(const spx_server_opts* opts, spx_server** out)
+{
+   if (!out) return SPX_ERR_ARG;
+   *out = nullptr;
+   int n = 0;
+   if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return SPX_ERR_DEVICE;
+   spx_server* S = new spx_server();
+   if (opts) S->o = *opts;
+   if (S->o.precision != SPX_PREC_FP64 && S->o.precision != SPX_PREC_MIXED) { delete S; return SPX_ERR_ARG; }
+   if (S->o.steps_per_launch <= 0) S->o.steps_per_launch = 500;
+   if (S->o.hbm_budget_bytes <= 0) S->o.hbm_budget_bytes = 4.0 * (1ull << 30);
+   if (S->o.max_queries <= 0) S->o.max_queries = 64;
+   if (S->o.launches_per_poll <= 0) S->o.launches_per_poll = 1;
+   S->t0 = std::chrono::steady_clock::now();
+   S->worker = std::thread(server_loop, S);
+   *out = S;
+   return SPX_OK;
+}
+
+int spx_server_submit(spx_server* S, const spx_sim* proto, uint64_t first, uint64_t n_traj, uint64_t seed, int64_t* query_id)
+{
+   if (!S || !proto || !query_id || n_traj == 0) return SPX_ERR_ARG;
+   spx_sim* s = const_cast<spx_sim*>(proto);
+   int rc = check_ready(s);
+   if (rc) { S->err = s->err; return rc; }
+   finalize_plan(s);
+   rc = check_transport(s);
+   if (rc) { S->err = s->err; return rc; }
+   SQuery* q = new SQuery();
+   q->plan = s->plan;
+   q->hist_bins = s->hist_bins;
+   q->first = first;
+   q->n_traj = n_traj;
+   q->rng.mode = 0;
+   q->rng.seed = seed;
+   q->footprint = QueryRes::footprint(q->plan, q->hist_bins, n_traj, &q->rng, false);
+   std::lock_guard<std::mutex> lk(S->mu);
+   if (S->failed) { delete q; return SPX_ERR_DEVICE; }
+   q->id = S->next_id++;
+   q->t_submit = S->now();
+   S->all[q->id] = q;
+   S->pending.push_back(q);
+   S->submitted++;
+   *query_id = q->id;
+   S->cv_work.notify_all();
+   return SPX_OK;
+}
+
+int spx_server_info(spx_server* S, int64_t id, spx_query_info* info)
+{
+   if (!S || !info) return SPX_ERR_ARG;
+   std::lock_guard<std::mutex> lk(S->mu);
+   auto it = S->all.find(id);
+   if (it == S->all.end()) return SPX_ERR_ARG;
+   const SQuery* q = it->second;
+   std::memset(info, 0, sizeof(*info));
+   info->state = q->state;
+   info->n_traj = q->n_traj;
+   info->done = q->ctl.done;
+   info->failed = q->ctl.failed;
+   info->steps_done = q->ctl.steps_done;
+   info->steps_executed = q->ctl.steps_executed;
+   info->t_submit = q->t_submit;
+   info->t_admit = q->t_admit;
+   info->t_done = q->t_done;
+   info->latency_s = q->t_done >= 0.0 ? q->t_done - q->t_submit : -1.0;
+   info->footprint_bytes = (int64_t)q->footprint;
+   return SPX_OK;
+}
+
+int spx_server_wait(spx_server* S, int64_t id, double timeout_s)
+{
+   if (!S) return SPX_ERR_ARG;
+   std::unique_lock<std::mutex> lk(S->mu);
+   auto it = S->all.find(id);
+   if (it == S->all.end()) return SPX_ERR_ARG;
+   SQuery* q = it->second;
+   auto pred = [&] { return q->state == SPX_Q_DONE || q->state == SPX_Q_FAILED; };
+   if (timeout_s < 0) S->cv_done.wait(lk, pred);
+   else if (!S->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), pred)) return SPX_ERR_LIMIT;
+   return q->state == SPX_Q_DONE ? SPX_OK : SPX_ERR_DEVICE;
+}
+
+int spx_server_result(spx_server* S, int64_t id, int d, int64_t* counts, double* sums, int64_t* n_events)
+{
+   if (!S) return SPX_ERR_ARG;
+   std::lock_guard<std::mutex> lk(S->mu);
+   auto it = S->all.find(id);
+   if (it == S->all.end()) return SPX_ERR_ARG;
+   const SQuery* q = it->second;
+   if (q->state != SPX_Q_DONE) return SPX_ERR_STATE;
+   if (d < 0 || d >= q->plan.n_dist) return SPX_ERR_ARG;
+   if (counts) std::memcpy(counts, q->counts[(size_t)d].data(), sizeof(int64_t) * q->counts[(size_t)d].size());
+   if (sums) std::memcpy(sums, q->sums[(size_t)d].data(), sizeof(double) * q->sums[(size_t)d].size());
+   if (n_events) *n_events = q->nev[(size_t)d];
+   return SPX_OK;
+}
+
+int spx_server_release(spx_server* S, int64_t id)
+{
+   if (!S) return SPX_ERR_ARG;
+   std::lock_guard<std::mutex> lk(S->mu);
+   auto it = S->all.find(id);
+   if (it == S->all.end()) return SPX_ERR_ARG;
+   if (it->second->state == SPX_Q_QUEUED || it->second->state == SPX_Q_RUNNING) return SPX_ERR_STATE;
+   delete it->second;
+   S->all.erase(it);
+   return SPX_OK;
+}
+
+int spx_server_stats_get(spx_server* S, spx_server_stats* st)
+{
+   if (!S || !st) return SPX_ERR_ARG;
+   std::lock_guard<std::mutex> lk(S->mu);
+   std::memset(st, 0, sizeof(*st));
+   st->submitted = S->submitted;
+   st->completed = S->completed;
+   for (Pool* p : S->pools) st->launches += (uint64_t)p->launches;
+   for (auto& kv : S->all) st->steps_executed += kv.second->ctl.steps_executed;
+   st->busy_s = S->busy_s;
+   st->wall_s = S->now();
+   st->pools = (int32_t)S->pools.size();
+   st->running = (int32_t)S->running.size();
+   st->queued = (int32_t)S->pending.size();
+   return SPX_OK;
+}
+
+const char* spx_server_last_error(spx_server* S) { return S ? S->err.c_str() : "null server"; }
+
+void spx_server_destroy(spx_server* S)
+{
+   if (!S) return;
+   {
+      std::lock_guard<std::mutex> lk(S->mu);
+      S->stop = true;
+   }
+   S->cv_work.notify_all();
+   if (S->worker.joinable()) S->worker.join();
+   for (auto& kv : S->all) { kv.second->res.release(); delete kv.second; }
+   for (Pool* p : S->pools) { p->release(); delete p; }
+   delete S;
 }
 
 }  // extern "C"

@@ -12,17 +12,23 @@
 namespace spx {
 
 typedef const SPX_CP Launch* LaunchPtr;
+typedef const SPX_CP QueryDesc* QueryPtr;
 
-// Member-wise copy of the run's random-stream description out of the constant launch block.
-__device__ __forceinline__ spx_rng_desc load_rng(LaunchPtr L)
+// The query of a wave (uniform by construction) and its plan, as constant-address-space pointers.
+__device__ __forceinline__ QueryPtr query_of(LaunchPtr L, int q) { return (QueryPtr)L->queries + q; }
+__device__ __forceinline__ PlanPtr plan_of(QueryPtr Q) { return (PlanPtr)Q->plan; }
+
+// Member-wise copy of a query's random-stream description out of the constant query table.
+__device__ __forceinline__ spx_rng_desc load_rng(QueryPtr Q)
 {
    spx_rng_desc r;
-   r.mode = L->rng.mode;
-   r.seed = L->rng.seed;
-   r.normals = L->rng.normals;
-   r.uniforms = L->rng.uniforms;
-   r.normal_stride = L->rng.normal_stride;
-   r.uniform_stride = L->rng.uniform_stride;
+   r.mode = Q->rng.mode;
+   r.pad_ = 0;
+   r.seed = Q->rng.seed;
+   r.normals = Q->rng.normals;
+   r.uniforms = Q->rng.uniforms;
+   r.normal_stride = Q->rng.normal_stride;
+   r.uniform_stride = Q->rng.uniform_stride;
    return r;
 }
 
@@ -144,7 +150,7 @@ __device__ __forceinline__ double& ev_field(LaunchPtr L, int cls, int f, uint64_
 // Bins the events of every distribution for a crossing of boundary `bnd`: value and weight per
 // distribution, exact integer counts and FP64 sums, and the per-slot last-event record.
 // ---------------------------------------------------------------------------------------------
-static __device__ int bin_events(PlanPtr P, LaunchPtr L, uint64_t slot, int bnd,
+static __device__ int bin_events(PlanPtr P, QueryPtr Q, LaunchPtr L, uint64_t slot, int bnd,
                           double t2, double x2, double y2, double z2, double m20, double m21, double m22, double amp, double logwgt)
 {
    const bool parker = (P->traj_type != 2);
@@ -230,12 +236,12 @@ static __device__ int bin_events(PlanPtr P, LaunchPtr L, uint64_t slot, int bnd,
       const double ew = exp(logwgt);
 #pragma unroll
       for (int k = 0; k < 9; k++) w[k] = w[k] * ew + amp;
-      const int64_t off = L->plan_host_hist_off[d] + lin;
-      atomicAdd(&L->hist_counts[off], 1ull);
+      const int64_t off = Q->hist_off[d] + lin;
+      atomicAdd(&Q->hist_counts[off], 1ull);
 #pragma unroll
       for (int k = 0; k < 9; k++)
-         if (k < D.width) atomicAdd(&L->hist_sums[off * 9 + k], w[k]);
-      atomicAdd(&L->hist_events[d], 1ull);
+         if (k < D.width) atomicAdd(&Q->hist_sums[off * 9 + k], w[k]);
+      atomicAdd(&Q->hist_events[d], 1ull);
       L->ev_bin[(uint64_t)d * L->n_lanes + slot] = lin;
       L->ev_w[(uint64_t)d * L->n_lanes + slot] = w[0];
    }
@@ -243,20 +249,21 @@ static __device__ int bin_events(PlanPtr P, LaunchPtr L, uint64_t slot, int bnd,
 }
 
 // Record a finished (or failed) trajectory: run statistics and, when asked, its endpoint.
-static __device__ void finish_record(LaunchPtr Lp, uint64_t slot, uint64_t traj, int ex, int attempts, int64_t steps, int n_events,
-                              double t, double x, double y, double z, double m0, double m1, double m2, double amp, double wgt)
+static __device__ void finish_record(QueryPtr Q, LaunchPtr Lp, uint64_t slot, uint64_t traj, int ex, int attempts, int64_t steps,
+                                     int n_events, double t, double x, double y, double z, double m0, double m1, double m2, double amp,
+                                     double wgt)
 {
    const SPX_CP Launch& L = *Lp;
    if (ex == 0) {
       const double el = fabs(t - L.t0[slot]);
-      atomic_min_pos(&L.ctl->tmin_bits, el);
-      atomic_max_pos(&L.ctl->tmax_bits, el);
-      atomicAdd(&L.ctl->steps_done, (unsigned long long)steps);
+      atomic_min_pos(&Q->ctl->tmin_bits, el);
+      atomic_max_pos(&Q->ctl->tmax_bits, el);
+      atomicAdd(&Q->ctl->steps_done, (unsigned long long)steps);
    }
-   else atomicAdd(&L.ctl->failed, 1ull);
-   atomicAdd(&L.ctl->done, 1ull);
-   if (L.records && traj >= L.rec_first && traj < L.rec_first + L.rec_n) {
-      spx_traj_record& r = L.records[traj - L.rec_first];
+   else atomicAdd(&Q->ctl->failed, 1ull);
+   atomicAdd(&Q->ctl->done, 1ull);
+   if (Q->records && traj >= Q->rec_first && traj < Q->rec_first + Q->rec_n) {
+      spx_traj_record& r = Q->records[traj - Q->rec_first];
       r.t = t; r.pos[0] = x; r.pos[1] = y; r.pos[2] = z; r.mom[0] = m0; r.mom[1] = m1; r.mom[2] = m2;
       r.amp = amp; r.wgt = wgt;
       r.t0 = L.t0[slot]; r.pos0[0] = L.x0[slot]; r.pos0[1] = L.y0[slot]; r.pos0[2] = L.z0[slot];
@@ -266,7 +273,7 @@ static __device__ void finish_record(LaunchPtr Lp, uint64_t slot, uint64_t traj,
       r.attempts = attempts;
       r.n_events = n_events;
       for (int d = 0; d < 4; d++) {
-         const bool has = ex == 0 && d < L.n_dist && L.ev_bin[(uint64_t)d * L.n_lanes + slot] >= 0;
+         const bool has = ex == 0 && d < Q->n_dist && L.ev_bin[(uint64_t)d * L.n_lanes + slot] >= 0;
          r.ev_bin[d] = has ? L.ev_bin[(uint64_t)d * L.n_lanes + slot] : -1;
          r.ev_weight[d] = has ? L.ev_w[(uint64_t)d * L.n_lanes + slot] : 0.0;
       }
@@ -385,71 +392,109 @@ __device__ __forceinline__ int handle_boundaries(PlanPtr P, LaunchPtr L, uint64_
 
 // ---------------------------------------------------------------------------------------------
 // Service pass, one thread per slot: bin the events the step kernel queued, finish or retry
-// trajectories, refill idle slots from the work counter and run SetStart through `set_start`
-// (a functor (rng, slot, traj, nu&) -> exception code of the transport).
+// trajectories, refill idle slots from their query's work counter (moving a fully idle wave to the
+// oldest admitted query with trajectories left) and run SetStart through `set_start`, a functor
+// (plan, query, rng, slot, traj, nu&) -> exception code of the transport.
 // ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long load_next(const RunCtl* ctl)
+{
+   return __hip_atomic_load(&ctl->next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <typename StartFn>
-__device__ __forceinline__ void service_slot(PlanPtr P, LaunchPtr L, StartFn&& set_start)
+__device__ __forceinline__ void service_slot(LaunchPtr L, StartFn&& set_start)
 {
    const uint64_t slot = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
-   const bool in_range = slot < L->n_lanes;
-   int st = in_range ? L->active[slot] : SLOT_IDLE;
-   const spx_rng_desc rng = load_rng(L);
-   // 1. pending distribution events of the last step kernel
-   if (st == SLOT_EVENT || st == SLOT_DONE) {
-      const int pend = L->ev_pending[slot];
-      int n_ev = L->n_events[slot];
-      for (int cls = 2; cls >= 0; cls--) {
-         if (!(pend & (1 << cls))) continue;
-         const int b = L->ev_bnd[cls * L->n_lanes + slot];
-         n_ev += bin_events(P, L, slot, b, ev_field(L, cls, EV_T, slot), ev_field(L, cls, EV_X, slot), ev_field(L, cls, EV_Y, slot),
-                            ev_field(L, cls, EV_Z, slot), ev_field(L, cls, EV_M0, slot), ev_field(L, cls, EV_M1, slot),
-                            ev_field(L, cls, EV_M2, slot), ev_field(L, cls, EV_AMP, slot), ev_field(L, cls, EV_WGT, slot));
+   int st = L->active[slot];
+   int q = __builtin_amdgcn_readfirstlane(L->slot_q[slot]);
+   if (q >= 0) {
+      const QueryPtr Q = query_of(L, q);
+      const PlanPtr P = plan_of(Q);
+      // 1. pending distribution events of the last step kernel
+      if (st == SLOT_EVENT || st == SLOT_DONE) {
+         const int pend = L->ev_pending[slot];
+         int n_ev = L->n_events[slot];
+         for (int cls = 2; cls >= 0; cls--) {
+            if (!(pend & (1 << cls))) continue;
+            const int b = L->ev_bnd[cls * L->n_lanes + slot];
+            n_ev += bin_events(P, Q, L, slot, b, ev_field(L, cls, EV_T, slot), ev_field(L, cls, EV_X, slot), ev_field(L, cls, EV_Y, slot),
+                               ev_field(L, cls, EV_Z, slot), ev_field(L, cls, EV_M0, slot), ev_field(L, cls, EV_M1, slot),
+                               ev_field(L, cls, EV_M2, slot), ev_field(L, cls, EV_AMP, slot), ev_field(L, cls, EV_WGT, slot));
+         }
+         L->ev_pending[slot] = 0;
+         L->n_events[slot] = n_ev;
+         if (st == SLOT_DONE) {
+            finish_record(Q, L, slot, L->traj[slot], 0, L->attempt[slot] + 1, L->steps[slot], n_ev, L->t[slot], L->x[slot], L->y[slot],
+                          L->z[slot], L->m0[slot], L->m1[slot], L->m2[slot], L->amp[slot], L->wgt[slot]);
+            st = SLOT_IDLE;
+         }
+         else st = SLOT_RUN;
       }
-      L->ev_pending[slot] = 0;
-      L->n_events[slot] = n_ev;
-      if (st == SLOT_DONE) {
-         finish_record(L, slot, L->traj[slot], 0, L->attempt[slot] + 1, L->steps[slot], n_ev, L->t[slot], L->x[slot], L->y[slot], L->z[slot],
-                       L->m0[slot], L->m1[slot], L->m2[slot], L->amp[slot], L->wgt[slot]);
-         st = SLOT_IDLE;
-      }
-      else st = SLOT_RUN;
-   }
-   // 2. discarded attempts: retry with continuing draws, or give up
-   if (st == SLOT_DISCARD) {
-      const int exc = L->ex_code[slot];
-      atomicAdd(&L->ctl->discards[exc], 1ull);
-      L->ev_pending[slot] = 0;
-      const int att = L->attempt[slot] + 1;
-      L->attempt[slot] = att;
-      if (att < P->max_attempts && exc != 5) st = SLOT_START;
-      else {
-         finish_record(L, slot, L->traj[slot], exc, att, L->steps[slot], L->n_events[slot], L->t[slot], L->x[slot], L->y[slot], L->z[slot],
-                       L->m0[slot], L->m1[slot], L->m2[slot], L->amp[slot], L->wgt[slot]);
-         st = SLOT_IDLE;
-      }
-   }
-   // 3. refill idle slots from the work counter
-   const bool need = in_range && st == SLOT_IDLE;
-   if (__ballot(need) != 0) {
-      const uint64_t id = wave_claim(L->ctl, need);
-      if (need && id < L->end) {
-         L->traj[slot] = id;
-         L->nu[slot] = 0;
-         L->attempt[slot] = 0;
-         st = SLOT_START;
+      // 2. discarded attempts: retry with continuing draws, or give up
+      if (st == SLOT_DISCARD) {
+         const int exc = L->ex_code[slot];
+         atomicAdd(&Q->ctl->discards[exc], 1ull);
+         L->ev_pending[slot] = 0;
+         const int att = L->attempt[slot] + 1;
+         L->attempt[slot] = att;
+         if (att < P->max_attempts && exc != 5) st = SLOT_START;
+         else {
+            finish_record(Q, L, slot, L->traj[slot], exc, att, L->steps[slot], L->n_events[slot], L->t[slot], L->x[slot], L->y[slot],
+                          L->z[slot], L->m0[slot], L->m1[slot], L->m2[slot], L->amp[slot], L->wgt[slot]);
+            st = SLOT_IDLE;
+         }
       }
    }
+   // 3. refill idle slots from the wave's query
+   if (q >= 0) {
+      const bool need = st == SLOT_IDLE;
+      if (__ballot(need) != 0) {
+         const QueryPtr Q = query_of(L, q);
+         const uint64_t id = wave_claim(Q->ctl, need);
+         if (need && id < Q->end) {
+            L->traj[slot] = id;
+            L->nu[slot] = 0;
+            L->attempt[slot] = 0;
+            st = SLOT_START;
+         }
+      }
+   }
+   // a wave with nothing running and nothing left in its query moves to the oldest admitted query with work
+   if (__ballot(st != SLOT_IDLE) == 0) {
+      int nq = -1;
+      if (lane_id() == 0) {
+         for (int i = 0; i < L->n_queue; i++) {
+            const int c = L->queue[i];
+            const QueryPtr C = query_of(L, c);
+            if (load_next(C->ctl) < C->end) { nq = c; break; }
+         }
+      }
+      q = __builtin_amdgcn_readfirstlane(__shfl(nq, 0));
+      if (q >= 0) {
+         const QueryPtr Q = query_of(L, q);
+         const uint64_t id = wave_claim(Q->ctl, true);
+         if (id < Q->end) {
+            L->traj[slot] = id;
+            L->nu[slot] = 0;
+            L->attempt[slot] = 0;
+            st = SLOT_START;
+         }
+      }
+   }
+   L->slot_q[slot] = q;
    // 4. SetStart, repeated with continuing draws when an attempt is discarded
    if (st == SLOT_START) {
+      const QueryPtr Q = query_of(L, q);
+      const PlanPtr P = plan_of(Q);
+      const spx_rng_desc rng = load_rng(Q);
       const uint64_t traj = L->traj[slot];
       uint32_t nu = L->nu[slot];
       int att = L->attempt[slot];
       int exc = 1;
       while (att < P->max_attempts) {
-         exc = set_start(rng, slot, traj, nu);
+         exc = set_start(P, Q, rng, slot, traj, nu);
          if (exc == 0) break;
-         atomicAdd(&L->ctl->discards[exc], 1ull);
+         atomicAdd(&Q->ctl->discards[exc], 1ull);
          att++;
          if (exc == 5) break;
       }
@@ -457,11 +502,11 @@ __device__ __forceinline__ void service_slot(PlanPtr P, LaunchPtr L, StartFn&& s
       L->attempt[slot] = att;
       if (exc == 0) st = SLOT_RUN;
       else {
-         finish_record(L, slot, traj, exc, att, 0, 0, P->it_t, P->is_pos[0], P->is_pos[1], P->is_pos[2], 0.0, 0.0, 0.0, 0.0, 0.0);
+         finish_record(Q, L, slot, traj, exc, att, 0, 0, P->it_t, P->is_pos[0], P->is_pos[1], P->is_pos[2], 0.0, 0.0, 0.0, 0.0, 0.0);
          st = SLOT_IDLE;
       }
    }
-   if (in_range) L->active[slot] = st;
+   L->active[slot] = st;
 }
 
 }  // namespace spx

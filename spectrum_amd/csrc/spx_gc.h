@@ -163,11 +163,11 @@ __device__ __forceinline__ void gc_mom_sample(PlanPtr P, const spx_rng_desc& rng
 // TrajectoryBase::SetStart (src/trajectory_base.cc:867-929) for the guiding centre: the momentum is
 // sampled twice (before and after the first field evaluation, :875 and :891), each time drawing mu.
 template <typename R, int BG>
-__device__ int gc_set_start(PlanPtr P, LaunchPtr L, const spx_rng_desc& rng, uint64_t slot,
+__device__ int gc_set_start(PlanPtr P, QueryPtr Q, LaunchPtr L, const spx_rng_desc& rng, uint64_t slot,
                             uint64_t traj, uint32_t& nu)
 {
    const SPX_CP spx_bg_t<R>& bg = PlanView<R>::bg(P);
-   const uint64_t first = L->first;
+   const uint64_t first = Q->first;
    const double t = P->it_t, x = P->is_pos[0], y = P->is_pos[1], z = P->is_pos[2];
    bool ex = false;
    double pp1, pl1, pp, pl;
@@ -202,12 +202,11 @@ __device__ int gc_set_start(PlanPtr P, LaunchPtr L, const spx_rng_desc& rng, uin
 }
 
 template <typename R, int BG>
-__global__ __launch_bounds__(BLOCK) void gc_service(const spx_plan_desc* __restrict__ Pg, const Launch* __restrict__ Lg)
+__global__ __launch_bounds__(BLOCK) void gc_service(const Launch* __restrict__ Lg)
 {
-   const PlanPtr P = (PlanPtr)Pg;
    const LaunchPtr L = (LaunchPtr)Lg;
-   service_slot(P, L, [&](const spx_rng_desc& rng, uint64_t slot, uint64_t traj, uint32_t& nu) {
-      return gc_set_start<R, BG>(P, L, rng, slot, traj, nu);
+   service_slot(L, [&](PlanPtr P, QueryPtr Q, const spx_rng_desc& rng, uint64_t slot, uint64_t traj, uint32_t& nu) {
+      return gc_set_start<R, BG>(P, Q, L, rng, slot, traj, nu);
    });
 }
 
@@ -319,18 +318,20 @@ __device__ __forceinline__ int gc_step(PlanPtr P, LaunchPtr L, const spx_rng_des
 }
 
 template <typename R, int BG>
-__global__ __launch_bounds__(BLOCK) void gc_step_kernel(const spx_plan_desc* __restrict__ Pg, const Launch* __restrict__ Lg)
+__global__ __launch_bounds__(BLOCK) void gc_step_kernel(const Launch* __restrict__ Lg)
 {
-   const PlanPtr P = (PlanPtr)Pg;
    const LaunchPtr Lp = (LaunchPtr)Lg;
    __shared__ ColdLDS C;
    const int tid = threadIdx.x;
    const uint64_t slot = (uint64_t)blockIdx.x * BLOCK + tid;
-   const bool in_range = slot < Lp->n_lanes;
-   const bool run = in_range && Lp->active[slot] == SLOT_RUN;
+   // the slots of a wave work for one query (the pool size is a multiple of the block size)
+   const int q = __builtin_amdgcn_readfirstlane(Lp->slot_q[slot]);
+   const bool run = q >= 0 && Lp->active[slot] == SLOT_RUN;
    if (__ballot(run) == 0) return;
-   const spx_rng_desc rng = load_rng(Lp);
-   const uint64_t first = Lp->first;
+   const QueryPtr Q = query_of(Lp, q);
+   const PlanPtr P = plan_of(Q);
+   const spx_rng_desc rng = load_rng(Q);
+   const uint64_t first = Q->first;
    GcHot<R> H;
    if (run) {
       const uint64_t n = Lp->n_lanes;
@@ -368,7 +369,7 @@ __global__ __launch_bounds__(BLOCK) void gc_step_kernel(const spx_plan_desc* __r
    asm volatile("" : "+s"(Ls));
    unsigned long long s = steps_here;
    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-   if (lane_id() == 0 && s) atomicAdd(&Ls->ctl->steps_executed, s);
+   if (lane_id() == 0 && s) atomicAdd(&Q->ctl->steps_executed, s);
    if (!run) return;
    const uint64_t n = Ls->n_lanes;
    Ls->active[slot] = st;
@@ -386,25 +387,25 @@ __global__ __launch_bounds__(BLOCK) void gc_step_kernel(const spx_plan_desc* __r
 }
 
 template <typename R, int BG>
-static hipError_t launch_gc_bg(const spx_plan_desc* d_plan, const Launch& L, const Launch* d_launch, hipStream_t st, hipEvent_t mid)
+static hipError_t launch_gc_bg(uint64_t n_lanes, const Launch* d_launch, hipStream_t st, hipEvent_t mid)
 {
-   const unsigned grid = (unsigned)((L.n_lanes + BLOCK - 1) / BLOCK);
-   hipLaunchKernelGGL((gc_service<R, BG>), dim3(grid), dim3(BLOCK), 0, st, d_plan, d_launch);
+   const unsigned grid = (unsigned)((n_lanes + BLOCK - 1) / BLOCK);
+   hipLaunchKernelGGL((gc_service<R, BG>), dim3(grid), dim3(BLOCK), 0, st, d_launch);
    if (mid) {
       hipError_t e = hipEventRecord(mid, st);
       if (e != hipSuccess) return e;
    }
-   hipLaunchKernelGGL((gc_step_kernel<R, BG>), dim3(grid), dim3(BLOCK), 0, st, d_plan, d_launch);
+   hipLaunchKernelGGL((gc_step_kernel<R, BG>), dim3(grid), dim3(BLOCK), 0, st, d_launch);
    return hipGetLastError();
 }
 
 template <typename R>
-hipError_t dispatch_gc(const spx_plan_desc& hp, const spx_plan_desc* d_plan, const Launch& L, const Launch* d_launch, hipStream_t st, hipEvent_t mid)
+hipError_t dispatch_gc(const spx_plan_desc& hp, uint64_t n_lanes, const Launch* d_launch, hipStream_t st, hipEvent_t mid)
 {
    if (hp.time_flow != 0) return hipErrorInvalidValue;
    switch (hp.bg.kind) {
-   case SPX_BG_SOLARWIND: return launch_gc_bg<R, SPX_BG_SOLARWIND>(d_plan, L, d_launch, st, mid);
-   case SPX_BG_UNIFORM: return launch_gc_bg<R, SPX_BG_UNIFORM>(d_plan, L, d_launch, st, mid);
+   case SPX_BG_SOLARWIND: return launch_gc_bg<R, SPX_BG_SOLARWIND>(n_lanes, d_launch, st, mid);
+   case SPX_BG_UNIFORM: return launch_gc_bg<R, SPX_BG_UNIFORM>(n_lanes, d_launch, st, mid);
    default: return hipErrorInvalidValue;
    }
 }

@@ -9,7 +9,7 @@
 
 namespace spx {
 
-// Device-side run control block (one per run / served batch).
+// Device-side control block of one run or served query.
 struct RunCtl {
    unsigned long long next;             // next trajectory id handed out
    unsigned long long done;             // trajectories finished (accepted or failed)
@@ -20,15 +20,32 @@ struct RunCtl {
    unsigned long long tmin_bits, tmax_bits;
 };
 
-// Everything a launch needs; passed by value as the kernel argument.
-struct Launch {
+// One run, or one served query: what its trajectories share.  Lives in device memory and is read
+// with scalar loads; each slot pool holds a table of them.
+struct QueryDesc {
+   const spx_plan_desc* plan;
    RunCtl* ctl;
    spx_rng_desc rng;                    // replay pointers are device pointers here
-   uint64_t first, end;                 // trajectory id range of this run
-   uint64_t n_lanes;
-   int steps_per_launch;
-   int n_dist;
-   int64_t plan_host_hist_off[SPX_MAX_DIST];
+   uint64_t first, end;                 // trajectory id range
+   int32_t n_dist, pad_;
+   int64_t hist_off[SPX_MAX_DIST];      // offsets of each distribution in this query's histograms
+   unsigned long long* hist_counts;
+   double* hist_sums;                   // 9 slots per bin
+   unsigned long long* hist_events;     // per distribution
+   spx_traj_record* records;            // optional per-trajectory records
+   uint64_t rec_first, rec_n;
+};
+
+// A pool of trajectory slots (kernel argument block, in device memory).  The slots of one wave always
+// work for the same query; a wave moves on to the oldest admitted query with trajectories left when
+// all of its slots are idle and its own query has none.
+struct Launch {
+   const QueryDesc* queries;            // query table
+   const int32_t* queue;                // admitted query indices, oldest first
+   int32_t n_queue;
+   int32_t steps_per_launch;
+   uint64_t n_lanes;                    // a multiple of the block size: waves are always full
+   int32_t* slot_q;                     // query index of each slot (-1: none)
    // lane state (structure of arrays, n_lanes each; boundary arrays SPX_MAX_BND * n_lanes)
    int32_t* active;
    double *t, *x, *y, *z, *m0, *m1, *m2, *amp, *wgt, *dt_prev, *src_old, *ds_prev, *dmax, *dt_adapt;
@@ -50,29 +67,22 @@ struct Launch {
    int32_t* ev_bnd;                     // 3 * n_lanes
    int32_t* ev_pending;                 // bit per class
    int32_t* ex_code;                    // discard reason of the last step kernel
-   // histograms
-   unsigned long long* hist_counts;
-   double* hist_sums;                   // 9 slots per bin
-   unsigned long long* hist_events;     // per distribution
-   // optional per-trajectory records
-   spx_traj_record* records;
-   uint64_t rec_first, rec_n;
 };
 
 // Host-side dispatch.  One engine launch is the service kernel followed by the step kernel on
 // `stream`; when `mid` is not null it is recorded between the two, so the caller can time each
 // kernel with events.  launch_f64 / launch_f32 pick the instantiation for (transport, background,
 // flow, diffusion) of each precision; they live in separately compiled units.
-hipError_t launch_f64(const spx_plan_desc& host_plan, const spx_plan_desc* dev_plan, const Launch& L, const Launch* dev_launch,
+hipError_t launch_f64(const spx_plan_desc& host_plan, uint64_t n_lanes, const Launch* dev_launch,
                       hipStream_t stream, hipEvent_t mid);
-hipError_t launch_f32(const spx_plan_desc& host_plan, const spx_plan_desc* dev_plan, const Launch& L, const Launch* dev_launch,
+hipError_t launch_f32(const spx_plan_desc& host_plan, uint64_t n_lanes, const Launch* dev_launch,
                       hipStream_t stream, hipEvent_t mid);
 
-inline hipError_t launch_transport(const spx_plan_desc& host_plan, const spx_plan_desc* dev_plan, const Launch& L, const Launch* dev_launch,
+inline hipError_t launch_transport(const spx_plan_desc& host_plan, uint64_t n_lanes, const Launch* dev_launch,
                                    int precision, hipStream_t stream, hipEvent_t mid)
 {
-   return precision == SPX_PREC_MIXED ? launch_f32(host_plan, dev_plan, L, dev_launch, stream, mid)
-                                      : launch_f64(host_plan, dev_plan, L, dev_launch, stream, mid);
+   return precision == SPX_PREC_MIXED ? launch_f32(host_plan, n_lanes, dev_launch, stream, mid)
+                                      : launch_f64(host_plan, n_lanes, dev_launch, stream, mid);
 }
 
 }  // namespace spx

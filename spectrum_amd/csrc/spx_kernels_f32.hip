@@ -8,11 +8,11 @@
 
 namespace spx {
 
-hipError_t launch_f32(const spx_plan_desc& hp, const spx_plan_desc* d_plan, const Launch& L, const Launch* d_launch, hipStream_t st,
+hipError_t launch_f32(const spx_plan_desc& hp, uint64_t n_lanes, const Launch* d_launch, hipStream_t st,
                       hipEvent_t mid)
 {
-   if (hp.traj_type == SPX_TRAJ_PARKER || hp.traj_type == SPX_TRAJ_PARKER_SOURCE) return dispatch_parker<float>(hp, d_plan, L, d_launch, st, mid);
-   if (hp.traj_type == SPX_TRAJ_GUIDING_DIFF_SCATT) return dispatch_gc<float>(hp, d_plan, L, d_launch, st, mid);
+   if (hp.traj_type == SPX_TRAJ_PARKER || hp.traj_type == SPX_TRAJ_PARKER_SOURCE) return dispatch_parker<float>(hp, n_lanes, d_launch, st, mid);
+   if (hp.traj_type == SPX_TRAJ_GUIDING_DIFF_SCATT) return dispatch_gc<float>(hp, n_lanes, d_launch, st, mid);
    return hipErrorInvalidValue;
 }
 

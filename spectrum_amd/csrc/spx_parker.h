@@ -119,11 +119,11 @@ __device__ __forceinline__ double parker_mom_sample(PlanPtr P, const spx_rng_des
 // + TrajectoryParkerSource::SetStart (src/trajectory_parker_source.cc:43-52).  Writes the slot's full
 // state to HBM.  Returns an exception code.
 template <typename R, int BG, int TT>
-__device__ int parker_set_start(PlanPtr P, LaunchPtr L, const spx_rng_desc& rng, uint64_t slot,
+__device__ int parker_set_start(PlanPtr P, QueryPtr Q, LaunchPtr L, const spx_rng_desc& rng, uint64_t slot,
                                 uint64_t traj, uint32_t& nu)
 {
    const SPX_CP spx_bg_t<R>& bg = PlanView<R>::bg(P);
-   const uint64_t first = L->first;
+   const uint64_t first = Q->first;
    const double t = P->it_t, x = P->is_pos[0], y = P->is_pos[1], z = P->is_pos[2];
    bool ex = false;
    // first sample along an arbitrary axis: only its magnitude enters the first field evaluation
@@ -164,12 +164,11 @@ __device__ int parker_set_start(PlanPtr P, LaunchPtr L, const spx_rng_desc& rng,
 }
 
 template <typename R, int BG, int TT>
-__global__ __launch_bounds__(BLOCK) void parker_service(const spx_plan_desc* __restrict__ Pg, const Launch* __restrict__ Lg)
+__global__ __launch_bounds__(BLOCK) void parker_service(const Launch* __restrict__ Lg)
 {
-   const PlanPtr P = (PlanPtr)Pg;
    const LaunchPtr L = (LaunchPtr)Lg;
-   service_slot(P, L, [&](const spx_rng_desc& rng, uint64_t slot, uint64_t traj, uint32_t& nu) {
-      return parker_set_start<R, BG, TT>(P, L, rng, slot, traj, nu);
+   service_slot(L, [&](PlanPtr P, QueryPtr Q, const spx_rng_desc& rng, uint64_t slot, uint64_t traj, uint32_t& nu) {
+      return parker_set_start<R, BG, TT>(P, Q, L, rng, slot, traj, nu);
    });
 }
 
@@ -298,18 +297,20 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
 }
 
 template <typename R, int BG, int FWD, int TT, int DF>
-__global__ __launch_bounds__(BLOCK) void parker_step_kernel(const spx_plan_desc* __restrict__ Pg, const Launch* __restrict__ Lg)
+__global__ __launch_bounds__(BLOCK) void parker_step_kernel(const Launch* __restrict__ Lg)
 {
-   const PlanPtr P = (PlanPtr)Pg;
    const LaunchPtr Lp = (LaunchPtr)Lg;
    __shared__ ColdLDS C;
    const int tid = threadIdx.x;
    const uint64_t slot = (uint64_t)blockIdx.x * BLOCK + tid;
-   const bool in_range = slot < Lp->n_lanes;
-   const bool run = in_range && Lp->active[slot] == SLOT_RUN;
+   // the slots of a wave work for one query (the pool size is a multiple of the block size)
+   const int q = __builtin_amdgcn_readfirstlane(Lp->slot_q[slot]);
+   const bool run = q >= 0 && Lp->active[slot] == SLOT_RUN;
    if (__ballot(run) == 0) return;
-   const spx_rng_desc rng = load_rng(Lp);
-   const uint64_t first = Lp->first;
+   const QueryPtr Q = query_of(Lp, q);
+   const PlanPtr P = plan_of(Q);
+   const spx_rng_desc rng = load_rng(Q);
+   const uint64_t first = Q->first;
    PkHot<R> H;
    if (run) {
       const uint64_t n = Lp->n_lanes;
@@ -353,7 +354,7 @@ __global__ __launch_bounds__(BLOCK) void parker_step_kernel(const spx_plan_desc*
    asm volatile("" : "+s"(Ls));
    unsigned long long s = steps_here;
    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-   if (lane_id() == 0 && s) atomicAdd(&Ls->ctl->steps_executed, s);
+   if (lane_id() == 0 && s) atomicAdd(&Q->ctl->steps_executed, s);
    if (!run) return;
    const uint64_t n = Ls->n_lanes;
    Ls->active[slot] = st;
@@ -374,44 +375,44 @@ __global__ __launch_bounds__(BLOCK) void parker_step_kernel(const spx_plan_desc*
 // Dispatch: one engine launch = service kernel + step kernel on the same stream
 // ---------------------------------------------------------------------------------------------
 template <typename R, int BG, int FWD, int TT, int DF>
-static hipError_t launch_parker(const spx_plan_desc* d_plan, const Launch& L, const Launch* d_launch, hipStream_t st, hipEvent_t mid)
+static hipError_t launch_parker(uint64_t n_lanes, const Launch* d_launch, hipStream_t st, hipEvent_t mid)
 {
-   const unsigned grid = (unsigned)((L.n_lanes + BLOCK - 1) / BLOCK);
-   hipLaunchKernelGGL((parker_service<R, BG, TT>), dim3(grid), dim3(BLOCK), 0, st, d_plan, d_launch);
+   const unsigned grid = (unsigned)((n_lanes + BLOCK - 1) / BLOCK);
+   hipLaunchKernelGGL((parker_service<R, BG, TT>), dim3(grid), dim3(BLOCK), 0, st, d_launch);
    if (mid) {
       hipError_t e = hipEventRecord(mid, st);
       if (e != hipSuccess) return e;
    }
-   hipLaunchKernelGGL((parker_step_kernel<R, BG, FWD, TT, DF>), dim3(grid), dim3(BLOCK), 0, st, d_plan, d_launch);
+   hipLaunchKernelGGL((parker_step_kernel<R, BG, FWD, TT, DF>), dim3(grid), dim3(BLOCK), 0, st, d_launch);
    return hipGetLastError();
 }
 
 template <typename R, int BG, int FWD, int TT>
-static hipError_t dispatch_df(const spx_plan_desc& hp, const spx_plan_desc* d_plan, const Launch& L, const Launch* d_launch, hipStream_t st, hipEvent_t mid)
+static hipError_t dispatch_df(const spx_plan_desc& hp, uint64_t n_lanes, const Launch* d_launch, hipStream_t st, hipEvent_t mid)
 {
    switch (hp.df.kind) {
-   case SPX_DF_RIGBMAG: return launch_parker<R, BG, FWD, TT, SPX_DF_RIGBMAG>(d_plan, L, d_launch, st, mid);
-   case SPX_DF_FULL: return launch_parker<R, BG, FWD, TT, SPX_DF_FULL>(d_plan, L, d_launch, st, mid);
-   default: return launch_parker<R, BG, FWD, TT, 0>(d_plan, L, d_launch, st, mid);
+   case SPX_DF_RIGBMAG: return launch_parker<R, BG, FWD, TT, SPX_DF_RIGBMAG>(n_lanes, d_launch, st, mid);
+   case SPX_DF_FULL: return launch_parker<R, BG, FWD, TT, SPX_DF_FULL>(n_lanes, d_launch, st, mid);
+   default: return launch_parker<R, BG, FWD, TT, 0>(n_lanes, d_launch, st, mid);
    }
 }
 
 template <typename R, int BG>
-static hipError_t dispatch_flow(const spx_plan_desc& hp, const spx_plan_desc* d_plan, const Launch& L, const Launch* d_launch, hipStream_t st, hipEvent_t mid)
+static hipError_t dispatch_flow(const spx_plan_desc& hp, uint64_t n_lanes, const Launch* d_launch, hipStream_t st, hipEvent_t mid)
 {
    const bool fwd = hp.time_flow == 0;
-   if (hp.traj_type == 1) return fwd ? dispatch_df<R, BG, 1, 1>(hp, d_plan, L, d_launch, st, mid) : dispatch_df<R, BG, 0, 1>(hp, d_plan, L, d_launch, st, mid);
-   return fwd ? dispatch_df<R, BG, 1, 0>(hp, d_plan, L, d_launch, st, mid) : dispatch_df<R, BG, 0, 0>(hp, d_plan, L, d_launch, st, mid);
+   if (hp.traj_type == 1) return fwd ? dispatch_df<R, BG, 1, 1>(hp, n_lanes, d_launch, st, mid) : dispatch_df<R, BG, 0, 1>(hp, n_lanes, d_launch, st, mid);
+   return fwd ? dispatch_df<R, BG, 1, 0>(hp, n_lanes, d_launch, st, mid) : dispatch_df<R, BG, 0, 0>(hp, n_lanes, d_launch, st, mid);
 }
 
 template <typename R>
-hipError_t dispatch_parker(const spx_plan_desc& hp, const spx_plan_desc* d_plan, const Launch& L, const Launch* d_launch, hipStream_t st, hipEvent_t mid)
+hipError_t dispatch_parker(const spx_plan_desc& hp, uint64_t n_lanes, const Launch* d_launch, hipStream_t st, hipEvent_t mid)
 {
    switch (hp.bg.kind) {
-   case SPX_BG_SOLARWIND: return dispatch_flow<R, SPX_BG_SOLARWIND>(hp, d_plan, L, d_launch, st, mid);
-   case SPX_BG_SMOOTHSHOCK: return dispatch_flow<R, SPX_BG_SMOOTHSHOCK>(hp, d_plan, L, d_launch, st, mid);
-   case SPX_BG_SHOCK: return dispatch_flow<R, SPX_BG_SHOCK>(hp, d_plan, L, d_launch, st, mid);
-   case SPX_BG_UNIFORM: return dispatch_flow<R, SPX_BG_UNIFORM>(hp, d_plan, L, d_launch, st, mid);
+   case SPX_BG_SOLARWIND: return dispatch_flow<R, SPX_BG_SOLARWIND>(hp, n_lanes, d_launch, st, mid);
+   case SPX_BG_SMOOTHSHOCK: return dispatch_flow<R, SPX_BG_SMOOTHSHOCK>(hp, n_lanes, d_launch, st, mid);
+   case SPX_BG_SHOCK: return dispatch_flow<R, SPX_BG_SHOCK>(hp, n_lanes, d_launch, st, mid);
+   case SPX_BG_UNIFORM: return dispatch_flow<R, SPX_BG_UNIFORM>(hp, n_lanes, d_launch, st, mid);
    default: return hipErrorInvalidValue;
    }
 }

@@ -170,3 +170,21 @@ def test_oracle_and_library_read_the_same_bytes(oracle_mod):
     assert (rec["status"] == 0).all()
     p0 = s.meta["p0"]
     np.testing.assert_allclose(rec["mom0"][:, 0], p0, rtol=1e-15)
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")
+def test_server_without_device_reports_device_error():
+    from spectrum_amd.server import Server
+    with pytest.raises(engine.SpxError, match="device error"):
+        Server()
+
+
+def test_server_argument_checks():
+    from spectrum_amd import server as S
+    L = S._bind()
+    h = C.c_void_p()
+    assert L.spx_server_create(None, None) == -1
+    assert L.spx_server_submit(None, None, 0, 1, 1, None) == -1
+    assert L.spx_server_wait(None, 1, 0.0) == -1
+    assert L.spx_server_result(None, 1, 0, None, None, None) == -1
+    L.spx_server_destroy(None)
