@@ -45,19 +45,19 @@ def load_json(path):
         return None
 
 
-def gpu_leg(E, cf, prec, args):
-    s = cf.parker_modulation(n_energies=N_ENERGIES)
+def gpu_leg(E, cf, prec, args, setup=None, n_traj=N_ENERGIES * PER_ENERGY):
+    s = setup if setup is not None else cf.parker_modulation(n_energies=N_ENERGIES)
     opts = dict(precision=PREC_ID[prec], steps_per_launch=args.steps_per_launch)
     if args.lanes:
         opts["lanes"] = args.lanes
     sim = E.Simulation.from_setup(s, **opts)
-    sim.begin(N_ENERGIES * PER_ENERGY, seed=1)
+    sim.begin(n_traj, seed=1)
     w = sim.advance(args.warmup)
     t0 = time.perf_counter()
     st = sim.advance(args.steps)
     wall = time.perf_counter() - t0
     steps = st.steps_executed - w.steps_executed
-    log(f"{prec}: lanes {st.lanes} launches {args.steps} trajectory-steps {steps} wall {wall:.3f}s "
+    log(f"{s.name} {prec}: lanes {st.lanes} launches {args.steps} trajectory-steps {steps} wall {wall:.3f}s "
         f"step-kernel {st.step_kernel_ms:.1f} ms service {st.service_kernel_ms:.1f} ms -> {steps / wall:.4e} steps/s")
     return dict(steps=steps, wall=wall, lanes=st.lanes, step_ms=st.step_kernel_ms, service_ms=st.service_kernel_ms,
                 done=st.done, failed=st.failed)
@@ -75,6 +75,25 @@ def roofline(leg, prec, flops, traffic):
     return {"bound": "valu", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
             "traffic": t, "kernel": f"parker_step_kernel ({prec})", "avg_launch_ms": round(per_launch_s * 1e3, 3),
             "flops_per_step": round(fps, 1), "transcendentals_per_step": round(flops["transcendentals_per_step"], 1)}
+
+
+def other_workloads(E, cf, args, flops_all):
+    """Steady-state throughput of the other two transports at their full trajectory counts: diffusive
+    shock acceleration (1e8 trajectories queued) and the guiding centre (1e7), both precisions."""
+    out = {}
+    for name, setup, n, key in (("dsa_smooth_shock_1e8", cf.dsa_shock(), 100_000_000, "dsa_smooth_shock"),
+                                ("gc_diff_scatt_1e7", cf.gc_diff_scatt(), 10_000_000, "gc_diff_scatt")):
+        res = {}
+        for p in ("fp64", "mixed"):
+            leg = gpu_leg(E, cf, p, args, setup, n)
+            leg["launches"] = args.steps
+            rf = roofline(leg, p, flops_all[key], None)
+            res[p] = {"value": leg["steps"] / leg["wall"], "avg_step_kernel_ms": rf["avg_launch_ms"], "roofline_frac": rf["frac"],
+                      "achieved_tflops": rf["achieved"]}
+        res["speedup_mixed_vs_fp64"] = round(res["mixed"]["value"] / res["fp64"]["value"], 3)
+        res["flops_per_step"] = round(flops_all[key]["flops_per_step"], 1)
+        out[name] = res
+    return out
 
 
 def cpu_leg(cf, seconds, threads):
@@ -165,6 +184,7 @@ def main():
     ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-other-workloads", action="store_true", help="skip the DSA and guiding-centre throughput legs")
     ap.add_argument("--serve-queries", type=int, default=40, help="served-stream leg: number of queries (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="mean query arrival rate (1/s)")
     ap.add_argument("--serve-scale", type=float, default=0.01,
@@ -189,7 +209,8 @@ def main():
     from spectrum_amd import configs as cf
     from spectrum_amd import engine as E
     log("device:", E.device_info())
-    flops = load_json(os.path.join(ROOT, "profiles", "flops_per_step.json"))["workloads"][WORKLOAD]
+    flops_all = load_json(os.path.join(ROOT, "profiles", "flops_per_step.json"))["workloads"]
+    flops = flops_all[WORKLOAD]
     traffic = load_json(os.path.join(ROOT, "profiles", "traffic_parker_step_kernel.json"))
 
     precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
@@ -224,6 +245,8 @@ def main():
                        "roofline": roofline(f, "fp64", flops, traffic)}
         if main_prec == "mixed":
             out["speedup_mixed_vs_fp64"] = round(value / out["fp64"]["value"], 3)
+    if not args.no_other_workloads:
+        out["other_workloads"] = other_workloads(E, cf, args, flops_all)
     if args.serve_queries > 0:
         out["serve"] = serve_leg(E, cf, args)
         out["p99_query_latency_s"] = out["serve"]["p99_latency_s"]

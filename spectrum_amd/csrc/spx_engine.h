@@ -106,9 +106,23 @@ __device__ __forceinline__ double next_uniform(const spx_rng_desc& rng, uint64_t
 // ---------------------------------------------------------------------------------------------
 // Boundaries (src/boundary_base.cc:105-191, src/boundary_{time,space,momentum}.cc)
 // ---------------------------------------------------------------------------------------------
+// With R = float the distance to a sphere or plane is formed in FP32 from the FP64 offset (the FP32
+// step's crossing test); time and momentum boundaries and every normal stay in FP64.
+template <typename R = double>
 __device__ __forceinline__ double bnd_eval(const SPX_CP spx_bnd_desc& b, int cross_left, bool gc, double t, double x, double y, double z,
                                            double m0, double m1, double m2, double* nrm)
 {
+   if constexpr (is_f32<R>()) {
+      if (!nrm && b.kind == SPX_BK_SPHERE) {
+         const v3<float> d = mk<float>((float)(x - b.origin[0]), (float)(y - b.origin[1]), (float)(z - b.origin[2]));
+         const float d2 = norm2(d);
+         return (double)(d2 * frsq(d2) - (float)b.radius);
+      }
+      if (!nrm && b.kind == SPX_BK_PLANE) {
+         const v3<float> d = mk<float>((float)(x - b.origin[0]), (float)(y - b.origin[1]), (float)(z - b.origin[2]));
+         return (double)dot(d, mk<float>((float)b.norm[0], (float)b.norm[1], (float)b.norm[2]));
+      }
+   }
    switch (b.kind) {
    case SPX_BK_TIME:
       if (b.status0 & SPX_BND_RECURRENT) return t - (double)(b.max_crossings - cross_left + 1) * b.timemark;  // src/boundary_time.cc:205-208
@@ -324,7 +338,7 @@ __device__ __forceinline__ int handle_boundaries(PlanPtr P, LaunchPtr L, uint64_
       if (b < P->n_bnd) {
          // ComputeBoundary (src/boundary_base.cc:105-127)
          const int left = (H.bst[b] & SPX_BND_RECURRENT) ? C.cross[b][tid] : 0;
-         const double d = bnd_eval(P->bnd[b], left, GC, H.t, H.x, H.y, H.z, H.m0(), 0.0, H.m2(), nullptr);
+         const double d = bnd_eval<typename HOT::real>(P->bnd[b], left, GC, H.t, H.x, H.y, H.z, H.m0(), 0.0, H.m2(), nullptr);
          const double prod = d * H.bdo[b];
          H.bd[b] = (prod == 0.0) ? -H.bdo[b] : d;
          H.bst[b] = (prod <= 0.0) ? (H.bst[b] | SPX_BND_CROSSED) : (H.bst[b] & ~SPX_BND_CROSSED);

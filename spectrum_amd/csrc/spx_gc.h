@@ -47,16 +47,26 @@ __device__ __forceinline__ bool gc_fields(const SPX_CP spx_bg_t<R>& bg, double t
       fa[1] = cross(fa[2], fa[0]);
       R g01 = R(0), g02 = R(0), g10 = R(0), g12 = R(0), g20 = R(0), g21 = R(0);
       v3<R> gbm = mk<R>(R(0), R(0), R(0));
+      v3<R> dc = mk<R>(R(0), R(0), R(0));
+      if constexpr (is_f32<R>()) dc = rel(bg, x, y, z);   // FP32 stencil points relative to the origin
 #pragma unroll 1
       for (int k = 0; k < 3; k++) {
          const v3<R> e = k == 0 ? fa[0] : (k == 1 ? fa[1] : fa[2]);
          R dr = r_g;
-         const double xf = x + (double)(dr * e.x), yf = y + (double)(dr * e.y), zf = z + (double)(dr * e.z);
          v3<R> Ud, Bf, Bb;
          R dsd;
-         bg_eval<R, BG>(bg, t, xf, yf, zf, 2, Ud, Bf, dsd);
-         dr *= R(2);
-         bg_eval<R, BG>(bg, t, xf - (double)(dr * e.x), yf - (double)(dr * e.y), zf - (double)(dr * e.z), 2, Ud, Bb, dsd);
+         if constexpr (is_f32<R>()) {
+            const v3<R> pf = dc + dr * e;
+            bg_eval_rel<R, BG>(bg, t, pf, 2, Ud, Bf, dsd);
+            dr *= R(2);
+            bg_eval_rel<R, BG>(bg, t, pf - dr * e, 2, Ud, Bb, dsd);
+         }
+         else {
+            const double xf = x + (double)(dr * e.x), yf = y + (double)(dr * e.y), zf = z + (double)(dr * e.z);
+            bg_eval<R, BG>(bg, t, xf, yf, zf, 2, Ud, Bf, dsd);
+            dr *= R(2);
+            bg_eval<R, BG>(bg, t, xf - (double)(dr * e.x), yf - (double)(dr * e.y), zf - (double)(dr * e.z), 2, Ud, Bb, dsd);
+         }
          const v3<R> gB = (Bf - Bb) / dr;
          const R gm = (norm(Bf) - norm(Bb)) / dr;
          if (k == 0) {
@@ -220,6 +230,7 @@ struct GcHot {
    int bst[SPX_MAX_BND];
    NormalStream<R> rs;
    int32_t steps;
+   typedef R real;
    __device__ __forceinline__ double m0() const { return pp; }
    __device__ __forceinline__ double m2() const { return pl; }
    __device__ __forceinline__ void reflect_mom() { pl = -pl; }

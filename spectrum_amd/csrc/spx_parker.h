@@ -39,9 +39,17 @@ __device__ __forceinline__ bool parker_fields(const SPX_CP spx_bg_t<R>& bg, doub
    R ds = ds_prev;
    bg_eval<R, BG>(bg, t, x, y, z, 3, F.U, B, ds);
    if (BG == SPX_BG_SMOOTHSHOCK) ds_prev = ds;
-   F.Bmag = norm(B);
-   if (F.Bmag < R(sp_tiny)) return false;
-   F.bhat = B / F.Bmag;
+   if constexpr (is_f32<R>()) {
+      const R b2 = norm2(B), ib = frsq(b2);
+      F.Bmag = b2 * ib;
+      if (F.Bmag < R(sp_tiny)) return false;
+      F.bhat = ib * B;
+   }
+   else {
+      F.Bmag = norm(B);
+      if (F.Bmag < R(sp_tiny)) return false;
+      F.bhat = B / F.Bmag;
+   }
    if (BG == SPX_BG_SOLARWIND) {
       const R r_g = fmin(LarmorRadius(p, F.Bmag), F.dmax);
       v3<R> fa[3];
@@ -50,16 +58,26 @@ __device__ __forceinline__ bool parker_fields(const SPX_CP spx_bg_t<R>& bg, doub
       fa[1] = cross(fa[2], fa[0]);
       // trace of transpose(fa) * G (common/matrix.hh:509-519, :186-189), summed per diagonal element
       R d0 = R(0), d1 = R(0), d2 = R(0);
+      v3<R> dc = mk<R>(R(0), R(0), R(0));
+      if constexpr (is_f32<R>()) dc = rel(bg, x, y, z);   // FP32 stencil points relative to the origin
 #pragma unroll 1
       for (int k = 0; k < 3; k++) {
          const v3<R> e = k == 0 ? fa[0] : (k == 1 ? fa[1] : fa[2]);
          R dr = r_g;
-         const double xf = x + (double)(dr * e.x), yf = y + (double)(dr * e.y), zf = z + (double)(dr * e.z);
          v3<R> Uf, Ub, Bd;
          R dsd;
-         bg_eval<R, BG>(bg, t, xf, yf, zf, 1, Uf, Bd, dsd);
-         dr *= R(2);
-         bg_eval<R, BG>(bg, t, xf - (double)(dr * e.x), yf - (double)(dr * e.y), zf - (double)(dr * e.z), 1, Ub, Bd, dsd);
+         if constexpr (is_f32<R>()) {
+            const v3<R> pf = dc + dr * e;
+            bg_eval_rel<R, BG>(bg, t, pf, 1, Uf, Bd, dsd);
+            dr *= R(2);
+            bg_eval_rel<R, BG>(bg, t, pf - dr * e, 1, Ub, Bd, dsd);
+         }
+         else {
+            const double xf = x + (double)(dr * e.x), yf = y + (double)(dr * e.y), zf = z + (double)(dr * e.z);
+            bg_eval<R, BG>(bg, t, xf, yf, zf, 1, Uf, Bd, dsd);
+            dr *= R(2);
+            bg_eval<R, BG>(bg, t, xf - (double)(dr * e.x), yf - (double)(dr * e.y), zf - (double)(dr * e.z), 1, Ub, Bd, dsd);
+         }
          const v3<R> G = (Uf - Ub) / dr;
          if (k == 0) { d0 = e.x * G.x; d1 = e.y * G.y; d2 = e.z * G.z; }
          else { d0 += e.x * G.x; d1 += e.y * G.y; d2 += e.z * G.z; }
@@ -182,6 +200,7 @@ struct PkHot {
    int bst[SPX_MAX_BND];                        // boundary status bits
    NormalStream<R> rs;
    int32_t steps;
+   typedef R real;
    __device__ __forceinline__ double m0() const { return p; }
    __device__ __forceinline__ double m2() const { return 0.0; }
    __device__ __forceinline__ void reflect_mom() {}
@@ -205,26 +224,36 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
    const bool pos_dep = (DF == SPX_DF_FULL) ? false : (df.pos_dep != 0);
    if (pos_dep || K0 != K1) {
       const int want = (df.kind == SPX_DF_FLOWMOM || df.kind == SPX_DF_KINRAD) ? 3 : 2;
+      // FP32 steps with a position-independent kappa evaluate the stencil relative to the origin
+      v3<R> dc = mk<R>(R(0), R(0), R(0));
+      if constexpr (is_f32<R>()) dc = rel(bg, H.x, H.y, H.z);
 #pragma unroll 1
       for (int j = 0; j < 3; j++) {
          const double ex = (j == 0) ? 1.0 : 0.0, ey = (j == 1) ? 1.0 : 0.0, ez = (j == 2) ? 1.0 : 0.0;
+         const v3<R> ej = mk<R>(R(ex), R(ey), R(ez));
          double xf = H.x + (double)delta * ex, yf = H.y + (double)delta * ey, zf = H.z + (double)delta * ez;
          v3<R> U2, B2;
          R dsd = H.ds_prev;
-         bg_eval<R, BG>(bg, H.t, xf, yf, zf, want, U2, B2, dsd);
-         const R Bmf = norm(B2);
+         if (is_f32<R>() && !pos_dep) bg_eval_rel<R, BG>(bg, H.t, dc + delta * ej, want, U2, B2, dsd);
+         else bg_eval<R, BG>(bg, H.t, xf, yf, zf, want, U2, B2, dsd);
+         R Bmf;
+         v3<R> bf;
+         if constexpr (is_f32<R>()) { const R b2 = norm2(B2), ib = frsq(b2); Bmf = b2 * ib; bf = ib * B2; }
+         else { Bmf = norm(B2); bf = B2 / Bmf; }
          if (Bmf < R(sp_tiny)) return 1;
-         const v3<R> bf = B2 / Bmf;
          R K0f = K0, K1f = K1;
          if (pos_dep) parker_kappa<R, DF>(df, p, Bmf, xf, yf, zf, U2, K0f, K1f);
          xf -= 2.0 * (double)delta * ex;
          yf -= 2.0 * (double)delta * ey;
          zf -= 2.0 * (double)delta * ez;
-         bg_eval<R, BG>(bg, H.t, xf, yf, zf, want, U2, B2, dsd);
+         if (is_f32<R>() && !pos_dep) bg_eval_rel<R, BG>(bg, H.t, dc + delta * ej - (R(2) * delta) * ej, want, U2, B2, dsd);
+         else bg_eval<R, BG>(bg, H.t, xf, yf, zf, want, U2, B2, dsd);
          if (BG == SPX_BG_SMOOTHSHOCK) H.ds_prev = dsd;
-         const R Bmb = norm(B2);
+         R Bmb;
+         v3<R> bb;
+         if constexpr (is_f32<R>()) { const R b2 = norm2(B2), ib = frsq(b2); Bmb = b2 * ib; bb = ib * B2; }
+         else { Bmb = norm(B2); bb = B2 / Bmb; }
          if (Bmb < R(sp_tiny)) return 1;
-         const v3<R> bb = B2 / Bmb;
          R K0b = K0, K1b = K1;
          if (pos_dep) parker_kappa<R, DF>(df, p, Bmb, xf, yf, zf, U2, K0b, K1b);
          const R bfj = j == 0 ? bf.x : (j == 1 ? bf.y : bf.z);

@@ -6,6 +6,8 @@
 // amplitude and weight are always accumulated in double.
 #pragma once
 
+#include <type_traits>
+
 #include "spx_plan.h"
 #include "spx_vec.h"
 
@@ -38,6 +40,11 @@ SPX_HD double rpow(double a, double b) { return pow(a, b); }
 SPX_HD float rpow(float a, float b) { return __powf(a, b); }
 SPX_HD double rtanh(double a) { return tanh(a); }
 SPX_HD float rtanh(float a) { return tanhf(a); }
+
+// The FP32 step may restructure its arithmetic (one reciprocal square root for a length and its
+// inverse, stencil points relative to the background origin); the FP64 step keeps the reference's.
+template <typename R> constexpr bool is_f32() { return std::is_same<R, float>::value; }
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
 // ---- relativistic kinematics (common/physics.hh:246-456) ---------------------------------------
 template <typename R> SPX_HD R EnrTot(R p) { return rsqrt_(sqr(p) + sqr(R(sp_mass * c_code))) * R(c_code); }
@@ -94,6 +101,25 @@ SPX_HD void solarwind_eval(const SPX_CP spx_bg_t<R>& b, v3<R> d, int want, v3<R>
    v3<R> ep[3] = {v3of(b.eprime[0]), v3of(b.eprime[1]), v3of(b.eprime[2])};
    const bool dg = b.ep_diag != 0;
    const v3<R> pp = dg ? mk<R>(d.x * b.eprime[0][0], d.y * b.eprime[1][1], d.z * b.eprime[2][2]) : to_basis(d, ep);
+   if constexpr (is_f32<R>()) {
+      const R r2 = norm2(pp), ir = frsq(r2), r = r2 * ir;
+      if (want & 1) {
+         const v3<R> u = (b.ur0 * ir) * pp;
+         U = dg ? mk<R>(u.x * b.eprime[0][0], u.y * b.eprime[1][1], u.z * b.eprime[2][2]) : from_basis(u, ep);
+      }
+      if (want & 2) {
+         const R costheta = pp.z * ir;
+         const R sintheta = rsqrt_(fmax(R(1) - sqr(costheta), R(0)));
+         const R s2 = sqr(pp.x) + sqr(pp.y);
+         R cosphi = R(0), sinphi = R(0);
+         if (s2 >= sqr(r * R(sp_tiny))) { const R is = frsq(s2); cosphi = pp.x * is; sinphi = pp.y * is; }
+         const R Br = b.B0[0] * sqr(b.r_ref * ir);
+         const R Bp = -Br * sintheta * (r - b.r_ref) * b.w0 / b.ur0;
+         const v3<R> Bl = mk<R>(Br * sintheta * cosphi - Bp * sinphi, Br * sintheta * sinphi + Bp * cosphi, Br * costheta);
+         B = dg ? mk<R>(Bl.x * b.eprime[0][0], Bl.y * b.eprime[1][1], Bl.z * b.eprime[2][2]) : from_basis(Bl, ep);
+      }
+      return;
+   }
    R r = norm(pp);
    if (want & 1) {
       const v3<R> u = b.ur0 * unit(pp);
@@ -118,29 +144,33 @@ SPX_HD void solarwind_eval(const SPX_CP spx_bg_t<R>& b, v3<R> d, int want, v3<R>
 template <typename R> SPX_HD R shock_transition(R x) { return R(0.5) * (R(1) + rtanh(R(8) * x)); }
 template <typename R> SPX_HD R shock_transition_deriv(R x) { return R(0.5) * R(8) * (R(1) - sqr(rtanh(R(8) * x))); }
 
-// Generic EvaluateBackground for U and B (want bits), used at the finite-difference points.
-// `ds` receives BackgroundSmoothShock's transition coordinate of this evaluation.
+// Generic EvaluateBackground for U and B (want bits) at d = x - r0, used at the finite-difference
+// points.  `ds` receives BackgroundSmoothShock's transition coordinate of this evaluation.
 template <typename R, int BG>
-SPX_HD void bg_eval(const SPX_CP spx_bg_t<R>& b, double t, double x, double y, double z, int want, v3<R>& U, v3<R>& B, R& ds)
+SPX_HD void bg_eval_rel(const SPX_CP spx_bg_t<R>& b, double t, v3<R> d, int want, v3<R>& U, v3<R>& B, R& ds)
 {
    if (BG == SPX_BG_SOLARWIND) {
-      solarwind_eval<R>(b, rel(b, x, y, z), want, U, B);
+      solarwind_eval<R>(b, d, want, U, B);
    }
    else if (BG == SPX_BG_UNIFORM) {   // src/background_uniform.cc:70-77
       U = v3of(b.u0); B = v3of(b.B0);
    }
    else if (BG == SPX_BG_SHOCK) {     // src/background_shock.cc:91-111
-      v3<R> d = rel(b, x, y, z);
       if (dot(d, v3of(b.n_shock)) - b.v_shock * R(t) > R(0)) { U = v3of(b.u0); B = v3of(b.B0); }
       else { U = v3of(b.u1); B = v3of(b.B1); }
    }
    else if (BG == SPX_BG_SMOOTHSHOCK) {   // src/background_smooth_shock.cc:139-154
-      v3<R> d = rel(b, x, y, z);
       ds = (dot(d, v3of(b.n_shock)) - b.v_shock * R(t)) / b.width_shock;
       R a1 = shock_transition(ds), a2 = R(1) - a1;
       U = a1 * v3of(b.u0) + a2 * v3of(b.u1);
       B = a1 * v3of(b.B0) + a2 * v3of(b.B1);
    }
+}
+
+template <typename R, int BG>
+SPX_HD void bg_eval(const SPX_CP spx_bg_t<R>& b, double t, double x, double y, double z, int want, v3<R>& U, v3<R>& B, R& ds)
+{
+   bg_eval_rel<R, BG>(b, t, rel(b, x, y, z), want, U, B, ds);
 }
 
 // EvaluateDmax per class (src/background_solarwind.cc:310-314, src/background_smooth_shock.cc:192-196,
