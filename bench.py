@@ -45,12 +45,18 @@ def load_json(path):
         return None
 
 
-def gpu_leg(E, cf, prec, args, setup=None, n_traj=N_ENERGIES * PER_ENERGY):
+def gpu_leg(E, cf, prec, args, setup=None, n_traj=N_ENERGIES * PER_ENERGY, autotune=False):
     s = setup if setup is not None else cf.parker_modulation(n_energies=N_ENERGIES)
     opts = dict(precision=PREC_ID[prec], steps_per_launch=args.steps_per_launch)
     if args.lanes:
         opts["lanes"] = args.lanes
     sim = E.Simulation.from_setup(s, **opts)
+    tuned = None
+    if autotune:
+        table = sim.autotune(n_probe=n_traj, seed=2)
+        best = max(table, key=lambda e: e[2])
+        tuned = {"lanes": best[0], "steps_per_launch": best[1], "candidates": len(table)}
+        log(f"{s.name} {prec}: autotuned lanes {best[0]} steps/launch {best[1]} ({best[2]:.3e} steps/s in the probe)")
     sim.begin(n_traj, seed=1)
     w = sim.advance(args.warmup)
     t0 = time.perf_counter()
@@ -60,7 +66,7 @@ def gpu_leg(E, cf, prec, args, setup=None, n_traj=N_ENERGIES * PER_ENERGY):
     log(f"{s.name} {prec}: lanes {st.lanes} launches {args.steps} trajectory-steps {steps} wall {wall:.3f}s "
         f"step-kernel {st.step_kernel_ms:.1f} ms service {st.service_kernel_ms:.1f} ms -> {steps / wall:.4e} steps/s")
     return dict(steps=steps, wall=wall, lanes=st.lanes, step_ms=st.step_kernel_ms, service_ms=st.service_kernel_ms,
-                done=st.done, failed=st.failed)
+                done=st.done, failed=st.failed, tuned=tuned)
 
 
 def roofline(leg, prec, flops, traffic):
@@ -81,15 +87,17 @@ def other_workloads(E, cf, args, flops_all):
     """Steady-state throughput of the other two transports at their full trajectory counts: diffusive
     shock acceleration (1e8 trajectories queued) and the guiding centre (1e7), both precisions."""
     out = {}
-    for name, setup, n, key in (("dsa_smooth_shock_1e8", cf.dsa_shock(), 100_000_000, "dsa_smooth_shock"),
-                                ("gc_diff_scatt_1e7", cf.gc_diff_scatt(), 10_000_000, "gc_diff_scatt")):
+    for name, setup, n, key, tune in (("dsa_smooth_shock_1e8", cf.dsa_shock(), 100_000_000, "dsa_smooth_shock", True),
+                                      ("gc_diff_scatt_1e7", cf.gc_diff_scatt(), 10_000_000, "gc_diff_scatt", False)):
         res = {}
         for p in ("fp64", "mixed"):
-            leg = gpu_leg(E, cf, p, args, setup, n)
+            leg = gpu_leg(E, cf, p, args, setup, n, autotune=tune)
             leg["launches"] = args.steps
             rf = roofline(leg, p, flops_all[key], None)
             res[p] = {"value": leg["steps"] / leg["wall"], "avg_step_kernel_ms": rf["avg_launch_ms"], "roofline_frac": rf["frac"],
-                      "achieved_tflops": rf["achieved"]}
+                      "achieved_tflops": rf["achieved"], "lanes": leg["lanes"]}
+            if leg["tuned"]:
+                res[p]["autotuned"] = leg["tuned"]
         res["speedup_mixed_vs_fp64"] = round(res["mixed"]["value"] / res["fp64"]["value"], 3)
         res["flops_per_step"] = round(flops_all[key]["flops_per_step"], 1)
         out[name] = res

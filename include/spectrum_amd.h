@@ -109,6 +109,18 @@ int spx_begin(spx_sim* sim, uint64_t first_traj, uint64_t n_traj, const spx_rng*
 int spx_advance(spx_sim* sim, int n_launches, spx_run_stats* stats);
 int spx_finish(spx_sim* sim, spx_traj_record* records, spx_run_stats* stats);
 
+/* Persistent-engine autotuner (no counterpart in the reference, whose MPI workers have no launch
+   shape): times a short probe run of `n_probe` trajectories of this configuration for resident slots of
+   1, 2, 3, 4, 6 and 8 waves per SIMD and 250 / 1000 / 4000 steps per launch, keeps the fastest as the
+   simulation's "lanes" and "steps_per_launch" options, and resets the distributions.  `table` (may be
+   NULL) receives up to `max_entries` measurements. */
+typedef struct {
+   int64_t lanes;
+   int32_t steps_per_launch, pad_;
+   double steps_per_s;
+} spx_tune_entry;
+int spx_autotune(spx_sim* sim, uint64_t n_probe, uint64_t seed, spx_tune_entry* table, int max_entries, int* n_entries);
+
 /* DistributionBase::NBins / GetCountsAddress / GetDistroAddress (src/distribution_base.hh,
    src/distribution_templated.cc).  counts: n_bins entries; sums: n_bins * width doubles. */
 int spx_distro_info(const spx_sim* sim, int d, int32_t* n_bins3, int32_t* width);

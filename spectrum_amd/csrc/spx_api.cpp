@@ -589,6 +589,64 @@ int spx_run(spx_sim* s, uint64_t first, uint64_t n_traj, const spx_rng* rng, spx
    return spx_finish(s, records, st);
 }
 
+int spx_autotune(spx_sim* s, uint64_t n_probe, uint64_t seed, spx_tune_entry* table, int max_entries, int* n_entries)
+{
+   if (!s || n_probe == 0) return SPX_ERR_ARG;
+   if (s->running) return fail(s, SPX_ERR_STATE, "autotune during a run");
+   int rc = ensure_device(s);
+   if (rc) return rc;
+   hipDeviceProp_t prop;
+   HIPCHK(s, hipGetDeviceProperties(&prop, 0));
+   const uint64_t cus = (uint64_t)prop.multiProcessorCount;
+   const int waves_per_simd[] = {1, 2, 3, 4, 6, 8};
+   const int spls[] = {250, 1000, 4000};
+   const int64_t keep_lanes = s->lanes_opt;
+   const int keep_spl = s->steps_per_launch, keep_graph = s->graph_launches;
+   s->graph_launches = 0;
+   double best = -1.0;
+   int64_t best_lanes = 0;
+   int best_spl = keep_spl;
+   int n = 0;
+   spx_rng rng{};
+   rng.mode = 0;
+   rng.seed = seed;
+   for (int w : waves_per_simd) {
+      for (int spl : spls) {
+         s->lanes_opt = (int64_t)(cus * 4 * 64 * (uint64_t)w);
+         s->steps_per_launch = spl;
+         rc = spx_begin(s, 0, n_probe, &rng, 0);
+         if (rc) break;
+         spx_run_stats w0, w1;
+         rc = spx_advance(s, 1, &w0);
+         if (!rc) rc = spx_advance(s, 2, &w1);
+         // drop the probe run: the slots are reset and the histograms zeroed below
+         HIPCHK(s, s->pool.remove_queue(0));
+         s->pool.free_index(0);
+         s->running = false;
+         if (rc) break;
+         const double rate = (double)(w1.steps_executed - w0.steps_executed) / std::max(w1.wall_s, 1e-9);
+         if (table && n < max_entries) {
+            table[n].lanes = s->lanes_opt;
+            table[n].steps_per_launch = spl;
+            table[n].steps_per_s = rate;
+         }
+         n++;
+         if (rate > best) { best = rate; best_lanes = s->lanes_opt; best_spl = spl; }
+      }
+      if (rc) break;
+   }
+   if (n_entries) *n_entries = n;
+   s->graph_launches = keep_graph;
+   if (rc) {
+      s->lanes_opt = keep_lanes;
+      s->steps_per_launch = keep_spl;
+      return rc;
+   }
+   s->lanes_opt = best_lanes;
+   s->steps_per_launch = best_spl;
+   return spx_reset_distros(s);
+}
+
 int spx_distro_info(const spx_sim* s, int d, int32_t* n_bins3, int32_t* width)
 {
    if (!s || d < 0 || d >= s->plan.n_dist) return SPX_ERR_ARG;

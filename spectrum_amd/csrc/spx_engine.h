@@ -161,103 +161,134 @@ __device__ __forceinline__ double& ev_field(LaunchPtr L, int cls, int f, uint64_
 
 // ---------------------------------------------------------------------------------------------
 // Distribution events (src/distribution_base.cc:174-194, src/distribution_templated.cc:145-180)
-// Bins the events of every distribution for a crossing of boundary `bnd`: value and weight per
-// distribution, exact integer counts and FP64 sums, and the per-slot last-event record.
+// Called by every lane of a wave together; `has` marks the lanes with an event at boundary `bnd`.
+// Each lane forms the value and weight of every distribution and its bin; the wave then merges the
+// lanes that hit the same bin (a shuffle reduction of the weights) and issues one atomic per bin:
+// exact integer counts and FP64 sums, plus the per-slot last-event record.
 // ---------------------------------------------------------------------------------------------
-static __device__ int bin_events(PlanPtr P, QueryPtr Q, LaunchPtr L, uint64_t slot, int bnd,
-                          double t2, double x2, double y2, double z2, double m20, double m21, double m22, double amp, double logwgt)
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+   return v;
+}
+
+static __device__ int bin_events(PlanPtr P, QueryPtr Q, LaunchPtr L, uint64_t slot, bool has, int bnd,
+                                 double t2, double x2, double y2, double z2, double m20, double m21, double m22, double amp, double logwgt)
 {
    const bool parker = (P->traj_type != 2);
    const double t1 = L->t0[slot], x1 = L->x0[slot], y1 = L->y0[slot], z1 = L->z0[slot];
    const double m10 = L->m00[slot], m11 = L->m01[slot], m12 = L->m02[slot];
-   const SPX_CP spx_bnd_desc& B = P->bnd[bnd];
    int n_ev = 0;
    for (int d = 0; d < P->n_dist; d++) {
-      const int action = d < B.n_act ? B.actions[d] : -1;
-      if (action < 0) continue;
-      n_ev++;
       const SPX_CP spx_ds_desc& D = P->dist[d];
-      const bool first = D.val_time == 0;
-      const double px = first ? x1 : x2, py = first ? y1 : y2, pz = first ? z1 : z2;
-      const double ma = first ? m10 : m20, mb = first ? m11 : m21, mc = first ? m12 : m22;
-      double v0 = 0.0, v1 = 0.0, v2 = 0.0;
-      switch (D.kind) {
-      case SPX_DS_SPECTRUM: v0 = EnrKin<double>(parker ? m10 : norm(mk<double>(m10, m11, m12))); break;   // src/distribution_other.cc:561-570
-      case SPX_DS_TIME: v0 = first ? t1 : t2; break;                                                      // :143-148
-      case SPX_DS_MOMENTUM:                                                                                // :262-290
-         if (D.val_coord == 0 || parker) { v0 = ma; v1 = mb; v2 = mc; }
-         else { v0 = norm(mk<double>(ma, mb, mc)); v1 = mc / v0; v2 = 0.0; }
-         break;
-      case SPX_DS_POSMOM:                                                                                  // :342-353
-         v0 = D.pos_idx == 3 ? norm(mk<double>(px, py, pz)) : (D.pos_idx == 0 ? px : (D.pos_idx == 1 ? py : pz));
-         v1 = D.mom_idx == 0 ? ma : (D.mom_idx == 1 ? mb : mc);
-         break;
-      case SPX_DS_POSPITCH:                                                                                // (|x|, p_par/|p|)
-         v0 = norm(mk<double>(px, py, pz));
-         v1 = mc / norm(mk<double>(ma, mb, mc));
-         break;
-      default: v0 = t2; break;                                                                             // :730-733, :777-780
+      int action = -1;
+      if (has) {
+         const SPX_CP spx_bnd_desc& B = P->bnd[bnd];
+         action = d < B.n_act ? B.actions[d] : -1;
       }
-      // EvaluateWeight through the action table
+      const bool ev = action >= 0;
+      n_ev += ev ? 1 : 0;
+      bool keep = ev;
       double w[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-      bool keep = true;
-      switch (D.kind) {
-      case SPX_DS_SPECTRUM:                                                                                // :575-594
-         if (action == 0) {
-            const double m2 = parker ? m20 : norm(mk<double>(m20, m21, m22));
-            const double T = EnrKin<double>(m2);
-            if (P->spectrum_type == 0) w[0] = D.J0 * VelP<double>(m2) * pow(T / D.T0, D.pow_law) / sqr(m2);
-            else if (P->spectrum_type == 1) w[0] = D.J0 * pow(T / D.T0, D.pow_law) / sqr(m2);
-            else w[0] = D.J0 * pow(T / D.T0, D.pow_law);
+      int lin = 0;
+      if (ev) {
+         const bool first = D.val_time == 0;
+         const double px = first ? x1 : x2, py = first ? y1 : y2, pz = first ? z1 : z2;
+         const double ma = first ? m10 : m20, mb = first ? m11 : m21, mc = first ? m12 : m22;
+         double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+         switch (D.kind) {
+         case SPX_DS_SPECTRUM: v0 = EnrKin<double>(parker ? m10 : norm(mk<double>(m10, m11, m12))); break;   // src/distribution_other.cc:561-570
+         case SPX_DS_TIME: v0 = first ? t1 : t2; break;                                                      // :143-148
+         case SPX_DS_MOMENTUM:                                                                                // :262-290
+            if (D.val_coord == 0 || parker) { v0 = ma; v1 = mb; v2 = mc; }
+            else { v0 = norm(mk<double>(ma, mb, mc)); v1 = mc / v0; v2 = 0.0; }
+            break;
+         case SPX_DS_POSMOM:                                                                                  // :342-353
+            v0 = D.pos_idx == 3 ? norm(mk<double>(px, py, pz)) : (D.pos_idx == 0 ? px : (D.pos_idx == 1 ? py : pz));
+            v1 = D.mom_idx == 0 ? ma : (D.mom_idx == 1 ? mb : mc);
+            break;
+         case SPX_DS_POSPITCH:                                                                                // (|x|, p_par/|p|)
+            v0 = norm(mk<double>(px, py, pz));
+            v1 = mc / norm(mk<double>(ma, mb, mc));
+            break;
+         default: v0 = t2; break;                                                                             // :730-733, :777-780
          }
-         else if (action == 1) w[0] = D.val_cold;
-         else keep = false;
-         break;
-      case SPX_DS_CUMO1:                                                                                   // :738-741
-         if (action != 0) { keep = false; break; }
-         w[0] = x2 - x1; w[1] = y2 - y1; w[2] = z2 - z1;
-         break;
-      case SPX_DS_CUMO2: {                                                                                 // :785-788
-         if (action != 0) { keep = false; break; }
-         const double dx = x2 - x1, dy = y2 - y1, dz = z2 - z1;
-         w[0] = dx * dx; w[1] = dx * dy; w[2] = dx * dz; w[4] = dy * dy; w[5] = dy * dz; w[8] = dz * dz;
-         w[3] = w[1]; w[6] = w[2]; w[7] = w[5];
-         break;
-      }
-      default:                                                                                             // :79-93
-         if (action == 0) w[0] = D.val_hot;
-         else if (action == 1) w[0] = D.val_cold;
-         else keep = false;
-      }
-      // AddEvent: bin index per active dimension
-      int bin[3] = {0, 0, 0};
-#pragma unroll
-      for (int ijk = 0; ijk < 3; ijk++) {
-         if (!keep || !(D.dims & (1 << ijk))) continue;
-         const double val = ijk == 0 ? v0 : (ijk == 1 ? v1 : v2);
-         const double v = (D.log_bins[ijk] ? log10(val) : val) - D.limits0[ijk];
-         if (v < 0.0 && !D.bin_outside[ijk]) { keep = false; continue; }
-         int b = (int)(v / D.bin_size[ijk]);
-         if (b < 0) b = 0;   // the reference can index below zero here (src/distribution_templated.cc:162-167)
-         if (b >= D.n_bins[ijk]) {
-            if (D.bin_outside[ijk]) b = D.n_bins[ijk] - 1;
+         // EvaluateWeight through the action table (w and keep = true are declared above)
+         switch (D.kind) {
+         case SPX_DS_SPECTRUM:                                                                                // :575-594
+            if (action == 0) {
+               const double m2 = parker ? m20 : norm(mk<double>(m20, m21, m22));
+               const double T = EnrKin<double>(m2);
+               if (P->spectrum_type == 0) w[0] = D.J0 * VelP<double>(m2) * pow(T / D.T0, D.pow_law) / sqr(m2);
+               else if (P->spectrum_type == 1) w[0] = D.J0 * pow(T / D.T0, D.pow_law) / sqr(m2);
+               else w[0] = D.J0 * pow(T / D.T0, D.pow_law);
+            }
+            else if (action == 1) w[0] = D.val_cold;
+            else keep = false;
+            break;
+         case SPX_DS_CUMO1:                                                                                   // :738-741
+            if (action != 0) { keep = false; break; }
+            w[0] = x2 - x1; w[1] = y2 - y1; w[2] = z2 - z1;
+            break;
+         case SPX_DS_CUMO2: {                                                                                 // :785-788
+            if (action != 0) { keep = false; break; }
+            const double dx = x2 - x1, dy = y2 - y1, dz = z2 - z1;
+            w[0] = dx * dx; w[1] = dx * dy; w[2] = dx * dz; w[4] = dy * dy; w[5] = dy * dz; w[8] = dz * dz;
+            w[3] = w[1]; w[6] = w[2]; w[7] = w[5];
+            break;
+         }
+         default:                                                                                             // :79-93
+            if (action == 0) w[0] = D.val_hot;
+            else if (action == 1) w[0] = D.val_cold;
             else keep = false;
          }
-         bin[ijk] = b;
+         // AddEvent: bin index per active dimension
+         int bin[3] = {0, 0, 0};
+#pragma unroll
+         for (int ijk = 0; ijk < 3; ijk++) {
+            if (!keep || !(D.dims & (1 << ijk))) continue;
+            const double val = ijk == 0 ? v0 : (ijk == 1 ? v1 : v2);
+            const double v = (D.log_bins[ijk] ? log10(val) : val) - D.limits0[ijk];
+            if (v < 0.0 && !D.bin_outside[ijk]) { keep = false; continue; }
+            int b = (int)(v / D.bin_size[ijk]);
+            if (b < 0) b = 0;   // the reference can index below zero here (src/distribution_templated.cc:162-167)
+            if (b >= D.n_bins[ijk]) {
+               if (D.bin_outside[ijk]) b = D.n_bins[ijk] - 1;
+               else keep = false;
+            }
+            bin[ijk] = b;
+         }
+         if (keep) {
+            lin = D.n_bins[2] * (D.n_bins[1] * bin[0] + bin[1]) + bin[2];
+            const double ew = exp(logwgt);
+#pragma unroll
+            for (int k = 0; k < 9; k++) w[k] = w[k] * ew + amp;
+            L->ev_bin[(uint64_t)d * L->n_lanes + slot] = lin;
+            L->ev_w[(uint64_t)d * L->n_lanes + slot] = w[0];
+         }
+         else L->ev_bin[(uint64_t)d * L->n_lanes + slot] = -1;
       }
-      if (!keep) { L->ev_bin[(uint64_t)d * L->n_lanes + slot] = -1; continue; }
-      const int lin = D.n_bins[2] * (D.n_bins[1] * bin[0] + bin[1]) + bin[2];
-      const double ew = exp(logwgt);
+      // one atomic per distinct bin of the wave
+      uint64_t todo = __ballot(keep);
+      if (todo == 0) continue;
+      if (lane_id() == __ffsll((unsigned long long)todo) - 1) atomicAdd(&Q->hist_events[d], (unsigned long long)__popcll(todo));
+      while (todo) {
+         const int leader = __ffsll((unsigned long long)todo) - 1;
+         const int key = __shfl(lin, leader);
+         const bool mine = keep && lin == key;
+         const uint64_t grp = __ballot(mine);
+         const int64_t off = Q->hist_off[d] + key;
 #pragma unroll
-      for (int k = 0; k < 9; k++) w[k] = w[k] * ew + amp;
-      const int64_t off = Q->hist_off[d] + lin;
-      atomicAdd(&Q->hist_counts[off], 1ull);
-#pragma unroll
-      for (int k = 0; k < 9; k++)
-         if (k < D.width) atomicAdd(&Q->hist_sums[off * 9 + k], w[k]);
-      atomicAdd(&Q->hist_events[d], 1ull);
-      L->ev_bin[(uint64_t)d * L->n_lanes + slot] = lin;
-      L->ev_w[(uint64_t)d * L->n_lanes + slot] = w[0];
+         for (int k = 0; k < 9; k++) {
+            if (k < D.width) {
+               const double v = wave_sum(mine ? w[k] : 0.0);
+               if (lane_id() == leader) atomicAdd(&Q->hist_sums[off * 9 + k], v);
+            }
+         }
+         if (lane_id() == leader) atomicAdd(&Q->hist_counts[off], (unsigned long long)__popcll(grp));
+         todo &= ~grp;
+      }
    }
    return n_ev;
 }
@@ -424,17 +455,22 @@ __device__ __forceinline__ void service_slot(LaunchPtr L, StartFn&& set_start)
    if (q >= 0) {
       const QueryPtr Q = query_of(L, q);
       const PlanPtr P = plan_of(Q);
-      // 1. pending distribution events of the last step kernel
-      if (st == SLOT_EVENT || st == SLOT_DONE) {
-         const int pend = L->ev_pending[slot];
-         int n_ev = L->n_events[slot];
-         for (int cls = 2; cls >= 0; cls--) {
-            if (!(pend & (1 << cls))) continue;
-            const int b = L->ev_bnd[cls * L->n_lanes + slot];
-            n_ev += bin_events(P, Q, L, slot, b, ev_field(L, cls, EV_T, slot), ev_field(L, cls, EV_X, slot), ev_field(L, cls, EV_Y, slot),
-                               ev_field(L, cls, EV_Z, slot), ev_field(L, cls, EV_M0, slot), ev_field(L, cls, EV_M1, slot),
-                               ev_field(L, cls, EV_M2, slot), ev_field(L, cls, EV_AMP, slot), ev_field(L, cls, EV_WGT, slot));
-         }
+      // 1. pending distribution events of the last step kernel (binned by the whole wave together)
+      const bool evl = (st == SLOT_EVENT || st == SLOT_DONE);
+      const int pend = evl ? L->ev_pending[slot] : 0;
+      int n_ev = evl ? L->n_events[slot] : 0;
+      for (int cls = 2; cls >= 0; cls--) {
+         const bool has = (pend >> cls) & 1;
+         if (__ballot(has) == 0) continue;
+         const int b = has ? L->ev_bnd[cls * L->n_lanes + slot] : 0;
+         const double f[EV_N] = {has ? ev_field(L, cls, EV_T, slot) : 0.0, has ? ev_field(L, cls, EV_X, slot) : 0.0,
+                                 has ? ev_field(L, cls, EV_Y, slot) : 0.0, has ? ev_field(L, cls, EV_Z, slot) : 0.0,
+                                 has ? ev_field(L, cls, EV_M0, slot) : 0.0, has ? ev_field(L, cls, EV_M1, slot) : 0.0,
+                                 has ? ev_field(L, cls, EV_M2, slot) : 0.0, has ? ev_field(L, cls, EV_AMP, slot) : 0.0,
+                                 has ? ev_field(L, cls, EV_WGT, slot) : 0.0};
+         n_ev += bin_events(P, Q, L, slot, has, b, f[EV_T], f[EV_X], f[EV_Y], f[EV_Z], f[EV_M0], f[EV_M1], f[EV_M2], f[EV_AMP], f[EV_WGT]);
+      }
+      if (evl) {
          L->ev_pending[slot] = 0;
          L->n_events[slot] = n_ev;
          if (st == SLOT_DONE) {

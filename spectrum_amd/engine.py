@@ -41,6 +41,10 @@ class SpxRunStats(C.Structure):
                 ("step_kernel_ms", C.c_double), ("service_kernel_ms", C.c_double)]
 
 
+class SpxTuneEntry(C.Structure):
+    _fields_ = [("lanes", C.c_int64), ("steps_per_launch", C.c_int32), ("pad_", C.c_int32), ("steps_per_s", C.c_double)]
+
+
 RECORD_DTYPE = np.dtype([("t", "f8"), ("pos", "f8", 3), ("mom", "f8", 3), ("amp", "f8"), ("wgt", "f8"),
                          ("t0", "f8"), ("pos0", "f8", 3), ("mom0", "f8", 3), ("steps", "i8"), ("status", "i4"),
                          ("attempts", "i4"), ("n_events", "i4"), ("ev_bin", "i4", 4), ("ev_weight", "f8", 4)], align=True)
@@ -70,6 +74,7 @@ def lib() -> C.CDLL:
         L.spx_last_error.argtypes = [C.c_void_p]
         L.spx_last_error.restype = C.c_char_p
         L.spx_device_info.argtypes = [C.c_char_p, C.c_size_t]
+        L.spx_autotune.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(SpxTuneEntry), C.c_int, C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -200,6 +205,13 @@ class Simulation:
         rec = np.zeros(self._n, dtype=RECORD_DTYPE) if records else None
         self._check(lib().spx_finish(self.h, rec.ctypes.data if records else None, C.byref(st)), "finish")
         return RunStats.of(st), rec
+
+    def autotune(self, n_probe: int, seed: int = 1):
+        """Pick resident slots and steps per launch by timing short probe runs; returns the table."""
+        table = (SpxTuneEntry * 64)()
+        n = C.c_int()
+        self._check(lib().spx_autotune(self.h, n_probe, seed, table, 64, C.byref(n)), "autotune")
+        return [(table[i].lanes, table[i].steps_per_launch, table[i].steps_per_s) for i in range(min(n.value, 64))]
 
     # --- distributions ----------------------------------------------------------------------
     def distro(self, d: int):

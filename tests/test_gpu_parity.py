@@ -177,3 +177,19 @@ def test_gc_uniform_ring_fp64_counter(device, oracle_mod):
     o = oracle_mod.Oracle(s)
     _, orec = o.run(n, seed=17, threads=THREADS)
     check_parity(st, rec, orec, sim, o, n)
+
+
+def test_autotune_picks_a_shape_and_keeps_results(device):
+    """The autotuner times 18 launch shapes on a probe run, keeps the fastest, and leaves the results
+    of later runs unchanged (counter streams are keyed by trajectory id, not by slot)."""
+    s = cf.parker_modulation(n_energies=8, t_limit_years=2.0 / 365.0, time_action=0)
+    ref = E.Simulation.from_setup(s, precision=E.PREC_MIXED)
+    _, rec_ref = ref.run(2000, seed=9)
+    sim = E.Simulation.from_setup(s, precision=E.PREC_MIXED)
+    table = sim.autotune(n_probe=2_000_000, seed=1)
+    print("[autotune]", [(l, k, f"{v:.3e}") for l, k, v in table], flush=True)
+    assert len(table) == 18 and all(v > 0 for _, _, v in table)
+    c, sm, ne = sim.distro(0)
+    assert ne == 0 and c.sum() == 0            # the probe's events were discarded
+    _, rec = sim.run(2000, seed=9)
+    np.testing.assert_array_equal(rec, rec_ref)
