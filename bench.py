@@ -193,6 +193,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-other-workloads", action="store_true", help="skip the DSA and guiding-centre throughput legs")
+    ap.add_argument("--autotune", action="store_true",
+                    help="autotune the modulation legs' launch shape first (the kernel trace then also holds the probe launches)")
     ap.add_argument("--serve-queries", type=int, default=40, help="served-stream leg: number of queries (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="mean query arrival rate (1/s)")
     ap.add_argument("--serve-scale", type=float, default=0.01,
@@ -224,7 +226,7 @@ def main():
     precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
     legs = {}
     for p in precs:
-        legs[p] = gpu_leg(E, cf, p, args)
+        legs[p] = gpu_leg(E, cf, p, args, autotune=args.autotune and not args.lanes)
         legs[p]["launches"] = args.steps
     main_prec = "mixed" if "mixed" in legs else "fp64"
     m = legs[main_prec]
@@ -244,7 +246,8 @@ def main():
         "data": "synthetic (counter-based Philox increments, seed 1)",
         "config": {"workload": WORKLOAD, "n_traj": N_ENERGIES * PER_ENERGY, "energies_GeV": [0.1, 10.0], "transport": "Parker backward",
                    "background": "SolarWind (Parker spiral)", "diffusion": "RigidityMagneticFieldPowerLaw",
-                   "lanes": m["lanes"], "steps_per_launch": args.steps_per_launch, "precision": main_prec},
+                   "lanes": m["lanes"], "steps_per_launch": (m["tuned"] or {}).get("steps_per_launch", args.steps_per_launch),
+                   "autotuned": bool(m["tuned"]), "precision": main_prec},
         "roofline": roofline(m, main_prec, flops, traffic),
     }
     if "fp64" in legs:
