@@ -1,0 +1,152 @@
+"""Small setups that together exercise every plugin class the engine lowers.
+
+Each builder returns a configs.Setup whose trajectories are short (a few thousand steps), so the
+oracle finishes a few hundred of them in seconds.  The GPU parity test runs every setup through the
+FP64 kernel and the oracle with the same counter streams.
+"""
+import numpy as np
+
+from spectrum_amd import configs as cf
+from spectrum_amd.data_container import DataContainer as DC
+
+
+def dist(n_bins, lo, hi, log=(0, 0, 0), outside=(0, 0, 0)):
+    return cf.distribution_base(DC(), n_bins, lo, hi, log, outside)
+
+
+def shock_flowmom_reflect():
+    """Forward Parker at a discontinuous shock; kappa ~ |U|^a p^b (position dependent through U);
+    log-uniform initial momenta; a reflecting plane upstream; time pass + expiry; (x, p) map."""
+    s = cf.Setup("shock_flowmom_reflect", cf.TRAJ_PARKER, cf.FORWARD)
+    u1, B = 0.013343, 16.0
+    dc = cf.background_base(DC(), 0.0, (0.0, 0.0, 0.0), (u1, 0.0, 0.0), (B, 0.0, 0.0), 1.0)
+    dc.insert_vec((-1.0, 0.0, 0.0)).insert_double(0.0).insert_double(3.0)
+    s.add(cf.BACKGROUND, "BackgroundShock", dc)
+    s.add(cf.DIFFUSION, "DiffusionFlowMomentumPowerLaw",
+          DC().insert_double(3.0).insert_double(u1).insert_double(0.5).insert_double(0.05).insert_double(0.3).insert_double(0.1))
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((-3.0, 1.0, -2.0)))
+    s.add(cf.INITIAL, "InitialMomentumThickShell", DC().insert_double(0.03).insert_double(0.3).insert_bool(True))
+    s.add(cf.BOUNDARY, "BoundaryPlaneReflect", cf.boundary(-1, [-1], ("vec", (-30.0, 0.0, 0.0)), ("vec", (1.0, 0.0, 0.0))))
+    s.add(cf.BOUNDARY, "BoundaryTimePass", cf.boundary(-1, [0], ("double", 150.0)))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", 400.0)))
+    d = dist((12, 0, 10), (-36.0, 0.0, 0.02), (12.0, 0.0, 1.0), (0, 0, 1), (1, 0, 1))
+    d.insert_double(1.0).insert_double(0.0).insert_int(1).insert_int(0).insert_int(0)
+    s.add(cf.DISTRIBUTION, "DistributionPositionMomentumUniform", d)
+    return s
+
+
+def solarwind_kinrad_reflect_recurrent():
+    """Backward Parker in the Parker spiral; kappa ~ T^a r^b (position dependent); a reflecting inner
+    sphere; a recurrent time boundary binning every 12 h; exit-time and <dx> distributions."""
+    s = cf.Setup("solarwind_kinrad_reflect_recurrent", cf.TRAJ_PARKER, cf.BACKWARD)
+    s.add(cf.BACKGROUND, "BackgroundSolarWind", cf.solarwind())
+    kd = DC().insert_double(1.5e22 / cf.unit_diffusion).insert_double(cf.GeV).insert_double(cf.one_au)
+    kd.insert_double(1.0).insert_double(1.0).insert_double(0.05).insert_int(0).insert_double(1.0).insert_double(0.0).insert_double(1.0)
+    s.add(cf.DIFFUSION, "DiffusionKineticEnergyRadialDistancePowerLaw", kd)
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((1.2 * cf.one_au, 0.3 * cf.one_au, -0.1 * cf.one_au)))
+    s.add(cf.INITIAL, "InitialMomentumShell", DC().insert_double(cf.mom_from_ekin(0.3 * cf.GeV)))
+    s.add(cf.BOUNDARY, "BoundarySphereReflect", cf.boundary(-1, [-1, -1], ("vec", (0.0, 0.0, 0.0)), ("double", 1.1 * cf.one_au)))
+    s.add(cf.BOUNDARY, "BoundaryTimeRecurrent", cf.boundary(-1, [0, 0], ("double", -0.5 * cf.one_day)))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0, 0], ("double", -2.25 * cf.one_day)))
+    d = dist((10, 0, 0), (-2.5 * cf.one_day, 0, 0), (0.0, 0, 0))
+    d.insert_double(1.0).insert_double(0.0).insert_int(1)
+    s.add(cf.DISTRIBUTION, "DistributionTimeUniform", d)
+    s.add(cf.DISTRIBUTION, "DistributionPositionCumulativeOrder1", dist((8, 0, 0), (-2.5 * cf.one_day, 0, 0), (0.0, 0, 0)))
+    return s
+
+
+def smoothshock_perp_momentum():
+    """Forward ParkerSource at a smooth shock with B tangential; perpendicular-only diffusion;
+    momentum pass and injection boundaries; <dx dx> and (p) distributions."""
+    s = cf.Setup("smoothshock_perp_momentum", cf.TRAJ_PARKER_SOURCE, cf.FORWARD)
+    u1, w = 0.013343, 2.5
+    dc = cf.background_base(DC(), 0.0, (0.0, 0.0, 0.0), (u1, 0.0, 0.0), (0.0, 16.0, 0.0), 10.0 * w)
+    dc.insert_vec((-1.0, 0.0, 0.0)).insert_double(0.0).insert_double(4.0).insert_double(w).insert_double(0.1)
+    s.add(cf.BACKGROUND, "BackgroundSmoothShock", dc)
+    s.add(cf.DIFFUSION, "DiffusionPerpConstant", DC().insert_double(3.3356))
+    p0 = cf.mom_from_ekin(10.0 * cf.keV)
+    s.add(cf.SOURCE, "SourceMomentumInjection", DC().insert_double(1.3 * p0).insert_double(2.0))
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((0.0, 0.0, 0.0)))
+    s.add(cf.INITIAL, "InitialMomentumShell", DC().insert_double(p0))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0, 0], ("double", 20000.0)))
+    s.add(cf.BOUNDARY, "BoundaryMomentumPass", cf.boundary(-1, [0, -1], ("double", 1.2 * p0)))
+    s.add(cf.BOUNDARY, "BoundaryMomentumInject", cf.boundary(1, [0, 0], ("double", 1.6 * p0)))
+    s.add(cf.DISTRIBUTION, "DistributionPositionCumulativeOrder2", dist((6, 0, 0), (0.0, 0, 0), (21000.0, 0, 0)))
+    d = dist((16, 1, 1), (p0, -1.0, -1.0), (2.0 * p0, 1.0, 1.0), (1, 0, 0), (1, 0, 0))
+    d.insert_double(1.0).insert_double(0.0).insert_int(1).insert_int(0)
+    s.add(cf.DISTRIBUTION, "DistributionMomentumUniform", d)
+    return s
+
+
+def uniform_para_plane_pass():
+    """Forward Parker in a uniform field with parallel-only diffusion and a pass-through plane."""
+    s = cf.Setup("uniform_para_plane_pass", cf.TRAJ_PARKER, cf.FORWARD)
+    s.add(cf.BACKGROUND, "BackgroundUniform", cf.background_base(DC(), 0.0, (0.0, 0.0, 0.0), (0.002, 0.001, 0.0), (3.0, 4.0, 12.0), 5.0))
+    s.add(cf.DIFFUSION, "DiffusionParaConstant", DC().insert_double(2.5))
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((0.0, 0.0, 0.0)))
+    s.add(cf.INITIAL, "InitialMomentumFixed", DC().insert_vec((cf.mom_from_ekin(cf.MeV), 0.0, 0.0)))
+    s.add(cf.BOUNDARY, "BoundaryPlanePass", cf.boundary(-1, [0], ("vec", (0.0, 0.0, 20.0)), ("vec", (0.0, 0.0, 1.0))))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", 600.0)))
+    d = dist((12, 0, 0), (0.0, 0, 0), (660.0, 0, 0))
+    d.insert_double(1.0).insert_double(0.5).insert_int(1)
+    s.add(cf.DISTRIBUTION, "DistributionTimeUniform", d)
+    return s
+
+
+def modulation_spectrum_type(t):
+    s = cf.parker_modulation(n_energies=8, t_limit_years=2.0 / 365.0, time_action=0, spectrum_type=t, name=f"modulation_type{t}")
+    return s
+
+
+def gc_rigidity_reflect():
+    """Guiding centre in the Parker spiral with a |B|-dependent rigidity law (perpendicular drift from
+    grad kappa), no pitch-angle scattering model term, a reflecting sphere and a (|p|, mu) map."""
+    s = cf.Setup("gc_rigidity_reflect", cf.TRAJ_GUIDING_DIFF_SCATT, cf.FORWARD)
+    s.add(cf.BACKGROUND, "BackgroundSolarWind", cf.solarwind())
+    lam0 = 3.0 * (1.5e22 / cf.unit_diffusion) / cf.c_code
+    dif = DC().insert_double(lam0).insert_double(cf.GV).insert_double(5.0e-5 / cf.unit_magnetic)
+    dif.insert_double(0.5).insert_double(-1.0).insert_double(0.02)
+    s.add(cf.DIFFUSION, "DiffusionRigidityMagneticFieldPowerLaw", dif)
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((cf.one_au, 0.0, 0.0)))
+    s.add(cf.INITIAL, "InitialMomentumShell", DC().insert_double(cf.mom_from_ekin(50.0 * cf.MeV)))
+    s.add(cf.BOUNDARY, "BoundarySphereReflect", cf.boundary(-1, [-1], ("vec", (0.0, 0.0, 0.0)), ("double", 0.9995 * cf.one_au)))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", 0.2 * cf.one_day)))
+    d = dist((8, 8, 1), (4.0, -1.0, -1.0), (4.3, 1.0, 1.0), (0, 0, 0), (1, 1, 1))
+    d.insert_double(1.0).insert_double(0.0).insert_int(1).insert_int(1)
+    s.add(cf.DISTRIBUTION, "DistributionMomentumUniform", d)
+    return s
+
+
+def gc_isotropic_only():
+    """Guiding centre with pitch-angle scattering only (no perpendicular diffusion)."""
+    s = cf.Setup("gc_isotropic_only", cf.TRAJ_GUIDING_DIFF_SCATT, cf.FORWARD)
+    s.add(cf.BACKGROUND, "BackgroundSolarWind", cf.solarwind())
+    s.add(cf.DIFFUSION, "DiffusionIsotropicConstant", DC().insert_double(5.0e-3))
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((2.0 * cf.one_au, 0.5 * cf.one_au, 0.0)))
+    s.add(cf.INITIAL, "InitialMomentumRing", DC().insert_double(cf.mom_from_ekin(20.0 * cf.MeV)).insert_double(0.4))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", 0.1 * cf.one_day)))
+    d = dist((16, 16, 0), (1.5 * cf.one_au, -1.0, 0.0), (3.0 * cf.one_au, 1.0, 1.0), (0, 0, 0), (1, 0, 0))
+    d.insert_double(1.0).insert_double(0.0).insert_int(1)
+    s.add(cf.DISTRIBUTION, "DistributionPositionPitchUniform", d)
+    return s
+
+
+ALL = [shock_flowmom_reflect, solarwind_kinrad_reflect_recurrent, smoothshock_perp_momentum, uniform_para_plane_pass,
+       lambda: modulation_spectrum_type(0), lambda: modulation_spectrum_type(2), gc_rigidity_reflect, gc_isotropic_only]
+NAMES = ["shock_flowmom_reflect", "solarwind_kinrad_reflect_recurrent", "smoothshock_perp_momentum", "uniform_para_plane_pass",
+         "modulation_type0", "modulation_type2", "gc_rigidity_reflect", "gc_isotropic_only"]
+
+
+def weight_atol(name, rec, rtol):
+    """Absolute bound on the difference of recorded event weights between two runs whose endpoints
+    agree to `rtol` relative: the <dx dx> weight x_i^2 moves by at most 2 |x| |dx| <= 2 rtol |x|^2."""
+    atol = np.zeros((rec.shape[0], 4))
+    if name == "smoothshock_perp_momentum":
+        atol[:, 0] = 2.0 * rtol * np.sum((rec["pos"] - rec["pos0"]) ** 2, axis=1) * (1.0 + 1e-6)
+    return atol

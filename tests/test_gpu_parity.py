@@ -37,7 +37,9 @@ def endpoint_rel(rec, orec):
     return np.maximum(np.maximum(dp, dm), dt)
 
 
-def check_parity(sim_stats, rec, orec, sim, o, n):
+def check_parity(sim_stats, rec, orec, sim, o, n, ev_weight_atol=0.0):
+    """`ev_weight_atol` bounds event weights that are ill-conditioned functions of the endpoint
+    (a squared position component near zero): an (n, 4) array or a scalar, added to the relative test."""
     print(f"[parity] n={n} steps={int(rec['steps'].sum())} discards={sim_stats.discards}", flush=True)
     assert sim_stats.done == n and sim_stats.failed == 0
     np.testing.assert_array_equal(rec["status"], orec["status"])
@@ -48,7 +50,8 @@ def check_parity(sim_stats, rec, orec, sim, o, n):
     r = endpoint_rel(rec, orec)
     print(f"[parity] endpoint rel max {r.max():.3e} median {np.median(r):.3e}", flush=True)
     assert r.max() <= ENDPOINT_RTOL
-    np.testing.assert_allclose(rec["ev_weight"], orec["ev_weight"], rtol=ENDPOINT_RTOL, atol=0)
+    dw = np.abs(rec["ev_weight"] - orec["ev_weight"])
+    assert (dw <= ENDPOINT_RTOL * np.abs(orec["ev_weight"]) + ev_weight_atol).all(), dw.max()
     assert sim_stats.steps_done == int(orec["steps"].sum())
     for d in range(sim.n_dist):
         c, s, ne = sim.distro(d)
