@@ -93,6 +93,19 @@ void spx_destroy(spx_sim* sim);
    `container` holds the DataContainer bytes of common/data_container.cc:133-240. */
 int spx_add_plugin(spx_sim* sim, int role, const char* class_name, const uint8_t* container, size_t len);
 
+/* Gridded background, BackgroundServerCartesian with first-order interpolation and no ghost cells
+   (src/background_server_cartesian.cc, src/server_cartesian.cc; benchmarks/run_tests.sh:383).
+   spx_add_background_file is SimulationWorker::AddBackground(proto, container, fname_pattern)
+   (src/simulation.hh:111, src/simulation.cc:387-396): it adds the background and, for the gridded class,
+   reads `fname_pattern`.info (blocks per axis, domain corners in au, variable count) and
+   `fname_pattern`.out (binary doubles, block by block) as ReadCartesianHeader / ReadCartesianData do
+   (src/reader_cartesian.cc:25-140).  spx_set_background_grid takes the same data from memory: blocks of
+   4x4x4 zones, `n_var` (= 6: flow, magnetic field, fluid units) doubles per zone, the .out order.  The
+   grid is kept in device memory once a run needs it; a position outside it discards the attempt. */
+int spx_add_background_file(spx_sim* sim, const char* class_name, const uint8_t* container, size_t len, const char* fname_pattern);
+int spx_set_background_grid(spx_sim* sim, const int32_t n_blocks[3], const double dom_min[3], const double dom_max[3], int32_t n_var,
+                            const double* data);
+
 /* Run-time replacement of compile-time switches and engine knobs:
    "DISTRO_KINETIC_ENERGY_POWER_LAW_TYPE" (src/distribution_other.hh:325), "max_trajectory_steps"
    (src/trajectory_base.hh:38), "max_attempts", "precision" (SPX_PREC_*), "block", "lanes",

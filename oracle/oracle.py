@@ -60,6 +60,8 @@ def lib(path: str = LIB_PATH):
         L.orc_background_fields.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_double), C.c_double,
                                             C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_kappa.restype = C.c_double
+        L.orc_set_grid.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32, C.c_void_p]
+        L.orc_grid_fields.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_kappa.argtypes = [C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double]
         L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.orc_counter_normal.restype = C.c_double
@@ -90,6 +92,14 @@ class Oracle:
         for k, v in setup.options.items():
             if L.orc_set_option(self.h, k.encode(), float(v)) != 0:
                 raise ValueError(f"unknown option {k}")
+        grid = getattr(setup, "grid", None)
+        if grid is not None:
+            self._grid = np.ascontiguousarray(grid.data, dtype=np.float64)
+            nb = (C.c_int32 * 3)(*grid.nb)
+            lo = (C.c_double * 3)(*grid.dom_min)
+            hi = (C.c_double * 3)(*grid.dom_max)
+            if L.orc_set_grid(self.h, nb, lo, hi, 6, self._grid.ctypes.data) != 0:
+                raise ValueError("oracle rejected the grid")
 
     def __del__(self):
         try:
@@ -146,6 +156,19 @@ class Oracle:
         dmax, divU = C.c_double(), C.c_double()
         self.L.orc_background_fields(self.h, t, P, p, U, B, C.byref(dmax), C.byref(divU))
         return np.array(U), np.array(B), dmax.value, divU.value
+
+    def grid_fields(self, pos):
+        """Gridded fields at one point: dict of U, B, Bmag, gradU, gradB, gradBmag, dmax; None where the
+        reference fails (outside the grid)."""
+        out = (C.c_double * 29)()
+        rc = self.L.orc_grid_fields(self.h, (C.c_double * 3)(*pos), out)
+        if rc < 0:
+            raise ValueError("no gridded background")
+        if rc:
+            return None
+        a = np.frombuffer(out, dtype=np.float64).copy()
+        return {"U": a[0:3], "B": a[3:6], "Bmag": a[6], "gradU": a[7:16].reshape(3, 3), "gradB": a[16:25].reshape(3, 3),
+                "gradBmag": a[25:28], "dmax": a[28]}
 
     def kappa(self, comp, pos, p, mu=0.0, Bmag=1.0, t=0.0):
         P = (C.c_double * 3)(*pos)

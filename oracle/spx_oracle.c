@@ -263,11 +263,14 @@ double orc_counter_normal(uint64_t seed, uint64_t traj, uint64_t draw)
  * ---------------------------------------------------------------------------------------------- */
 typedef struct { const uint8_t* p; const uint8_t* end; int err; } dcread;
 
+/* DataContainer::Read (common/data_container.cc:214-222) copies the record's own length; a shorter
+   record fills the low bytes of a zeroed value, a longer one is rejected. */
 static void rd_raw(dcread* r, void* dst, size_t want)
 {
    if (r->err || r->p >= r->end) { r->err = 1; return; }
    size_t len = *r->p;
-   if (r->p + 1 + len > r->end || len != want) { r->err = 1; return; }
+   if (r->p + 1 + len > r->end || len > want || len == 0) { r->err = 1; return; }
+   memset(dst, 0, want);
    memcpy(dst, r->p + 1, len);
    r->p += len + 1;
 }
@@ -296,7 +299,18 @@ static void rd_skip(dcread* r)
 /* ------------------------------------------------------------------------------------------------
  * Plugin parameter blocks
  * ---------------------------------------------------------------------------------------------- */
-enum { BG_NONE = 0, BG_UNIFORM, BG_SOLARWIND, BG_SHOCK, BG_SMOOTHSHOCK };
+enum { BG_NONE = 0, BG_UNIFORM, BG_SOLARWIND, BG_SHOCK, BG_SMOOTHSHOCK, BG_GRID };
+
+/* BackgroundServerCartesian data, stored as the reference's reader keeps it
+   (src/reader_cartesian.cc:60-103): blocks of 4x4x4 zones, block index x fastest, each block's
+   zones x fastest with 6 doubles (flow, magnetic field) per zone, fluid units. */
+#define GRID_BS 4
+#define GRID_NVAR 6
+typedef struct {
+   int nb[3];
+   double dom_min[3], dom_max[3], block_length[3];
+   double* data;
+} grid_t;
 enum { DF_NONE = 0, DF_ISO, DF_PARA, DF_PERP, DF_FULL, DF_FLOWMOM, DF_KINRAD, DF_RIGBMAG, DF_PERPISO };
 enum { BK_TIME = 0, BK_SPHERE, BK_PLANE, BK_MOMENTUM };
 enum { IT_FIXED = 0 };
@@ -316,6 +330,8 @@ typedef struct {
    double width_shock;
    /* Uniform (src/background_uniform.cc:58-64) */
    vec3 Eu;
+   /* ServerCartesian */
+   const grid_t* grid;
 } bg_t;
 
 typedef struct {
@@ -356,6 +372,7 @@ struct orc_sim {
    int64_t max_steps;          /* max_trajectory_steps, src/trajectory_base.hh:38 */
    int max_attempts;
    int64_t discards[8];
+   grid_t grid; int have_grid;
 };
 
 /* ------------------------------------------------------------------------------------------------
@@ -546,11 +563,111 @@ static void smooth_shock_derivatives(const bg_t* bg, lane_t* L, spd_t* sp, uint1
    if (mask & BG_dBdt) { sp->dBdt = vscl(shock_transition_deriv(ds) * bg->v_shock / bg->width_shock, vsub(bg->B1, bg->B0)); }
 }
 
+/* ServerCartesianFront::GetVariables + GetGradients with SERVER_INTERP_ORDER 1 and no ghost cells
+   (src/server_cartesian.cc:570-610, :715-740), on the reference's block storage.  Returns nonzero where
+   the reference fails: a position outside the domain (ExServerError, src/server_cartesian.cc:1058) or
+   a stencil that needs a block beyond the domain (which the reference does not guard). */
+static int grid_fields(const grid_t* G, vec3 pos, spd_t* sp)
+{
+   /* the block of the position: ReadCartesianGetNode (src/reader_cartesian.cc:255-270), fluid units */
+   const double ulen_ratio = AU_CGS / unit_length;                    /* Fluid / Particle length */
+   const double uvel_ratio = 1.0E7 / unit_velocity;                   /* Fluid / Particle velocity */
+   const double umag_ratio = (1.0E7 * sqrt(MP_CGS * 1.0)) / unit_magnetic;
+   const double c_fluid = C_CGS / 1.0E7;
+   int a, iz, v, uvw, node[3];
+   for (a = 0; a < 3; a++) {
+      double pf = pos.v[a] / AU_CGS * unit_length;
+      if (pf < G->dom_min[a] || pf > G->dom_max[a]) return 1;
+      node[a] = (int)((pf - G->dom_min[a]) / G->block_length[a]);
+      if (node[a] >= G->nb[a]) return 1;
+   }
+   /* its geometry: ReadCartesianGetBlockCorners (src/reader_cartesian.cc:277-290), BlockBase::LoadDimensions
+      and ConfigureProperties (src/block_base.cc:110-146) */
+   vec3 face_min, face_max, zone_length, cent_min, offset_lo, offset_hi;
+   for (a = 0; a < 3; a++) {
+      face_min.v[a] = G->dom_min[a] + G->block_length[a] * node[a];
+      face_max.v[a] = face_min.v[a] + G->block_length[a];
+      face_min.v[a] *= ulen_ratio;
+      face_max.v[a] *= ulen_ratio;
+      zone_length.v[a] = (face_max.v[a] - face_min.v[a]) / GRID_BS;
+      cent_min.v[a] = face_min.v[a] + 0.5 * zone_length.v[a];
+   }
+   /* BlockBase::GetZoneOffset (src/block_base.cc:164-176) */
+   int zone_lo[3];
+   for (a = 0; a < 3; a++) {
+      offset_lo.v[a] = (pos.v[a] - cent_min.v[a]) / zone_length.v[a];
+      zone_lo[a] = (int)offset_lo.v[a];
+      if (offset_lo.v[a] < 0.0) zone_lo[a]--;
+      offset_lo.v[a] = offset_lo.v[a] - zone_lo[a];
+      offset_hi.v[a] = 1.0 - offset_lo.v[a];
+   }
+   /* InteriorInterpolationStencil (src/server_cartesian.cc:135-183): corner iz sits at zone_lo + (i, j, k)
+      with i = iz & 1, j = (iz >> 1) & 1, k = iz >> 2 */
+   static const int sh[8][3] = {{0,0,0},{1,0,0},{0,1,0},{1,1,0},{0,0,1},{1,0,1},{0,1,1},{1,1,1}};
+   double w[8], der[24];
+   const double* d = zone_length.v;
+   const double *lo = offset_lo.v, *hi = offset_hi.v;
+   w[0] = hi[0] * hi[1] * hi[2]; der[0] = -hi[1] * hi[2] / d[0]; der[1] = -hi[0] * hi[2] / d[1]; der[2] = -hi[0] * hi[1] / d[2];
+   w[1] = lo[0] * hi[1] * hi[2]; der[3] = hi[1] * hi[2] / d[0]; der[4] = -lo[0] * hi[2] / d[1]; der[5] = -lo[0] * hi[1] / d[2];
+   w[2] = hi[0] * lo[1] * hi[2]; der[6] = -lo[1] * hi[2] / d[0]; der[7] = hi[0] * hi[2] / d[1]; der[8] = -hi[0] * lo[1] / d[2];
+   w[3] = lo[0] * lo[1] * hi[2]; der[9] = lo[1] * hi[2] / d[0]; der[10] = lo[0] * hi[2] / d[1]; der[11] = -lo[0] * lo[1] / d[2];
+   w[4] = hi[0] * hi[1] * lo[2]; der[12] = -hi[1] * lo[2] / d[0]; der[13] = -hi[0] * lo[2] / d[1]; der[14] = hi[0] * hi[1] / d[2];
+   w[5] = lo[0] * hi[1] * lo[2]; der[15] = hi[1] * lo[2] / d[0]; der[16] = -lo[0] * lo[2] / d[1]; der[17] = lo[0] * hi[1] / d[2];
+   w[6] = hi[0] * lo[1] * lo[2]; der[18] = -lo[1] * lo[2] / d[0]; der[19] = hi[0] * lo[2] / d[1]; der[20] = hi[0] * lo[1] / d[2];
+   w[7] = lo[0] * lo[1] * lo[2]; der[21] = lo[1] * lo[2] / d[0]; der[22] = lo[0] * lo[2] / d[1]; der[23] = lo[0] * lo[1] / d[2];
+   /* BuildInterpolationStencil with no ghost cells (src/server_cartesian.cc:215-246): a zone index of -1 or
+      4 moves the corner into the previous / next block along that axis, the weights unchanged */
+   double vars[GRID_NVAR], grads[GRID_NVAR][3], Bmag = 0.0, gBm[3] = {0.0, 0.0, 0.0};
+   for (v = 0; v < GRID_NVAR; v++) { vars[v] = 0.0; grads[v][0] = grads[v][1] = grads[v][2] = 0.0; }
+   for (iz = 0; iz < 8; iz++) {
+      int zc[3], bc[3];
+      for (a = 0; a < 3; a++) {
+         zc[a] = zone_lo[a] + sh[iz][a];
+         bc[a] = node[a];
+         if (zc[a] == -1) { zc[a] = GRID_BS - 1; bc[a]--; }
+         else if (zc[a] == GRID_BS) { zc[a] = 0; bc[a]++; }
+         if (bc[a] < 0 || bc[a] >= G->nb[a]) return 1;
+      }
+      const int64_t blk = bc[0] + (int64_t)G->nb[0] * (bc[1] + (int64_t)G->nb[1] * bc[2]);
+      /* BlockBase::DataLoc (src/block_base.cc:192-196) */
+      const double* val = G->data + blk * (GRID_NVAR * GRID_BS * GRID_BS * GRID_BS) + GRID_NVAR * (zc[0] + GRID_BS * (zc[1] + GRID_BS * zc[2]));
+      for (v = 0; v < GRID_NVAR; v++) vars[v] += w[iz] * val[v];
+      const double bm = sqrt(Sqr(val[3]) + Sqr(val[4]) + Sqr(val[5]));
+      Bmag += w[iz] * bm;
+      for (v = 0; v < GRID_NVAR; v++)
+         for (uvw = 0; uvw < 3; uvw++) grads[v][uvw] += der[3 * iz + uvw] * val[v];
+      for (uvw = 0; uvw < 3; uvw++) gBm[uvw] += der[3 * iz + uvw] * bm;
+   }
+   /* SERVER_VAR_INDEX_FLO 0, SERVER_VAR_INDEX_MAG 3 (src/server_base.hh:32-35); E = -(U x B) / c in fluid units */
+   vec3 U = V(vars[0], vars[1], vars[2]), B = V(vars[3], vars[4], vars[5]);
+   vec3 E = vdiv(vneg(vcross(U, B)), c_fluid);
+   sp->U = vscl(uvel_ratio, U);
+   sp->B = vscl(umag_ratio, B);
+   sp->E = vscl(umag_ratio, E);
+   sp->Bmag = Bmag * umag_ratio;
+   for (uvw = 0; uvw < 3; uvw++) {
+      for (a = 0; a < 3; a++) {
+         sp->gradU.row[uvw].v[a] = grads[a][uvw] * uvel_ratio;
+         sp->gradB.row[uvw].v[a] = grads[3 + a][uvw] * umag_ratio;
+      }
+      sp->gradBmag.v[uvw] = gBm[uvw] * umag_ratio;
+   }
+   sp->dmax = fmin(sp->dmax, fmin(fmin(zone_length.v[0], zone_length.v[1]), zone_length.v[2]));
+   return 0;
+}
+
 /* BackgroundBase::GetFields (src/background_base.cc:425-456).  Returns nonzero on ExFieldError. */
 static int get_fields(const bg_t* bg, lane_t* L, double t, vec3 pos, double pmag, uint16_t mask, spd_t* sp)
 {
    sp->mask = mask;
    sp->dmax = bg_dmax(bg, L, pos);
+   if (bg->kind == BG_GRID) {
+      /* BackgroundServer (src/background_server.cc:80-110): |B| is interpolated, not recomputed */
+      if (!bg->grid || grid_fields(bg->grid, pos, sp)) return 1;
+      if (sp->Bmag < sp_tiny) return 1;
+      sp->bhat = vdiv(sp->B, sp->Bmag);
+      return 0;
+   }
    bg_evaluate(bg, L, t, pos, mask, sp);
    if (mask & BG_B) {
       sp->Bmag = vnorm(sp->B);
@@ -1223,10 +1340,47 @@ orc_sim* orc_create(int traj_type, int time_flow)
    return s;
 }
 
+int orc_set_grid(orc_sim* s, const int32_t nb[3], const double dom_min[3], const double dom_max[3], int32_t n_var, const double* data)
+{
+   int a;
+   if (!s || n_var != GRID_NVAR) return -1;
+   for (a = 0; a < 3; a++) if (nb[a] < 1 || !(dom_max[a] > dom_min[a])) return -1;
+   const size_t n = (size_t)GRID_NVAR * GRID_BS * GRID_BS * GRID_BS * (size_t)nb[0] * nb[1] * nb[2];
+   free(s->grid.data);
+   s->grid.data = (double*)malloc(n * sizeof(double));
+   if (!s->grid.data) return -1;
+   memcpy(s->grid.data, data, n * sizeof(double));
+   for (a = 0; a < 3; a++) {
+      s->grid.nb[a] = nb[a];
+      s->grid.dom_min[a] = dom_min[a];
+      s->grid.dom_max[a] = dom_max[a];
+      s->grid.block_length[a] = (dom_max[a] - dom_min[a]) / nb[a];   /* src/reader_cartesian.cc:88 */
+   }
+   s->have_grid = 1;
+   if (s->bg.kind == BG_GRID) s->bg.grid = &s->grid;
+   return 0;
+}
+
+int orc_grid_fields(orc_sim* s, const double pos[3], double out[29])
+{
+   spd_t sp; memset(&sp, 0, sizeof(sp));
+   int k, j;
+   if (!s || s->bg.kind != BG_GRID || !s->bg.grid) return -1;
+   sp.dmax = s->bg.dmax0;
+   if (grid_fields(s->bg.grid, V(pos[0], pos[1], pos[2]), &sp)) return 1;
+   for (k = 0; k < 3; k++) { out[k] = sp.U.v[k]; out[3 + k] = sp.B.v[k]; out[25 + k] = sp.gradBmag.v[k]; }
+   out[6] = sp.Bmag;
+   for (k = 0; k < 3; k++)
+      for (j = 0; j < 3; j++) { out[7 + 3 * k + j] = sp.gradU.row[k].v[j]; out[16 + 3 * k + j] = sp.gradB.row[k].v[j]; }
+   out[28] = sp.dmax;
+   return 0;
+}
+
 void orc_destroy(orc_sim* s)
 {
    int d;
    if (!s) return;
+   free(s->grid.data);
    for (d = 0; d < s->n_dist; d++) { free(s->dist[d].counts); free(s->dist[d].sums); }
    free(s);
 }
@@ -1272,6 +1426,10 @@ static int setup_background(orc_sim* s, const char* cls, dcread* r)
       bg->E0 = vdiv(vneg(vcross(bg->u0, bg->B0)), c_code);
       bg->E1 = vdiv(vneg(vcross(bg->u1, bg->B1)), c_code);
       if (bg->kind == BG_SMOOTHSHOCK) { bg->width_shock = rd_d(r); bg->dmax_fraction = rd_d(r); }
+   }
+   else if (!strcmp(cls, "BackgroundServerCartesian")) {  /* src/background_server.cc:62-75: base parameters only */
+      bg->kind = BG_GRID;
+      bg->grid = s->have_grid ? &s->grid : NULL;
    }
    else return -1;
    s->have_bg = 1;

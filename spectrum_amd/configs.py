@@ -75,6 +75,7 @@ class Setup:
     plugins: list = field(default_factory=list)     # (role, class name, bytes)
     options: dict = field(default_factory=dict)     # runtime replacements of compile-time switches
     meta: dict = field(default_factory=dict)
+    grid: object = None                             # cartesian.CartesianGrid of a BackgroundServerCartesian
 
     def add(self, role: int, cls: str, dc: DataContainer) -> "Setup":
         self.plugins.append((role, cls, dc.bytes()))
@@ -261,4 +262,34 @@ def uniform_walk(D=2.0, U=(0.0, 0.0, 0.0), B=(3.0, 4.0, 0.0), t_limit=1000.0, dm
     dst.insert_double(1.0).insert_double(0.0).insert_int(1)
     s.add(DISTRIBUTION, "DistributionTimeUniform", dst)
     s.meta.update(D=D, U=U, t_limit=t_limit)
+    return s
+
+
+def parker_modulation_cartesian(grid, n_bins=100, T1_MeV=10.0, T2_MeV=5000.0, inner_au=0.05, outer_au=80.0,
+                                t_limit_years=1.0, name="parker_modulation_cartesian") -> Setup:
+    """Backward Parker modulation on a gridded solar wind, benchmarks/main_test_modulation_cartesian_parker.cc:
+    BackgroundServerCartesian (dmax 1 au) on `grid` (cartesian.parker_spiral() is the reference's
+    25^3-block background), kappa = 1.5e22 cm^2/s (T / 1 GeV) (r / 1 au)^2 isotropic, log-uniform
+    10 MeV - 5 GeV from 1 au, absorbing spheres at 0.05 au (action 1) and 80 au (the hot boundary,
+    action 0), a 1-year limit (not binned), and a 100-bin spectrum with J0 T^-1.8 at the outer boundary.
+    The unused shock parameters of the diffusion model go in as ints, as the benchmark inserts them."""
+    s = Setup(name, TRAJ_PARKER, BACKWARD, grid=grid)
+    s.add(BACKGROUND, "BackgroundServerCartesian", background_base(DataContainer(), 0.0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), one_au))
+    s.add(INITIAL, "InitialTimeFixed", DataContainer().insert_double(0.0))
+    s.add(INITIAL, "InitialSpaceFixed", DataContainer().insert_vec((one_au, 0.0, 0.0)))
+    p1, p2 = mom_from_ekin(T1_MeV * MeV), mom_from_ekin(T2_MeV * MeV)
+    s.add(INITIAL, "InitialMomentumThickShell", DataContainer().insert_double(p1).insert_double(p2).insert_bool(True))
+    s.add(BOUNDARY, "BoundarySphereAbsorb", boundary(1, [1], ("vec", (0.0, 0.0, 0.0)), ("double", inner_au * one_au)))
+    s.add(BOUNDARY, "BoundarySphereAbsorb", boundary(1, [0], ("vec", (0.0, 0.0, 0.0)), ("double", outer_au * one_au)))
+    s.add(BOUNDARY, "BoundaryTimeExpire", boundary(1, [-1], ("double", -t_limit_years * one_year)))
+    dif = DataContainer().insert_double(1.5e22 / unit_diffusion).insert_double(GeV).insert_double(one_au)
+    dif.insert_double(1.0).insert_double(2.0).insert_double(1.0).insert_int(0)
+    dif.insert_int(1).insert_int(0).insert_int(1)
+    s.add(DIFFUSION, "DiffusionKineticEnergyRadialDistancePowerLaw", dif)
+    unit_distro = 1.0 / (unit_length ** 2 * unit_time * 4.0 * math.pi * unit_energy)
+    dst = distribution_base(DataContainer(), (n_bins, 0, 0), (T1_MeV * MeV, 0.0, 0.0), (T2_MeV * MeV, 0.0, 0.0), (1, 0, 0), (0, 0, 0),
+                            unit_distro, (unit_energy, 1.0, 1.0), False)
+    dst.insert_double(1.0 / unit_distro).insert_double(GeV).insert_double(-1.8).insert_double(0.0)
+    s.add(DISTRIBUTION, "DistributionSpectrumKineticEnergyPowerLaw", dst)
+    s.meta.update(T1=T1_MeV * MeV, T2=T2_MeV * MeV, n_bins=n_bins)
     return s

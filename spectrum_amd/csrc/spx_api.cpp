@@ -12,7 +12,9 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <fstream>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -31,6 +33,10 @@ constexpr double c_code_h = C_CGS_H / unit_velocity_h;
 constexpr double sp_tiny_h = 1.0E-15;
 
 // ---- DataContainer reader (common/data_container.cc:214-238): [u8 len][len bytes] records -------------
+// DataContainer::Read copies the record's own length into the value.  A record shorter than the value
+// (an int where the class reads a double, as benchmarks/main_test_modulation_cartesian_parker.cc:190-200
+// inserts its unused shock parameters) fills the low bytes; the rest, which the reference leaves as it
+// was, is zero here.  A longer record would overrun the value and is rejected.
 struct Reader {
    const uint8_t* p;
    const uint8_t* end;
@@ -39,7 +45,8 @@ struct Reader {
    {
       if (err || p >= end) { err = true; return; }
       size_t len = *p;
-      if (p + 1 + len > end || len != want) { err = true; return; }
+      if (p + 1 + len > end || len > want || len == 0) { err = true; return; }
+      std::memset(dst, 0, want);
       std::memcpy(dst, p + 1, len);
       p += len + 1;
    }
@@ -90,6 +97,56 @@ inline void induced_e(const double* u, const double* B, double* E)
    E[0] = -c[0] / c_code_h; E[1] = -c[1] / c_code_h; E[2] = -c[2] / c_code_h;
 }
 
+// ---- gridded background (BackgroundServerCartesian) -------------------------------------------------
+// Unit factors of the grid data (common/physics.hh:20-56 fluid units, common/specie.hh:30-64 code
+// units), each formed as the reference's compile-time expressions are.
+constexpr double MP_CGS_H = 1.67262192596E-24;
+constexpr double E_CGS_H = 4.8032044E-10;
+constexpr double AU_CGS_H = 1.495978707E+13;
+constexpr double unit_length_h = 1.0E10;
+inline void grid_units(spx_grid_desc& g)
+{
+   const double unit_momentum = MP_CGS_H * unit_velocity_h;
+   const double unit_time = unit_length_h / unit_velocity_h;
+   const double unit_magnetic = unit_momentum / (E_CGS_H * unit_time);
+   const double fluid_magnetic = 1.0E7 * std::sqrt(MP_CGS_H * 1.0);
+   g.f_len = AU_CGS_H / unit_length_h;
+   g.f_vel = 1.0E7 / unit_velocity_h;
+   g.f_mag = fluid_magnetic / unit_magnetic;
+}
+
+// Zone records of a grid on the host and, once a run needs them, in device memory (FP64 and FP32
+// copies).  Shared by a simulation and the queries served from it, so it lives as long as any of them.
+struct GridData {
+   spx_grid_desc desc{};
+   std::vector<double> rec;            // [nz2][nz1][nz0][SPX_GRID_REC]
+   DevBuf<double> d64;
+   DevBuf<float> d32;
+   std::mutex mu;
+   bool uploaded = false;
+   size_t bytes() const { return rec.size() * (sizeof(double) + sizeof(float)); }
+   hipError_t upload()
+   {
+      std::lock_guard<std::mutex> lk(mu);
+      if (uploaded) return hipSuccess;
+      hipError_t e = d64.alloc(rec.size());
+      if (e != hipSuccess) return e;
+      e = d32.alloc(rec.size());
+      if (e != hipSuccess) return e;
+      e = hipMemcpy(d64.p, rec.data(), sizeof(double) * rec.size(), hipMemcpyHostToDevice);
+      if (e != hipSuccess) return e;
+      std::vector<float> f(rec.size());
+      for (size_t i = 0; i < rec.size(); i++) f[i] = (float)rec[i];
+      e = hipMemcpy(d32.p, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice);
+      if (e != hipSuccess) return e;
+      desc.z64 = d64.p;
+      desc.z32 = d32.p;
+      uploaded = true;
+      return hipSuccess;
+   }
+   ~GridData() { d64.release(); d32.release(); }
+};
+
 }  // namespace
 
 struct spx_sim {
@@ -109,6 +166,7 @@ struct spx_sim {
    QueryRes q;
    DevBuf<unsigned long long> hist_counts, hist_events;
    DevBuf<double> hist_sums;
+   std::shared_ptr<GridData> grid;     // BackgroundServerCartesian data
    // run state
    bool running = false;
    uint64_t run_first = 0, run_n = 0;
@@ -179,6 +237,10 @@ static int add_background(spx_sim* s, const std::string& cls, Reader& r)
       induced_e(g.u0, g.B0, g.E0);
       induced_e(g.u1, g.B1, g.E1);
       if (g.kind == SPX_BG_SMOOTHSHOCK) { g.width_shock = r.d(); g.dmax_fraction = r.d(); }   // src/background_smooth_shock.cc:139-146
+   }
+   else if (cls == "BackgroundServerCartesian") {   // src/background_server.cc:62-75: the base parameters only
+      g.kind = SPX_BG_GRID;
+      s->grid.reset();                  // the grid comes with spx_set_background_grid / spx_add_background_file
    }
    else return fail(s, SPX_ERR_CLASS, "unknown background class " + cls);
    s->have_bg = true;
@@ -332,6 +394,16 @@ static int ensure_device(spx_sim* s)
    return SPX_OK;
 }
 
+// Device copy of the grid, and its addresses in the plan (both precision views).
+static int bind_grid(spx_sim* s)
+{
+   if (s->plan.bg.kind != SPX_BG_GRID) return SPX_OK;
+   HIPCHK(s, s->grid->upload());
+   s->plan.bg.grid = s->grid->desc;
+   s->plan.bgf.grid = s->grid->desc;
+   return SPX_OK;
+}
+
 // Resident slots per device: 4 waves per SIMD on every CU.  The state is register resident, so a few
 // waves per SIMD hide latency; more would only lengthen the tail of a run.
 static uint64_t auto_lanes()
@@ -358,6 +430,7 @@ static void finalize_plan(spx_sim* s)
    f.dmax0 = (float)g.dmax0; f.r_ref = (float)g.r_ref; f.dmax_fraction = (float)g.dmax_fraction; f.ur0 = (float)g.ur0;
    f.w0 = (float)g.w0; f.hp_rad = (float)g.hp_rad; f.v_shock = (float)g.v_shock; f.compression = (float)g.compression;
    f.width_shock = (float)g.width_shock;
+   f.grid = g.grid;
    spx_df_t<double>& d = P.df;
    d.pos_dep = 0;
    if (d.kind == SPX_DF_FLOWMOM) d.pos_dep = (d.pow_law_U != 0.0);
@@ -380,6 +453,7 @@ static int check_ready(spx_sim* s)
    if (s->plan.n_bt < 1) return fail(s, SPX_ERR_NOT_READY, "at least one time boundary is required");
    if (!s->have_df) return fail(s, SPX_ERR_NOT_READY, "no diffusion model");
    if (s->plan.traj_type == SPX_TRAJ_PARKER_SOURCE && s->plan.src_kind == SPX_SRC_NONE) return fail(s, SPX_ERR_NOT_READY, "no source");
+   if (s->plan.bg.kind == SPX_BG_GRID && !s->grid) return fail(s, SPX_ERR_NOT_READY, "BackgroundServerCartesian has no grid data");
    return SPX_OK;
 }
 
@@ -388,6 +462,8 @@ static int check_transport(spx_sim* s)
 {
    if (s->plan.traj_type == SPX_TRAJ_GUIDING_DIFF_SCATT) {
       if (s->plan.time_flow != SPX_FLOW_FORWARD) return fail(s, SPX_ERR_ARG, "guiding-centre transport runs forward in time");
+      if (s->plan.bg.kind == SPX_BG_GRID)
+         return fail(s, SPX_ERR_ARG, "guiding-centre transport on a gridded background is not implemented (Parker transports only)");
       if (s->plan.bg.kind != SPX_BG_SOLARWIND && s->plan.bg.kind != SPX_BG_UNIFORM)
          return fail(s, SPX_ERR_ARG, "guiding-centre transport supports the static SolarWind and Uniform backgrounds");
       if (s->plan.df.kind == SPX_DF_FLOWMOM || s->plan.df.kind == SPX_DF_KINRAD)
@@ -446,6 +522,78 @@ int spx_add_plugin(spx_sim* s, int role, const char* class_name, const uint8_t* 
    return rc;
 }
 
+int spx_set_background_grid(spx_sim* s, const int32_t n_blocks[3], const double dom_min[3], const double dom_max[3], int32_t n_var,
+                            const double* data)
+{
+   if (!s || !n_blocks || !dom_min || !dom_max || !data) return SPX_ERR_ARG;
+   if (s->running) return fail(s, SPX_ERR_STATE, "the background cannot change during a run");
+   if (s->plan.bg.kind != SPX_BG_GRID) return fail(s, SPX_ERR_STATE, "add a BackgroundServerCartesian background first");
+   // BlockCartesian holds n_variables_cartesian = 6 values per zone: flow (0-2) and magnetic field (3-5)
+   // (src/block_cartesian.hh:12, src/server_base.hh:32-35)
+   if (n_var != 6) return fail(s, SPX_ERR_ARG, "a Cartesian grid holds 6 variables per zone (U, B)");
+   auto G = std::make_shared<GridData>();
+   spx_grid_desc& g = G->desc;
+   int64_t nzone = 1;
+   for (int a = 0; a < 3; a++) {
+      if (n_blocks[a] < 1 || n_blocks[a] > (1 << 20) / SPX_GRID_BLOCK) return fail(s, SPX_ERR_ARG, "blocks per axis out of range");
+      if (!(dom_max[a] > dom_min[a])) return fail(s, SPX_ERR_ARG, "empty domain");
+      g.nb[a] = n_blocks[a];
+      g.nz[a] = SPX_GRID_BLOCK * n_blocks[a];
+      g.dom_min[a] = dom_min[a];
+      g.dom_max[a] = dom_max[a];
+      g.block_length[a] = (dom_max[a] - dom_min[a]) / (double)n_blocks[a];   // src/reader_cartesian.cc:88
+      nzone *= g.nz[a];
+   }
+   grid_units(g);
+   if (nzone > ((int64_t)1 << 34) / SPX_GRID_REC) return fail(s, SPX_ERR_LIMIT, "grid too large");
+   G->rec.assign((size_t)nzone * SPX_GRID_REC, 0.0);
+   // ReadCartesianData's reordering (src/reader_cartesian.cc:103-140): blocks x fastest, zones x fastest
+   const int B = SPX_GRID_BLOCK;
+   const int64_t bsize = (int64_t)n_var * B * B * B;
+   int64_t node = 0;
+   for (int kb = 0; kb < g.nb[2]; kb++)
+      for (int jb = 0; jb < g.nb[1]; jb++)
+         for (int ib = 0; ib < g.nb[0]; ib++, node++)
+            for (int kz = 0; kz < B; kz++)
+               for (int jz = 0; jz < B; jz++)
+                  for (int iz = 0; iz < B; iz++) {
+                     const double* v = data + node * bsize + (int64_t)n_var * (iz + B * (jz + B * kz));
+                     const int64_t zx = (int64_t)B * ib + iz, zy = (int64_t)B * jb + jz, zz = (int64_t)B * kb + kz;
+                     double* r = G->rec.data() + (zx + g.nz[0] * (zy + (int64_t)g.nz[1] * zz)) * SPX_GRID_REC;
+                     for (int m = 0; m < 6; m++) r[m] = v[m];
+                     // |B| of the zone as GetVariablesInterp1 forms it (src/server_cartesian.cc:470-473)
+                     double b2 = v[3] * v[3];
+                     b2 = b2 + v[4] * v[4];
+                     b2 = b2 + v[5] * v[5];
+                     r[6] = std::sqrt(b2);
+                  }
+   s->grid = G;
+   return SPX_OK;
+}
+
+int spx_add_background_file(spx_sim* s, const char* class_name, const uint8_t* container, size_t len, const char* fname_pattern)
+{
+   if (!s || !class_name || !fname_pattern) return SPX_ERR_ARG;
+   int rc = spx_add_plugin(s, SPX_BACKGROUND, class_name, container, len);
+   if (rc) return rc;
+   if (s->plan.bg.kind != SPX_BG_GRID) return SPX_OK;   // analytic backgrounds have no files
+   // ReadCartesianHeader (src/reader_cartesian.cc:25-95): blocks per axis, domain corners, variables
+   const std::string pat(fname_pattern);
+   std::ifstream hdr(pat + ".info");
+   int32_t nb[3] = {0, 0, 0}, nvar = 0;
+   double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+   hdr >> nb[0] >> nb[1] >> nb[2] >> lo[0] >> lo[1] >> lo[2] >> hi[0] >> hi[1] >> hi[2] >> nvar;
+   if (!hdr) return fail(s, SPX_ERR_ARG, "cannot read grid header " + pat + ".info");
+   if (nb[0] < 1 || nb[1] < 1 || nb[2] < 1 || nvar < 1 || nvar > 64) return fail(s, SPX_ERR_ARG, "bad grid header " + pat + ".info");
+   // ReadCartesianData (src/reader_cartesian.cc:98-103): the binary doubles, block by block
+   const size_t n = (size_t)nvar * SPX_GRID_BLOCK * SPX_GRID_BLOCK * SPX_GRID_BLOCK * (size_t)nb[0] * nb[1] * nb[2];
+   std::vector<double> data(n);
+   std::ifstream dat(pat + ".out", std::ios::binary);
+   dat.read(reinterpret_cast<char*>(data.data()), (std::streamsize)(n * sizeof(double)));
+   if (!dat || (size_t)dat.gcount() != n * sizeof(double)) return fail(s, SPX_ERR_ARG, "grid data file " + pat + ".out is short");
+   return spx_set_background_grid(s, nb, lo, hi, nvar, data.data());
+}
+
 int spx_set_option(spx_sim* s, const char* name, double v)
 {
    if (!s || !name) return SPX_ERR_ARG;
@@ -473,6 +621,8 @@ int spx_begin(spx_sim* s, uint64_t first, uint64_t n_traj, const spx_rng* rng, i
    rc = check_transport(s);
    if (rc) return rc;
    rc = ensure_device(s);
+   if (rc) return rc;
+   rc = bind_grid(s);
    if (rc) return rc;
    uint64_t lanes = s->lanes_opt > 0 ? (uint64_t)s->lanes_opt : auto_lanes();
    lanes = (lanes + s->block - 1) / s->block * s->block;
@@ -733,6 +883,7 @@ struct SQuery {
    std::vector<std::vector<int64_t>> counts;
    std::vector<std::vector<double>> sums;
    std::vector<int64_t> nev;
+   std::shared_ptr<GridData> grid;     // keeps a gridded background's device data alive
 };
 
 }  // namespace
@@ -917,14 +1068,21 @@ int spx_server_submit(spx_server* S, const spx_sim* proto, uint64_t first, uint6
    finalize_plan(s);
    rc = check_transport(s);
    if (rc) { S->err = s->err; return rc; }
+   if (s->plan.bg.kind == SPX_BG_GRID) {
+      rc = ensure_device(s);
+      if (!rc) rc = bind_grid(s);
+      if (rc) { S->err = s->err; return rc; }
+   }
    SQuery* q = new SQuery();
    q->plan = s->plan;
+   q->grid = s->grid;
    q->hist_bins = s->hist_bins;
    q->first = first;
    q->n_traj = n_traj;
    q->rng.mode = 0;
    q->rng.seed = seed;
    q->footprint = QueryRes::footprint(q->plan, q->hist_bins, n_traj, &q->rng, false);
+   if (q->grid) q->footprint += q->grid->bytes();   // charged to every query that uses the grid
    std::lock_guard<std::mutex> lk(S->mu);
    if (S->failed) { delete q; return SPX_ERR_DEVICE; }
    q->id = S->next_id++;

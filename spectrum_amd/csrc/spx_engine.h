@@ -480,7 +480,7 @@ __device__ __forceinline__ void service_slot(LaunchPtr L, StartFn&& set_start)
          }
          else st = SLOT_RUN;
       }
-      // 2. discarded attempts: retry with continuing draws, or give up
+      // 2. discarded attempts: retry with continuing draws (a process-wide stream in the reference), or give up
       if (st == SLOT_DISCARD) {
          const int exc = L->ex_code[slot];
          atomicAdd(&Q->ctl->discards[exc], 1ull);
@@ -504,6 +504,7 @@ __device__ __forceinline__ void service_slot(LaunchPtr L, StartFn&& set_start)
          if (need && id < Q->end) {
             L->traj[slot] = id;
             L->nu[slot] = 0;
+            L->nn[slot] = 0;
             L->attempt[slot] = 0;
             st = SLOT_START;
          }
@@ -526,6 +527,7 @@ __device__ __forceinline__ void service_slot(LaunchPtr L, StartFn&& set_start)
          if (id < Q->end) {
             L->traj[slot] = id;
             L->nu[slot] = 0;
+            L->nn[slot] = 0;
             L->attempt[slot] = 0;
             st = SLOT_START;
          }

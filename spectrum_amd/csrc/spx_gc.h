@@ -192,7 +192,7 @@ __device__ int gc_set_start(PlanPtr P, QueryPtr Q, LaunchPtr L, const spx_rng_de
    L->amp[slot] = 0.0; L->wgt[slot] = 0.0; L->dt_prev[slot] = 0.0; L->src_old[slot] = 0.0; L->ds_prev[slot] = 0.0;
    L->dmax[slot] = (double)F.dmax;
    L->pf[slot] = pm1;
-   L->steps[slot] = 0; L->nn[slot] = 0; L->n_events[slot] = 0;
+   L->steps[slot] = 0; L->n_events[slot] = 0;   // the draw counters continue across attempts
    L->dt_adapt[slot] = sp_large * (double)F.dmax / c_code;
    L->t0[slot] = t; L->x0[slot] = x; L->y0[slot] = y; L->z0[slot] = z; L->m00[slot] = pp; L->m01[slot] = 0.0; L->m02[slot] = pl;
    for (int d = 0; d < SPX_MAX_DIST; d++) L->ev_bin[(uint64_t)d * n + slot] = -1;

@@ -35,6 +35,19 @@ template <typename R, int BG>
 __device__ __forceinline__ bool parker_fields(const SPX_CP spx_bg_t<R>& bg, double t, double x, double y, double z, R p, R& ds_prev, PkFields<R>& F)
 {
    F.dmax = bg_dmax<R, BG>(bg, x, y, z, ds_prev);
+   if constexpr (BG == SPX_BG_GRID) {
+      // BackgroundServer: interpolated U, B and |B|, the stencil's grad U, dmax limited by the zone size
+      GridFields<R> G;
+      if (!grid_eval<R, true>(bg.grid, x, y, z, G)) return false;
+      F.dmax = fmin(F.dmax, R(G.zl_min));
+      F.U = G.U;
+      F.Bmag = G.Bmag;
+      if (F.Bmag < R(sp_tiny)) return false;
+      if constexpr (is_f32<R>()) F.bhat = (R(1) / F.Bmag) * G.B;
+      else F.bhat = G.B / F.Bmag;
+      F.divU = G.divU;
+      return true;
+   }
    v3<R> B;
    R ds = ds_prev;
    bg_eval<R, BG>(bg, t, x, y, z, 3, F.U, B, ds);
@@ -159,7 +172,7 @@ __device__ int parker_set_start(PlanPtr P, QueryPtr Q, LaunchPtr L, const spx_rn
    L->t[slot] = t; L->x[slot] = x; L->y[slot] = y; L->z[slot] = z; L->m0[slot] = p; L->m1[slot] = 0.0; L->m2[slot] = 0.0;
    L->amp[slot] = 0.0; L->wgt[slot] = 0.0; L->dt_prev[slot] = 0.0;
    L->ds_prev[slot] = (double)ds; L->dmax[slot] = (double)F.dmax;
-   L->steps[slot] = 0; L->nn[slot] = 0; L->n_events[slot] = 0;
+   L->steps[slot] = 0; L->n_events[slot] = 0;   // the draw counters continue across attempts
    // the adaptive step starts large so that dt is the physical step (src/trajectory_base.cc:894-895)
    L->dt_adapt[slot] = sp_large * (double)F.dmax / c_code;
    L->t0[slot] = t; L->x0[slot] = x; L->y0[slot] = y; L->z0[slot] = z; L->m00[slot] = p; L->m01[slot] = 0.0; L->m02[slot] = 0.0;
@@ -222,7 +235,50 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
    const R delta = fmin(LarmorRadius(p, H.F.Bmag), H.F.dmax);
    v3<R> divK = mk<R>(R(0), R(0), R(0));
    const bool pos_dep = (DF == SPX_DF_FULL) ? false : (df.pos_dep != 0);
-   if (pos_dep || K0 != K1) {
+   if (BG == SPX_BG_GRID && (pos_dep || K0 != K1)) {
+      // the same stencil on the gridded background: interpolated B and |B| at the derivative points
+#pragma unroll 1
+      for (int j = 0; j < 3; j++) {
+         const double ex = (j == 0) ? 1.0 : 0.0, ey = (j == 1) ? 1.0 : 0.0, ez = (j == 2) ? 1.0 : 0.0;
+         double xf = H.x + (double)delta * ex, yf = H.y + (double)delta * ey, zf = H.z + (double)delta * ez;
+         GridFields<R> Gf, Gb;
+         if (!grid_eval<R, false>(bg.grid, xf, yf, zf, Gf)) return 1;
+         if (Gf.Bmag < R(sp_tiny)) return 1;
+         const v3<R> bf = is_f32<R>() ? (R(1) / Gf.Bmag) * Gf.B : Gf.B / Gf.Bmag;
+         R K0f = K0, K1f = K1;
+         if (pos_dep) parker_kappa<R, DF>(df, p, Gf.Bmag, xf, yf, zf, Gf.U, K0f, K1f);
+         xf -= 2.0 * (double)delta * ex;
+         yf -= 2.0 * (double)delta * ey;
+         zf -= 2.0 * (double)delta * ez;
+         if (!grid_eval<R, false>(bg.grid, xf, yf, zf, Gb)) return 1;
+         if (Gb.Bmag < R(sp_tiny)) return 1;
+         const v3<R> bb = is_f32<R>() ? (R(1) / Gb.Bmag) * Gb.B : Gb.B / Gb.Bmag;
+         R K0b = K0, K1b = K1;
+         if (pos_dep) parker_kappa<R, DF>(df, p, Gb.Bmag, xf, yf, zf, Gb.U, K0b, K1b);
+         const R bfj = j == 0 ? bf.x : (j == 1 ? bf.y : bf.z);
+         const R bbj = j == 0 ? bb.x : (j == 1 ? bb.y : bb.z);
+         const R cf = (K1f - K0f) * bfj, cb = (K1b - K0b) * bbj;
+         divK.x += R(0.5) * ((K0f * (j == 0 ? R(1) : R(0)) + cf * bf.x) - (K0b * (j == 0 ? R(1) : R(0)) + cb * bb.x)) / delta;
+         divK.y += R(0.5) * ((K0f * (j == 1 ? R(1) : R(0)) + cf * bf.y) - (K0b * (j == 1 ? R(1) : R(0)) + cb * bb.y)) / delta;
+         divK.z += R(0.5) * ((K0f * (j == 2 ? R(1) : R(0)) + cf * bf.z) - (K0b * (j == 2 ? R(1) : R(0)) + cb * bb.z)) / delta;
+      }
+   }
+   else if (BG == SPX_BG_GRID) {
+      // divK vanishes identically, but the reference still evaluates the six derivative points, and one
+      // outside the grid discards the attempt
+#pragma unroll 1
+      for (int j = 0; j < 3; j++) {
+         const double ex = (j == 0) ? 1.0 : 0.0, ey = (j == 1) ? 1.0 : 0.0, ez = (j == 2) ? 1.0 : 0.0;
+         double xf = H.x + (double)delta * ex, yf = H.y + (double)delta * ey, zf = H.z + (double)delta * ez;
+         GridFields<R> G;
+         if (!grid_eval<R, false>(bg.grid, xf, yf, zf, G) || G.Bmag < R(sp_tiny)) return 1;
+         xf -= 2.0 * (double)delta * ex;
+         yf -= 2.0 * (double)delta * ey;
+         zf -= 2.0 * (double)delta * ez;
+         if (!grid_eval<R, false>(bg.grid, xf, yf, zf, G) || G.Bmag < R(sp_tiny)) return 1;
+      }
+   }
+   else if (pos_dep || K0 != K1) {
       const int want = (df.kind == SPX_DF_FLOWMOM || df.kind == SPX_DF_KINRAD) ? 3 : 2;
       // FP32 steps with a position-independent kappa evaluate the stencil relative to the origin
       v3<R> dc = mk<R>(R(0), R(0), R(0));
@@ -434,10 +490,21 @@ static hipError_t dispatch_flow(const spx_plan_desc& hp, uint64_t n_lanes, const
    return fwd ? dispatch_df<R, BG, 1, 0>(hp, n_lanes, d_launch, st, mid) : dispatch_df<R, BG, 0, 0>(hp, n_lanes, d_launch, st, mid);
 }
 
+// The gridded background runs the generic diffusion dispatch only (every class through kappa01).
+template <typename R>
+static hipError_t dispatch_grid(const spx_plan_desc& hp, uint64_t n_lanes, const Launch* d_launch, hipStream_t st, hipEvent_t mid)
+{
+   const bool fwd = hp.time_flow == 0;
+   constexpr int G = SPX_BG_GRID;
+   if (hp.traj_type == 1) return fwd ? launch_parker<R, G, 1, 1, 0>(n_lanes, d_launch, st, mid) : launch_parker<R, G, 0, 1, 0>(n_lanes, d_launch, st, mid);
+   return fwd ? launch_parker<R, G, 1, 0, 0>(n_lanes, d_launch, st, mid) : launch_parker<R, G, 0, 0, 0>(n_lanes, d_launch, st, mid);
+}
+
 template <typename R>
 hipError_t dispatch_parker(const spx_plan_desc& hp, uint64_t n_lanes, const Launch* d_launch, hipStream_t st, hipEvent_t mid)
 {
    switch (hp.bg.kind) {
+   case SPX_BG_GRID: return dispatch_grid<R>(hp, n_lanes, d_launch, st, mid);
    case SPX_BG_SOLARWIND: return dispatch_flow<R, SPX_BG_SOLARWIND>(hp, n_lanes, d_launch, st, mid);
    case SPX_BG_SMOOTHSHOCK: return dispatch_flow<R, SPX_BG_SMOOTHSHOCK>(hp, n_lanes, d_launch, st, mid);
    case SPX_BG_SHOCK: return dispatch_flow<R, SPX_BG_SHOCK>(hp, n_lanes, d_launch, st, mid);

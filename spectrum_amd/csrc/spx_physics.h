@@ -183,6 +183,101 @@ SPX_HD R bg_dmax(const SPX_CP spx_bg_t<R>& b, double x, double y, double z, R ds
    return b.dmax0;
 }
 
+// ---- gridded background: BackgroundServerCartesian, SERVER_INTERP_ORDER 1, no ghost cells -----------
+// The block of a position follows ReadCartesianGetNode (src/reader_cartesian.cc:255-270) on the
+// position in fluid units; its geometry ReadCartesianGetBlockCorners (src/reader_cartesian.cc:277-290),
+// BlockBase::LoadDimensions and ConfigureProperties (src/block_base.cc:110-146); the offsets
+// BlockBase::GetZoneOffset (src/block_base.cc:164-176); the weights and derivative weights
+// ServerCartesianFront::InteriorInterpolationStencil (src/server_cartesian.cc:135-183), which the
+// reference keeps when the stencil reaches into neighbour blocks (:215-246), so a corner's zone is
+// 4 * block + local zone in the global zone array; the sums GetVariablesInterp1 (:454-560) and
+// GetGradientsInterp1 (:630-705) in corner order; the unit conversions GetVariables / GetGradients
+// (:570-610, :715-740).  A position outside the domain (ExServerError, src/server_cartesian.cc:1058)
+// or whose stencil leaves the grid (the reference then reads a block that does not exist) returns
+// false: a field error that discards the attempt.
+template <typename R>
+struct GridFields {
+   v3<R> U, B;
+   R Bmag;
+   R divU;            // trace of grad U (GRAD only)
+   double zl_min;     // smallest zone length of the position's block (ServerCartesianFront dmax limit)
+};
+
+typedef double grid_d2 __attribute__((ext_vector_type(2)));
+typedef float grid_f4 __attribute__((ext_vector_type(4)));
+
+// One zone record: U, B, |B| (fluid units).
+template <typename R> __device__ __forceinline__ void grid_rec(const SPX_CP spx_grid_desc& g, int64_t zone, R v[7]);
+template <> __device__ __forceinline__ void grid_rec<double>(const SPX_CP spx_grid_desc& g, int64_t zone, double v[7])
+{
+   const grid_d2* r = reinterpret_cast<const grid_d2*>(g.z64) + zone * (SPX_GRID_REC / 2);
+   const grid_d2 a = r[0], b = r[1], c = r[2], d = r[3];
+   v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y; v[4] = c.x; v[5] = c.y; v[6] = d.x;
+}
+template <> __device__ __forceinline__ void grid_rec<float>(const SPX_CP spx_grid_desc& g, int64_t zone, float v[7])
+{
+   const grid_f4* r = reinterpret_cast<const grid_f4*>(g.z32) + zone * (SPX_GRID_REC / 4);
+   const grid_f4 a = r[0], b = r[1];
+   v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z;
+}
+
+template <typename R, bool GRAD>
+__device__ __forceinline__ bool grid_eval(const SPX_CP spx_grid_desc& g, double x, double y, double z, GridFields<R>& o)
+{
+   const double pos[3] = {x, y, z};
+   R lo[3], hi[3], dl[3];
+   int64_t zone0 = 0;
+   double zl_min = 0.0;
+#pragma unroll
+   for (int a = 0; a < 3; a++) {
+      const double pf = pos[a] / AU_CGS * unit_length;          // server side: position in fluid units
+      if (!(pf >= g.dom_min[a] && pf <= g.dom_max[a])) return false;
+      const int ib = (int)((pf - g.dom_min[a]) / g.block_length[a]);
+      if (ib >= g.nb[a]) return false;                          // on the upper face of the domain
+      const double f0 = g.dom_min[a] + g.block_length[a] * (double)ib;
+      const double face_lo = f0 * g.f_len, face_hi = (f0 + g.block_length[a]) * g.f_len;
+      const double zl = (face_hi - face_lo) / (double)SPX_GRID_BLOCK;
+      const double cmin = face_lo + 0.5 * zl;
+      double off = (pos[a] - cmin) / zl;
+      int zi = (int)off;
+      if (off < 0.0) zi--;
+      off = off - (double)zi;
+      const int gz = SPX_GRID_BLOCK * ib + zi;
+      if (gz < 0 || gz + 1 >= g.nz[a]) return false;
+      lo[a] = R(off);
+      hi[a] = R(1.0 - off);
+      dl[a] = R(zl);
+      zl_min = (a == 0) ? zl : fmin(zl_min, zl);
+      zone0 += (a == 0) ? (int64_t)gz : (a == 1 ? (int64_t)gz * g.nz[0] : (int64_t)gz * g.nz[0] * g.nz[1]);
+   }
+   const int64_t sy = g.nz[0], sz = (int64_t)g.nz[0] * g.nz[1];
+   R acc[7], dUx = R(0), dUy = R(0), dUz = R(0);
+#pragma unroll
+   for (int c = 0; c < 8; c++) {
+      const int i = c & 1, j = (c >> 1) & 1, k = c >> 2;
+      R v[7];
+      grid_rec<R>(g, zone0 + i + j * sy + k * sz, v);
+      const R fx = i ? lo[0] : hi[0], fy = j ? lo[1] : hi[1], fz = k ? lo[2] : hi[2];
+      const R w = fx * fy * fz;
+#pragma unroll
+      for (int m = 0; m < 7; m++) acc[m] = (c == 0) ? w * v[m] : acc[m] + w * v[m];
+      if (GRAD) {   // the diagonal of grad U: d(Ux)/dx, d(Uy)/dy, d(Uz)/dz
+         const R sx = i ? R(1) : R(-1), sy_ = j ? R(1) : R(-1), sz_ = k ? R(1) : R(-1);
+         const R ddx = sx * (fy * fz) / dl[0], ddy = sy_ * (fx * fz) / dl[1], ddz = sz_ * (fx * fy) / dl[2];
+         dUx = (c == 0) ? ddx * v[0] : dUx + ddx * v[0];
+         dUy = (c == 0) ? ddy * v[1] : dUy + ddy * v[1];
+         dUz = (c == 0) ? ddz * v[2] : dUz + ddz * v[2];
+      }
+   }
+   const R fv = R(g.f_vel), fm = R(g.f_mag);
+   o.U = mk<R>(acc[0] * fv, acc[1] * fv, acc[2] * fv);
+   o.B = mk<R>(acc[3] * fm, acc[4] * fm, acc[5] * fm);
+   o.Bmag = acc[6] * fm;
+   if (GRAD) o.divU = dUx * fv + dUy * fv + dUz * fv;
+   o.zl_min = zl_min;
+   return true;
+}
+
 // ---- diffusion: DiffusionBase::GetComponent (src/diffusion_base.cc:105-121) + models ----------------
 // Kappa components 0 (perp) and 1 (para) at one point; r is |x| (KINRAD), Umag |U|.
 template <typename R>

@@ -13,7 +13,7 @@
 #define SPX_MAX_DIST 4    // distributions per simulation
 #define SPX_MAX_ACT 4     // one action per distribution
 
-enum spx_bg_kind { SPX_BG_NONE = 0, SPX_BG_UNIFORM = 1, SPX_BG_SOLARWIND = 2, SPX_BG_SHOCK = 3, SPX_BG_SMOOTHSHOCK = 4 };
+enum spx_bg_kind { SPX_BG_NONE = 0, SPX_BG_UNIFORM = 1, SPX_BG_SOLARWIND = 2, SPX_BG_SHOCK = 3, SPX_BG_SMOOTHSHOCK = 4, SPX_BG_GRID = 5 };
 enum spx_df_kind {
    SPX_DF_NONE = 0, SPX_DF_ISO, SPX_DF_PARA, SPX_DF_PERP, SPX_DF_FULL, SPX_DF_FLOWMOM, SPX_DF_KINRAD, SPX_DF_RIGBMAG, SPX_DF_PERPISO
 };
@@ -29,6 +29,20 @@ enum spx_src_kind { SPX_SRC_NONE = 0, SPX_SRC_MOMINJ = 1 };
 #define SPX_BND_CROSSED 0x0200
 #define SPX_BND_RECURRENT 0x0400
 
+// Uniform Cartesian grid of BackgroundServerCartesian (src/reader_cartesian.cc:25-110): blocks of
+// 4x4x4 cell-centred zones, stored in HBM as one zone array, x fastest, one record per zone of
+// U(3), B(3), |B| and a pad in fluid units (64 B in double, 32 B in float).
+#define SPX_GRID_BLOCK 4
+#define SPX_GRID_REC 8
+typedef struct {
+   int32_t nb[3], nz[3];                // blocks and zones per axis
+   double dom_min[3], dom_max[3];       // domain corners, fluid units (au)
+   double block_length[3];              // (dom_max - dom_min) / nb, fluid units
+   double f_len, f_vel, f_mag;          // Fluid / Particle units of length, velocity, magnetic field
+   const double* z64;                   // device zone records (double)
+   const float* z32;                    // the same records rounded to float
+} spx_grid_desc;
+
 // Background parameters in the step type T (double for the drop-in kernel, float for the mixed one).
 template <typename T>
 struct spx_bg_t {
@@ -42,6 +56,8 @@ struct spx_bg_t {
    T n_shock[3], v_shock, compression, u1[3], B1[3], E0[3], E1[3], width_shock;
    // BackgroundUniform (src/background_uniform.cc:58-64)
    T Eu[3];
+   // BackgroundServerCartesian (src/background_server_cartesian.cc, src/server_cartesian.cc)
+   spx_grid_desc grid;
 };
 typedef spx_bg_t<double> spx_bg_desc;
 

@@ -74,6 +74,8 @@ def lib() -> C.CDLL:
         L.spx_last_error.argtypes = [C.c_void_p]
         L.spx_last_error.restype = C.c_char_p
         L.spx_device_info.argtypes = [C.c_char_p, C.c_size_t]
+        L.spx_set_background_grid.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32, C.c_void_p]
+        L.spx_add_background_file.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, C.c_size_t, C.c_char_p]
         L.spx_autotune.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(SpxTuneEntry), C.c_int, C.POINTER(C.c_int)]
         _lib = L
     return _lib
@@ -126,6 +128,8 @@ class Simulation:
         sim = cls(setup.traj_type, setup.time_flow)
         for role, name, blob in setup.plugins:
             sim.add_plugin(role, name, blob)
+        if setup.grid is not None:
+            sim.set_background_grid(setup.grid)
         for k, v in {**setup.options, **options}.items():
             sim.set_option(k, v)
         return sim
@@ -152,7 +156,23 @@ class Simulation:
         if role == cf.DISTRIBUTION:
             self.n_dist += 1
 
-    def add_background(self, class_name, container): self.add_plugin(cf.BACKGROUND, class_name, container)
+    def add_background(self, class_name, container, fname_pattern: str = None) -> None:
+        """SimulationWorker::AddBackground (src/simulation.hh:111); a file pattern loads the grid of a
+        BackgroundServerCartesian from ``pattern``.info / ``pattern``.out."""
+        if fname_pattern is None:
+            self.add_plugin(cf.BACKGROUND, class_name, container)
+            return
+        blob = container.bytes() if isinstance(container, DataContainer) else bytes(container)
+        self._check(lib().spx_add_background_file(self.h, class_name.encode(), blob, len(blob), fname_pattern.encode()), class_name)
+
+    def set_background_grid(self, grid) -> None:
+        """The grid of a BackgroundServerCartesian from memory (a cartesian.CartesianGrid)."""
+        nb = (C.c_int32 * 3)(*grid.nb)
+        lo = (C.c_double * 3)(*grid.dom_min)
+        hi = (C.c_double * 3)(*grid.dom_max)
+        data = np.ascontiguousarray(grid.data, dtype=np.float64)
+        self._check(lib().spx_set_background_grid(self.h, nb, lo, hi, 6, data.ctypes.data), "set_background_grid")
+
     def add_diffusion(self, class_name, container): self.add_plugin(cf.DIFFUSION, class_name, container)
     def add_source(self, class_name, container): self.add_plugin(cf.SOURCE, class_name, container)
     def add_boundary(self, class_name, container): self.add_plugin(cf.BOUNDARY, class_name, container)
