@@ -30,6 +30,7 @@ sys.path.insert(0, ROOT)
 WORKLOAD = "parker_modulation_64x156250"
 N_ENERGIES, PER_ENERGY = 64, 156250
 PEAK_TFLOPS = {"fp64": 78.6, "mixed": 157.3}      # MI355X vector FP64 / FP32 peaks
+PEAK_HBM_GBPS = 8000.0                              # MI355X HBM3E
 PREC_ID = {"fp64": 0, "mixed": 1}
 
 
@@ -101,6 +102,41 @@ def other_workloads(E, cf, args, flops_all):
         res["speedup_mixed_vs_fp64"] = round(res["mixed"]["value"] / res["fp64"]["value"], 3)
         res["flops_per_step"] = round(flops_all[key]["flops_per_step"], 1)
         out[name] = res
+    return out
+
+
+def grid_workloads(E, cf, args, traffic_all):
+    """Parker modulation on a gridded solar wind (BackgroundServerCartesian), the reference's
+    Cartesian benchmark: its 25^3-block (100^3-zone, 64 MB of FP64 zone records) grid over +-100 au,
+    and a 64^3-block (256^3-zone, 1.07 GB) grid that no cache holds.  Each step interpolates the grid
+    at the new position and at the six derivative points of div(kappa): 7 gathers of 8 zones.  The
+    algorithmic gather is 8 zones x 6 values x 8 B (FP64) or 4 B (FP32) per interpolation; the
+    measured HBM traffic per launch comes from the FETCH_SIZE / WRITE_SIZE profile when present."""
+    from spectrum_amd import cartesian as G
+    out = {}
+    for nb in args.grid_blocks:
+        grid = G.parker_spiral(nb=nb)
+        setup = cf.parker_modulation_cartesian(grid)
+        name = f"parker_modulation_cartesian_{nb}^3_blocks"
+        res = {"zones": 4 * nb, "zone_records_bytes": {"fp64": 64 * (4 * nb) ** 3, "mixed": 32 * (4 * nb) ** 3}}
+        for p in (("fp64", "mixed") if args.precision == "both" else (args.precision,)):
+            leg = gpu_leg(E, cf, p, args, setup, 10_000_000)
+            per_launch_s = leg["step_ms"] * 1e-3 / args.steps
+            steps_per_launch = leg["steps"] / args.steps
+            gather = 7 * 8 * 6 * (8 if p == "fp64" else 4)
+            r = {"value": leg["steps"] / leg["wall"], "avg_step_kernel_ms": round(per_launch_s * 1e3, 3), "lanes": leg["lanes"],
+                 "gather_bytes_per_step": gather,
+                 "gather_GBps": round(gather * steps_per_launch / per_launch_s / 1e9, 1)}
+            t = (traffic_all or {}).get(name, {}).get(p)
+            if t:
+                r["hbm_bytes_per_launch"] = t["bytes_per_launch"]
+                r["hbm_GBps"] = round(t["bytes_per_launch"] / (t["avg_launch_ms"] * 1e-3) / 1e9, 1)
+                r["hbm_frac"] = round(r["hbm_GBps"] / PEAK_HBM_GBPS, 4)
+            res[p] = r
+        if "fp64" in res and "mixed" in res:
+            res["speedup_mixed_vs_fp64"] = round(res["mixed"]["value"] / res["fp64"]["value"], 3)
+        out[name] = res
+        del grid, setup
     return out
 
 
@@ -193,6 +229,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-other-workloads", action="store_true", help="skip the DSA and guiding-centre throughput legs")
+    ap.add_argument("--no-grid-workloads", action="store_true", help="skip the gridded-background (Cartesian) legs")
+    ap.add_argument("--grid-blocks", type=lambda v: [int(x) for x in v.split(",")], default=[25, 64],
+                    help="blocks per axis of the gridded legs (comma list; 25 is the reference's Cartesian benchmark)")
+    ap.add_argument("--grid-only", action="store_true",
+                    help="run only the gridded-background legs and print their results (for profiling their kernels)")
     ap.add_argument("--autotune", action="store_true",
                     help="autotune the modulation legs' launch shape first (the kernel trace then also holds the probe launches)")
     ap.add_argument("--serve-queries", type=int, default=40, help="served-stream leg: number of queries (0 = skip)")
@@ -222,6 +263,10 @@ def main():
     flops_all = load_json(os.path.join(ROOT, "profiles", "flops_per_step.json"))["workloads"]
     flops = flops_all[WORKLOAD]
     traffic = load_json(os.path.join(ROOT, "profiles", "traffic_parker_step_kernel.json"))
+    traffic_grid = load_json(os.path.join(ROOT, "profiles", "traffic_grid_step_kernel.json"))
+    if args.grid_only:
+        print(json.dumps({"grid_workloads": grid_workloads(E, cf, args, traffic_grid)}), flush=True)
+        return
 
     precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
     legs = {}
@@ -258,6 +303,8 @@ def main():
             out["speedup_mixed_vs_fp64"] = round(value / out["fp64"]["value"], 3)
     if not args.no_other_workloads:
         out["other_workloads"] = other_workloads(E, cf, args, flops_all)
+    if not args.no_grid_workloads:
+        out["grid_workloads"] = grid_workloads(E, cf, args, traffic_grid)
     if args.serve_queries > 0:
         out["serve"] = serve_leg(E, cf, args)
         out["p99_query_latency_s"] = out["serve"]["p99_latency_s"]

@@ -8,6 +8,7 @@ profiles/flops_per_step.json.  Floating-point operations are add/sub, mul, div a
 comparisons, negations, fabs/fmin/fmax are not flops.
 
     python oracle/flops_per_step.py            # all workloads
+    python oracle/flops_per_step.py NAME ...   # only these, merged into the existing file
 """
 from __future__ import annotations
 
@@ -24,6 +25,7 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, HERE)
 
 import oracle as O  # noqa: E402
+from spectrum_amd import cartesian as G  # noqa: E402
 from spectrum_amd import configs as cf  # noqa: E402
 
 FLOP = ("add", "mul", "div", "sqrt")
@@ -36,6 +38,7 @@ def workloads():
         ("parker_modulation_64x156250", cf.parker_modulation(n_energies=64), 64, 1),
         ("dsa_smooth_shock", cf.dsa_shock(), 32, 1),
         ("gc_diff_scatt", cf.gc_diff_scatt(), 16, 1),
+        ("parker_modulation_cartesian_25^3_blocks", cf.parker_modulation_cartesian(G.parker_spiral(nb=25)), 256, 1),
     ]
 
 
@@ -65,7 +68,13 @@ def main():
                             "calls (exp, log, pow, tanh, trig) are listed separately and not charged as flops",
               "command": "make -C oracle && python oracle/flops_per_step.py",
               "workloads": {}}
+    only = sys.argv[1:]
+    if only and os.path.exists(out_path):
+        with open(out_path) as f:
+            result["workloads"] = json.load(f)["workloads"]
     for name, setup, n, seed in workloads():
+        if only and name not in only:
+            continue
         r = count(setup, n, seed)
         print(name, json.dumps(r), flush=True)
         result["workloads"][name] = r
