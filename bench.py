@@ -109,9 +109,10 @@ def grid_workloads(E, cf, args, traffic_all):
     """Parker modulation on a gridded solar wind (BackgroundServerCartesian), the reference's
     Cartesian benchmark: its 25^3-block (100^3-zone, 64 MB of FP64 zone records) grid over +-100 au,
     and a 64^3-block (256^3-zone, 1.07 GB) grid that no cache holds.  Each step interpolates the grid
-    at the new position and at the six derivative points of div(kappa): 7 gathers of 8 zones.  The
-    algorithmic gather is 8 zones x 6 values x 8 B (FP64) or 4 B (FP32) per interpolation; the
-    measured HBM traffic per launch comes from the FETCH_SIZE / WRITE_SIZE profile when present."""
+    at the new position (kappa is isotropic here, so the six derivative points of div(kappa) need only
+    their positions checked): one gather of 8 zones.  The algorithmic gather is 8 zones x 6 values x
+    8 B (FP64) or 4 B (FP32) per interpolation; the measured HBM traffic per launch comes from the
+    FETCH_SIZE / WRITE_SIZE profile when present."""
     from spectrum_amd import cartesian as G
     out = {}
     for nb in args.grid_blocks:
@@ -123,7 +124,7 @@ def grid_workloads(E, cf, args, traffic_all):
             leg = gpu_leg(E, cf, p, args, setup, 10_000_000)
             per_launch_s = leg["step_ms"] * 1e-3 / args.steps
             steps_per_launch = leg["steps"] / args.steps
-            gather = 7 * 8 * 6 * (8 if p == "fp64" else 4)
+            gather = setup.meta["grid_gathers_per_step"] * 8 * 6 * (8 if p == "fp64" else 4)
             r = {"value": leg["steps"] / leg["wall"], "avg_step_kernel_ms": round(per_launch_s * 1e3, 3), "lanes": leg["lanes"],
                  "gather_bytes_per_step": gather,
                  "gather_GBps": round(gather * steps_per_launch / per_launch_s / 1e9, 1)}

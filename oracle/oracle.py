@@ -14,6 +14,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 FLOPS_LIB_PATH = os.path.join(HERE, "liboracle_flops.so")   # same source, operation-counting scalar
+FMA_LIB_PATH = os.path.join(HERE, "liboracle_fma.so")       # same source, multiply-adds contracted (last-bit perturbation)
 
 RNG_COUNTER, RNG_REPLAY = 0, 1
 EX_NAMES = {0: "none", 1: "field", 2: "step_too_small", 3: "step_nan", 4: "max_steps", 5: "replay_exhausted", 6: "boundary"}

@@ -266,13 +266,14 @@ def uniform_walk(D=2.0, U=(0.0, 0.0, 0.0), B=(3.0, 4.0, 0.0), t_limit=1000.0, dm
 
 
 def parker_modulation_cartesian(grid, n_bins=100, T1_MeV=10.0, T2_MeV=5000.0, inner_au=0.05, outer_au=80.0,
-                                t_limit_years=1.0, name="parker_modulation_cartesian") -> Setup:
+                                t_limit_years=1.0, kap_rat=1.0, name="parker_modulation_cartesian") -> Setup:
     """Backward Parker modulation on a gridded solar wind, benchmarks/main_test_modulation_cartesian_parker.cc:
     BackgroundServerCartesian (dmax 1 au) on `grid` (cartesian.parker_spiral() is the reference's
     25^3-block background), kappa = 1.5e22 cm^2/s (T / 1 GeV) (r / 1 au)^2 isotropic, log-uniform
     10 MeV - 5 GeV from 1 au, absorbing spheres at 0.05 au (action 1) and 80 au (the hot boundary,
     action 0), a 1-year limit (not binned), and a 100-bin spectrum with J0 T^-1.8 at the outer boundary.
-    The unused shock parameters of the diffusion model go in as ints, as the benchmark inserts them."""
+    The unused shock parameters of the diffusion model go in as ints, as the benchmark inserts them.
+    ``kap_rat`` != 1 makes kappa anisotropic (the benchmark's is isotropic)."""
     s = Setup(name, TRAJ_PARKER, BACKWARD, grid=grid)
     s.add(BACKGROUND, "BackgroundServerCartesian", background_base(DataContainer(), 0.0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), one_au))
     s.add(INITIAL, "InitialTimeFixed", DataContainer().insert_double(0.0))
@@ -283,7 +284,7 @@ def parker_modulation_cartesian(grid, n_bins=100, T1_MeV=10.0, T2_MeV=5000.0, in
     s.add(BOUNDARY, "BoundarySphereAbsorb", boundary(1, [0], ("vec", (0.0, 0.0, 0.0)), ("double", outer_au * one_au)))
     s.add(BOUNDARY, "BoundaryTimeExpire", boundary(1, [-1], ("double", -t_limit_years * one_year)))
     dif = DataContainer().insert_double(1.5e22 / unit_diffusion).insert_double(GeV).insert_double(one_au)
-    dif.insert_double(1.0).insert_double(2.0).insert_double(1.0).insert_int(0)
+    dif.insert_double(1.0).insert_double(2.0).insert_double(kap_rat).insert_int(0)
     dif.insert_int(1).insert_int(0).insert_int(1)
     s.add(DIFFUSION, "DiffusionKineticEnergyRadialDistancePowerLaw", dif)
     unit_distro = 1.0 / (unit_length ** 2 * unit_time * 4.0 * math.pi * unit_energy)
@@ -291,5 +292,7 @@ def parker_modulation_cartesian(grid, n_bins=100, T1_MeV=10.0, T2_MeV=5000.0, in
                             unit_distro, (unit_energy, 1.0, 1.0), False)
     dst.insert_double(1.0 / unit_distro).insert_double(GeV).insert_double(-1.8).insert_double(0.0)
     s.add(DISTRIBUTION, "DistributionSpectrumKineticEnergyPowerLaw", dst)
-    s.meta.update(T1=T1_MeV * MeV, T2=T2_MeV * MeV, n_bins=n_bins)
+    # grid interpolations per step: the new position, plus the six derivative points of div(kappa)
+    # unless kappa is isotropic and reads no field there (then only their positions are checked)
+    s.meta.update(T1=T1_MeV * MeV, T2=T2_MeV * MeV, n_bins=n_bins, grid_gathers_per_step=1 if kap_rat == 1.0 else 7)
     return s

@@ -399,8 +399,22 @@ static int bind_grid(spx_sim* s)
 {
    if (s->plan.bg.kind != SPX_BG_GRID) return SPX_OK;
    HIPCHK(s, s->grid->upload());
-   s->plan.bg.grid = s->grid->desc;
-   s->plan.bgf.grid = s->grid->desc;
+   spx_grid_desc g = s->grid->desc;
+   // TrajectoryParker::DiffusionCoeff (src/trajectory_parker.cc:100-119) evaluates the background at the
+   // six derivative points.  Their fields enter div(kappa) only through (kappa_par - kappa_perp) bhat
+   // and through kappa's own U or |B| dependence; when neither can matter, the only observable effects
+   // of those evaluations are the domain check and GetFields' |B| < sp_tiny check.  The interpolated
+   // |B| is a convex combination of zone magnitudes, so the latter cannot fire when the smallest zone
+   // |B| clears sp_tiny with margin.  Then the kernel checks the points' positions and loads nothing.
+   const spx_df_desc& d = s->plan.df;
+   const bool k_equal = ((d.kind == SPX_DF_FLOWMOM || d.kind == SPX_DF_KINRAD || d.kind == SPX_DF_RIGBMAG) && d.kap_rat == 1.0) ||
+                        (d.kind == SPX_DF_FULL && d.Dperp == d.Dpara);
+   const bool need_u = d.kind == SPX_DF_FLOWMOM || (d.kind == SPX_DF_KINRAD && d.stream_dep_idx != 0);
+   const bool need_b = d.kind == SPX_DF_RIGBMAG && d.pow_law_B != 0.0;
+   const bool bmag_safe = g.bmag_min * g.f_mag * 0.999 > sp_tiny_h;
+   g.fd_fields = (k_equal && !need_u && !need_b && bmag_safe) ? 0 : 1;
+   s->plan.bg.grid = g;
+   s->plan.bgf.grid = g;
    return SPX_OK;
 }
 
@@ -567,6 +581,9 @@ int spx_set_background_grid(spx_sim* s, const int32_t n_blocks[3], const double 
                      b2 = b2 + v[5] * v[5];
                      r[6] = std::sqrt(b2);
                   }
+   double bmin = INFINITY;
+   for (int64_t z = 0; z < nzone; z++) bmin = std::min(bmin, G->rec[(size_t)z * SPX_GRID_REC + 6]);
+   g.bmag_min = bmin;
    s->grid = G;
    return SPX_OK;
 }

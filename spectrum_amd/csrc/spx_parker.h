@@ -241,18 +241,23 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
       for (int j = 0; j < 3; j++) {
          const double ex = (j == 0) ? 1.0 : 0.0, ey = (j == 1) ? 1.0 : 0.0, ez = (j == 2) ? 1.0 : 0.0;
          double xf = H.x + (double)delta * ex, yf = H.y + (double)delta * ey, zf = H.z + (double)delta * ez;
+         // with fd_fields == 0 the fields at these points cannot change the result (the host has shown it):
+         // only the positions are checked, and bhat = 0 meets a zero (kappa_par - kappa_perp)
+         const bool loads = bg.grid.fd_fields != 0;
          GridFields<R> Gf, Gb;
-         if (!grid_eval<R, false>(bg.grid, xf, yf, zf, Gf)) return 1;
-         if (Gf.Bmag < R(sp_tiny)) return 1;
-         const v3<R> bf = is_f32<R>() ? (R(1) / Gf.Bmag) * Gf.B : Gf.B / Gf.Bmag;
+         Gf.U = Gb.U = Gf.B = Gb.B = mk<R>(R(0), R(0), R(0));
+         Gf.Bmag = Gb.Bmag = H.F.Bmag;
+         if (loads ? !grid_eval<R, false>(bg.grid, xf, yf, zf, Gf) : !grid_inside<R>(bg.grid, xf, yf, zf)) return 1;
+         if (loads && Gf.Bmag < R(sp_tiny)) return 1;
+         const v3<R> bf = !loads ? Gf.B : (is_f32<R>() ? (R(1) / Gf.Bmag) * Gf.B : Gf.B / Gf.Bmag);
          R K0f = K0, K1f = K1;
          if (pos_dep) parker_kappa<R, DF>(df, p, Gf.Bmag, xf, yf, zf, Gf.U, K0f, K1f);
          xf -= 2.0 * (double)delta * ex;
          yf -= 2.0 * (double)delta * ey;
          zf -= 2.0 * (double)delta * ez;
-         if (!grid_eval<R, false>(bg.grid, xf, yf, zf, Gb)) return 1;
-         if (Gb.Bmag < R(sp_tiny)) return 1;
-         const v3<R> bb = is_f32<R>() ? (R(1) / Gb.Bmag) * Gb.B : Gb.B / Gb.Bmag;
+         if (loads ? !grid_eval<R, false>(bg.grid, xf, yf, zf, Gb) : !grid_inside<R>(bg.grid, xf, yf, zf)) return 1;
+         if (loads && Gb.Bmag < R(sp_tiny)) return 1;
+         const v3<R> bb = !loads ? Gb.B : (is_f32<R>() ? (R(1) / Gb.Bmag) * Gb.B : Gb.B / Gb.Bmag);
          R K0b = K0, K1b = K1;
          if (pos_dep) parker_kappa<R, DF>(df, p, Gb.Bmag, xf, yf, zf, Gb.U, K0b, K1b);
          const R bfj = j == 0 ? bf.x : (j == 1 ? bf.y : bf.z);
@@ -270,12 +275,13 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
       for (int j = 0; j < 3; j++) {
          const double ex = (j == 0) ? 1.0 : 0.0, ey = (j == 1) ? 1.0 : 0.0, ez = (j == 2) ? 1.0 : 0.0;
          double xf = H.x + (double)delta * ex, yf = H.y + (double)delta * ey, zf = H.z + (double)delta * ez;
+         const bool loads = bg.grid.fd_fields != 0;
          GridFields<R> G;
-         if (!grid_eval<R, false>(bg.grid, xf, yf, zf, G) || G.Bmag < R(sp_tiny)) return 1;
+         if (loads ? (!grid_eval<R, false>(bg.grid, xf, yf, zf, G) || G.Bmag < R(sp_tiny)) : !grid_inside<R>(bg.grid, xf, yf, zf)) return 1;
          xf -= 2.0 * (double)delta * ex;
          yf -= 2.0 * (double)delta * ey;
          zf -= 2.0 * (double)delta * ez;
-         if (!grid_eval<R, false>(bg.grid, xf, yf, zf, G) || G.Bmag < R(sp_tiny)) return 1;
+         if (loads ? (!grid_eval<R, false>(bg.grid, xf, yf, zf, G) || G.Bmag < R(sp_tiny)) : !grid_inside<R>(bg.grid, xf, yf, zf)) return 1;
       }
    }
    else if (pos_dep || K0 != K1) {

@@ -221,13 +221,15 @@ template <> __device__ __forceinline__ void grid_rec<float>(const SPX_CP spx_gri
    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z;
 }
 
-template <typename R, bool GRAD>
-__device__ __forceinline__ bool grid_eval(const SPX_CP spx_grid_desc& g, double x, double y, double z, GridFields<R>& o)
+// The position's block, its stencil's first zone and the offsets within it.  False where the
+// reference fails (outside the domain, or a stencil beyond the grid).
+template <typename R>
+__device__ __forceinline__ bool grid_locate(const SPX_CP spx_grid_desc& g, double x, double y, double z, R lo[3], R hi[3], R dl[3],
+                                            int64_t& zone0, double& zl_min)
 {
    const double pos[3] = {x, y, z};
-   R lo[3], hi[3], dl[3];
-   int64_t zone0 = 0;
-   double zl_min = 0.0;
+   zone0 = 0;
+   zl_min = 0.0;
 #pragma unroll
    for (int a = 0; a < 3; a++) {
       const double pf = pos[a] / AU_CGS * unit_length;          // server side: position in fluid units
@@ -250,6 +252,25 @@ __device__ __forceinline__ bool grid_eval(const SPX_CP spx_grid_desc& g, double 
       zl_min = (a == 0) ? zl : fmin(zl_min, zl);
       zone0 += (a == 0) ? (int64_t)gz : (a == 1 ? (int64_t)gz * g.nz[0] : (int64_t)gz * g.nz[0] * g.nz[1]);
    }
+   return true;
+}
+
+template <typename R>
+__device__ __forceinline__ bool grid_inside(const SPX_CP spx_grid_desc& g, double x, double y, double z)
+{
+   R lo[3], hi[3], dl[3];
+   int64_t zone0;
+   double zl_min;
+   return grid_locate<R>(g, x, y, z, lo, hi, dl, zone0, zl_min);
+}
+
+template <typename R, bool GRAD>
+__device__ __forceinline__ bool grid_eval(const SPX_CP spx_grid_desc& g, double x, double y, double z, GridFields<R>& o)
+{
+   R lo[3], hi[3], dl[3];
+   int64_t zone0;
+   double zl_min;
+   if (!grid_locate<R>(g, x, y, z, lo, hi, dl, zone0, zl_min)) return false;
    const int64_t sy = g.nz[0], sz = (int64_t)g.nz[0] * g.nz[1];
    R acc[7], dUx = R(0), dUy = R(0), dUz = R(0);
 #pragma unroll

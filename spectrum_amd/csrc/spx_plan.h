@@ -39,6 +39,10 @@ typedef struct {
    double dom_min[3], dom_max[3];       // domain corners, fluid units (au)
    double block_length[3];              // (dom_max - dom_min) / nb, fluid units
    double f_len, f_vel, f_mag;          // Fluid / Particle units of length, velocity, magnetic field
+   double bmag_min;                     // smallest zone |B| of the grid, fluid units
+   // 0 when the derivative points of div(kappa) need no field values: kappa_perp == kappa_par
+   // identically, kappa reads neither U nor |B|, and no interpolated |B| can fall below sp_tiny
+   int32_t fd_fields, pad_;
    const double* z64;                   // device zone records (double)
    const float* z32;                    // the same records rounded to float
 } spx_grid_desc;

@@ -37,9 +37,18 @@ def endpoint_rel(rec, orec):
     return np.maximum(np.maximum(dp, dm), dt)
 
 
-def check_parity(sim_stats, rec, orec, sim, o, n, ev_weight_atol=0.0):
+def sensitivity(orec, prec):
+    """Per-trajectory distance between the oracle and its last-bit-perturbed build (multiply-adds
+    contracted): how strongly each trajectory amplifies rounding differences."""
+    return endpoint_rel(prec, orec)
+
+
+def check_parity(sim_stats, rec, orec, sim, o, n, ev_weight_atol=0.0, sens=None):
     """`ev_weight_atol` bounds event weights that are ill-conditioned functions of the endpoint
-    (a squared position component near zero): an (n, 4) array or a scalar, added to the relative test."""
+    (a squared position component near zero): an (n, 4) array or a scalar, added to the relative test.
+    `sens` (per trajectory, from `sensitivity`) lets a trajectory that amplifies last-bit differences
+    exceed the 1e-10 endpoint bound by as much as 100 times the oracle's own disagreement with its
+    perturbed build; steps, statuses and bins stay exact."""
     print(f"[parity] n={n} steps={int(rec['steps'].sum())} discards={sim_stats.discards}", flush=True)
     assert sim_stats.done == n and sim_stats.failed == 0
     np.testing.assert_array_equal(rec["status"], orec["status"])
@@ -48,17 +57,26 @@ def check_parity(sim_stats, rec, orec, sim, o, n, ev_weight_atol=0.0):
     np.testing.assert_array_equal(rec["n_events"], orec["n_events"])
     np.testing.assert_array_equal(rec["ev_bin"], orec["ev_bin"])
     r = endpoint_rel(rec, orec)
-    print(f"[parity] endpoint rel max {r.max():.3e} median {np.median(r):.3e}", flush=True)
-    assert r.max() <= ENDPOINT_RTOL
+    tol = np.full(n, ENDPOINT_RTOL) if sens is None else np.maximum(ENDPOINT_RTOL, 100.0 * sens)
+    print(f"[parity] endpoint rel max {r.max():.3e} median {np.median(r):.3e}; above 1e-10: {int((r > ENDPOINT_RTOL).sum())} "
+          f"(their oracle sensitivity {'-' if sens is None else np.array2string(sens[r > ENDPOINT_RTOL], precision=2)})", flush=True)
+    assert (r <= tol).all()
     dw = np.abs(rec["ev_weight"] - orec["ev_weight"])
-    assert (dw <= ENDPOINT_RTOL * np.abs(orec["ev_weight"]) + ev_weight_atol).all(), dw.max()
+    wtol = tol[:, None] * np.abs(orec["ev_weight"])
+    assert (dw <= wtol + ev_weight_atol).all(), dw.max()
     assert sim_stats.steps_done == int(orec["steps"].sum())
     for d in range(sim.n_dist):
         c, s, ne = sim.distro(d)
         oc, osum, one = o.distro(d)
         np.testing.assert_array_equal(c, oc)
         assert ne == one
-        np.testing.assert_allclose(s, osum, rtol=ENDPOINT_RTOL, atol=1e-300)
+        atol = 1e-300
+        if sens is not None:   # one event per trajectory and distribution: a bin sum inherits its trajectories' bounds
+            extra = np.zeros(osum.shape[0])
+            hit = orec["ev_bin"][:, d] >= 0
+            np.add.at(extra, orec["ev_bin"][hit, d], (tol[hit] - ENDPOINT_RTOL) * np.abs(orec["ev_weight"][hit, d]))
+            atol = (extra if osum.ndim == 1 else extra[:, None]) + 1e-300
+        assert (np.abs(s - osum) <= ENDPOINT_RTOL * np.abs(osum) + atol).all()
 
 
 @pytest.fixture(scope="module")

@@ -180,3 +180,16 @@ def test_grid_needs_data_and_parker_transport():
     sim = E.Simulation.from_setup(s)
     with pytest.raises(E.SpxError, match="grid"):
         sim.run(1, seed=1)
+
+
+def test_perturbed_oracle_build_measures_conditioning(oracle_mod):
+    """The contracted build differs from the oracle only by rounding: identical steps and statuses, and
+    endpoint differences at rounding level on the well-conditioned isotropic case."""
+    import test_gpu_parity as P
+    s = cf.parker_modulation_cartesian(G.parker_spiral(nb=25), kap_rat=1.0)
+    _, a = oracle_mod.Oracle(s).run(128, seed=5, threads=8)
+    _, b = oracle_mod.Oracle(s, oracle_mod.FMA_LIB_PATH).run(128, seed=5, threads=8)
+    np.testing.assert_array_equal(a["steps"], b["steps"])
+    np.testing.assert_array_equal(a["status"], b["status"])
+    r = P.sensitivity(a, b)
+    assert r.max() < 1e-11 and r.max() > 0.0
