@@ -109,11 +109,10 @@ __device__ __forceinline__ bool parker_fields(const SPX_CP spx_bg_t<R>& bg, doub
 // DiffusionBase::GetComponent for components 0 and 1 at a position.  DF is the diffusion class when it
 // is known at compile time (0 = any, dispatched at run time).
 template <typename R, int DF>
-__device__ __forceinline__ void parker_kappa(const SPX_CP spx_df_t<R>& df, R p, R Bmag, double x, double y, double z, v3<R> U, R& K0, R& K1)
+__device__ __forceinline__ void parker_kappa(const SPX_CP spx_df_t<R>& df, R km, R Bmag, double x, double y, double z, v3<R> U, R& K0, R& K1)
 {
    if (DF == SPX_DF_RIGBMAG) {   // src/diffusion_other.cc:804-809; pow(x, 0) == 1 exactly
-      const R vmag = VelP(p);
-      K1 = (df.lam0 * vmag / R(3)) * rpow(Rigidity(p) / df.R0, df.pow_law_R);
+      K1 = km;
       K1 = K1 * ((df.pow_law_B != R(0)) ? rpow(Bmag / df.B0, df.pow_law_B) : R(1));
       K0 = df.kap_rat * K1;
       return;
@@ -122,7 +121,7 @@ __device__ __forceinline__ void parker_kappa(const SPX_CP spx_df_t<R>& df, R p, 
    R r = R(0), Umag = R(0);
    if (df.kind == SPX_DF_KINRAD) r = R(norm(mk<double>(x, y, z)));
    if (df.kind == SPX_DF_FLOWMOM || (df.kind == SPX_DF_KINRAD && df.stream_dep_idx != 0)) Umag = norm(U);
-   kappa01(df, p, Bmag, r, Umag, K0, K1);
+   kappa01(df, km, Bmag, r, Umag, K0, K1);
 }
 
 // Momentum sample of the initial condition (src/initial_momentum.cc:283-410), Parker layout (p, 0, 0).
@@ -231,7 +230,8 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
    const R p = R(H.p);
    // DriftCoeff + DiffusionCoeff (src/trajectory_parker.cc:189-206, 83-153)
    R K0 = R(0), K1 = R(0);
-   parker_kappa<R, DF>(df, p, H.F.Bmag, H.x, H.y, H.z, H.F.U, K0, K1);
+   const R km = (DF == SPX_DF_FULL) ? R(0) : kappa_mom(df, p);
+   parker_kappa<R, DF>(df, km, H.F.Bmag, H.x, H.y, H.z, H.F.U, K0, K1);
    const R delta = fmin(LarmorRadius(p, H.F.Bmag), H.F.dmax);
    v3<R> divK = mk<R>(R(0), R(0), R(0));
    const bool pos_dep = (DF == SPX_DF_FULL) ? false : (df.pos_dep != 0);
@@ -251,7 +251,7 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
          if (loads && Gf.Bmag < R(sp_tiny)) return 1;
          const v3<R> bf = !loads ? Gf.B : (is_f32<R>() ? (R(1) / Gf.Bmag) * Gf.B : Gf.B / Gf.Bmag);
          R K0f = K0, K1f = K1;
-         if (pos_dep) parker_kappa<R, DF>(df, p, Gf.Bmag, xf, yf, zf, Gf.U, K0f, K1f);
+         if (pos_dep) parker_kappa<R, DF>(df, km, Gf.Bmag, xf, yf, zf, Gf.U, K0f, K1f);
          xf -= 2.0 * (double)delta * ex;
          yf -= 2.0 * (double)delta * ey;
          zf -= 2.0 * (double)delta * ez;
@@ -259,7 +259,7 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
          if (loads && Gb.Bmag < R(sp_tiny)) return 1;
          const v3<R> bb = !loads ? Gb.B : (is_f32<R>() ? (R(1) / Gb.Bmag) * Gb.B : Gb.B / Gb.Bmag);
          R K0b = K0, K1b = K1;
-         if (pos_dep) parker_kappa<R, DF>(df, p, Gb.Bmag, xf, yf, zf, Gb.U, K0b, K1b);
+         if (pos_dep) parker_kappa<R, DF>(df, km, Gb.Bmag, xf, yf, zf, Gb.U, K0b, K1b);
          const R bfj = j == 0 ? bf.x : (j == 1 ? bf.y : bf.z);
          const R bbj = j == 0 ? bb.x : (j == 1 ? bb.y : bb.z);
          const R cf = (K1f - K0f) * bfj, cb = (K1b - K0b) * bbj;
@@ -304,7 +304,7 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
          else { Bmf = norm(B2); bf = B2 / Bmf; }
          if (Bmf < R(sp_tiny)) return 1;
          R K0f = K0, K1f = K1;
-         if (pos_dep) parker_kappa<R, DF>(df, p, Bmf, xf, yf, zf, U2, K0f, K1f);
+         if (pos_dep) parker_kappa<R, DF>(df, km, Bmf, xf, yf, zf, U2, K0f, K1f);
          xf -= 2.0 * (double)delta * ex;
          yf -= 2.0 * (double)delta * ey;
          zf -= 2.0 * (double)delta * ez;
@@ -317,7 +317,7 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
          else { Bmb = norm(B2); bb = B2 / Bmb; }
          if (Bmb < R(sp_tiny)) return 1;
          R K0b = K0, K1b = K1;
-         if (pos_dep) parker_kappa<R, DF>(df, p, Bmb, xf, yf, zf, U2, K0b, K1b);
+         if (pos_dep) parker_kappa<R, DF>(df, km, Bmb, xf, yf, zf, U2, K0b, K1b);
          const R bfj = j == 0 ? bf.x : (j == 1 ? bf.y : bf.z);
          const R bbj = j == 0 ? bb.x : (j == 1 ? bb.y : bb.z);
          const R cf = (K1f - K0f) * bfj, cb = (K1b - K0b) * bbj;

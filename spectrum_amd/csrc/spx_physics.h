@@ -36,8 +36,23 @@ SPX_HD double rsqrt_(double x) { return sqrt(x); }
 SPX_HD float rsqrt_(float x) { return __builtin_sqrtf(x); }
 SPX_HD double rdiv(double a, double b) { return a / b; }
 SPX_HD float rdiv(float a, float b) { return a / b; }
-SPX_HD double rpow(double a, double b) { return pow(a, b); }
-SPX_HD float rpow(float a, float b) { return __powf(a, b); }
+// pow with the identities every pow meets exactly (x^1 = x, x^0 = 1 as C99 F.9.4.4 fixes), and in the
+// FP32 step also x^2 = x * x and x^(1/2) = sqrt(x), more accurate than the approximate powf.  The
+// exponents are plan constants, so these branches are uniform.
+SPX_HD double rpow(double a, double b)
+{
+   if (b == 1.0) return a;
+   if (b == 0.0) return 1.0;
+   return pow(a, b);
+}
+SPX_HD float rpow(float a, float b)
+{
+   if (b == 1.0f) return a;
+   if (b == 0.0f) return 1.0f;
+   if (b == 2.0f) return a * a;
+   if (b == 0.5f) return __builtin_sqrtf(a);
+   return __powf(a, b);
+}
 SPX_HD double rtanh(double a) { return tanh(a); }
 SPX_HD float rtanh(float a) { return tanhf(a); }
 
@@ -300,32 +315,42 @@ __device__ __forceinline__ bool grid_eval(const SPX_CP spx_grid_desc& g, double 
 }
 
 // ---- diffusion: DiffusionBase::GetComponent (src/diffusion_base.cc:105-121) + models ----------------
-// Kappa components 0 (perp) and 1 (para) at one point; r is |x| (KINRAD), Umag |U|.
+// The momentum-only factor of kappa_par: the same at every point of a step's derivative stencil, so
+// it is formed once per step (the same inputs give the same value).
 template <typename R>
-SPX_HD void kappa01(const SPX_CP spx_df_t<R>& d, R p, R Bmag, R r, R Umag, R& K0, R& K1)
+SPX_HD R kappa_mom(const SPX_CP spx_df_t<R>& d, R p)
+{
+   switch (d.kind) {
+   case SPX_DF_FLOWMOM: return rpow(p / d.p0, d.pow_law_p);                                   // :622-627
+   case SPX_DF_KINRAD: return d.kap0 * rpow(EnrKin(p) / d.T0, d.pow_law_T);                   // :713-723
+   case SPX_DF_RIGBMAG: return (d.lam0 * VelP(p) / R(3)) * rpow(Rigidity(p) / d.R0, d.pow_law_R);   // :804-809
+   default: return R(0);
+   }
+}
+
+// Kappa components 0 (perp) and 1 (para) at one point, in the reference's operation order; km is
+// kappa_mom, r is |x| (KINRAD), Umag |U|.
+template <typename R>
+SPX_HD void kappa01(const SPX_CP spx_df_t<R>& d, R km, R Bmag, R r, R Umag, R& K0, R& K1)
 {
    switch (d.kind) {
    case SPX_DF_PARA: K1 = d.D0; break;                                      // :381-385
    case SPX_DF_PERP: K0 = d.D0; break;                                      // :459-463
    case SPX_DF_FULL: K0 = d.Dperp; K1 = d.Dpara; break;                     // :538-543
    case SPX_DF_FLOWMOM:                                                      // :622-627
-      K1 = d.kap0 * rpow(Umag / d.U0, d.pow_law_U) * rpow(p / d.p0, d.pow_law_p);
+      K1 = d.kap0 * rpow(Umag / d.U0, d.pow_law_U) * km;
       K0 = d.kap_rat * K1;
       break;
-   case SPX_DF_KINRAD: {                                                     // :713-723
-      R kap_mom = d.kap0 * rpow(EnrKin(p) / d.T0, d.pow_law_T);
-      if (d.stream_dep_idx == 0 || r < d.r0) K1 = kap_mom * rpow(r / d.r0, d.pow_law_r);
-      else if (r < d.r0 + d.w_sh) K1 = kap_mom * sqr(Umag / d.u_up);
-      else K1 = kap_mom * d.dn_up_rat;
+   case SPX_DF_KINRAD:                                                       // :713-723
+      if (d.stream_dep_idx == 0 || r < d.r0) K1 = km * rpow(r / d.r0, d.pow_law_r);
+      else if (r < d.r0 + d.w_sh) K1 = km * sqr(Umag / d.u_up);
+      else K1 = km * d.dn_up_rat;
       K0 = d.kap_rat * K1;
       break;
-   }
-   case SPX_DF_RIGBMAG: {                                                    // :804-809
-      R vmag = VelP(p);
-      K1 = (d.lam0 * vmag / R(3)) * rpow(Rigidity(p) / d.R0, d.pow_law_R) * rpow(Bmag / d.B0, d.pow_law_B);
+   case SPX_DF_RIGBMAG:                                                      // :804-809
+      K1 = km * rpow(Bmag / d.B0, d.pow_law_B);
       K0 = d.kap_rat * K1;
       break;
-   }
    case SPX_DF_PERPISO: K0 = d.Dperp; break;
    default: break;
    }
