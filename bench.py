@@ -46,11 +46,12 @@ def load_json(path):
         return None
 
 
-def gpu_leg(E, cf, prec, args, setup=None, n_traj=N_ENERGIES * PER_ENERGY, autotune=False):
+def gpu_leg(E, cf, prec, args, setup=None, n_traj=N_ENERGIES * PER_ENERGY, autotune=False, lanes=0, steps_per_launch=0):
     s = setup if setup is not None else cf.parker_modulation(n_energies=N_ENERGIES)
-    opts = dict(precision=PREC_ID[prec], steps_per_launch=args.steps_per_launch)
-    if args.lanes:
-        opts["lanes"] = args.lanes
+    spl = steps_per_launch or args.steps_per_launch
+    opts = dict(precision=PREC_ID[prec], steps_per_launch=spl)
+    if lanes or args.lanes:
+        opts["lanes"] = lanes or args.lanes
     sim = E.Simulation.from_setup(s, **opts)
     tuned = None
     if autotune:
@@ -58,16 +59,19 @@ def gpu_leg(E, cf, prec, args, setup=None, n_traj=N_ENERGIES * PER_ENERGY, autot
         best = max(table, key=lambda e: e[2])
         tuned = {"lanes": best[0], "steps_per_launch": best[1], "candidates": len(table)}
         log(f"{s.name} {prec}: autotuned lanes {best[0]} steps/launch {best[1]} ({best[2]:.3e} steps/s in the probe)")
+        spl = best[1]
+    # a shorter launch gets proportionally more timed launches, so every leg times about as many steps per slot
+    launches = max(args.steps, (args.steps * args.steps_per_launch) // spl)
     sim.begin(n_traj, seed=1)
     w = sim.advance(args.warmup)
     t0 = time.perf_counter()
-    st = sim.advance(args.steps)
+    st = sim.advance(launches)
     wall = time.perf_counter() - t0
     steps = st.steps_executed - w.steps_executed
-    log(f"{s.name} {prec}: lanes {st.lanes} launches {args.steps} trajectory-steps {steps} wall {wall:.3f}s "
+    log(f"{s.name} {prec}: lanes {st.lanes} steps/launch {spl} launches {launches} trajectory-steps {steps} wall {wall:.3f}s "
         f"step-kernel {st.step_kernel_ms:.1f} ms service {st.service_kernel_ms:.1f} ms -> {steps / wall:.4e} steps/s")
     return dict(steps=steps, wall=wall, lanes=st.lanes, step_ms=st.step_kernel_ms, service_ms=st.service_kernel_ms,
-                done=st.done, failed=st.failed, tuned=tuned)
+                done=st.done, failed=st.failed, tuned=tuned, launches=launches, steps_per_launch=spl)
 
 
 def roofline(leg, prec, flops, traffic):
@@ -93,7 +97,6 @@ def other_workloads(E, cf, args, flops_all):
         res = {}
         for p in ("fp64", "mixed"):
             leg = gpu_leg(E, cf, p, args, setup, n, autotune=tune)
-            leg["launches"] = args.steps
             rf = roofline(leg, p, flops_all[key], None)
             res[p] = {"value": leg["steps"] / leg["wall"], "avg_step_kernel_ms": rf["avg_launch_ms"], "roofline_frac": rf["frac"],
                       "achieved_tflops": rf["achieved"], "lanes": leg["lanes"]}
@@ -103,6 +106,14 @@ def other_workloads(E, cf, args, flops_all):
         res["flops_per_step"] = round(flops_all[key]["flops_per_step"], 1)
         out[name] = res
     return out
+
+
+def grid_lanes(E, args):
+    """Resident slots of the gridded legs: --grid-waves waves per SIMD on every CU (0 = engine default)."""
+    if not args.grid_waves:
+        return 0
+    cus = int(E.device_info().split("cu=")[1].split("|")[0])
+    return cus * 4 * 64 * args.grid_waves
 
 
 def grid_workloads(E, cf, args, traffic_all):
@@ -121,11 +132,13 @@ def grid_workloads(E, cf, args, traffic_all):
         name = f"parker_modulation_cartesian_{nb}^3_blocks"
         res = {"zones": 4 * nb, "zone_records_bytes": {"fp64": 64 * (4 * nb) ** 3, "mixed": 32 * (4 * nb) ** 3}}
         for p in (("fp64", "mixed") if args.precision == "both" else (args.precision,)):
-            leg = gpu_leg(E, cf, p, args, setup, 10_000_000)
-            per_launch_s = leg["step_ms"] * 1e-3 / args.steps
-            steps_per_launch = leg["steps"] / args.steps
+            leg = gpu_leg(E, cf, p, args, setup, 10_000_000, autotune=args.autotune and not args.lanes,
+                          lanes=grid_lanes(E, args), steps_per_launch=args.grid_steps_per_launch)
+            per_launch_s = leg["step_ms"] * 1e-3 / leg["launches"]
+            steps_per_launch = leg["steps"] / leg["launches"]
             gather = setup.meta["grid_gathers_per_step"] * 8 * 6 * (8 if p == "fp64" else 4)
             r = {"value": leg["steps"] / leg["wall"], "avg_step_kernel_ms": round(per_launch_s * 1e3, 3), "lanes": leg["lanes"],
+                 "steps_per_launch": leg["steps_per_launch"], "timed_launches": leg["launches"], "autotuned": leg["tuned"],
                  "gather_bytes_per_step": gather,
                  "gather_GBps": round(gather * steps_per_launch / per_launch_s / 1e9, 1)}
             t = (traffic_all or {}).get(name, {}).get(p)
@@ -233,6 +246,10 @@ def main():
     ap.add_argument("--no-grid-workloads", action="store_true", help="skip the gridded-background (Cartesian) legs")
     ap.add_argument("--grid-blocks", type=lambda v: [int(x) for x in v.split(",")], default=[25, 64],
                     help="blocks per axis of the gridded legs (comma list; 25 is the reference's Cartesian benchmark)")
+    ap.add_argument("--grid-waves", type=int, default=8,
+                    help="waves per SIMD of the gridded legs' slots (the autotuner's choice for them; 0 = engine default)")
+    ap.add_argument("--grid-steps-per-launch", type=int, default=250,
+                    help="steps per slot per launch of the gridded legs (the autotuner's choice for them)")
     ap.add_argument("--grid-only", action="store_true",
                     help="run only the gridded-background legs and print their results (for profiling their kernels)")
     ap.add_argument("--autotune", action="store_true",
@@ -273,7 +290,6 @@ def main():
     legs = {}
     for p in precs:
         legs[p] = gpu_leg(E, cf, p, args, autotune=args.autotune and not args.lanes)
-        legs[p]["launches"] = args.steps
     main_prec = "mixed" if "mixed" in legs else "fp64"
     m = legs[main_prec]
     value = m["steps"] / m["wall"]
@@ -282,9 +298,9 @@ def main():
         "value": value,
         "unit": "trajectory-steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": m["launches"],
         "warmup": args.warmup,
-        "ms_per_step": m["wall"] * 1e3 / args.steps,
+        "ms_per_step": m["wall"] * 1e3 / m["launches"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -292,13 +308,13 @@ def main():
         "data": "synthetic (counter-based Philox increments, seed 1)",
         "config": {"workload": WORKLOAD, "n_traj": N_ENERGIES * PER_ENERGY, "energies_GeV": [0.1, 10.0], "transport": "Parker backward",
                    "background": "SolarWind (Parker spiral)", "diffusion": "RigidityMagneticFieldPowerLaw",
-                   "lanes": m["lanes"], "steps_per_launch": (m["tuned"] or {}).get("steps_per_launch", args.steps_per_launch),
+                   "lanes": m["lanes"], "steps_per_launch": m["steps_per_launch"],
                    "autotuned": bool(m["tuned"]), "precision": main_prec},
         "roofline": roofline(m, main_prec, flops, traffic),
     }
     if "fp64" in legs:
         f = legs["fp64"]
-        out["fp64"] = {"value": f["steps"] / f["wall"], "ms_per_step": f["wall"] * 1e3 / args.steps,
+        out["fp64"] = {"value": f["steps"] / f["wall"], "ms_per_step": f["wall"] * 1e3 / f["launches"],
                        "roofline": roofline(f, "fp64", flops, traffic)}
         if main_prec == "mixed":
             out["speedup_mixed_vs_fp64"] = round(value / out["fp64"]["value"], 3)

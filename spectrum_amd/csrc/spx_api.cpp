@@ -559,6 +559,13 @@ int spx_set_background_grid(spx_sim* s, const int32_t n_blocks[3], const double 
       nzone *= g.nz[a];
    }
    grid_units(g);
+   for (int a = 0; a < 3; a++) {
+      const double zl = g.block_length[a] / SPX_GRID_BLOCK;
+      const double lo = (g.dom_min[a] + 0.5 * zl) * g.f_len, hi = (g.dom_max[a] - 0.5 * zl) * g.f_len;
+      const double margin = 1e-9 * (std::fabs(lo) + std::fabs(hi) + (hi - lo));
+      g.inner_lo[a] = lo + margin;
+      g.inner_hi[a] = hi - margin;
+   }
    if (nzone > ((int64_t)1 << 34) / SPX_GRID_REC) return fail(s, SPX_ERR_LIMIT, "grid too large");
    G->rec.assign((size_t)nzone * SPX_GRID_REC, 0.0);
    // ReadCartesianData's reordering (src/reader_cartesian.cc:103-140): blocks x fastest, zones x fastest

@@ -273,6 +273,9 @@ __device__ __forceinline__ bool grid_locate(const SPX_CP spx_grid_desc& g, doubl
 template <typename R>
 __device__ __forceinline__ bool grid_inside(const SPX_CP spx_grid_desc& g, double x, double y, double z)
 {
+   // well inside the zone-centre box the answer is yes without the exact block arithmetic
+   if (x > g.inner_lo[0] && x < g.inner_hi[0] && y > g.inner_lo[1] && y < g.inner_hi[1] && z > g.inner_lo[2] && z < g.inner_hi[2])
+      return true;
    R lo[3], hi[3], dl[3];
    int64_t zone0;
    double zl_min;

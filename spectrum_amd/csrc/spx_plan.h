@@ -40,6 +40,9 @@ typedef struct {
    double block_length[3];              // (dom_max - dom_min) / nb, fluid units
    double f_len, f_vel, f_mag;          // Fluid / Particle units of length, velocity, magnetic field
    double bmag_min;                     // smallest zone |B| of the grid, fluid units
+   // a box in code units, a relative 1e-9 inside the first and last zone centres: a position in it has
+   // a stencil within the grid, whatever the rounding of the exact block arithmetic
+   double inner_lo[3], inner_hi[3];
    // 0 when the derivative points of div(kappa) need no field values: kappa_perp == kappa_par
    // identically, kappa reads neither U nor |B|, and no interpolated |B| can fall below sp_tiny
    int32_t fd_fields, pad_;

@@ -91,8 +91,9 @@ def test_continuous_batching_short_query_overtakes_long_one():
     long_q = cf.parker_modulation(n_energies=16, r_start_au=1.0, t_limit_years=40.0 / 365.0, time_action=0, name="long")
     short_q = cf.parker_modulation(n_energies=4, r_start_au=10.0, t_limit_years=0.5 / 365.0, time_action=0, name="short")
     with Server(precision=E.PREC_MIXED) as srv:
-        a = srv.submit(long_q, 200000, seed=1)
+        a = srv.submit(long_q, 2000000, seed=1)
         time.sleep(0.3)
+        assert srv.info(a)["state"] == "running"      # the short query arrives while the long one runs
         b = srv.submit(short_q, 2000, seed=2)
         assert srv.wait(b, timeout=120.0)
         ib = srv.info(b)
