@@ -388,7 +388,7 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
 }
 
 template <typename R, int BG, int FWD, int TT, int DF>
-__global__ __launch_bounds__(BLOCK) void parker_step_kernel(const Launch* __restrict__ Lg)
+__device__ __forceinline__ void parker_step_body(const Launch* __restrict__ Lg)
 {
    const LaunchPtr Lp = (LaunchPtr)Lg;
    __shared__ ColdLDS C;
@@ -462,6 +462,28 @@ __global__ __launch_bounds__(BLOCK) void parker_step_kernel(const Launch* __rest
    }
 }
 
+template <typename R, int BG, int FWD, int TT, int DF>
+__global__ __launch_bounds__(BLOCK) void parker_step_kernel(const Launch* __restrict__ Lg)
+{
+   parker_step_body<R, BG, FWD, TT, DF>(Lg);
+}
+
+// The gridded background's step kernel waits on gathers: it asks the compiler for room for
+// SPX_GRID_WAVES resident waves per SIMD (0: the compiler's own register budget).
+#ifndef SPX_GRID_WAVES
+#define SPX_GRID_WAVES 0
+#endif
+#if SPX_GRID_WAVES > 0
+#define SPX_GRID_ATTR __attribute__((amdgpu_waves_per_eu(SPX_GRID_WAVES)))
+#else
+#define SPX_GRID_ATTR
+#endif
+template <typename R, int FWD, int TT>
+__global__ __launch_bounds__(BLOCK) SPX_GRID_ATTR void parker_grid_step_kernel(const Launch* __restrict__ Lg)
+{
+   parker_step_body<R, SPX_BG_GRID, FWD, TT, 0>(Lg);
+}
+
 // ---------------------------------------------------------------------------------------------
 // Dispatch: one engine launch = service kernel + step kernel on the same stream
 // ---------------------------------------------------------------------------------------------
@@ -474,7 +496,8 @@ static hipError_t launch_parker(uint64_t n_lanes, const Launch* d_launch, hipStr
       hipError_t e = hipEventRecord(mid, st);
       if (e != hipSuccess) return e;
    }
-   hipLaunchKernelGGL((parker_step_kernel<R, BG, FWD, TT, DF>), dim3(grid), dim3(BLOCK), 0, st, d_launch);
+   if constexpr (BG == SPX_BG_GRID) hipLaunchKernelGGL((parker_grid_step_kernel<R, FWD, TT>), dim3(grid), dim3(BLOCK), 0, st, d_launch);
+   else hipLaunchKernelGGL((parker_step_kernel<R, BG, FWD, TT, DF>), dim3(grid), dim3(BLOCK), 0, st, d_launch);
    return hipGetLastError();
 }
 

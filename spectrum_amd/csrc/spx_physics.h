@@ -290,22 +290,34 @@ __device__ __forceinline__ bool grid_eval(const SPX_CP spx_grid_desc& g, double 
    double zl_min;
    if (!grid_locate<R>(g, x, y, z, lo, hi, dl, zone0, zl_min)) return false;
    const int64_t sy = g.nz[0], sz = (int64_t)g.nz[0] * g.nz[1];
+   // corners in the reference's order c = i + 2j + 4k, one row (j, k) of two x-neighbours at a time: the
+   // sums start from zero as the reference's do, and a rolled loop keeps two records in flight instead
+   // of eight, which leaves registers for more resident waves to hide the gather latency
    R acc[7], dUx = R(0), dUy = R(0), dUz = R(0);
 #pragma unroll
-   for (int c = 0; c < 8; c++) {
-      const int i = c & 1, j = (c >> 1) & 1, k = c >> 2;
-      R v[7];
-      grid_rec<R>(g, zone0 + i + j * sy + k * sz, v);
-      const R fx = i ? lo[0] : hi[0], fy = j ? lo[1] : hi[1], fz = k ? lo[2] : hi[2];
-      const R w = fx * fy * fz;
+   for (int m = 0; m < 7; m++) acc[m] = R(0);
+#pragma unroll 1
+   for (int r = 0; r < 4; r++) {
+      const int j = r & 1, k = r >> 1;
+      const int64_t zr = zone0 + j * sy + k * sz;
+      R v0[7], v1[7];
+      grid_rec<R>(g, zr, v0);
+      grid_rec<R>(g, zr + 1, v1);
+      const R fy = j ? lo[1] : hi[1], fz = k ? lo[2] : hi[2];
 #pragma unroll
-      for (int m = 0; m < 7; m++) acc[m] = (c == 0) ? w * v[m] : acc[m] + w * v[m];
-      if (GRAD) {   // the diagonal of grad U: d(Ux)/dx, d(Uy)/dy, d(Uz)/dz
-         const R sx = i ? R(1) : R(-1), sy_ = j ? R(1) : R(-1), sz_ = k ? R(1) : R(-1);
-         const R ddx = sx * (fy * fz) / dl[0], ddy = sy_ * (fx * fz) / dl[1], ddz = sz_ * (fx * fy) / dl[2];
-         dUx = (c == 0) ? ddx * v[0] : dUx + ddx * v[0];
-         dUy = (c == 0) ? ddy * v[1] : dUy + ddy * v[1];
-         dUz = (c == 0) ? ddz * v[2] : dUz + ddz * v[2];
+      for (int i = 0; i < 2; i++) {
+         const R* v = i ? v1 : v0;
+         const R fx = i ? lo[0] : hi[0];
+         const R w = fx * fy * fz;
+#pragma unroll
+         for (int m = 0; m < 7; m++) acc[m] = acc[m] + w * v[m];
+         if (GRAD) {   // the diagonal of grad U: d(Ux)/dx, d(Uy)/dy, d(Uz)/dz
+            const R sx = i ? R(1) : R(-1), sy_ = j ? R(1) : R(-1), sz_ = k ? R(1) : R(-1);
+            const R ddx = sx * (fy * fz) / dl[0], ddy = sy_ * (fx * fz) / dl[1], ddz = sz_ * (fx * fy) / dl[2];
+            dUx = dUx + ddx * v[0];
+            dUy = dUy + ddy * v[1];
+            dUz = dUz + ddz * v[2];
+         }
       }
    }
    const R fv = R(g.f_vel), fm = R(g.f_mag);
