@@ -141,6 +141,22 @@ int spx_get_distro(const spx_sim* sim, int d, int64_t* counts, double* sums, int
 /* DistributionTemplated::ResetDistribution (src/distribution_templated.cc:212-235) for every distribution. */
 int spx_reset_distros(spx_sim* sim);
 
+/* Distribution files, byte for byte the reference's formats:
+   spx_dump_distro is DistributionTemplated::Dump (src/distribution_templated.cc:394-422), the binary
+   file the reference writes per distribution at the end of MainLoop (src/simulation.cc:790-793): class
+   name, the configuration DataContainer, int event count, int counts and the bin sums (per-event
+   records are not kept: both record counts are 0).  spx_restore_distro is Restore (:430-467), the
+   checkpoint/resume path (SimulationWorker::RestoreDistro, src/simulation.cc:323): the file's counts,
+   sums and event count replace distribution d's, and later runs add to them; the class and the number
+   of bins must match.  spx_print_distro1d / 2d are Print1D / Print2D (:546-632) behind
+   SimulationMaster::PrintDistro1D / 2D (src/simulation.cc:856-874): one text row per bin of dimension
+   ijk (or per pair ijk1, ijk2), the other dimensions summed, with bin centre, mean, sum and count,
+   in physical units when phys_units != 0. */
+int spx_dump_distro(const spx_sim* sim, int d, const char* file_name);
+int spx_restore_distro(spx_sim* sim, int d, const char* file_name);
+int spx_print_distro1d(const spx_sim* sim, int d, int ijk, const char* file_name, int phys_units);
+int spx_print_distro2d(const spx_sim* sim, int d, int ijk1, int ijk2, const char* file_name, int phys_units);
+
 const char* spx_last_error(const spx_sim* sim);
 /* Name, compute units and architecture of device 0 ("" and SPX_ERR_DEVICE without a GPU). */
 int spx_device_info(char* buf, size_t len);
@@ -159,7 +175,7 @@ enum { SPX_Q_QUEUED = 0, SPX_Q_RUNNING = 1, SPX_Q_DONE = 2, SPX_Q_FAILED = 3 };
 typedef struct {
    int32_t precision;          /* SPX_PREC_* of every query */
    int32_t steps_per_launch;   /* steps per slot per engine launch (0: 500) */
-   int64_t lanes;              /* resident slots per kernel pool (0: 4 waves per SIMD on every CU) */
+   int64_t lanes;              /* resident slots per kernel pool (0: 8 waves per SIMD on every CU) */
    double hbm_budget_bytes;    /* device memory the admitted queries may hold (0: 4 GiB) */
    int32_t max_queries;        /* queries admitted at once per pool (0: 64) */
    int32_t launches_per_poll;  /* engine launches between completion checks (0: 1) */

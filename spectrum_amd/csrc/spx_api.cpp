@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <condition_variable>
 #include <cstdio>
@@ -147,6 +148,16 @@ struct GridData {
    ~GridData() { d64.release(); d32.release(); }
 };
 
+// What the distribution file formats need beyond the plan: the class name, the configuration bytes
+// (DistributionBase::Dump writes them), and the unit factors of Print1D / Print2D.
+struct DistMeta {
+   std::string cls;
+   std::vector<uint8_t> container;
+   uint32_t n_records = 0;
+   double unit_distro[9] = {1, 1, 1, 1, 1, 1, 1, 1, 1};
+   double unit_val[3] = {1, 1, 1};
+};
+
 }  // namespace
 
 struct spx_sim {
@@ -167,6 +178,7 @@ struct spx_sim {
    DevBuf<unsigned long long> hist_counts, hist_events;
    DevBuf<double> hist_sums;
    std::shared_ptr<GridData> grid;     // BackgroundServerCartesian data
+   std::vector<DistMeta> dmeta;        // per distribution, for the file formats
    // run state
    bool running = false;
    uint64_t run_first = 0, run_n = 0;
@@ -345,9 +357,12 @@ static int add_distribution(spx_sim* s, const std::string& cls, Reader& r)
    else if (cls == "DistributionPositionCumulativeOrder2") width = 9;
    // DistributionTemplated::SetupDistribution (src/distribution_templated.cc:92-139)
    double minval[3], maxval[3], unit_val[3];
+   DistMeta M;
    r.mi(D.n_bins); r.v(minval); r.v(maxval); r.mi(D.log_bins); r.mi(D.bin_outside);
-   r.skip();   // unit_distro
+   // unit_distro has the distribution's value type: 1, 3 or 9 doubles (DistributionTemplated::ReadUnitDistro)
+   r.raw(M.unit_distro, sizeof(double) * width);
    r.v(unit_val); (void)r.b();
+   for (int k = 0; k < 3; k++) M.unit_val[k] = unit_val[k];
    D.dims = 0;
    for (int k = 0; k < 3; k++) {
       if (D.n_bins[k] > 0) D.dims |= (1 << k);
@@ -378,6 +393,8 @@ static int add_distribution(spx_sim* s, const std::string& cls, Reader& r)
    P.hist_off[P.n_dist] = s->hist_bins;
    s->hist_bins += D.nb;
    P.n_dist++;
+   M.cls = cls;
+   s->dmeta.push_back(M);
    return SPX_OK;
 }
 
@@ -531,7 +548,19 @@ int spx_add_plugin(spx_sim* s, int role, const char* class_name, const uint8_t* 
       break;
    case SPX_BOUNDARY: rc = add_boundary(s, cls, r); break;
    case SPX_INITIAL: rc = add_initial(s, cls, r); break;
-   case SPX_DISTRIBUTION: rc = add_distribution(s, cls, r); break;
+   case SPX_DISTRIBUTION:
+      rc = add_distribution(s, cls, r);
+      if (rc == SPX_OK && r.err) {   // a malformed container: take the distribution back out
+         s->plan.n_dist--;
+         s->hist_bins -= s->plan.dist[s->plan.n_dist].nb;
+         s->dmeta.pop_back();
+      }
+      else if (rc == SPX_OK) {
+         DistMeta& M = s->dmeta.back();
+         M.container.assign(container, container + len);
+         for (size_t i = 0; i < len; i += 1 + container[i]) M.n_records++;   // DataContainer::size()
+      }
+      break;
    default: return fail(s, SPX_ERR_ARG, "unknown role");
    }
    if (rc == SPX_OK && r.err) return fail(s, SPX_ERR_CONTAINER, cls + ": container does not match the class parameters");
@@ -862,6 +891,202 @@ int spx_reset_distros(spx_sim* s)
    HIPCHK(s, hipMemsetAsync(s->hist_events.p, 0, sizeof(unsigned long long) * SPX_MAX_DIST, s->pool.stream));
    HIPCHK(s, hipStreamSynchronize(s->pool.stream));
    return SPX_OK;
+}
+
+// ---- distribution files (src/distribution_templated.cc:394-632) --------------------------------------
+static int ensure_hist(spx_sim* s)
+{
+   if (s->hist_counts.p) return SPX_OK;
+   int rc = ensure_device(s);
+   if (rc) return rc;
+   if (!s->pool.stream) HIPCHK(s, hipStreamCreate(&s->pool.stream));
+   const size_t nb = (size_t)std::max<int64_t>(s->hist_bins, 1);
+   HIPCHK(s, s->hist_counts.alloc(nb));
+   HIPCHK(s, s->hist_sums.alloc(nb * 9));
+   HIPCHK(s, s->hist_events.alloc(SPX_MAX_DIST));
+   HIPCHK(s, hipMemsetAsync(s->hist_counts.p, 0, sizeof(unsigned long long) * nb, s->pool.stream));
+   HIPCHK(s, hipMemsetAsync(s->hist_sums.p, 0, sizeof(double) * nb * 9, s->pool.stream));
+   HIPCHK(s, hipMemsetAsync(s->hist_events.p, 0, sizeof(unsigned long long) * SPX_MAX_DIST, s->pool.stream));
+   HIPCHK(s, hipStreamSynchronize(s->pool.stream));
+   return SPX_OK;
+}
+
+int spx_dump_distro(const spx_sim* sc, int d, const char* file_name)
+{
+   spx_sim* s = const_cast<spx_sim*>(sc);
+   if (!s || !file_name || d < 0 || d >= s->plan.n_dist) return SPX_ERR_ARG;
+   const spx_ds_desc& D = s->plan.dist[d];
+   const DistMeta& M = s->dmeta[d];
+   std::vector<int64_t> c((size_t)D.nb);
+   std::vector<double> sm((size_t)D.nb * D.width);
+   int64_t nev = 0;
+   int rc = spx_get_distro(s, d, c.data(), sm.data(), &nev);
+   if (rc) return rc;
+   // the reference keeps int counts and an int event count
+   if (nev > INT32_MAX) return fail(s, SPX_ERR_LIMIT, "event count exceeds the file format's 32-bit integer");
+   std::vector<int32_t> c32((size_t)D.nb);
+   for (size_t b = 0; b < c.size(); b++) {
+      if (c[b] > INT32_MAX) return fail(s, SPX_ERR_LIMIT, "bin count exceeds the file format's 32-bit integer");
+      c32[b] = (int32_t)c[b];
+   }
+   std::ofstream f(file_name, std::ios::binary);
+   if (!f) return fail(s, SPX_ERR_ARG, std::string("cannot write ") + file_name);
+   const uint32_t name_len = (uint32_t)M.cls.size(), size = M.n_records, length = (uint32_t)M.container.size(), zero = 0;
+   const int32_t nev32 = (int32_t)nev;
+   const uint8_t specie = 0;   // sizeof(Specie<...>) == 1: the empty specie object is one byte
+   f.write((const char*)&name_len, 4);
+   f.write(M.cls.data(), name_len);
+   f.write((const char*)&specie, 1);
+   f.write((const char*)&size, 4);
+   f.write((const char*)&length, 4);
+   f.write((const char*)M.container.data(), length);
+   f.write((const char*)&nev32, 4);
+   f.write((const char*)c32.data(), sizeof(int32_t) * c32.size());
+   f.write((const char*)sm.data(), sizeof(double) * sm.size());
+   f.write((const char*)&zero, 4);   // values_record: per-event records are not kept
+   f.write((const char*)&zero, 4);   // weights_record
+   if (!f) return fail(s, SPX_ERR_ARG, std::string("write failed: ") + file_name);
+   return SPX_OK;
+}
+
+int spx_restore_distro(spx_sim* s, int d, const char* file_name)
+{
+   if (!s || !file_name || d < 0 || d >= s->plan.n_dist) return SPX_ERR_ARG;
+   if (s->running) return fail(s, SPX_ERR_STATE, "restore during a run");
+   const spx_ds_desc& D = s->plan.dist[d];
+   const DistMeta& M = s->dmeta[d];
+   std::ifstream f(file_name, std::ios::binary);
+   if (!f) return fail(s, SPX_ERR_ARG, std::string("cannot read ") + file_name);
+   uint32_t name_len = 0, size = 0, length = 0;
+   f.read((char*)&name_len, 4);
+   if (!f || name_len > 4096) return fail(s, SPX_ERR_ARG, "not a distribution file");
+   std::string cls(name_len, '\0');
+   f.read(&cls[0], name_len);
+   // DistributionTemplated::Restore leaves the distribution alone when the class differs
+   if (cls != M.cls) return fail(s, SPX_ERR_ARG, "distribution file holds " + cls + ", not " + M.cls);
+   uint8_t specie = 0;
+   f.read((char*)&specie, 1);
+   f.read((char*)&size, 4);
+   f.read((char*)&length, 4);
+   if (!f || length > (1u << 20)) return fail(s, SPX_ERR_ARG, "bad container in distribution file");
+   std::vector<uint8_t> cont(length);
+   f.read((char*)cont.data(), length);
+   // the binning must be this distribution's (the reference would re-read its setup from the file)
+   Reader r{cont.data(), cont.data() + cont.size()};
+   int32_t nb3[3];
+   r.mi(nb3);
+   int64_t nb = 1;
+   for (int k = 0; k < 3; k++) nb *= std::max(nb3[k], 1);
+   if (r.err || nb != D.nb) return fail(s, SPX_ERR_ARG, "distribution file has a different binning");
+   int32_t nev32 = 0;
+   std::vector<int32_t> c32((size_t)D.nb);
+   std::vector<double> sm((size_t)D.nb * D.width);
+   f.read((char*)&nev32, 4);
+   f.read((char*)c32.data(), sizeof(int32_t) * c32.size());
+   f.read((char*)sm.data(), sizeof(double) * sm.size());
+   if (!f) return fail(s, SPX_ERR_ARG, "distribution file is short");
+   int rc = ensure_hist(s);
+   if (rc) return rc;
+   std::vector<unsigned long long> c64((size_t)D.nb);
+   std::vector<double> s9((size_t)D.nb * 9, 0.0);
+   for (int64_t b = 0; b < D.nb; b++) {
+      c64[b] = (unsigned long long)std::max(c32[b], 0);
+      for (int k = 0; k < D.width; k++) s9[b * 9 + k] = sm[b * D.width + k];
+   }
+   const unsigned long long ev = (unsigned long long)std::max(nev32, 0);
+   const int64_t off = s->plan.hist_off[d];
+   HIPCHK(s, hipMemcpyAsync(s->hist_counts.p + off, c64.data(), sizeof(unsigned long long) * D.nb, hipMemcpyHostToDevice, s->pool.stream));
+   HIPCHK(s, hipMemcpyAsync(s->hist_sums.p + off * 9, s9.data(), sizeof(double) * D.nb * 9, hipMemcpyHostToDevice, s->pool.stream));
+   HIPCHK(s, hipMemcpyAsync(s->hist_events.p + d, &ev, sizeof(ev), hipMemcpyHostToDevice, s->pool.stream));
+   HIPCHK(s, hipStreamSynchronize(s->pool.stream));
+   return SPX_OK;
+}
+
+// Print1D / Print2D (src/distribution_templated.cc:546-632): the same columns, widths and precision.
+namespace {
+double bin_cent(const spx_ds_desc& D, int ijk, int bin)   // DistributionBase::BinCent (src/distribution_base.hh:267-272)
+{
+   if (bin < 0 || bin >= D.n_bins[ijk]) return 0.0;
+   if (D.log_bins[ijk]) return std::pow(10.0, D.limits0[ijk] + (bin + 0.5) * D.bin_size[ijk]);
+   return D.limits0[ijk] + (bin + 0.5) * D.bin_size[ijk];
+}
+void put_d(std::string& o, double v) { char b[64]; std::snprintf(b, sizeof b, "%20.10g", v); o += b; }
+void put_l(std::string& o, long v) { char b[64]; std::snprintf(b, sizeof b, "%20ld", v); o += b; }
+// PrintSumDistro (:477-537): mean and sum per component
+void put_sums(std::string& o, const DistMeta& M, int width, long n, const double* sum, bool phys)
+{
+   if (width == 1) {
+      put_d(o, (n ? sum[0] / n : 0.0) * (phys ? M.unit_distro[0] : 1.0));
+      put_d(o, sum[0] * (phys ? M.unit_distro[0] : 1.0));
+      return;
+   }
+   for (int pass = 0; pass < 2; pass++)
+      for (int k = 0; k < width; k++) {
+         if (!n) { put_d(o, 0.0); continue; }
+         const double v = pass == 0 ? sum[k] / n : sum[k];
+         put_d(o, phys ? v * M.unit_distro[k] : v);
+      }
+}
+}  // namespace
+
+static int print_distro(spx_sim* s, int d, int a1, int a2, const char* file_name, int phys)
+{
+   if (!s || !file_name || d < 0 || d >= s->plan.n_dist || a1 < 0 || a1 > 2 || a2 < 0 || a2 > 2) return SPX_ERR_ARG;
+   const spx_ds_desc& D = s->plan.dist[d];
+   const DistMeta& M = s->dmeta[d];
+   std::vector<int64_t> c((size_t)D.nb);
+   std::vector<double> sm((size_t)D.nb * D.width);
+   int64_t nev = 0;
+   int rc = spx_get_distro(s, d, c.data(), sm.data(), &nev);
+   if (rc) return rc;
+   const int* nb = D.n_bins;
+   auto lin = [&](const int* bin) { return (int64_t)nb[2] * (nb[1] * bin[0] + bin[1]) + bin[2]; };   // MultiIndex::LinIdx
+   std::string o = "# Total number of events: " + std::to_string(nev) + "\n\n";
+   const bool one = (a1 == a2);
+   const int c1 = one ? (a1 + 1) % 3 : 3 - a1 - a2, c2 = one ? (a1 + 2) % 3 : -1;   // collapsed dimensions
+   int bin[3] = {0, 0, 0};
+   double sum[9];
+   for (bin[a1] = 0; bin[a1] < nb[a1]; bin[a1]++) {
+      for (int j = 0; j < (one ? 1 : nb[a2]); j++) {
+         if (!one) bin[a2] = j;
+         long n = 0;
+         for (int k = 0; k < D.width; k++) sum[k] = 0.0;
+         if (one) {
+            for (bin[c1] = 0; bin[c1] < nb[c1]; bin[c1]++)
+               for (bin[c2] = 0; bin[c2] < nb[c2]; bin[c2]++) {
+                  const int64_t l = lin(bin);
+                  n += (long)c[l];
+                  for (int k = 0; k < D.width; k++) sum[k] += sm[l * D.width + k];
+               }
+         }
+         else {
+            for (bin[c1] = 0; bin[c1] < nb[c1]; bin[c1]++) {
+               const int64_t l = lin(bin);
+               n += (long)c[l];
+               for (int k = 0; k < D.width; k++) sum[k] += sm[l * D.width + k];
+            }
+         }
+         put_d(o, bin_cent(D, a1, bin[a1]) * (phys ? M.unit_val[a1] : 1.0));
+         if (!one) put_d(o, bin_cent(D, a2, bin[a2]) * (phys ? M.unit_val[a2] : 1.0));
+         put_sums(o, M, D.width, n, sum, phys != 0);
+         put_l(o, n);
+         o += "\n";
+      }
+   }
+   std::ofstream f(file_name);
+   if (!f) return fail(s, SPX_ERR_ARG, std::string("cannot write ") + file_name);
+   f << o;
+   return f ? SPX_OK : fail(s, SPX_ERR_ARG, std::string("write failed: ") + file_name);
+}
+
+int spx_print_distro1d(const spx_sim* s, int d, int ijk, const char* file_name, int phys_units)
+{
+   return print_distro(const_cast<spx_sim*>(s), d, ijk, ijk, file_name, phys_units);
+}
+
+int spx_print_distro2d(const spx_sim* s, int d, int ijk1, int ijk2, const char* file_name, int phys_units)
+{
+   return print_distro(const_cast<spx_sim*>(s), d, ijk1, ijk2, file_name, phys_units);
 }
 
 const char* spx_last_error(const spx_sim* s) { return s ? s->err.c_str() : "null simulation"; }

@@ -76,6 +76,10 @@ def lib() -> C.CDLL:
         L.spx_device_info.argtypes = [C.c_char_p, C.c_size_t]
         L.spx_set_background_grid.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32, C.c_void_p]
         L.spx_add_background_file.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, C.c_size_t, C.c_char_p]
+        L.spx_dump_distro.argtypes = [C.c_void_p, C.c_int, C.c_char_p]
+        L.spx_restore_distro.argtypes = [C.c_void_p, C.c_int, C.c_char_p]
+        L.spx_print_distro1d.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_char_p, C.c_int]
+        L.spx_print_distro2d.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_int]
         L.spx_autotune.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(SpxTuneEntry), C.c_int, C.POINTER(C.c_int)]
         _lib = L
     return _lib
@@ -248,3 +252,16 @@ class Simulation:
 
     def reset_distros(self) -> None:
         self._check(lib().spx_reset_distros(self.h), "reset")
+
+    # --- distribution files (DistributionTemplated::Dump / Restore / Print1D / Print2D) --------------
+    def dump_distro(self, d: int, file_name: str) -> None:
+        self._check(lib().spx_dump_distro(self.h, d, file_name.encode()), "dump_distro")
+
+    def restore_distro(self, d: int, file_name: str) -> None:
+        self._check(lib().spx_restore_distro(self.h, d, file_name.encode()), "restore_distro")
+
+    def print_distro_1d(self, d: int, ijk: int, file_name: str, phys_units: bool = False) -> None:
+        self._check(lib().spx_print_distro1d(self.h, d, ijk, file_name.encode(), int(phys_units)), "print_distro_1d")
+
+    def print_distro_2d(self, d: int, ijk1: int, ijk2: int, file_name: str, phys_units: bool = False) -> None:
+        self._check(lib().spx_print_distro2d(self.h, d, ijk1, ijk2, file_name.encode(), int(phys_units)), "print_distro_2d")
