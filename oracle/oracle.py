@@ -159,9 +159,9 @@ class Oracle:
         return np.array(U), np.array(B), dmax.value, divU.value
 
     def grid_fields(self, pos):
-        """Gridded fields at one point: dict of U, B, Bmag, gradU, gradB, gradBmag, dmax; None where the
+        """Gridded fields at one point: dict of U, B, Bmag, gradU, gradB, gradBmag, dmax, E; None where the
         reference fails (outside the grid)."""
-        out = (C.c_double * 29)()
+        out = (C.c_double * 32)()
         rc = self.L.orc_grid_fields(self.h, (C.c_double * 3)(*pos), out)
         if rc < 0:
             raise ValueError("no gridded background")
@@ -169,7 +169,7 @@ class Oracle:
             return None
         a = np.frombuffer(out, dtype=np.float64).copy()
         return {"U": a[0:3], "B": a[3:6], "Bmag": a[6], "gradU": a[7:16].reshape(3, 3), "gradB": a[16:25].reshape(3, 3),
-                "gradBmag": a[25:28], "dmax": a[28]}
+                "gradBmag": a[25:28], "dmax": a[28], "E": a[29:32]}
 
     def kappa(self, comp, pos, p, mu=0.0, Bmag=1.0, t=0.0):
         P = (C.c_double * 3)(*pos)

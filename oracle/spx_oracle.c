@@ -1361,7 +1361,7 @@ int orc_set_grid(orc_sim* s, const int32_t nb[3], const double dom_min[3], const
    return 0;
 }
 
-int orc_grid_fields(orc_sim* s, const double pos[3], double out[29])
+int orc_grid_fields(orc_sim* s, const double pos[3], double out[32])
 {
    spd_t sp; memset(&sp, 0, sizeof(sp));
    int k, j;
@@ -1373,6 +1373,7 @@ int orc_grid_fields(orc_sim* s, const double pos[3], double out[29])
    for (k = 0; k < 3; k++)
       for (j = 0; j < 3; j++) { out[7 + 3 * k + j] = sp.gradU.row[k].v[j]; out[16 + 3 * k + j] = sp.gradB.row[k].v[j]; }
    out[28] = sp.dmax;
+   for (k = 0; k < 3; k++) out[29 + k] = sp.E.v[k];
    return 0;
 }
 

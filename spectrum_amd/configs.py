@@ -296,3 +296,21 @@ def parker_modulation_cartesian(grid, n_bins=100, T1_MeV=10.0, T2_MeV=5000.0, in
     # unless kappa is isotropic and reads no field there (then only their positions are checked)
     s.meta.update(T1=T1_MeV * MeV, T2=T2_MeV * MeV, n_bins=n_bins, grid_gathers_per_step=1 if kap_rat == 1.0 else 7)
     return s
+
+
+def gc_cartesian(grid, T_MeV=100.0, D0=1.668e-4, kperp_cgs=8.6e19, t_limit_days=2.0, r_start_au=1.0, name="gc_cartesian") -> Setup:
+    """Guiding centre with perpendicular diffusion and pitch-angle scattering on a gridded solar wind
+    (BackgroundServerCartesian, dmax 1 au): the gc_diff_scatt workload with the analytic spiral replaced
+    by its sampled grid; (r, mu) map at the time limit."""
+    s = Setup(name, TRAJ_GUIDING_DIFF_SCATT, FORWARD, grid=grid)
+    s.add(BACKGROUND, "BackgroundServerCartesian", background_base(DataContainer(), 0.0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), one_au))
+    s.add(DIFFUSION, "DiffusionPerpConstantPitchIsotropic", DataContainer().insert_double(kperp_cgs / unit_diffusion).insert_double(D0))
+    s.add(INITIAL, "InitialTimeFixed", DataContainer().insert_double(0.0))
+    s.add(INITIAL, "InitialSpaceFixed", DataContainer().insert_vec((r_start_au * one_au, 0.0, 0.0)))
+    s.add(INITIAL, "InitialMomentumShell", DataContainer().insert_double(mom_from_ekin(T_MeV * MeV)))
+    s.add(BOUNDARY, "BoundarySphereAbsorb", boundary(1, [0], ("vec", (0.0, 0.0, 0.0)), ("double", 60.0 * one_au)))
+    s.add(BOUNDARY, "BoundaryTimeExpire", boundary(1, [0], ("double", t_limit_days * one_day)))
+    dst = distribution_base(DataContainer(), (32, 64, 0), (0.0, -1.0, 0.0), (10.0 * one_au, 1.0, 1.0), (0, 0, 0), (1, 0, 0))
+    dst.insert_double(1.0).insert_double(0.0).insert_int(1)
+    s.add(DISTRIBUTION, "DistributionPositionPitchUniform", dst)
+    return s

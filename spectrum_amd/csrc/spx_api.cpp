@@ -495,10 +495,8 @@ static int check_transport(spx_sim* s)
 {
    if (s->plan.traj_type == SPX_TRAJ_GUIDING_DIFF_SCATT) {
       if (s->plan.time_flow != SPX_FLOW_FORWARD) return fail(s, SPX_ERR_ARG, "guiding-centre transport runs forward in time");
-      if (s->plan.bg.kind == SPX_BG_GRID)
-         return fail(s, SPX_ERR_ARG, "guiding-centre transport on a gridded background is not implemented (Parker transports only)");
-      if (s->plan.bg.kind != SPX_BG_SOLARWIND && s->plan.bg.kind != SPX_BG_UNIFORM)
-         return fail(s, SPX_ERR_ARG, "guiding-centre transport supports the static SolarWind and Uniform backgrounds");
+      if (s->plan.bg.kind != SPX_BG_SOLARWIND && s->plan.bg.kind != SPX_BG_UNIFORM && s->plan.bg.kind != SPX_BG_GRID)
+         return fail(s, SPX_ERR_ARG, "guiding-centre transport supports the static SolarWind, Uniform and Cartesian-grid backgrounds");
       if (s->plan.df.kind == SPX_DF_FLOWMOM || s->plan.df.kind == SPX_DF_KINRAD)
          return fail(s, SPX_ERR_ARG, "guiding-centre transport: diffusion model needs grad U, which its field mask does not compute");
    }

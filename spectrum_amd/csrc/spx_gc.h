@@ -3,7 +3,7 @@
 // TrajectoryGuidingDiffScatt::Advance (src/trajectory_guiding_diff_scatt.cc:76-174), forward in
 // time, on the engine of spx_engine.h: one lane per trajectory slot, service kernel + step kernel
 // per engine launch.  The momentum is kept in the reference's guiding-centre layout
-// (p_perp, 0, p_par).  Static backgrounds only (SolarWind, Uniform): their time derivatives
+// (p_perp, 0, p_par).  Static backgrounds only (SolarWind, Uniform, the Cartesian grid): their time derivatives
 // vanish exactly, so the dB/dt terms of the modified fields and of dp_perp/dt are left out,
 // which changes no result bit beyond the sign of a zero.
 #pragma once
@@ -30,6 +30,19 @@ template <typename R, int BG>
 __device__ __forceinline__ bool gc_fields(const SPX_CP spx_bg_t<R>& bg, double t, double x, double y, double z, R pm, GcFields<R>& F)
 {
    F.dmax = bg_dmax<R, BG>(bg, x, y, z, R(0));
+   if constexpr (BG == SPX_BG_GRID) {
+      // BackgroundServer: interpolated fields and the stencil's derivatives, no finite differences
+      v3<R> U;
+      R gB[6];
+      double zl_min;
+      if (!grid_eval_gc<R>(bg.grid, x, y, z, U, F.B, F.Bmag, F.E, F.gradBmag, gB, zl_min)) return false;
+      F.dmax = fmin(F.dmax, R(zl_min));
+      if (F.Bmag < R(sp_tiny)) return false;
+      F.bhat = F.B / F.Bmag;
+      const v3<R> curlB = mk<R>(gB[3] - gB[5], gB[4] - gB[1], gB[0] - gB[2]);
+      F.curlbhat = (curlB - cross(F.gradBmag, F.bhat)) / F.Bmag;
+      return true;
+   }
    v3<R> U, B;
    R ds = R(0);
    bg_eval<R, BG>(bg, t, x, y, z, 3, U, B, ds);
@@ -417,6 +430,7 @@ hipError_t dispatch_gc(const spx_plan_desc& hp, uint64_t n_lanes, const Launch* 
    switch (hp.bg.kind) {
    case SPX_BG_SOLARWIND: return launch_gc_bg<R, SPX_BG_SOLARWIND>(n_lanes, d_launch, st, mid);
    case SPX_BG_UNIFORM: return launch_gc_bg<R, SPX_BG_UNIFORM>(n_lanes, d_launch, st, mid);
+   case SPX_BG_GRID: return launch_gc_bg<R, SPX_BG_GRID>(n_lanes, d_launch, st, mid);
    default: return hipErrorInvalidValue;
    }
 }

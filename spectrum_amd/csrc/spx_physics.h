@@ -329,6 +329,64 @@ __device__ __forceinline__ bool grid_eval(const SPX_CP spx_grid_desc& g, double 
    return true;
 }
 
+// The guiding centre's fields from the grid: U, B, the interpolated |B|, E = -(U x B) / c formed in fluid
+// units before the unit conversion (GetVariablesInterp1, src/server_cartesian.cc:538-560, :590-609), and
+// from the stencil's derivative weights (GetGradientsInterp1, :630-705) grad|B| and the off-diagonal
+// d B_j / d x_i that curl B needs.  gB holds (01, 02, 10, 12, 20, 21) of gradB[i][j] = d B_j / d x_i.
+template <typename R>
+__device__ __forceinline__ bool grid_eval_gc(const SPX_CP spx_grid_desc& g, double x, double y, double z, v3<R>& U, v3<R>& B, R& Bmag,
+                                             v3<R>& E, v3<R>& gradBmag, R gB[6], double& zl_min)
+{
+   R lo[3], hi[3], dl[3];
+   int64_t zone0;
+   if (!grid_locate<R>(g, x, y, z, lo, hi, dl, zone0, zl_min)) return false;
+   const int64_t sy = g.nz[0], sz = (int64_t)g.nz[0] * g.nz[1];
+   R acc[7], gm[3], g6[6];
+#pragma unroll
+   for (int m = 0; m < 7; m++) acc[m] = R(0);
+#pragma unroll
+   for (int m = 0; m < 3; m++) gm[m] = R(0);
+#pragma unroll
+   for (int m = 0; m < 6; m++) g6[m] = R(0);
+#pragma unroll 1
+   for (int r = 0; r < 4; r++) {
+      const int j = r & 1, k = r >> 1;
+      const int64_t zr = zone0 + j * sy + k * sz;
+      R v0[7], v1[7];
+      grid_rec<R>(g, zr, v0);
+      grid_rec<R>(g, zr + 1, v1);
+      const R fy = j ? lo[1] : hi[1], fz = k ? lo[2] : hi[2];
+#pragma unroll
+      for (int i = 0; i < 2; i++) {
+         const R* v = i ? v1 : v0;
+         const R fx = i ? lo[0] : hi[0];
+         const R w = fx * fy * fz;
+#pragma unroll
+         for (int m = 0; m < 7; m++) acc[m] = acc[m] + w * v[m];
+         const R sx = i ? R(1) : R(-1), sy_ = j ? R(1) : R(-1), sz_ = k ? R(1) : R(-1);
+         const R dx = sx * (fy * fz) / dl[0], dy = sy_ * (fx * fz) / dl[1], dz = sz_ * (fx * fy) / dl[2];
+         gm[0] = gm[0] + dx * v[6]; gm[1] = gm[1] + dy * v[6]; gm[2] = gm[2] + dz * v[6];
+         g6[0] = g6[0] + dx * v[4];   // d By / dx
+         g6[1] = g6[1] + dx * v[5];   // d Bz / dx
+         g6[2] = g6[2] + dy * v[3];   // d Bx / dy
+         g6[3] = g6[3] + dy * v[5];   // d Bz / dy
+         g6[4] = g6[4] + dz * v[3];   // d Bx / dz
+         g6[5] = g6[5] + dz * v[4];   // d By / dz
+      }
+   }
+   const R fv = R(g.f_vel), fm = R(g.f_mag), cf = R(C_CGS / 1.0E7);
+   const v3<R> Uf = mk<R>(acc[0], acc[1], acc[2]), Bf = mk<R>(acc[3], acc[4], acc[5]);
+   const v3<R> Ef = (-cross(Uf, Bf)) / cf;
+   U = mk<R>(Uf.x * fv, Uf.y * fv, Uf.z * fv);
+   B = mk<R>(Bf.x * fm, Bf.y * fm, Bf.z * fm);
+   E = mk<R>(Ef.x * fm, Ef.y * fm, Ef.z * fm);
+   Bmag = acc[6] * fm;
+   gradBmag = mk<R>(gm[0] * fm, gm[1] * fm, gm[2] * fm);
+#pragma unroll
+   for (int m = 0; m < 6; m++) gB[m] = g6[m] * fm;
+   return true;
+}
+
 // ---- diffusion: DiffusionBase::GetComponent (src/diffusion_base.cc:105-121) + models ----------------
 // The momentum-only factor of kappa_par: the same at every point of a step's derivative stencil, so
 // it is formed once per step (the same inputs give the same value).

@@ -42,6 +42,23 @@ def test_cartesian_modulation_fp64_parity(oracle_mod, spiral, kap_rat):
     check_parity(st, rec, orec, sim, o, n, sens=sensitivity(orec, prec))
 
 
+def test_guiding_centre_on_the_grid_fp64_parity(oracle_mod, spiral):
+    # the guiding centre takes grad B and grad|B| from the stencil (curl bhat, grad-B drift) and E from
+    # the interpolated U and B
+    s = cf.gc_cartesian(spiral, t_limit_days=0.5)
+    n = 256
+    print(f"[grid] guiding centre parity n={n}", flush=True)
+    sim = E.Simulation.from_setup(s, precision=E.PREC_FP64, lanes=1024, steps_per_launch=500)
+    st, rec = sim.run(n, seed=6)
+    o = oracle_mod.Oracle(s)
+    _, orec = o.run(n, seed=6, threads=THREADS)
+    _, prec = oracle_mod.Oracle(s, oracle_mod.FMA_LIB_PATH).run(n, seed=6, threads=THREADS)
+    check_parity(st, rec, orec, sim, o, n, sens=sensitivity(orec, prec))
+    # and the mixed kernel finishes every trajectory
+    st32, _ = E.Simulation.from_setup(s, precision=E.PREC_MIXED).run(4 * n, seed=6, records=False)
+    assert st32.done == 4 * n and st32.failed == 0
+
+
 def test_small_grid_discards_match(oracle_mod):
     # a +-6 au grid inside an 8 au absorbing sphere: most trajectories leave the grid, each exit is a
     # field error that discards the attempt and retries with fresh draws, up to max_attempts

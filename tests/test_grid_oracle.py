@@ -108,6 +108,9 @@ def test_values_and_gradients_equal_a_global_trilinear_interpolation(oracle_mod,
         np.testing.assert_allclose(f["gradU"], grad[:, 0:3] / G.VELOCITY_RATIO * G.LENGTH_RATIO, rtol=1e-10, atol=1e-12)
         np.testing.assert_allclose(f["gradB"], grad[:, 3:6] / G.MAGNETIC_RATIO * G.LENGTH_RATIO, rtol=1e-10, atol=1e-12)
         np.testing.assert_allclose(f["gradBmag"], grad[:, 6] / G.MAGNETIC_RATIO * G.LENGTH_RATIO, rtol=1e-10, atol=1e-12)
+        # E is formed in fluid units and converted; in exact arithmetic it is -(U x B) / c in code units
+        Ec = -np.cross(f["U"], f["B"]) / cf.c_code
+        np.testing.assert_allclose(f["E"], Ec, rtol=1e-12, atol=1e-14 * np.abs(Ec).max())
         assert f["dmax"] == pytest.approx(zl.min() / G.LENGTH_RATIO, rel=1e-14)
 
 
