@@ -226,6 +226,7 @@ static int add_background(spx_sim* s, const std::string& cls, Reader& r)
       for (int k = 0; k < 3; k++)
          for (int j = 0; j < 3; j++)
             if ((j != k && g.eprime[k][j] != 0.0) || (j == k && std::fabs(g.eprime[k][j]) != 1.0)) g.ep_diag = 0;
+      if (g.ep_diag && g.eprime[0][0] == 1.0 && g.eprime[1][1] == 1.0 && g.eprime[2][2] == 1.0) g.ep_diag = 2;   // x1 is exact: skipped
       g.ur0 = std::fabs(g.u0[0]);
       g.hp_rad = 117.0 * 1.495978707E+13 / 1.0E10;
    }
@@ -464,6 +465,10 @@ static void finalize_plan(spx_sim* s)
    f.w0 = (float)g.w0; f.hp_rad = (float)g.hp_rad; f.v_shock = (float)g.v_shock; f.compression = (float)g.compression;
    f.width_shock = (float)g.width_shock;
    f.grid = g.grid;
+   if (g.kind == SPX_BG_SOLARWIND) {
+      f.sw_c1 = (float)(g.B0[0] * g.r_ref * g.r_ref);
+      f.sw_k = (float)(g.w0 / g.ur0);
+   }
    spx_df_t<double>& d = P.df;
    d.pos_dep = 0;
    if (d.kind == SPX_DF_FLOWMOM) d.pos_dep = (d.pow_law_U != 0.0);

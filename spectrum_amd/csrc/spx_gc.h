@@ -80,8 +80,9 @@ __device__ __forceinline__ bool gc_fields(const SPX_CP spx_bg_t<R>& bg, double t
             dr *= R(2);
             bg_eval<R, BG>(bg, t, xf - (double)(dr * e.x), yf - (double)(dr * e.y), zf - (double)(dr * e.z), 2, Ud, Bb, dsd);
          }
-         const v3<R> gB = (Bf - Bb) / dr;
-         const R gm = (norm(Bf) - norm(Bb)) / dr;
+         const R idr = is_f32<R>() ? R(0.5) / r_g : R(0);   // FP32: one reciprocal per point
+         const v3<R> gB = is_f32<R>() ? idr * (Bf - Bb) : (Bf - Bb) / dr;
+         const R gm = is_f32<R>() ? (norm(Bf) - norm(Bb)) * idr : (norm(Bf) - norm(Bb)) / dr;
          if (k == 0) {
             g01 = e.x * gB.y; g02 = e.x * gB.z; g10 = e.y * gB.x; g12 = e.y * gB.z; g20 = e.z * gB.x; g21 = e.z * gB.y;
             gbm = mk<R>(e.x * gm, e.y * gm, e.z * gm);

@@ -91,7 +91,7 @@ __device__ __forceinline__ bool parker_fields(const SPX_CP spx_bg_t<R>& bg, doub
             dr *= R(2);
             bg_eval<R, BG>(bg, t, xf - (double)(dr * e.x), yf - (double)(dr * e.y), zf - (double)(dr * e.z), 1, Ub, Bd, dsd);
          }
-         const v3<R> G = (Uf - Ub) / dr;
+         const v3<R> G = is_f32<R>() ? (R(0.5) / r_g) * (Uf - Ub) : (Uf - Ub) / dr;   // FP32: one reciprocal per point
          if (k == 0) { d0 = e.x * G.x; d1 = e.y * G.y; d2 = e.z * G.z; }
          else { d0 += e.x * G.x; d1 += e.y * G.y; d2 += e.z * G.z; }
       }
@@ -233,6 +233,7 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
    const R km = (DF == SPX_DF_FULL) ? R(0) : kappa_mom(df, p);
    parker_kappa<R, DF>(df, km, H.F.Bmag, H.x, H.y, H.z, H.F.U, K0, K1);
    const R delta = fmin(LarmorRadius(p, H.F.Bmag), H.F.dmax);
+   const R hdi = is_f32<R>() ? R(0.5) / delta : R(0);
    v3<R> divK = mk<R>(R(0), R(0), R(0));
    const bool pos_dep = (DF == SPX_DF_FULL) ? false : (df.pos_dep != 0);
    if (BG == SPX_BG_GRID && (pos_dep || K0 != K1)) {
@@ -263,9 +264,16 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
          const R bfj = j == 0 ? bf.x : (j == 1 ? bf.y : bf.z);
          const R bbj = j == 0 ? bb.x : (j == 1 ? bb.y : bb.z);
          const R cf = (K1f - K0f) * bfj, cb = (K1b - K0b) * bbj;
-         divK.x += R(0.5) * ((K0f * (j == 0 ? R(1) : R(0)) + cf * bf.x) - (K0b * (j == 0 ? R(1) : R(0)) + cb * bb.x)) / delta;
-         divK.y += R(0.5) * ((K0f * (j == 1 ? R(1) : R(0)) + cf * bf.y) - (K0b * (j == 1 ? R(1) : R(0)) + cb * bb.y)) / delta;
-         divK.z += R(0.5) * ((K0f * (j == 2 ? R(1) : R(0)) + cf * bf.z) - (K0b * (j == 2 ? R(1) : R(0)) + cb * bb.z)) / delta;
+         if constexpr (is_f32<R>()) {   // one reciprocal of delta for the whole stencil
+            divK.x += hdi * ((K0f * (j == 0 ? R(1) : R(0)) + cf * bf.x) - (K0b * (j == 0 ? R(1) : R(0)) + cb * bb.x));
+            divK.y += hdi * ((K0f * (j == 1 ? R(1) : R(0)) + cf * bf.y) - (K0b * (j == 1 ? R(1) : R(0)) + cb * bb.y));
+            divK.z += hdi * ((K0f * (j == 2 ? R(1) : R(0)) + cf * bf.z) - (K0b * (j == 2 ? R(1) : R(0)) + cb * bb.z));
+         }
+         else {
+            divK.x += R(0.5) * ((K0f * (j == 0 ? R(1) : R(0)) + cf * bf.x) - (K0b * (j == 0 ? R(1) : R(0)) + cb * bb.x)) / delta;
+            divK.y += R(0.5) * ((K0f * (j == 1 ? R(1) : R(0)) + cf * bf.y) - (K0b * (j == 1 ? R(1) : R(0)) + cb * bb.y)) / delta;
+            divK.z += R(0.5) * ((K0f * (j == 2 ? R(1) : R(0)) + cf * bf.z) - (K0b * (j == 2 ? R(1) : R(0)) + cb * bb.z)) / delta;
+         }
       }
    }
    else if (BG == SPX_BG_GRID) {
@@ -321,9 +329,16 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
          const R bfj = j == 0 ? bf.x : (j == 1 ? bf.y : bf.z);
          const R bbj = j == 0 ? bb.x : (j == 1 ? bb.y : bb.z);
          const R cf = (K1f - K0f) * bfj, cb = (K1b - K0b) * bbj;
-         divK.x += R(0.5) * ((K0f * (j == 0 ? R(1) : R(0)) + cf * bf.x) - (K0b * (j == 0 ? R(1) : R(0)) + cb * bb.x)) / delta;
-         divK.y += R(0.5) * ((K0f * (j == 1 ? R(1) : R(0)) + cf * bf.y) - (K0b * (j == 1 ? R(1) : R(0)) + cb * bb.y)) / delta;
-         divK.z += R(0.5) * ((K0f * (j == 2 ? R(1) : R(0)) + cf * bf.z) - (K0b * (j == 2 ? R(1) : R(0)) + cb * bb.z)) / delta;
+         if constexpr (is_f32<R>()) {   // one reciprocal of delta for the whole stencil
+            divK.x += hdi * ((K0f * (j == 0 ? R(1) : R(0)) + cf * bf.x) - (K0b * (j == 0 ? R(1) : R(0)) + cb * bb.x));
+            divK.y += hdi * ((K0f * (j == 1 ? R(1) : R(0)) + cf * bf.y) - (K0b * (j == 1 ? R(1) : R(0)) + cb * bb.y));
+            divK.z += hdi * ((K0f * (j == 2 ? R(1) : R(0)) + cf * bf.z) - (K0b * (j == 2 ? R(1) : R(0)) + cb * bb.z));
+         }
+         else {
+            divK.x += R(0.5) * ((K0f * (j == 0 ? R(1) : R(0)) + cf * bf.x) - (K0b * (j == 0 ? R(1) : R(0)) + cb * bb.x)) / delta;
+            divK.y += R(0.5) * ((K0f * (j == 1 ? R(1) : R(0)) + cf * bf.y) - (K0b * (j == 1 ? R(1) : R(0)) + cb * bb.y)) / delta;
+            divK.z += R(0.5) * ((K0f * (j == 2 ? R(1) : R(0)) + cf * bf.z) - (K0b * (j == 2 ? R(1) : R(0)) + cb * bb.z)) / delta;
+         }
       }
    }
    else if (BG == SPX_BG_SMOOTHSHOCK) {

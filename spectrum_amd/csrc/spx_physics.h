@@ -106,40 +106,25 @@ template <typename R> __device__ v3<R> v3of(const SPX_CP R* a) { return mk<R>(a[
 template <typename R>
 SPX_HD v3<R> rel(const SPX_CP spx_bg_t<R>& b, double x, double y, double z) { return mk<R>(R(x - b.r0[0]), R(y - b.r0[1]), R(z - b.r0[2])); }
 
-// BackgroundSolarWind::EvaluateBackground (src/background_solarwind.cc:111-255), every switch at 0.
-// Flags select what is evaluated: U (bit 0), B (bit 1), E (bit 2).
+// BackgroundSolarWind::EvaluateBackground (src/background_solarwind.cc:111-255), every switch at 0, in
+// the frame of the rotation axis: pp is the position in that frame, u and Bl the fields in it.
 template <typename R>
-SPX_HD void solarwind_eval(const SPX_CP spx_bg_t<R>& b, v3<R> d, int want, v3<R>& U, v3<R>& B)
+SPX_HD void solarwind_local(const SPX_CP spx_bg_t<R>& b, v3<R> pp, int want, v3<R>& u, v3<R>& Bl)
 {
-   // With a diagonal +-1 basis, to_basis / from_basis reduce to sign flips: every product with an
-   // off-diagonal zero adds a signed zero, so the results agree bit for bit up to the sign of a zero.
-   v3<R> ep[3] = {v3of(b.eprime[0]), v3of(b.eprime[1]), v3of(b.eprime[2])};
-   const bool dg = b.ep_diag != 0;
-   const v3<R> pp = dg ? mk<R>(d.x * b.eprime[0][0], d.y * b.eprime[1][1], d.z * b.eprime[2][2]) : to_basis(d, ep);
    if constexpr (is_f32<R>()) {
+      // The spiral in closed form: sin(theta) cos(phi) = x / r, sin(theta) sin(phi) = y / r, cos(theta) = z / r,
+      // so B = B0 r_ref^2 / r^3 (x + q y, y - q x, z) with q = (r - r_ref) w0 / ur0 (one reciprocal square root)
       const R r2 = norm2(pp), ir = frsq(r2), r = r2 * ir;
-      if (want & 1) {
-         const v3<R> u = (b.ur0 * ir) * pp;
-         U = dg ? mk<R>(u.x * b.eprime[0][0], u.y * b.eprime[1][1], u.z * b.eprime[2][2]) : from_basis(u, ep);
-      }
+      if (want & 1) u = (b.ur0 * ir) * pp;
       if (want & 2) {
-         const R costheta = pp.z * ir;
-         const R sintheta = rsqrt_(fmax(R(1) - sqr(costheta), R(0)));
-         const R s2 = sqr(pp.x) + sqr(pp.y);
-         R cosphi = R(0), sinphi = R(0);
-         if (s2 >= sqr(r * R(sp_tiny))) { const R is = frsq(s2); cosphi = pp.x * is; sinphi = pp.y * is; }
-         const R Br = b.B0[0] * sqr(b.r_ref * ir);
-         const R Bp = -Br * sintheta * (r - b.r_ref) * b.w0 / b.ur0;
-         const v3<R> Bl = mk<R>(Br * sintheta * cosphi - Bp * sinphi, Br * sintheta * sinphi + Bp * cosphi, Br * costheta);
-         B = dg ? mk<R>(Bl.x * b.eprime[0][0], Bl.y * b.eprime[1][1], Bl.z * b.eprime[2][2]) : from_basis(Bl, ep);
+         const R Bri = b.sw_c1 * (ir * ir) * ir;
+         const R q = (r - b.r_ref) * b.sw_k;
+         Bl = mk<R>(Bri * (pp.x + q * pp.y), Bri * (pp.y - q * pp.x), Bri * pp.z);
       }
       return;
    }
    R r = norm(pp);
-   if (want & 1) {
-      const v3<R> u = b.ur0 * unit(pp);
-      U = dg ? mk<R>(u.x * b.eprime[0][0], u.y * b.eprime[1][1], u.z * b.eprime[2][2]) : from_basis(u, ep);
-   }
+   if (want & 1) u = b.ur0 * unit(pp);
    if (want & 2) {
       R r_mns = r - b.r_ref;
       R costheta = pp.z / r;
@@ -150,9 +135,28 @@ SPX_HD void solarwind_eval(const SPX_CP spx_bg_t<R>& b, v3<R> d, int want, v3<R>
       else { cosphi = pp.x / s; sinphi = pp.y / s; }
       R Br = b.B0[0] * sqr(b.r_ref / r);
       R Bp = -Br * sintheta * r_mns * b.w0 / b.ur0;
-      v3<R> Bl = mk<R>(Br * sintheta * cosphi - Bp * sinphi, Br * sintheta * sinphi + Bp * cosphi, Br * costheta);
-      B = dg ? mk<R>(Bl.x * b.eprime[0][0], Bl.y * b.eprime[1][1], Bl.z * b.eprime[2][2]) : from_basis(Bl, ep);
+      Bl = mk<R>(Br * sintheta * cosphi - Bp * sinphi, Br * sintheta * sinphi + Bp * cosphi, Br * costheta);
    }
+}
+
+// Flags select what is evaluated: U (bit 0), B (bit 1).  With a diagonal +-1 basis, to_basis /
+// from_basis reduce to sign flips: every product with an off-diagonal zero adds a signed zero, so the
+// results agree bit for bit up to the sign of a zero.  The identity basis (rotation axis along +z,
+// a uniform branch) needs no products at all: x * 1 == x exactly.
+template <typename R>
+SPX_HD void solarwind_eval(const SPX_CP spx_bg_t<R>& b, v3<R> d, int want, v3<R>& U, v3<R>& B)
+{
+   if (b.ep_diag == 2) {
+      solarwind_local<R>(b, d, want, U, B);
+      return;
+   }
+   v3<R> ep[3] = {v3of(b.eprime[0]), v3of(b.eprime[1]), v3of(b.eprime[2])};
+   const bool dg = b.ep_diag != 0;
+   auto diag = [&](v3<R> a) { return mk<R>(a.x * b.eprime[0][0], a.y * b.eprime[1][1], a.z * b.eprime[2][2]); };
+   v3<R> u, Bl;
+   solarwind_local<R>(b, dg ? diag(d) : to_basis(d, ep), want, u, Bl);
+   if (want & 1) U = dg ? diag(u) : from_basis(u, ep);
+   if (want & 2) B = dg ? diag(Bl) : from_basis(Bl, ep);
 }
 
 // src/background_smooth_shock.cc:46-75 (SMOOTH_SHOCK_ORDER 4)

@@ -54,11 +54,12 @@ typedef struct {
 template <typename T>
 struct spx_bg_t {
    int32_t kind;
-   int32_t ep_diag;                    // SolarWind: eprime is diagonal with entries +-1 (rotation axis along +-z)
+   int32_t ep_diag;                    // SolarWind: eprime is diagonal with entries +-1 (rotation axis along +-z); 2: the identity
    double t0, r0[3];                   // always double: positions are accumulated in double
    T u0[3], B0[3], dmax0;
    // BackgroundSolarWind (src/background_solarwind.cc:58-84)
    T r_ref, dmax_fraction, eprime[3][3], ur0, w0, hp_rad;
+   T sw_c1, sw_k;                      // FP32 step: B0 r_ref^2 and w0 / ur0 of the closed-form spiral
    // BackgroundShock / BackgroundSmoothShock (src/background_shock.cc:58-86, src/background_smooth_shock.cc:139-146)
    T n_shock[3], v_shock, compression, u1[3], B1[3], E0[3], E1[3], width_shock;
    // BackgroundUniform (src/background_uniform.cc:58-64)
