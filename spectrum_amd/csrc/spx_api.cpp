@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cfloat>
 #include <climits>
 #include <cmath>
 #include <condition_variable>
@@ -468,6 +469,9 @@ static void finalize_plan(spx_sim* s)
    if (g.kind == SPX_BG_SOLARWIND) {
       f.sw_c1 = (float)(g.B0[0] * g.r_ref * g.r_ref);
       f.sw_k = (float)(g.w0 / g.ur0);
+      // |B| = |B0| r_ref^2 |w| / r^3 with |w| >= r, so |B| >= |B0| r_ref^2 / r^2
+      const double r2_ok = std::fabs(g.B0[0] * g.r_ref * g.r_ref) / sp_tiny_h;
+      f.sw_r2_ok = r2_ok > (double)FLT_MAX ? INFINITY : (float)r2_ok;
    }
    spx_df_t<double>& d = P.df;
    d.pos_dep = 0;

@@ -159,6 +159,61 @@ SPX_HD void solarwind_eval(const SPX_CP spx_bg_t<R>& b, v3<R> d, int want, v3<R>
    if (want & 2) B = dg ? diag(Bl) : from_basis(Bl, ep);
 }
 
+// The Parker step's two SolarWind stencils in the FP32 step with the identity basis, restated from the
+// closed forms above.  At the points c +- h e (c = x - r0, |e| = 1), |c +- h e|^2 = |c|^2 + h^2 +- 2h e.c,
+// so each point costs one reciprocal square root and no vector products.
+//
+// NumericalDerivatives' trace for div U (src/background_base.cc:228-312 with the basis e_k):
+// sum_k e_k . [U(c + h e_k) - U(c - h e_k)], with e . U(p) = ur0 (e . p) / |p|.
+template <typename R>
+SPX_HD R sw_divu_stencil(const SPX_CP spx_bg_t<R>& b, v3<R> c, R h, const v3<R> (&e)[3])
+{
+   const R c2h = norm2(c) + h * h, h2 = R(2) * h;
+   R acc = R(0);
+#pragma unroll
+   for (int k = 0; k < 3; k++) {
+      const R s = dot(e[k], c);
+      acc += frsq(c2h + h2 * s) * (s + h) - frsq(c2h - h2 * s) * (s - h);
+   }
+   return b.ur0 * acc;
+}
+
+// The b b part of div(kappa)'s stencil (TrajectoryParker::DiffusionCoeff, src/trajectory_parker.cc:120-151,
+// with kappa position independent): sum_j [b_j b](c + h e_j) - [b_j b](c - h e_j) over the Cartesian axes.
+// b is the direction of w = (x + q y, y - q x, z), q = (r - r_ref) w0 / ur0: the radial factor of B cancels
+// in b_j b = w_j w / |w|^2.
+// True when no point c +- h e_j can meet GetFields' |B| < sp_tiny (src/background_base.cc:444-448): every
+// |c +- h e_j|^2 is at most |c|^2 + h^2 + 2h (|c_x| + |c_y| + |c_z|).
+template <typename R>
+SPX_HD bool sw_stencil_clear(const SPX_CP spx_bg_t<R>& b, v3<R> c, R h)
+{
+   return norm2(c) + h * h + (R(2) * h) * (fabs(c.x) + fabs(c.y) + fabs(c.z)) <= b.sw_r2_ok;
+}
+
+template <typename R>
+SPX_HD v3<R> sw_bb_stencil(const SPX_CP spx_bg_t<R>& b, v3<R> c, R h)
+{
+   const R c2h = norm2(c) + h * h, h2 = R(2) * h;
+   v3<R> acc = mk<R>(R(0), R(0), R(0));
+#pragma unroll
+   for (int j = 0; j < 3; j++) {
+#pragma unroll
+      for (int s = 0; s < 2; s++) {
+         const R sh = s == 0 ? h : -h;
+         const R cj = j == 0 ? c.x : (j == 1 ? c.y : c.z);
+         const v3<R> p = mk<R>(j == 0 ? c.x + sh : c.x, j == 1 ? c.y + sh : c.y, j == 2 ? c.z + sh : c.z);
+         const R r2 = c2h + (s == 0 ? h2 : -h2) * cj;
+         const R ir = frsq(r2), r = r2 * ir;
+         const R q = (r - b.r_ref) * b.sw_k;
+         const v3<R> w = mk<R>(p.x + q * p.y, p.y - q * p.x, p.z);
+         const R wj = j == 0 ? w.x : (j == 1 ? w.y : w.z);
+         const R t = wj / norm2(w);
+         acc = acc + (s == 0 ? t : -t) * w;
+      }
+   }
+   return acc;
+}
+
 // src/background_smooth_shock.cc:46-75 (SMOOTH_SHOCK_ORDER 4)
 template <typename R> SPX_HD R shock_transition(R x) { return R(0.5) * (R(1) + rtanh(R(8) * x)); }
 template <typename R> SPX_HD R shock_transition_deriv(R x) { return R(0.5) * R(8) * (R(1) - sqr(rtanh(R(8) * x))); }
