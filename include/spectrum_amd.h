@@ -157,6 +157,24 @@ int spx_restore_distro(spx_sim* sim, int d, const char* file_name);
 int spx_print_distro1d(const spx_sim* sim, int d, int ijk, const char* file_name, int phys_units);
 int spx_print_distro2d(const spx_sim* sim, int d, int ijk1, int ijk2, const char* file_name, int phys_units);
 
+/* SimulationMaster's task loop in its serial form (one worker: the device).
+   spx_set_tasks is SetTasks (src/simulation.cc:509-519) with its clamps: a negative count becomes 0, a
+   batch size outside [1, n_traj] becomes min(1, n_traj), and a per-worker maximum below n_traj becomes
+   0 (none).  spx_distro_file_name is DistroFileName (:496-499, default "distribution_"), and
+   spx_restore_distro_at_start is SimulationMaster::RestoreDistro (:842-845): the main loop restores
+   distribution d from <base><d>.out before it starts.
+   spx_main_loop is MainLoop (:818-835): MasterStart (:699-737) resets the distributions, restores the
+   flagged ones and prints the trajectory count; the run integrates n_traj trajectories (ids continue
+   from the previous main loop's); whenever the completed percentage, counted in whole batches as
+   DecrementTrajectoryCount counts them (:558-643), goes up, every distribution is dumped to
+   <base><d>.out and the progress block is printed to stderr; MasterFinish (:776-811) dumps the final
+   distributions and prints the shortest and longest elapsed trajectory time (s), the integration
+   time per trajectory (ms, kernel time here) and the total time.  No workload manager is detected. */
+int spx_set_tasks(spx_sim* sim, int64_t n_traj, int64_t batch_size, int64_t max_traj_per_worker);
+int spx_distro_file_name(spx_sim* sim, const char* base);
+int spx_restore_distro_at_start(spx_sim* sim, int d);
+int spx_main_loop(spx_sim* sim, const spx_rng* rng, spx_run_stats* stats);
+
 const char* spx_last_error(const spx_sim* sim);
 /* Name, compute units and architecture of device 0 ("" and SPX_ERR_DEVICE without a GPU). */
 int spx_device_info(char* buf, size_t len);

@@ -15,9 +15,11 @@
 #include <cstring>
 #include <deque>
 #include <fstream>
+#include <iomanip>
 #include <map>
 #include <memory>
 #include <mutex>
+#include <sstream>
 #include <string>
 #include <thread>
 #include <vector>
@@ -184,6 +186,12 @@ struct spx_sim {
    bool running = false;
    uint64_t run_first = 0, run_n = 0;
    bool keep_records = false;
+   // SimulationMaster's tasks (src/simulation.hh:196-237): trajectory count, batch size, per-worker
+   // maximum, the base name of the distribution files, the distributions to restore at the start
+   int64_t task_n = 0, task_batch = 0, task_max_per_worker = 0;
+   std::string distro_file_name = "distribution_";
+   uint32_t restore_at_start = 0;
+   uint64_t next_traj = 0;             // the main loop's trajectories continue from here
 };
 
 static int fail(spx_sim* s, int code, const std::string& msg)
@@ -1094,6 +1102,137 @@ int spx_print_distro1d(const spx_sim* s, int d, int ijk, const char* file_name, 
 int spx_print_distro2d(const spx_sim* s, int d, int ijk1, int ijk2, const char* file_name, int phys_units)
 {
    return print_distro(const_cast<spx_sim*>(s), d, ijk1, ijk2, file_name, phys_units);
+}
+
+// ---- SimulationMaster's task loop (src/simulation.cc:496-519, :558-643, :699-835), serial form ---------
+int spx_set_tasks(spx_sim* s, int64_t n_traj, int64_t batch_size, int64_t max_traj_per_worker)
+{
+   if (!s) return SPX_ERR_ARG;
+   if (s->running) return fail(s, SPX_ERR_STATE, "tasks cannot change during a run");
+   if (n_traj < 0) n_traj = 0;
+   if (batch_size < 1 || batch_size > n_traj) batch_size = std::min<int64_t>(1, n_traj);
+   if (max_traj_per_worker * 1 < n_traj) max_traj_per_worker = 0;   // one worker: the device
+   s->task_n = n_traj;
+   s->task_batch = batch_size;
+   s->task_max_per_worker = max_traj_per_worker;
+   return SPX_OK;
+}
+
+int spx_distro_file_name(spx_sim* s, const char* base)
+{
+   if (!s || !base) return SPX_ERR_ARG;
+   s->distro_file_name = base;
+   return SPX_OK;
+}
+
+int spx_restore_distro_at_start(spx_sim* s, int d)
+{
+   if (!s || d < 0 || d >= s->plan.n_dist) return SPX_ERR_ARG;
+   s->restore_at_start |= 1u << d;
+   return SPX_OK;
+}
+
+static std::string distro_file(const spx_sim* s, int d) { return s->distro_file_name + std::to_string(d) + ".out"; }
+
+static int dump_all(spx_sim* s)
+{
+   for (int d = 0; d < s->plan.n_dist; d++) {
+      const int rc = spx_dump_distro(s, d, distro_file(s, d).c_str());
+      if (rc) return rc;
+   }
+   return SPX_OK;
+}
+
+int spx_main_loop(spx_sim* s, const spx_rng* rng, spx_run_stats* st)
+{
+   if (!s || !rng) return SPX_ERR_ARG;
+   if (s->running) return fail(s, SPX_ERR_STATE, "main loop during a run");
+   const bool fwd = s->plan.time_flow == SPX_FLOW_FORWARD;
+   const double unit_time = unit_length_h / unit_velocity_h;
+   const auto t_start = std::chrono::system_clock::now();
+   auto seconds_since_start = [&]() {
+      return (long)std::chrono::duration_cast<std::chrono::seconds>(std::chrono::system_clock::now() - t_start).count();
+   };
+   // MasterStart (:699-737): fresh distributions, then the ones flagged by RestoreDistro from their files
+   int rc = spx_reset_distros(s);
+   if (rc) return rc;
+   for (int d = 0; d < s->plan.n_dist; d++) {
+      if (!((s->restore_at_start >> d) & 1u)) continue;
+      // Restore leaves the distribution empty when there is no file yet (src/distribution_templated.cc:436-437)
+      if (!std::ifstream(distro_file(s, d), std::ios::binary).is_open()) continue;
+      rc = spx_restore_distro(s, d, distro_file(s, d).c_str());
+      if (rc) return rc;
+   }
+   double shortest = fwd ? 1.0E300 : -1.0E300, longest = 0.0;
+   double integ_ms = 0.0;   // the serial master's elapsed_time: time spent integrating, here the kernels' time
+   const int64_t n_total = s->task_n;
+   std::fprintf(stderr, "Trajectories left: %s\n", std::to_string(n_total).c_str());
+   spx_run_stats run{};
+   if (n_total > 0) {
+      rc = spx_begin(s, s->next_traj, (uint64_t)n_total, rng, 0);
+      if (rc) return rc;
+      const auto w0 = std::chrono::steady_clock::now();
+      s->pool.step_ms = s->pool.service_ms = 0.0;
+      const double guard = 64.0 + (double)s->plan.max_attempts * ((double)s->plan.max_steps + 1.0) / s->steps_per_launch *
+                                      std::max(1.0, (double)n_total / (double)s->pool.n_lanes + 1.0);
+      int percentage_done = 0, launches = 0, burst = 1;
+      while (true) {
+         RunCtl ctl;
+         HIPCHK(s, s->q.read_ctl(ctl, s->pool.stream));
+         // DecrementTrajectoryCount (:558-643): the serial master counts whole batches as completed
+         const int64_t done = (int64_t)ctl.done;
+         const int64_t completed = done >= n_total ? n_total : done / s->task_batch * s->task_batch;
+         const int percentage_new = (int)(100 * completed / n_total);
+         if (percentage_new > percentage_done) {
+            percentage_done = percentage_new;
+            rc = dump_all(s);   // the partial distributions
+            if (rc) return rc;
+            const long elapsed = seconds_since_start();
+            const long remaining = (long)(n_total - completed);
+            const long time_left = remaining * (elapsed + 1) / (long)completed;
+            const long integ_s = (long)(0.001 * integ_ms);
+            std::ostringstream m;
+            m << "\n";
+            m << "Trajectories completed: " << std::setw(20) << percentage_done << " %" << "\n";
+            m << "Est. time to finish:    " << std::setw(20) << time_left << " seconds" << "\n";
+            m << "Time allocated for job: " << std::setw(20) << "inf" << " (No WM)" << "\n";   // no workload manager
+            m << "SIMULATION EFFICIENCY: " << "\n" << "    " << 100 * integ_s / (elapsed + 1) << " %"
+              << " of time was spent computing trajectories." << "\n";
+            std::fputs(m.str().c_str(), stderr);
+         }
+         if (done >= n_total) break;
+         if (launches > guard) return fail(s, SPX_ERR_STATE, "run did not drain");
+         float ms = 0.f;
+         HIPCHK(s, s->pool.launch(burst, &ms));
+         integ_ms += ms;
+         launches += burst;
+         if (burst < 8) burst *= 2;
+      }
+      const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();
+      rc = spx_finish(s, nullptr, &run);
+      if (rc) return rc;
+      run.wall_s = wall;
+      run.kernel_ms = integ_ms;
+      run.launches = launches;
+      s->next_traj += (uint64_t)n_total;
+      // WorkerDuties (:266-273) keeps the extremes of ElapsedTime() = t - t0, negative for backward runs
+      if (run.done > run.failed) {
+         shortest = fwd ? run.t_min : 0.0 - run.t_min;
+         longest = fwd ? run.t_max : 0.0 - run.t_max;
+      }
+   }
+   if (st) *st = run;
+   // MasterFinish (:776-811)
+   std::fprintf(stderr, "%s:%d: info message: Simulation completed\n", __FILE__, __LINE__);
+   rc = dump_all(s);
+   if (rc) return rc;
+   std::ostringstream m;
+   m << "Shortest simulated trajectory time = " << shortest * unit_time << " s" << "\n";
+   m << "Longest simulated trajectory time = " << longest * unit_time << " s" << "\n";
+   m << "Time per trajectory integration = " << integ_ms / (double)n_total << " ms" << "\n";
+   m << "Total time of this simulation: " << std::setw(20) << seconds_since_start() << " seconds." << "\n";
+   std::fputs(m.str().c_str(), stderr);
+   return SPX_OK;
 }
 
 const char* spx_last_error(const spx_sim* s) { return s ? s->err.c_str() : "null simulation"; }

@@ -80,6 +80,10 @@ def lib() -> C.CDLL:
         L.spx_restore_distro.argtypes = [C.c_void_p, C.c_int, C.c_char_p]
         L.spx_print_distro1d.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_char_p, C.c_int]
         L.spx_print_distro2d.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_int]
+        L.spx_set_tasks.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_int64]
+        L.spx_distro_file_name.argtypes = [C.c_void_p, C.c_char_p]
+        L.spx_restore_distro_at_start.argtypes = [C.c_void_p, C.c_int]
+        L.spx_main_loop.argtypes = [C.c_void_p, C.POINTER(SpxRng), C.POINTER(SpxRunStats)]
         L.spx_autotune.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(SpxTuneEntry), C.c_int, C.POINTER(C.c_int)]
         _lib = L
     return _lib
@@ -229,6 +233,28 @@ class Simulation:
         rec = np.zeros(self._n, dtype=RECORD_DTYPE) if records else None
         self._check(lib().spx_finish(self.h, rec.ctypes.data if records else None, C.byref(st)), "finish")
         return RunStats.of(st), rec
+
+    # --- SimulationMaster's task loop (src/simulation.cc:496-519, :818-845) --------------------------------
+    def set_tasks(self, n_traj: int, batch_size: int = 1, max_traj_per_worker: int = 0) -> None:
+        """SetTasks: the main loop's trajectory count and batch size (with the reference's clamps)."""
+        self._check(lib().spx_set_tasks(self.h, n_traj, batch_size, max_traj_per_worker), "set_tasks")
+
+    def distro_file_name(self, base: str) -> None:
+        """DistroFileName: distributions are written to <base><d>.out (default "distribution_")."""
+        self._check(lib().spx_distro_file_name(self.h, base.encode()), "distro_file_name")
+
+    def restore_distro_at_start(self, d: int) -> None:
+        """SimulationMaster::RestoreDistro: the main loop starts from <base><d>.out."""
+        self._check(lib().spx_restore_distro_at_start(self.h, d), "restore_distro_at_start")
+
+    def main_loop(self, seed: int = 1, normals=None, uniforms=None) -> RunStats:
+        """MainLoop: reset (and restore) the distributions, integrate the tasks with progress dumps and
+        reports on stderr, dump the final distributions."""
+        keep: list = []
+        rng = self._rng(seed, normals, uniforms, keep)
+        st = SpxRunStats()
+        self._check(lib().spx_main_loop(self.h, C.byref(rng), C.byref(st)), "main_loop")
+        return RunStats.of(st)
 
     def autotune(self, n_probe: int, seed: int = 1):
         """Pick resident slots and steps per launch by timing short probe runs; returns the table."""
