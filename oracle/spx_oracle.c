@@ -802,7 +802,7 @@ static double bnd_eval(const bnd_t* b, bst_t* st, int traj_type, double t, vec3 
       st->normal = b->norm;
       return vdot(vsub(pos, b->origin), b->norm);
    case BK_MOMENTUM:  /* src/boundary_momentum.cc:69-76 */
-      if (traj_type == ORC_TRAJ_GUIDING_DIFF_SCATT) return vnorm(mom) - b->momentum;
+      if (traj_type >= ORC_TRAJ_GUIDING_DIFF_SCATT) return vnorm(mom) - b->momentum;
       return mom.v[0] - b->momentum;
    }
    return 0.0;
@@ -854,7 +854,7 @@ static void dist_event(const orc_sim* s, const dist_t* d, int action,
    vec3 value = V(0, 0, 0);
    double w[9] = {0};
    int traj_type = s->traj_type;
-   int parker = (traj_type != ORC_TRAJ_GUIDING_DIFF_SCATT);
+   int parker = (traj_type < ORC_TRAJ_GUIDING_DIFF_SCATT);
    ev->valid = 0;
    /* EvaluateValue */
    switch (d->kind) {
@@ -960,7 +960,7 @@ static inline vec3 gc_convert(vec3 m) { return V(sqrt(Sqr(m.v[0]) + Sqr(m.v[2]))
 
 static int common_fields(const orc_sim* s, lane_t* L, traj_t* T, uint16_t mask)
 {
-   double pm = (s->traj_type == ORC_TRAJ_GUIDING_DIFF_SCATT) ? gc_convert(T->mom).v[0] : T->mom.v[0];
+   double pm = (s->traj_type >= ORC_TRAJ_GUIDING_DIFF_SCATT) ? gc_convert(T->mom).v[0] : T->mom.v[0];
    return get_fields(&s->bg, L, T->t, T->pos, pm, mask, &T->sp);
 }
 
@@ -1012,7 +1012,7 @@ static void handle_boundaries(const orc_sim* s, traj_t* T)
          active = b;
          if (st & BND_REFLECT) {
             T->pos = vsub(T->pos, vscl(2.0 * T->bs[b].delta, T->bs[b].normal));
-            if (s->traj_type == ORC_TRAJ_GUIDING_DIFF_SCATT) T->mom.v[2] = -T->mom.v[2];  /* src/trajectory_guiding.hh:121-124 */
+            if (s->traj_type >= ORC_TRAJ_GUIDING_DIFF_SCATT) T->mom.v[2] = -T->mom.v[2];  /* src/trajectory_guiding.hh:121-124 */
             T->n_refl++;
             bnd_compute(&s->bs[b], &T->bs[b], s->traj_type, T->t, T->pos, T->mom);
             T->bs[b].crossings_left--;
@@ -1257,10 +1257,130 @@ static int gcds_advance(const orc_sim* s, lane_t* L, traj_t* T)
    return ORC_EX_NONE;
 }
 
+/* TrajectoryGuiding::Slopes (src/trajectory_guiding.cc:109-124, PPERP_METHOD 1) with the drift of the
+   present momentum */
+static void gcg_slopes(traj_t* T, vec3 vel, vec3* drift, vec3* slope_mom)
+{
+   const spd_t* sp = &T->sp;
+   vec3 Es, Bs;
+   gc_drift(T, vel, drift, &Es, &Bs);
+   slope_mom->v[0] = 0.5 * T->mom.v[0] / sp->Bmag * (sp->dBmagdt + vdot(*drift, sp->gradBmag));
+   slope_mom->v[1] = 0.0;
+   slope_mom->v[2] = sp_charge * vdot(Es, Bs) / vdot(Bs, sp->bhat);
+}
+
+/* RKStep with the Euler table (src/trajectory_base.cc:362-389), forward flow, then HandleBoundaries and
+   the end-of-step fields (src/trajectory_base.cc:566-573) */
+static int gc_finish_step(const orc_sim* s, lane_t* L, traj_t* T, vec3 slope_pos, vec3 slope_mom)
+{
+   double dtv = T->dt * 1.0;
+   T->t += T->dt;
+   T->pos = vadd(T->pos, vscl(dtv, slope_pos));
+   T->mom = vadd(T->mom, vscl(dtv, slope_mom));
+   T->amp += dtv * 0.0 * exp(T->wgt);
+   T->wgt += dtv * 0.0;
+   return 0;
+}
+
+static int gc_end_step(const orc_sim* s, lane_t* L, traj_t* T)
+{
+   handle_boundaries(s, T);
+   if (!(T->status & TRAJ_FINISH)) {
+      if (common_fields(s, L, T, BG_ALL | BG_gradB | BG_dBdt)) EX(ORC_EX_FIELD);
+   }
+   T->segs++;
+   return ORC_EX_NONE;
+}
+
+/* TrajectoryGuiding::Advance = RKAdvance (src/trajectory_guiding.cc:131-134, src/trajectory_base.cc:545-579):
+   Slopes, PhysicalStep (src/trajectory_guiding.cc:96-100), the time step, RKStep; no random numbers */
+static int gcg_advance(const orc_sim* s, lane_t* L, traj_t* T)
+{
+   spd_t* sp = &T->sp;
+   vec3 vel = VelCart(T->mom);
+   vec3 drift, slope_mom;
+   gcg_slopes(T, vel, &drift, &slope_mom);
+   T->dt_physical = 0.5 * sp->dmax / (vnorm(drift) + 0.5 * vnorm(vel));
+   T->dt = fmin(T->dt_physical, T->dt_adaptive);
+   time_boundary_proximity(s, T);
+   gc_finish_step(s, L, T, drift, slope_mom);
+   return gc_end_step(s, L, T);
+}
+
+/* TrajectoryGuidingScatt::Advance (src/trajectory_guiding_scatt.cc:272-340): DiffusionCoeff (D_mumu and
+   its mu derivative), DriftCoeff, TrajectoryGuiding::PhysicalStep and the pitch-angle limits (:255-262),
+   EulerPitchAngleScatt (:81-106), Slopes with the scattered momentum, RKStep */
+static int gcs_advance(const orc_sim* s, lane_t* L, traj_t* T)
+{
+   spd_t* sp = &T->sp;
+   kap_t kp = {{0, 0, 0}};
+   vec3 vel = VelCart(T->mom);
+   vec3 mc = gc_convert(T->mom);
+   double Dmumu = df_component(&s->df, &kp, 2, T->pos, mc.v[0], mc.v[1], sp);
+   double Vmu = df_mu_deriv(&s->df, mc.v[1]);
+   vec3 drift, Es, Bs;
+   gc_drift(T, vel, &drift, &Es, &Bs);
+   T->dt_physical = 0.5 * sp->dmax / (vnorm(drift) + 0.5 * vnorm(vel));
+   {
+      const double dthetamax = 2.0 * M_PI / 180.0;
+      double amu = fabs(T->mom.v[2] / vnorm(T->mom));
+      double dmumax = sqrt(1 - amu) * dthetamax + 0.5 * amu * Sqr(dthetamax);
+      T->dt_physical = fmin(T->dt_physical, 0.5 * Sqr(dmumax) / Dmumu);
+      T->dt_physical = fmin(T->dt_physical, 0.5 * dmumax / fabs(Vmu));
+   }
+   T->dt = fmin(T->dt_physical, T->dt_adaptive);
+   time_boundary_proximity(s, T);
+   {
+      double mu_new;
+      vec3 m = gc_convert(T->mom);
+      int guard = 0;
+      do {
+         mu_new = m.v[1] + Vmu * T->dt + sqrt(2.0 * Dmumu * T->dt) * draw_normal(L);
+         if (L->exhausted) EX(ORC_EX_REPLAY_EXHAUSTED);
+         if (++guard > 100000) EX(ORC_EX_STEP_NAN);
+      } while (fabs(mu_new) > 1.0);
+      T->mom.v[0] = m.v[0] * sqrt(1 - Sqr(mu_new));
+      T->mom.v[2] = m.v[0] * mu_new;
+      vel = VelCyl(T->mom);
+   }
+   vec3 slope_mom;
+   gcg_slopes(T, vel, &drift, &slope_mom);
+   gc_finish_step(s, L, T, drift, slope_mom);
+   return gc_end_step(s, L, T);
+}
+
+/* TrajectoryGuidingDiff::Advance (src/trajectory_guiding_diff.cc:324-370), forward flow: Slopes with
+   V_perp (:290-299), PhysicalStep (:306-314), EulerPerpDiffSlopes (:123-140), RKStep, the perpendicular
+   displacement */
+static int gcd_advance(const orc_sim* s, lane_t* L, traj_t* T)
+{
+   spd_t* sp = &T->sp;
+   kap_t kp = {{0, 0, 0}};
+   vec3 fa[3], Vperp, drift, slope_mom;
+   double Dperp;
+   vec3 vel = VelCart(T->mom);
+   gcg_slopes(T, vel, &drift, &slope_mom);
+   gcd_diffcoeff(s, T, &kp, fa, &Dperp, &Vperp);
+   vec3 slope_pos = vadd(drift, Vperp);
+   T->dt_physical = 0.5 * sp->dmax / (vnorm(vadd(drift, Vperp)) + 0.5 * vnorm(vel));
+   T->dt_physical = fmin(T->dt_physical, 0.5 * Sqr(sp->dmax) / Dperp);
+   T->dt = fmin(T->dt_physical, T->dt_adaptive);
+   time_boundary_proximity(s, T);
+   vec3 mconv = gc_convert(T->mom);
+   double dWx = sqrt(T->dt) * draw_normal(L);
+   double dWy = sqrt(T->dt) * draw_normal(L);
+   if (L->exhausted) EX(ORC_EX_REPLAY_EXHAUSTED);
+   Dperp = df_component(&s->df, &kp, 0, T->pos, mconv.v[0], mconv.v[1], sp);
+   vec3 dr = from_basis(V(sqrt(2.0 * Dperp) * dWx, sqrt(2.0 * Dperp) * dWy, 0.0), fa);
+   gc_finish_step(s, L, T, slope_pos, slope_mom);
+   T->pos = vadd(T->pos, dr);
+   return gc_end_step(s, L, T);
+}
+
 /* Initial momentum sample (src/initial_momentum.cc:283-410) */
 static vec3 mom_sample(const orc_sim* s, lane_t* L)
 {
-   int gc = (s->traj_type == ORC_TRAJ_GUIDING_DIFF_SCATT);
+   int gc = (s->traj_type >= ORC_TRAJ_GUIDING_DIFF_SCATT);
    double p;
    switch (s->im_kind) {
    case IM_FIXED: return s->im_mom;
@@ -1291,7 +1411,6 @@ static vec3 mom_sample(const orc_sim* s, lane_t* L)
    TrajectoryBase::Integrate (src/trajectory_base.cc:946-995).  Returns exception code. */
 static int integrate_one(const orc_sim* s, lane_t* L, traj_t* T)
 {
-   int gc = (s->traj_type == ORC_TRAJ_GUIDING_DIFF_SCATT);
    int b;
    int64_t** hc = T->hc; double** hs = T->hs; int64_t* hev = T->hev;
    memset(T, 0, sizeof(*T));
@@ -1317,7 +1436,14 @@ static int integrate_one(const orc_sim* s, lane_t* L, traj_t* T)
    if (s->traj_type == ORC_TRAJ_PARKER_SOURCE) { L->src_del_old = 0.0; (void)source_term(s, L, T->mom.v[0], 1.0); }
 
    while (!(T->status & (TRAJ_FINISH | TRAJ_DISCARD))) {
-      int ex = gc ? gcds_advance(s, L, T) : parker_advance(s, L, T);
+      int ex;
+      switch (s->traj_type) {
+      case ORC_TRAJ_GUIDING_DIFF_SCATT: ex = gcds_advance(s, L, T); break;
+      case ORC_TRAJ_GUIDING: ex = gcg_advance(s, L, T); break;
+      case ORC_TRAJ_GUIDING_SCATT: ex = gcs_advance(s, L, T); break;
+      case ORC_TRAJ_GUIDING_DIFF: ex = gcd_advance(s, L, T); break;
+      default: ex = parker_advance(s, L, T);
+      }
       if (ex) return ex;
       if (T->segs > s->max_steps) EX(ORC_EX_MAX_STEPS);
       if (T->dt < sp_tiny * T->sp.dmax / c_code) EX(ORC_EX_STEP_TOO_SMALL);
@@ -1582,7 +1708,7 @@ int orc_add(orc_sim* s, int role, const char* class_name, const uint8_t* bytes, 
 int64_t orc_run(orc_sim* s, uint64_t first_traj, uint64_t n_traj, const orc_rng* rng, int n_threads, orc_traj* per_traj)
 {
    if (!s->have_bg || s->n_t < 1) return -1;
-   if (s->traj_type != ORC_TRAJ_GUIDING_DIFF_SCATT && !s->have_df) return -1;
+   if (s->traj_type < ORC_TRAJ_GUIDING_DIFF_SCATT && !s->have_df) return -1;
    int nthr = n_threads > 0 ? n_threads : 1;
    int64_t total_steps = 0;
    int d;

@@ -23,7 +23,8 @@ extern "C" {
 enum { ORC_BACKGROUND = 0, ORC_DIFFUSION = 1, ORC_SOURCE = 2, ORC_BOUNDARY = 3, ORC_INITIAL = 4, ORC_DISTRIBUTION = 5 };
 
 /* Transport families (configure.ac:237-261 TRAJ_TYPE) and time flow (configure.ac:264-280). */
-enum { ORC_TRAJ_PARKER = 0, ORC_TRAJ_PARKER_SOURCE = 1, ORC_TRAJ_GUIDING_DIFF_SCATT = 2 };
+enum { ORC_TRAJ_PARKER = 0, ORC_TRAJ_PARKER_SOURCE = 1, ORC_TRAJ_GUIDING_DIFF_SCATT = 2, ORC_TRAJ_GUIDING = 3,
+       ORC_TRAJ_GUIDING_SCATT = 4, ORC_TRAJ_GUIDING_DIFF = 5 };
 enum { ORC_FLOW_FORWARD = 0, ORC_FLOW_BACKWARD = 1 };
 
 /* RNG modes: counter-based Philox4x32-10 streams, or replay of supplied draws. */

@@ -501,7 +501,8 @@ static int check_ready(spx_sim* s)
    if (!s->have_bg) return fail(s, SPX_ERR_NOT_READY, "no background");
    if (!s->have_it || !s->have_is || !s->have_im) return fail(s, SPX_ERR_NOT_READY, "initial condition missing");
    if (s->plan.n_bt < 1) return fail(s, SPX_ERR_NOT_READY, "at least one time boundary is required");
-   if (!s->have_df) return fail(s, SPX_ERR_NOT_READY, "no diffusion model");
+   // TrajectoryGuiding (no diffusion, src/trajectory_guiding.hh) integrates without a diffusion model
+   if (!s->have_df && s->plan.traj_type != SPX_TRAJ_GUIDING) return fail(s, SPX_ERR_NOT_READY, "no diffusion model");
    if (s->plan.traj_type == SPX_TRAJ_PARKER_SOURCE && s->plan.src_kind == SPX_SRC_NONE) return fail(s, SPX_ERR_NOT_READY, "no source");
    if (s->plan.bg.kind == SPX_BG_GRID && !s->grid) return fail(s, SPX_ERR_NOT_READY, "BackgroundServerCartesian has no grid data");
    return SPX_OK;
@@ -510,7 +511,7 @@ static int check_ready(spx_sim* s)
 // Combinations the kernels implement (the reference rejects the others at compile time).
 static int check_transport(spx_sim* s)
 {
-   if (s->plan.traj_type == SPX_TRAJ_GUIDING_DIFF_SCATT) {
+   if (s->plan.traj_type >= SPX_TRAJ_GUIDING_DIFF_SCATT) {
       if (s->plan.time_flow != SPX_FLOW_FORWARD) return fail(s, SPX_ERR_ARG, "guiding-centre transport runs forward in time");
       if (s->plan.bg.kind != SPX_BG_SOLARWIND && s->plan.bg.kind != SPX_BG_UNIFORM && s->plan.bg.kind != SPX_BG_GRID)
          return fail(s, SPX_ERR_ARG, "guiding-centre transport supports the static SolarWind, Uniform and Cartesian-grid backgrounds");
@@ -527,7 +528,7 @@ extern "C" {
 
 int spx_create(int traj_type, int time_flow, spx_sim** out)
 {
-   if (!out || traj_type < 0 || traj_type > 2 || (time_flow != 0 && time_flow != 1)) return SPX_ERR_ARG;
+   if (!out || traj_type < 0 || traj_type > SPX_TRAJ_GUIDING_DIFF || (time_flow != 0 && time_flow != 1)) return SPX_ERR_ARG;
    spx_sim* s = new spx_sim();
    std::memset(&s->plan, 0, sizeof(s->plan));
    s->plan.traj_type = traj_type;

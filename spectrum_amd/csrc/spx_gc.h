@@ -260,7 +260,13 @@ __device__ __forceinline__ int gc_step(PlanPtr P, LaunchPtr L, const spx_rng_des
 {
    const SPX_CP spx_bg_t<R>& bg = PlanView<R>::bg(P);
    const SPX_CP spx_df_t<R>& df = PlanView<R>::df(P);
-   const bool scatt = (df.kind == SPX_DF_ISO || df.kind == SPX_DF_PERPISO);
+   // the four guiding-centre transports differ in two terms: pitch-angle scattering (GuidingScatt,
+   // GuidingDiffScatt) and perpendicular diffusion (GuidingDiff, GuidingDiffScatt); without them the
+   // step draws no numbers for them and its coefficients are zero
+   const int tt = P->traj_type;
+   const bool pa_step = (tt == SPX_TRAJ_GUIDING_DIFF_SCATT || tt == SPX_TRAJ_GUIDING_SCATT);
+   const bool perp_step = (tt == SPX_TRAJ_GUIDING_DIFF_SCATT || tt == SPX_TRAJ_GUIDING_DIFF);
+   const bool scatt = pa_step && (df.kind == SPX_DF_ISO || df.kind == SPX_DF_PERPISO);
    R pp = R(H.pp), pl = R(H.pl);
    const double pmag_d = sqrt(H.pp * H.pp + H.pl * H.pl);
    v3<R> vel = gc_vel(pp, pl);
@@ -271,17 +277,18 @@ __device__ __forceinline__ int gc_step(PlanPtr P, LaunchPtr L, const spx_rng_des
    // DiffusionCoeff: D_perp, V_perp, D_mumu and its mu derivative (src/diffusion_other.cc:64-79)
    R pmag = R(pmag_d);
    const R mu = pl / norm(mk<R>(pp, R(0), pl));
-   R Dperp;
-   v3<R> Vperp;
-   gc_perp(df, H.F, fa, pmag, Dperp, Vperp);
+   R Dperp = R(0);
+   v3<R> Vperp = mk<R>(R(0), R(0), R(0));
+   if (perp_step) gc_perp(df, H.F, fa, pmag, Dperp, Vperp);
    const R Dmumu = scatt ? df.D0 * (R(1) - sqr(mu)) : R(0);
    const R Vmu = scatt ? R(-2) * df.D0 * mu : R(0);
    v3<R> drift, Es, Bs;
    gc_drift(H.F, vel, pp, pl, drift, Es, Bs);
-   // PhysicalStep
-   R dtp = R(0.5) * H.F.dmax / (norm(drift + Vperp) + R(0.5) * norm(vel));
-   dtp = fmin(dtp, R(0.5) * sqr(H.F.dmax) / Dperp);
-   {
+   // PhysicalStep: TrajectoryGuiding's (src/trajectory_guiding.cc:96-100), GuidingDiff's with V_perp and
+   // D_perp (src/trajectory_guiding_diff.cc:306-314), GuidingScatt's pitch-angle limits (:255-262)
+   R dtp = R(0.5) * H.F.dmax / ((perp_step ? norm(drift + Vperp) : norm(drift)) + R(0.5) * norm(vel));
+   if (perp_step) dtp = fmin(dtp, R(0.5) * sqr(H.F.dmax) / Dperp);
+   if (pa_step) {
       const R dthetamax = R(2.0 * M_PI / 180.0);
       const R amu = fabs(pl / norm(mk<R>(pp, R(0), pl)));
       const R dmumax = rsqrt_(R(1) - amu) * dthetamax + R(0.5) * amu * sqr(dthetamax);
@@ -292,8 +299,8 @@ __device__ __forceinline__ int gc_step(PlanPtr P, LaunchPtr L, const spx_rng_des
    dt = time_proximity<1>(P, H, dt, (double)H.F.dmax);
    const R dtr = R(dt);
    // EulerPitchAngleScatt: redraw until |mu'| <= 1
-   double ppd, pld;
-   {
+   double ppd = H.pp, pld = H.pl;
+   if (pa_step) {
       const R sd = rsqrt_(R(2) * Dmumu * dtr);
       R mu_new;
       int guard = 0;
@@ -314,22 +321,26 @@ __device__ __forceinline__ int gc_step(PlanPtr P, LaunchPtr L, const spx_rng_des
    const R slope_pp = R(0.5) * pp / H.F.Bmag * dot(drift, H.F.gradBmag);
    const R slope_pl = R(sp_charge) * dot(Es, Bs) / dot(Bs, H.F.bhat);
    pmag = rsqrt_(sqr(pp) + sqr(pl));
-   gc_perp(df, H.F, fa, pmag, Dperp, Vperp);
-   const v3<R> slope_pos = drift + Vperp;
-   // EulerPerpDiffSlopes
-   const R sdt = rsqrt_(dtr);
-   const R dWx = sdt * next_normal<R>(rng, first, H.rs);
-   const R dWy = sdt * next_normal<R>(rng, first, H.rs);
-   if (H.rs.exhausted) return 5;
-   Dperp = gc_kperp(df, pmag, H.F.Bmag);
-   const v3<R> dr = from_basis(mk<R>(rsqrt_(R(2) * Dperp) * dWx, rsqrt_(R(2) * Dperp) * dWy, R(0)), fa);
+   v3<R> slope_pos = drift;
+   v3<R> dr = mk<R>(R(0), R(0), R(0));
+   if (perp_step) {
+      gc_perp(df, H.F, fa, pmag, Dperp, Vperp);
+      slope_pos = drift + Vperp;
+      // EulerPerpDiffSlopes
+      const R sdt = rsqrt_(dtr);
+      const R dWx = sdt * next_normal<R>(rng, first, H.rs);
+      const R dWy = sdt * next_normal<R>(rng, first, H.rs);
+      if (H.rs.exhausted) return 5;
+      Dperp = gc_kperp(df, pmag, H.F.Bmag);
+      dr = from_basis(mk<R>(rsqrt_(R(2) * Dperp) * dWx, rsqrt_(R(2) * Dperp) * dWy, R(0)), fa);
+   }
    // RKStep (Euler), then the perpendicular displacement
    const R dtv = dtr * R(1);
    H.t += dt;
    H.x += (double)(dtv * slope_pos.x); H.y += (double)(dtv * slope_pos.y); H.z += (double)(dtv * slope_pos.z);
    H.pp = ppd + (double)(dtv * slope_pp);
    H.pl = pld + (double)(dtv * slope_pl);
-   H.x += (double)dr.x; H.y += (double)dr.y; H.z += (double)dr.z;
+   if (perp_step) { H.x += (double)dr.x; H.y += (double)dr.y; H.z += (double)dr.z; }
    bflags = handle_boundaries<true>(P, L, slot, H, C, tid);
    if (!(bflags & 1)) {
       H.pf = sqrt(H.pp * H.pp + H.pl * H.pl);

@@ -97,7 +97,8 @@ typedef struct {
 } spx_ds_desc;
 
 typedef struct {
-   int32_t traj_type, time_flow;        // 0 Parker, 1 ParkerSource, 2 GuidingDiffScatt; 0 forward, 1 backward
+   int32_t traj_type, time_flow;        // 0 Parker, 1 ParkerSource, 2 GuidingDiffScatt, 3 Guiding, 4 GuidingScatt,
+                                        // 5 GuidingDiff; 0 forward, 1 backward
    int32_t n_bnd, n_dist;
    int32_t spectrum_type;               // DISTRO_KINETIC_ENERGY_POWER_LAW_TYPE
    int32_t max_attempts;

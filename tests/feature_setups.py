@@ -137,10 +137,50 @@ def gc_isotropic_only():
     return s
 
 
+def guiding_spiral():
+    """TrajectoryGuiding (no diffusion): gradient and curvature drifts of a 20 MeV ring in the Parker
+    spiral, no diffusion model, a time limit and a (|p|, mu) map."""
+    s = cf.Setup("guiding_spiral", cf.TRAJ_GUIDING, cf.FORWARD)
+    s.add(cf.BACKGROUND, "BackgroundSolarWind", cf.solarwind())
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((1.5 * cf.one_au, 0.3 * cf.one_au, 0.2 * cf.one_au)))
+    s.add(cf.INITIAL, "InitialMomentumShell", DC().insert_double(cf.mom_from_ekin(20.0 * cf.MeV)))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", 0.05 * cf.one_day)))
+    d = dist((8, 8, 1), (3.0, -1.0, -1.0), (3.5, 1.0, 1.0), (0, 0, 0), (1, 1, 1))
+    d.insert_double(1.0).insert_double(0.0).insert_int(1).insert_int(1)
+    s.add(cf.DISTRIBUTION, "DistributionMomentumUniform", d)
+    return s
+
+
+def guiding_scatt_uniform():
+    """TrajectoryGuidingScatt: isotropic pitch-angle scattering in a uniform field along z, no flow."""
+    s = cf.Setup("guiding_scatt_uniform", cf.TRAJ_GUIDING_SCATT, cf.FORWARD)
+    s.add(cf.BACKGROUND, "BackgroundUniform", cf.background_base(DC(), 0.0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, 3.0), 50.0))
+    s.add(cf.DIFFUSION, "DiffusionIsotropicConstant", DC().insert_double(2.0e-3))
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((1.0, 2.0, 0.0)))
+    s.add(cf.INITIAL, "InitialMomentumRing", DC().insert_double(cf.mom_from_ekin(10.0 * cf.MeV)).insert_double(0.6))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", 300.0)))
+    d = dist((16, 16, 0), (-500.0, -1.0, 0.0), (500.0, 1.0, 1.0), (0, 0, 0), (1, 0, 0))
+    d.insert_double(1.0).insert_double(0.0).insert_int(1)
+    s.add(cf.DISTRIBUTION, "DistributionPositionPitchUniform", d)
+    return s
+
+
+def guiding_diff_rigidity():
+    """TrajectoryGuidingDiff: perpendicular diffusion with a |B|-dependent rigidity law (grad kappa drift)
+    in the Parker spiral, a reflecting sphere, no pitch-angle scattering."""
+    s = gc_rigidity_reflect()
+    s.name, s.traj_type = "guiding_diff_rigidity", cf.TRAJ_GUIDING_DIFF
+    return s
+
+
 ALL = [shock_flowmom_reflect, solarwind_kinrad_reflect_recurrent, smoothshock_perp_momentum, uniform_para_plane_pass,
-       lambda: modulation_spectrum_type(0), lambda: modulation_spectrum_type(2), gc_rigidity_reflect, gc_isotropic_only]
+       lambda: modulation_spectrum_type(0), lambda: modulation_spectrum_type(2), gc_rigidity_reflect, gc_isotropic_only,
+       guiding_spiral, guiding_scatt_uniform, guiding_diff_rigidity]
 NAMES = ["shock_flowmom_reflect", "solarwind_kinrad_reflect_recurrent", "smoothshock_perp_momentum", "uniform_para_plane_pass",
-         "modulation_type0", "modulation_type2", "gc_rigidity_reflect", "gc_isotropic_only"]
+         "modulation_type0", "modulation_type2", "gc_rigidity_reflect", "gc_isotropic_only",
+         "guiding_spiral", "guiding_scatt_uniform", "guiding_diff_rigidity"]
 
 
 def weight_atol(name, rec, rtol):
