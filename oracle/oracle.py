@@ -14,6 +14,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 FLOPS_LIB_PATH = os.path.join(HERE, "liboracle_flops.so")   # same source, operation-counting scalar
+LIBM_LIB_PATH = os.path.join(HERE, "liboracle_libm.so")     # same source, pow / cbrt one ulp larger
 FMA_LIB_PATH = os.path.join(HERE, "liboracle_fma.so")       # same source, multiply-adds contracted (last-bit perturbation)
 
 RNG_COUNTER, RNG_REPLAY = 0, 1
@@ -63,6 +64,8 @@ def lib(path: str = LIB_PATH):
         L.orc_kappa.restype = C.c_double
         L.orc_set_grid.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32, C.c_void_p]
         L.orc_grid_fields.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_kappa_at.argtypes = [C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_double), C.c_double, C.c_double]
+        L.orc_kappa_at.restype = C.c_double
         L.orc_kappa.argtypes = [C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double]
         L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.orc_counter_normal.restype = C.c_double
@@ -170,6 +173,10 @@ class Oracle:
         a = np.frombuffer(out, dtype=np.float64).copy()
         return {"U": a[0:3], "B": a[3:6], "Bmag": a[6], "gradU": a[7:16].reshape(3, 3), "gradB": a[16:25].reshape(3, 3),
                 "gradBmag": a[25:28], "dmax": a[28], "E": a[29:32]}
+
+    def kappa_at(self, comp, pos, p, mu=0.0, t=0.0):
+        """Diffusion component `comp` with the background's own fields and region indicators at pos."""
+        return self.L.orc_kappa_at(self.h, comp, t, (C.c_double * 3)(*pos), p, mu)
 
     def kappa(self, comp, pos, p, mu=0.0, Bmag=1.0, t=0.0):
         P = (C.c_double * 3)(*pos)

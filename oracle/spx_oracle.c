@@ -24,6 +24,15 @@
 #include <stdio.h>
 #ifdef _OPENMP
 #include <omp.h>
+
+#ifdef ORC_PERTURB_LIBM
+/* The last-bit-perturbed build of the power functions (a GPU's libm may round them one unit differently):
+   test infrastructure for measuring how strongly a trajectory amplifies such differences. */
+static double orc_pow_1ulp(double a, double b) { return pow(a, b) * (1.0 + 2.220446049250313e-16); }
+static double orc_cbrt_1ulp(double a) { return cbrt(a) * (1.0 + 2.220446049250313e-16); }
+#define pow orc_pow_1ulp
+#define cbrt orc_cbrt_1ulp
+#endif
 #endif
 
 /* ------------------------------------------------------------------------------------------------
@@ -91,6 +100,7 @@ static const double unit_charge = E_CGS;
 #define MAX_ACT 8
 
 static inline double Sqr(double x) { return x * x; }
+static inline double Cube(double x) { return x * x * x; }   /* common/definitions.hh:198-201 */
 
 /* ------------------------------------------------------------------------------------------------
  * GeoVector / GeoMatrix restated (common/vectors.hh, common/simple_array.hh, common/matrix.hh)
@@ -311,7 +321,7 @@ typedef struct {
    double dom_min[3], dom_max[3], block_length[3];
    double* data;
 } grid_t;
-enum { DF_NONE = 0, DF_ISO, DF_PARA, DF_PERP, DF_FULL, DF_FLOWMOM, DF_KINRAD, DF_RIGBMAG, DF_PERPISO };
+enum { DF_NONE = 0, DF_ISO, DF_PARA, DF_PERP, DF_FULL, DF_FLOWMOM, DF_KINRAD, DF_RIGBMAG, DF_PERPISO, DF_STRAUSS, DF_POTGIETER, DF_QLT };
 enum { BK_TIME = 0, BK_SPHERE, BK_PLANE, BK_MOMENTUM };
 enum { IT_FIXED = 0 };
 enum { IS_FIXED = 0 };
@@ -340,6 +350,8 @@ typedef struct {
    double kap0, U0, pow_law_U, p0, pow_law_p, kap_rat;
    double T0, r0, pow_law_T, pow_law_r; int stream_dep_idx; double u_up, w_sh, s_sh, dn_up_rat;
    double lam0, R0, B0, pow_law_R, pow_law_B;
+   int LISM_idx; double lam_in, lam_out, kap_rat_in, kap_rat_out;   /* Strauss (lam) / Potgieter (kappa) */
+   double A2A, l_max, ps_index, k_min, ps_minus;                     /* QLTConstant */
 } df_t;
 
 typedef struct {
@@ -749,6 +761,28 @@ static double df_component(const df_t* df, kap_t* kp, int comp, vec3 pos, double
       if (comp == 0) kp->K[0] = df->Dperp;
       else if (comp == 2) kp->K[2] = df->D0 * (1.0 - Sqr(mu));
       break;
+   case DF_STRAUSS: case DF_POTGIETER: {  /* :886-906, :968-988 */
+      if (comp == 2) break;
+      double LISM_ind = (df->LISM_idx < 0) ? 0.0 : (sp->region[df->LISM_idx] > 0.0 ? 0.0 : 1.0);
+      double lam_para = LISM_ind * df->lam_out + (1.0 - LISM_ind) * df->lam_in;
+      double B0_eff = LISM_ind * sp->Bmag + (1.0 - LISM_ind) * df->B0;
+      double rig = Rigidity(p);
+      if (df->kind == DF_STRAUSS)
+         kp->K[1] = (lam_para * vmag / 3.0) * (rig < df->R0 ? cbrt(rig / df->R0) : rig / df->R0) * (B0_eff / sp->Bmag);
+      else
+         kp->K[1] = lam_para * (vmag / c_code) * (rig < df->R0 ? 1.0 : sqrt(Cube(rig / df->R0))) * (B0_eff / sp->Bmag);
+      if (comp == 0) {
+         double kap_rat = LISM_ind * df->kap_rat_out + (1.0 - LISM_ind) * df->kap_rat_in;
+         kp->K[0] = kap_rat * kp->K[1];
+      }
+      break;
+   }
+   case DF_QLT: {  /* :145-149 */
+      if (comp == 0 || comp == 1) break;
+      double Omega = CyclotronFrequency(vmag, sp->Bmag);
+      kp->K[2] = 0.25 * M_PI * df->ps_minus * fabs(Omega) * st2 * pow(vmag * df->k_min * fabs(mu / Omega), df->ps_minus) * df->A2A;
+      break;
+   }
    default: break;
    }
    return kp->K[comp];
@@ -771,11 +805,19 @@ static double df_dir_deriv(const df_t* df, const kap_t* kp, int comp, int xyz, v
    }
 }
 
-/* GetMuDerivative overrides */
-static double df_mu_deriv(const df_t* df, double mu)
+/* GetMuDerivative: the overrides, and DiffusionBase's forward difference (src/diffusion_base.cc:226-255)
+   for the classes without one (QLTConstant); kp holds the component 2 just evaluated at mu */
+static double df_mu_deriv(const df_t* df, kap_t* kp, vec3 pos, double p, double mu, const spd_t* sp)
 {
    switch (df->kind) {
    case DF_ISO: case DF_PERPISO: return -2.0 * df->D0 * mu;  /* src/diffusion_other.cc:76-79 */
+   case DF_QLT: {
+      kap_t saved = *kp;
+      double dmu = sp_small * (mu + sp_small < 1.0 ? 1.0 : -1.0);
+      double derivative = (df_component(df, kp, 2, pos, p, mu + dmu, sp) - saved.K[2]) / dmu;
+      *kp = saved;
+      return derivative;
+   }
    default: return 0.0;
    }
 }
@@ -1186,7 +1228,7 @@ static int gcds_advance(const orc_sim* s, lane_t* L, traj_t* T)
    gcd_diffcoeff(s, T, &kp, fa, &Dperp, &Vperp);
    vec3 mc = gc_convert(T->mom);
    Dmumu = df_component(&s->df, &kp, 2, T->pos, mc.v[0], mc.v[1], sp);
-   Vmu = df_mu_deriv(&s->df, mc.v[1]);
+   Vmu = df_mu_deriv(&s->df, &kp, T->pos, mc.v[0], mc.v[1], sp);
    /* DriftCoeff */
    gc_drift(T, vel, &drift, &Es, &Bs);
    /* PhysicalStep: GuidingDiff (src/trajectory_guiding_diff.cc:324-332) then GuidingScatt (src/trajectory_guiding_scatt.cc:255-262) */
@@ -1317,7 +1359,7 @@ static int gcs_advance(const orc_sim* s, lane_t* L, traj_t* T)
    vec3 vel = VelCart(T->mom);
    vec3 mc = gc_convert(T->mom);
    double Dmumu = df_component(&s->df, &kp, 2, T->pos, mc.v[0], mc.v[1], sp);
-   double Vmu = df_mu_deriv(&s->df, mc.v[1]);
+   double Vmu = df_mu_deriv(&s->df, &kp, T->pos, mc.v[0], mc.v[1], sp);
    vec3 drift, Es, Bs;
    gc_drift(T, vel, &drift, &Es, &Bs);
    T->dt_physical = 0.5 * sp->dmax / (vnorm(drift) + 0.5 * vnorm(vel));
@@ -1586,6 +1628,18 @@ static int setup_diffusion(orc_sim* s, const char* cls, dcread* r)
       df->lam0 = rd_d(r); df->R0 = rd_d(r); df->B0 = rd_d(r); df->pow_law_R = rd_d(r); df->pow_law_B = rd_d(r); df->kap_rat = rd_d(r);
    }
    else if (!strcmp(cls, "DiffusionPerpConstantPitchIsotropic")) { df->kind = DF_PERPISO; df->Dperp = rd_d(r); df->D0 = rd_d(r); }
+   else if (!strcmp(cls, "DiffusionStraussEtAl2013") || !strcmp(cls, "DiffusionPotgieterEtAl2015")) {
+      /* src/diffusion_other.cc:869-880, :950-961 */
+      df->kind = !strcmp(cls, "DiffusionStraussEtAl2013") ? DF_STRAUSS : DF_POTGIETER;
+      df->LISM_idx = rd_i(r); df->lam_in = rd_d(r); df->lam_out = rd_d(r); df->R0 = rd_d(r); df->B0 = rd_d(r);
+      df->kap_rat_in = rd_d(r); df->kap_rat_out = rd_d(r);
+   }
+   else if (!strcmp(cls, "DiffusionQLTConstant")) {  /* src/diffusion_other.cc:130-140 */
+      df->kind = DF_QLT;
+      df->A2A = rd_d(r); df->l_max = rd_d(r); df->ps_index = rd_d(r);
+      df->k_min = 2.0 * M_PI / df->l_max;
+      df->ps_minus = df->ps_index - 1.0;
+   }
    else return -1;
    s->have_df = 1;
    return 0;
@@ -1821,6 +1875,18 @@ void orc_background_fields(orc_sim* s, double t, const double pos[3], double p, 
    for (k = 0; k < 3; k++) { out_U[k] = sp.U.v[k]; out_B[k] = sp.B.v[k]; }
    if (out_dmax) *out_dmax = sp.dmax;
    if (out_divU) *out_divU = divU(&sp);
+}
+
+/* One diffusion component with the background's fields (incl. its region indicators) at pos */
+double orc_kappa_at(orc_sim* s, int comp, double t, const double pos[3], double p, double mu)
+{
+   kap_t kp = {{0, 0, 0}};
+   spd_t sp; memset(&sp, 0, sizeof(sp));
+   lane_t L; memset(&L, 0, sizeof(L));
+   vec3 x = V(pos[0], pos[1], pos[2]);
+   bg_evaluate(&s->bg, &L, t, x, BG_ALL, &sp);
+   sp.Bmag = vnorm(sp.B);
+   return df_component(&s->df, &kp, comp, x, p, mu, &sp);
 }
 
 double orc_kappa(orc_sim* s, int comp, double t, const double pos[3], double p, double mu, double Bmag)

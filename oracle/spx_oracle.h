@@ -83,6 +83,7 @@ int orc_set_grid(orc_sim* s, const int32_t nb[3], const double dom_min[3], const
 /* The gridded fields at one point: U(3), B(3), |B|, grad U (9, row = derivative direction),
    grad B (9), grad |B| (3), dmax, E(3).  Returns 0, 1 where the reference fails (field error), -1 without a grid. */
 int orc_grid_fields(orc_sim* s, const double pos[3], double out[32]);
+double orc_kappa_at(orc_sim* s, int comp, double t, const double pos[3], double p, double mu);
 
 /* Building blocks exposed for closed-form tests. */
 void orc_background_fields(orc_sim* s, double t, const double pos[3], double p, double out_U[3], double out_B[3], double* out_dmax, double* out_divU);

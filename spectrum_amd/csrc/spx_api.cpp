@@ -292,6 +292,18 @@ static int add_diffusion(spx_sim* s, const std::string& cls, Reader& r)
       d.lam0 = r.d(); d.R0 = r.d(); d.B0 = r.d(); d.pow_law_R = r.d(); d.pow_law_B = r.d(); d.kap_rat = r.d();
    }
    else if (cls == "DiffusionPerpConstantPitchIsotropic") { d.kind = SPX_DF_PERPISO; d.Dperp = r.d(); d.D0 = r.d(); }
+   else if (cls == "DiffusionStraussEtAl2013" || cls == "DiffusionPotgieterEtAl2015") {   // src/diffusion_other.cc:869-880, :950-961
+      d.kind = cls == "DiffusionStraussEtAl2013" ? SPX_DF_STRAUSS : SPX_DF_POTGIETER;
+      d.region_idx = r.i(); d.lam_in = r.d(); d.lam_out = r.d(); d.R0 = r.d(); d.B0 = r.d();
+      d.kap_rat_in = r.d(); d.kap_rat_out = r.d();
+   }
+   else if (cls == "DiffusionQLTConstant") {   // src/diffusion_other.cc:130-140
+      d.kind = SPX_DF_QLT;
+      d.A2A = r.d();
+      const double l_max = r.d(), ps_index = r.d();
+      d.k_min = (2.0 * M_PI) / l_max;
+      d.ps_minus = ps_index - 1.0;
+   }
    else return fail(s, SPX_ERR_CLASS, "unknown diffusion class " + cls);
    s->have_df = true;
    return SPX_OK;
@@ -437,7 +449,7 @@ static int bind_grid(spx_sim* s)
    const bool k_equal = ((d.kind == SPX_DF_FLOWMOM || d.kind == SPX_DF_KINRAD || d.kind == SPX_DF_RIGBMAG) && d.kap_rat == 1.0) ||
                         (d.kind == SPX_DF_FULL && d.Dperp == d.Dpara);
    const bool need_u = d.kind == SPX_DF_FLOWMOM || (d.kind == SPX_DF_KINRAD && d.stream_dep_idx != 0);
-   const bool need_b = d.kind == SPX_DF_RIGBMAG && d.pow_law_B != 0.0;
+   const bool need_b = (d.kind == SPX_DF_RIGBMAG && d.pow_law_B != 0.0) || d.kind == SPX_DF_STRAUSS || d.kind == SPX_DF_POTGIETER;
    const bool bmag_safe = g.bmag_min * g.f_mag * 0.999 > sp_tiny_h;
    g.fd_fields = (k_equal && !need_u && !need_b && bmag_safe) ? 0 : 1;
    s->plan.bg.grid = g;
@@ -486,8 +498,11 @@ static void finalize_plan(spx_sim* s)
    if (d.kind == SPX_DF_FLOWMOM) d.pos_dep = (d.pow_law_U != 0.0);
    if (d.kind == SPX_DF_KINRAD) d.pos_dep = 1;
    if (d.kind == SPX_DF_RIGBMAG) d.pos_dep = (d.pow_law_B != 0.0);
+   if (d.kind == SPX_DF_STRAUSS || d.kind == SPX_DF_POTGIETER) d.pos_dep = 1;   // through |B| and the region
    spx_df_t<float>& e = P.dff;
-   e.kind = d.kind; e.stream_dep_idx = d.stream_dep_idx; e.pos_dep = d.pos_dep;
+   e.kind = d.kind; e.stream_dep_idx = d.stream_dep_idx; e.pos_dep = d.pos_dep; e.region_idx = d.region_idx;
+   e.lam_in = (float)d.lam_in; e.lam_out = (float)d.lam_out; e.kap_rat_in = (float)d.kap_rat_in; e.kap_rat_out = (float)d.kap_rat_out;
+   e.A2A = (float)d.A2A; e.k_min = (float)d.k_min; e.ps_minus = (float)d.ps_minus;
    e.D0 = (float)d.D0; e.Dperp = (float)d.Dperp; e.Dpara = (float)d.Dpara; e.kap0 = (float)d.kap0; e.U0 = (float)d.U0;
    e.pow_law_U = (float)d.pow_law_U; e.p0 = (float)d.p0; e.pow_law_p = (float)d.pow_law_p; e.kap_rat = (float)d.kap_rat;
    e.T0 = (float)d.T0; e.r0 = (float)d.r0; e.pow_law_T = (float)d.pow_law_T; e.pow_law_r = (float)d.pow_law_r;
@@ -517,6 +532,17 @@ static int check_transport(spx_sim* s)
          return fail(s, SPX_ERR_ARG, "guiding-centre transport supports the static SolarWind, Uniform and Cartesian-grid backgrounds");
       if (s->plan.df.kind == SPX_DF_FLOWMOM || s->plan.df.kind == SPX_DF_KINRAD)
          return fail(s, SPX_ERR_ARG, "guiding-centre transport: diffusion model needs grad U, which its field mask does not compute");
+      if (s->plan.df.kind == SPX_DF_STRAUSS || s->plan.df.kind == SPX_DF_POTGIETER)
+         return fail(s, SPX_ERR_ARG, "guiding-centre transport: the Strauss / Potgieter models run with the Parker transport here");
+   }
+   else if (s->plan.df.kind == SPX_DF_QLT)   // src/diffusion_other.cc:83 compiles it for the guiding-centre transports only
+      return fail(s, SPX_ERR_ARG, "DiffusionQLTConstant needs a pitch-angle transport");
+   if (s->plan.df.kind == SPX_DF_STRAUSS || s->plan.df.kind == SPX_DF_POTGIETER) {
+      // the region indicators the backgrounds define: SolarWind sets region[0] (heliopause) and region[1] (-1) only
+      // (src/background_solarwind.cc:144-148); the gridded server carries none here
+      const int ri = s->plan.df.region_idx;
+      if (ri > 2 || (ri == 2 && s->plan.bg.kind == SPX_BG_SOLARWIND) || (ri >= 0 && s->plan.bg.kind == SPX_BG_GRID))
+         return fail(s, SPX_ERR_ARG, "LISM_idx names a region indicator this background does not set");
    }
    return SPX_OK;
 }

@@ -121,6 +121,30 @@ __device__ __forceinline__ R gc_kperp(const SPX_CP spx_df_t<R>& df, R pmag, R Bm
    }
 }
 
+// TrajectoryGuidingScatt::DiffusionCoeff (src/trajectory_guiding_scatt.cc:61-67): D_mumu = GetComponent(2)
+// and its mu derivative, analytic for the isotropic model (src/diffusion_other.cc:64-79) and the forward
+// difference of DiffusionBase::GetMuDerivative (src/diffusion_base.cc:226-255) for DiffusionQLTConstant
+// (src/diffusion_other.cc:145-149); the other models have no component 2.
+template <typename R>
+__device__ __forceinline__ void gc_dmumu(const SPX_CP spx_df_t<R>& df, R pmag, R Bmag, R mu, R& Dmumu, R& Vmu)
+{
+   if (df.kind == SPX_DF_ISO || df.kind == SPX_DF_PERPISO) {
+      Dmumu = df.D0 * (R(1) - sqr(mu));
+      Vmu = R(-2) * df.D0 * mu;
+   }
+   else if (df.kind == SPX_DF_QLT) {
+      const R vmag = VelP(pmag);
+      const R Omega = CyclotronFrequency(vmag, Bmag);
+      auto k2 = [&](R m) {
+         const R st2 = R(1) - sqr(m);
+         return R(0.25) * R(M_PI) * df.ps_minus * R(fabs(Omega)) * st2 * rpow(R(vmag * df.k_min * R(fabs(m / Omega))), R(df.ps_minus)) * df.A2A;
+      };
+      Dmumu = k2(mu);
+      const R dmu = R(sp_small) * ((mu + R(sp_small) < R(1)) ? R(1) : R(-1));
+      Vmu = (k2(mu + dmu) - Dmumu) / dmu;
+   }
+}
+
 // TrajectoryGuidingDiff::DiffusionCoeff (src/trajectory_guiding_diff.cc:72-98): D_perp and the
 // drift V_perp = grad_perp D_perp in the field-aligned frame.  Only the rigidity power law has a
 // spatial derivative (src/diffusion_other.cc:818-823), and only through |B|.
@@ -266,7 +290,6 @@ __device__ __forceinline__ int gc_step(PlanPtr P, LaunchPtr L, const spx_rng_des
    const int tt = P->traj_type;
    const bool pa_step = (tt == SPX_TRAJ_GUIDING_DIFF_SCATT || tt == SPX_TRAJ_GUIDING_SCATT);
    const bool perp_step = (tt == SPX_TRAJ_GUIDING_DIFF_SCATT || tt == SPX_TRAJ_GUIDING_DIFF);
-   const bool scatt = pa_step && (df.kind == SPX_DF_ISO || df.kind == SPX_DF_PERPISO);
    R pp = R(H.pp), pl = R(H.pl);
    const double pmag_d = sqrt(H.pp * H.pp + H.pl * H.pl);
    v3<R> vel = gc_vel(pp, pl);
@@ -280,8 +303,8 @@ __device__ __forceinline__ int gc_step(PlanPtr P, LaunchPtr L, const spx_rng_des
    R Dperp = R(0);
    v3<R> Vperp = mk<R>(R(0), R(0), R(0));
    if (perp_step) gc_perp(df, H.F, fa, pmag, Dperp, Vperp);
-   const R Dmumu = scatt ? df.D0 * (R(1) - sqr(mu)) : R(0);
-   const R Vmu = scatt ? R(-2) * df.D0 * mu : R(0);
+   R Dmumu = R(0), Vmu = R(0);
+   if (pa_step) gc_dmumu(df, pmag, H.F.Bmag, mu, Dmumu, Vmu);
    v3<R> drift, Es, Bs;
    gc_drift(H.F, vel, pp, pl, drift, Es, Bs);
    // PhysicalStep: TrajectoryGuiding's (src/trajectory_guiding.cc:96-100), GuidingDiff's with V_perp and

@@ -112,10 +112,31 @@ __device__ __forceinline__ bool parker_fields(const SPX_CP spx_bg_t<R>& bg, doub
    return true;
 }
 
+// LISM_ind of the Strauss / Potgieter models at x (src/diffusion_other.cc:891-892, :973-974): SolarWind sets
+// region[0] = +1 inside the heliopause and -1 beyond it, region[1] = -1 (src/background_solarwind.cc:144-146);
+// the other analytic backgrounds set positive regions only (src/background_uniform.cc:68,
+// src/background_shock.cc:99-107, src/background_smooth_shock.cc:151).  r is the norm in the rotated frame;
+// the diagonal +-1 bases leave it unchanged.
+template <typename R, int BG>
+__device__ __forceinline__ bool lism_outer(const SPX_CP spx_bg_t<R>& bg, const SPX_CP spx_df_t<R>& df, double x, double y, double z)
+{
+   if (BG != SPX_BG_SOLARWIND || df.region_idx < 0) return false;
+   if (df.region_idx == 1) return true;
+   const v3<R> d = rel(bg, x, y, z);
+   R r;
+   if (bg.ep_diag != 0) r = norm(d);
+   else {
+      const v3<R> ep[3] = {v3of(bg.eprime[0]), v3of(bg.eprime[1]), v3of(bg.eprime[2])};
+      r = norm(to_basis(d, ep));
+   }
+   return !(r < R(hp_rad_sw));
+}
+
 // DiffusionBase::GetComponent for components 0 and 1 at a position.  DF is the diffusion class when it
-// is known at compile time (0 = any, dispatched at run time).
-template <typename R, int DF>
-__device__ __forceinline__ void parker_kappa(const SPX_CP spx_df_t<R>& df, R km, R Bmag, double x, double y, double z, v3<R> U, R& K0, R& K1)
+// is known at compile time (0 = any, dispatched at run time); km2 is the LISM factor of the region models.
+template <typename R, int BG, int DF>
+__device__ __forceinline__ void parker_kappa(const SPX_CP spx_bg_t<R>& bg, const SPX_CP spx_df_t<R>& df, R km, R km2, R Bmag,
+                                             double x, double y, double z, v3<R> U, R& K0, R& K1)
 {
    if (DF == SPX_DF_RIGBMAG) {   // src/diffusion_other.cc:804-809; pow(x, 0) == 1 exactly
       K1 = km;
@@ -124,6 +145,12 @@ __device__ __forceinline__ void parker_kappa(const SPX_CP spx_df_t<R>& df, R km,
       return;
    }
    if (DF == SPX_DF_FULL) { K0 = df.Dperp; K1 = df.Dpara; return; }   // :538-543
+   if (df.kind == SPX_DF_STRAUSS || df.kind == SPX_DF_POTGIETER) {       // :886-906, :968-988
+      const bool outer = lism_outer<R, BG>(bg, df, x, y, z);
+      K1 = (outer ? km2 : km) * ((outer ? Bmag : df.B0) / Bmag);
+      K0 = (outer ? df.kap_rat_out : df.kap_rat_in) * K1;
+      return;
+   }
    R r = R(0), Umag = R(0);
    if (df.kind == SPX_DF_KINRAD) r = R(norm(mk<double>(x, y, z)));
    if (df.kind == SPX_DF_FLOWMOM || (df.kind == SPX_DF_KINRAD && df.stream_dep_idx != 0)) Umag = norm(U);
@@ -177,6 +204,7 @@ __device__ int parker_set_start(PlanPtr P, QueryPtr Q, LaunchPtr L, const spx_rn
    L->t[slot] = t; L->x[slot] = x; L->y[slot] = y; L->z[slot] = z; L->m0[slot] = p; L->m1[slot] = 0.0; L->m2[slot] = 0.0;
    L->amp[slot] = 0.0; L->wgt[slot] = 0.0; L->dt_prev[slot] = 0.0;
    L->ds_prev[slot] = (double)ds; L->dmax[slot] = (double)F.dmax;
+   L->pf[slot] = p1;   // the first step uses these fields, evaluated with the first sample's magnitude
    L->steps[slot] = 0; L->n_events[slot] = 0;   // the draw counters continue across attempts
    // the adaptive step starts large so that dt is the physical step (src/trajectory_base.cc:894-895)
    L->dt_adapt[slot] = sp_large * (double)F.dmax / c_code;
@@ -237,7 +265,10 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
    // DriftCoeff + DiffusionCoeff (src/trajectory_parker.cc:189-206, 83-153)
    R K0 = R(0), K1 = R(0);
    const R km = (DF == SPX_DF_FULL) ? R(0) : kappa_mom(df, p);
-   parker_kappa<R, DF>(df, km, H.F.Bmag, H.x, H.y, H.z, H.F.U, K0, K1);
+   // the LISM parameters of the Strauss / Potgieter models, when a region indicator can select them
+   const R km2 = (DF == 0 && (df.kind == SPX_DF_STRAUSS || df.kind == SPX_DF_POTGIETER) && df.region_idx >= 0)
+                    ? kappa_mom_region(df, p, true) : R(0);
+   parker_kappa<R, BG, DF>(bg, df, km, km2, H.F.Bmag, H.x, H.y, H.z, H.F.U, K0, K1);
    const R delta = fmin(LarmorRadius(p, H.F.Bmag), H.F.dmax);
    const R hdi = is_f32<R>() ? R(0.5) / delta : R(0);
    v3<R> divK = mk<R>(R(0), R(0), R(0));
@@ -258,7 +289,7 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
          if (loads && Gf.Bmag < R(sp_tiny)) return 1;
          const v3<R> bf = !loads ? Gf.B : (is_f32<R>() ? (R(1) / Gf.Bmag) * Gf.B : Gf.B / Gf.Bmag);
          R K0f = K0, K1f = K1;
-         if (pos_dep) parker_kappa<R, DF>(df, km, Gf.Bmag, xf, yf, zf, Gf.U, K0f, K1f);
+         if (pos_dep) parker_kappa<R, BG, DF>(bg, df, km, km2, Gf.Bmag, xf, yf, zf, Gf.U, K0f, K1f);
          xf -= 2.0 * (double)delta * ex;
          yf -= 2.0 * (double)delta * ey;
          zf -= 2.0 * (double)delta * ez;
@@ -266,7 +297,7 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
          if (loads && Gb.Bmag < R(sp_tiny)) return 1;
          const v3<R> bb = !loads ? Gb.B : (is_f32<R>() ? (R(1) / Gb.Bmag) * Gb.B : Gb.B / Gb.Bmag);
          R K0b = K0, K1b = K1;
-         if (pos_dep) parker_kappa<R, DF>(df, km, Gb.Bmag, xf, yf, zf, Gb.U, K0b, K1b);
+         if (pos_dep) parker_kappa<R, BG, DF>(bg, df, km, km2, Gb.Bmag, xf, yf, zf, Gb.U, K0b, K1b);
          const R bfj = j == 0 ? bf.x : (j == 1 ? bf.y : bf.z);
          const R bbj = j == 0 ? bb.x : (j == 1 ? bb.y : bb.z);
          const R cf = (K1f - K0f) * bfj, cb = (K1b - K0b) * bbj;
@@ -323,7 +354,7 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
          else { Bmf = norm(B2); bf = B2 / Bmf; }
          if (Bmf < R(sp_tiny)) return 1;
          R K0f = K0, K1f = K1;
-         if (pos_dep) parker_kappa<R, DF>(df, km, Bmf, xf, yf, zf, U2, K0f, K1f);
+         if (pos_dep) parker_kappa<R, BG, DF>(bg, df, km, km2, Bmf, xf, yf, zf, U2, K0f, K1f);
          xf -= 2.0 * (double)delta * ex;
          yf -= 2.0 * (double)delta * ey;
          zf -= 2.0 * (double)delta * ez;
@@ -336,7 +367,7 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
          else { Bmb = norm(B2); bb = B2 / Bmb; }
          if (Bmb < R(sp_tiny)) return 1;
          R K0b = K0, K1b = K1;
-         if (pos_dep) parker_kappa<R, DF>(df, km, Bmb, xf, yf, zf, U2, K0b, K1b);
+         if (pos_dep) parker_kappa<R, BG, DF>(bg, df, km, km2, Bmb, xf, yf, zf, U2, K0b, K1b);
          const R bfj = j == 0 ? bf.x : (j == 1 ? bf.y : bf.z);
          const R bbj = j == 0 ? bb.x : (j == 1 ? bb.y : bb.z);
          const R cf = (K1f - K0f) * bfj, cb = (K1b - K0b) * bbj;
@@ -442,10 +473,11 @@ __device__ __forceinline__ void parker_step_body(const Launch* __restrict__ Lg)
          H.bd[b] = Lp->bdelta[b * n + slot]; H.bdo[b] = Lp->bdelta_old[b * n + slot];
          C.cross[b][tid] = Lp->bcross[b * n + slot]; H.bst[b] = Lp->bstat[b * n + slot];
       }
-      // The fields at the current point are a function of (t, x, p) and are recomputed bit for bit;
-      // dmax (which may depend on the previous evaluation) is restored.
+      // The fields at the current point are a function of (t, x, p of their evaluation) and are recomputed
+      // bit for bit: p is the trajectory's momentum after a step, the first momentum sample before the
+      // first (src/trajectory_base.cc:874-879); dmax (which may depend on the previous evaluation) is restored.
       R ds_tmp = H.ds_prev;
-      (void)parker_fields<R, BG>(PlanView<R>::bg(P), H.t, H.x, H.y, H.z, R(H.p), ds_tmp, H.F);
+      (void)parker_fields<R, BG>(PlanView<R>::bg(P), H.t, H.x, H.y, H.z, R(Lp->pf[slot]), ds_tmp, H.F);
       H.F.dmax = R(Lp->dmax[slot]);
    }
    int st = run ? SLOT_RUN : SLOT_IDLE;
@@ -478,6 +510,7 @@ __device__ __forceinline__ void parker_step_body(const Launch* __restrict__ Lg)
    Ls->ex_code[slot] = exc;
    Ls->t[slot] = H.t; Ls->x[slot] = H.x; Ls->y[slot] = H.y; Ls->z[slot] = H.z; Ls->m0[slot] = H.p;
    Ls->steps[slot] = H.steps; Ls->nn[slot] = H.rs.nn;
+   if (steps_here) Ls->pf[slot] = H.p;   // every step ends with fields at the new momentum
    Ls->ds_prev[slot] = (double)H.ds_prev; Ls->dmax[slot] = (double)H.F.dmax;
    Ls->amp[slot] = C.amp[tid]; Ls->wgt[slot] = C.wgt[tid];
    Ls->dt_prev[slot] = C.dt_prev[tid]; Ls->src_old[slot] = C.src_old[tid];

@@ -447,6 +447,22 @@ __device__ __forceinline__ bool grid_eval_gc(const SPX_CP spx_grid_desc& g, doub
 }
 
 // ---- diffusion: DiffusionBase::GetComponent (src/diffusion_base.cc:105-121) + models ----------------
+SPX_HD double rcbrt(double x) { return cbrt(x); }
+SPX_HD float rcbrt(float x) { return cbrtf(x); }
+
+// DiffusionStraussEtAl2013 (src/diffusion_other.cc:886-906) and DiffusionPotgieterEtAl2015 (:968-988):
+// kappa_par = (lam v / 3) f(R) (B0_eff / |B|), resp. kappa (v / c) g(R) (B0_eff / |B|); this is the factor
+// before (B0_eff / |B|) for the inner (heliosphere) or outer (LISM) parameters, in the reference's order.
+template <typename R>
+SPX_HD R kappa_mom_region(const SPX_CP spx_df_t<R>& d, R p, bool outer)
+{
+   const R lam = outer ? d.lam_out : d.lam_in;
+   const R rig = Rigidity(p), vmag = VelP(p);
+   if (d.kind == SPX_DF_STRAUSS) return (lam * vmag / R(3)) * (rig < d.R0 ? rcbrt(rig / d.R0) : rig / d.R0);
+   const R x = rig / d.R0;
+   return lam * (vmag / R(c_code)) * (rig < d.R0 ? R(1) : rsqrt_(x * x * x));
+}
+
 // The momentum-only factor of kappa_par: the same at every point of a step's derivative stencil, so
 // it is formed once per step (the same inputs give the same value).
 template <typename R>
@@ -456,6 +472,7 @@ SPX_HD R kappa_mom(const SPX_CP spx_df_t<R>& d, R p)
    case SPX_DF_FLOWMOM: return rpow(p / d.p0, d.pow_law_p);                                   // :622-627
    case SPX_DF_KINRAD: return d.kap0 * rpow(EnrKin(p) / d.T0, d.pow_law_T);                   // :713-723
    case SPX_DF_RIGBMAG: return (d.lam0 * VelP(p) / R(3)) * rpow(Rigidity(p) / d.R0, d.pow_law_R);   // :804-809
+   case SPX_DF_STRAUSS: case SPX_DF_POTGIETER: return kappa_mom_region(d, p, false);
    default: return R(0);
    }
 }

@@ -15,7 +15,8 @@
 
 enum spx_bg_kind { SPX_BG_NONE = 0, SPX_BG_UNIFORM = 1, SPX_BG_SOLARWIND = 2, SPX_BG_SHOCK = 3, SPX_BG_SMOOTHSHOCK = 4, SPX_BG_GRID = 5 };
 enum spx_df_kind {
-   SPX_DF_NONE = 0, SPX_DF_ISO, SPX_DF_PARA, SPX_DF_PERP, SPX_DF_FULL, SPX_DF_FLOWMOM, SPX_DF_KINRAD, SPX_DF_RIGBMAG, SPX_DF_PERPISO
+   SPX_DF_NONE = 0, SPX_DF_ISO, SPX_DF_PARA, SPX_DF_PERP, SPX_DF_FULL, SPX_DF_FLOWMOM, SPX_DF_KINRAD, SPX_DF_RIGBMAG, SPX_DF_PERPISO,
+   SPX_DF_STRAUSS, SPX_DF_POTGIETER, SPX_DF_QLT
 };
 enum spx_bnd_class { SPX_BC_TIME = 0, SPX_BC_SPACE = 1, SPX_BC_MOMENTUM = 2 };
 enum spx_bnd_kind { SPX_BK_TIME = 0, SPX_BK_SPHERE = 1, SPX_BK_PLANE = 2, SPX_BK_MOMENTUM = 3 };
@@ -72,11 +73,16 @@ typedef spx_bg_t<double> spx_bg_desc;
 
 template <typename T>
 struct spx_df_t {
-   int32_t kind, stream_dep_idx, pos_dep, pad_;
+   int32_t kind, stream_dep_idx, pos_dep;
+   int32_t region_idx;                  // Strauss / Potgieter: LISM_idx, the region indicator read (-1: none)
    T D0, Dperp, Dpara;
    T kap0, U0, pow_law_U, p0, pow_law_p, kap_rat;
    T T0, r0, pow_law_T, pow_law_r, u_up, w_sh, s_sh, dn_up_rat;
    T lam0, R0, B0, pow_law_R, pow_law_B;
+   // DiffusionStraussEtAl2013 (lam_in / lam_out) and DiffusionPotgieterEtAl2015 (kappa_in / kappa_out)
+   T lam_in, lam_out, kap_rat_in, kap_rat_out;
+   // DiffusionQLTConstant: A2A, k_min = 2 pi / l_max, ps_minus = ps_index - 1
+   T A2A, k_min, ps_minus;
 };
 typedef spx_df_t<double> spx_df_desc;
 

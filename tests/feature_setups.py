@@ -175,12 +175,75 @@ def guiding_diff_rigidity():
     return s
 
 
+def parker_strauss_heliopause():
+    """Parker (backward) on the SolarWind background across the 117 au heliopause with the Strauss et al. 2013
+    model selecting inner / outer parameters by region[0], a sphere at 100 au and a spectrum."""
+    s = cf.Setup("parker_strauss_heliopause", cf.TRAJ_PARKER, cf.BACKWARD)
+    s.add(cf.BACKGROUND, "BackgroundSolarWind", cf.solarwind())
+    p0 = cf.mom_from_ekin(1.0 * cf.GeV)
+    dc = DC().insert_int(0).insert_double(0.5 * cf.one_au).insert_double(2.0 * cf.one_au).insert_double(p0 * cf.c_code)
+    dc.insert_double(5.0e-6 / cf.unit_magnetic).insert_double(0.02).insert_double(0.2)
+    s.add(cf.DIFFUSION, "DiffusionStraussEtAl2013", dc)
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((0.6 * 116.8 * cf.one_au, 0.64 * 116.8 * cf.one_au, 0.48 * 116.8 * cf.one_au)))
+    s.add(cf.INITIAL, "InitialMomentumThickShell", DC().insert_double(0.3 * p0).insert_double(3.0 * p0).insert_bool(True))
+    s.add(cf.BOUNDARY, "BoundarySphereAbsorb", cf.boundary(1, [0], ("vec", (0.0, 0.0, 0.0)), ("double", 100.0 * cf.one_au)))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", -5.0 * cf.one_day)))
+    d = dist((8, 1, 1), (0.1 * p0, -1.0, -1.0), (10.0 * p0, 1.0, 1.0), (1, 0, 0), (1, 0, 0))
+    d.insert_double(1.0).insert_double(0.0).insert_int(1).insert_int(0)
+    s.add(cf.DISTRIBUTION, "DistributionMomentumUniform", d)
+    return s
+
+
+def parker_potgieter_uniform():
+    """Parker (forward) in a uniform flow and field with the Potgieter et al. 2015 model (no region) and a
+    reflecting plane."""
+    s = cf.Setup("parker_potgieter_uniform", cf.TRAJ_PARKER, cf.FORWARD)
+    s.add(cf.BACKGROUND, "BackgroundUniform", cf.background_base(DC(), 0.0, (0.0, 0.0, 0.0), (0.002, 0.0, 0.001), (2.0, 1.0, 2.0), 0.5))
+    p0 = cf.mom_from_ekin(50.0 * cf.MeV)
+    dc = DC().insert_int(-1).insert_double(2.0).insert_double(20.0).insert_double(p0 * cf.c_code)
+    dc.insert_double(4.5).insert_double(0.05).insert_double(0.5)
+    s.add(cf.DIFFUSION, "DiffusionPotgieterEtAl2015", dc)
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((0.0, 0.0, 0.0)))
+    s.add(cf.INITIAL, "InitialMomentumThickShell", DC().insert_double(0.5 * p0).insert_double(2.0 * p0).insert_bool(True))
+    s.add(cf.BOUNDARY, "BoundaryPlaneReflect", cf.boundary(-1, [-1], ("vec", (-3.0, 0.0, 0.0)), ("vec", (1.0, 0.0, 0.0))))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", 200.0)))
+    s.add(cf.DISTRIBUTION, "DistributionPositionCumulativeOrder1", dist((10, 0, 0), (0.0, 0, 0), (201.0, 0, 0)))
+    return s
+
+
+def gc_qlt_spiral():
+    """Guiding centre (GuidingDiffScatt) in the Parker spiral with QLT pitch-angle scattering, whose mu
+    derivative is DiffusionBase's forward difference."""
+    s = cf.Setup("gc_qlt_spiral", cf.TRAJ_GUIDING_DIFF_SCATT, cf.FORWARD)
+    s.add(cf.BACKGROUND, "BackgroundSolarWind", cf.solarwind())
+    s.add(cf.DIFFUSION, "DiffusionQLTConstant", DC().insert_double(0.05).insert_double(0.01 * cf.one_au).insert_double(5.0 / 3.0))
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((1.2 * cf.one_au, 0.4 * cf.one_au, 0.1 * cf.one_au)))
+    s.add(cf.INITIAL, "InitialMomentumShell", DC().insert_double(cf.mom_from_ekin(20.0 * cf.MeV)))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", 0.01 * cf.one_day)))
+    d = dist((16, 16, 0), (1.0 * cf.one_au, -1.0, 0.0), (1.5 * cf.one_au, 1.0, 1.0), (0, 0, 0), (1, 0, 0))
+    d.insert_double(1.0).insert_double(0.0).insert_int(1)
+    s.add(cf.DISTRIBUTION, "DistributionPositionPitchUniform", d)
+    return s
+
+
 ALL = [shock_flowmom_reflect, solarwind_kinrad_reflect_recurrent, smoothshock_perp_momentum, uniform_para_plane_pass,
        lambda: modulation_spectrum_type(0), lambda: modulation_spectrum_type(2), gc_rigidity_reflect, gc_isotropic_only,
-       guiding_spiral, guiding_scatt_uniform, guiding_diff_rigidity]
+       guiding_spiral, guiding_scatt_uniform, guiding_diff_rigidity, parker_strauss_heliopause, parker_potgieter_uniform,
+       gc_qlt_spiral]
 NAMES = ["shock_flowmom_reflect", "solarwind_kinrad_reflect_recurrent", "smoothshock_perp_momentum", "uniform_para_plane_pass",
          "modulation_type0", "modulation_type2", "gc_rigidity_reflect", "gc_isotropic_only",
-         "guiding_spiral", "guiding_scatt_uniform", "guiding_diff_rigidity"]
+         "guiding_spiral", "guiding_scatt_uniform", "guiding_diff_rigidity", "parker_strauss_heliopause", "parker_potgieter_uniform",
+         "gc_qlt_spiral"]
+
+
+# Setups whose endpoints amplify last-bit differences far beyond 1e-10: QLT's mu derivative is
+# DiffusionBase::GetMuDerivative's forward difference with dmu = 1e-8 (src/diffusion_base.cc:238), which
+# multiplies rounding differences by ~1e8.  Their parity bound is scaled by the oracle's own disagreement
+# with its last-bit-perturbed build (test_gpu_parity.sensitivity).
+ILL_CONDITIONED = {"gc_qlt_spiral"}
 
 
 def weight_atol(name, rec, rtol):

@@ -12,7 +12,7 @@ import pytest
 
 import feature_setups as F
 from spectrum_amd import engine as E
-from test_gpu_parity import check_parity
+from test_gpu_parity import check_parity, sensitivity
 
 pytestmark = pytest.mark.gpu
 THREADS = min(16, os.cpu_count() or 1)
@@ -27,7 +27,12 @@ def test_feature_fp64_parity(oracle_mod, k):
     st, rec = sim.run(N, seed=21)
     o = oracle_mod.Oracle(s)
     _, orec = o.run(N, seed=21, threads=THREADS)
-    check_parity(st, rec, orec, sim, o, N, ev_weight_atol=F.weight_atol(F.NAMES[k], orec, 1e-10))
+    sens = None
+    if F.NAMES[k] in F.ILL_CONDITIONED:
+        _, prec = oracle_mod.Oracle(s, oracle_mod.FMA_LIB_PATH).run(N, seed=21, threads=THREADS)
+        _, lrec = oracle_mod.Oracle(s, oracle_mod.LIBM_LIB_PATH).run(N, seed=21, threads=THREADS)
+        sens = np.maximum(sensitivity(orec, prec), sensitivity(orec, lrec))
+    check_parity(st, rec, orec, sim, o, N, ev_weight_atol=F.weight_atol(F.NAMES[k], orec, 1e-10), sens=sens)
 
 
 @pytest.mark.parametrize("k", range(len(F.ALL)), ids=F.NAMES)

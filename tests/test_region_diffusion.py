@@ -1,0 +1,169 @@
+"""The oracle's DiffusionStraussEtAl2013, DiffusionPotgieterEtAl2015 (src/diffusion_other.cc:869-993) and
+DiffusionQLTConstant (:130-149) checked against what their published forms and the transports fix:
+
+* Strauss / Potgieter in a uniform field with the Parker transport: the displacement variances
+  <dz^2> = 2 kappa_par T along B and <dx^2 + dy^2> = 4 kappa_perp T across it, with
+  kappa_par = (lam v / 3) (R/R0)^(1/3) (B0/B) below R0 and (lam v / 3) (R/R0) (B0/B) above (Strauss et al. 2013),
+  kappa_par = kappa (v/c) (B0/B) below R0 and kappa (v/c) (R/R0)^(3/2) (B0/B) above (Potgieter et al. 2015);
+* the LISM switch on the SolarWind background: inside the 117 au heliopause the inner parameters and
+  B0 / |B| with |B| of the Parker spiral, beyond it the outer ones with no |B| factor; LISM_idx 1 (a
+  unipolar-field indicator, always -1 here) selects the outer ones everywhere, -1 none;
+* QLT pitch-angle scattering with GuidingScatt in a uniform field: over a short time <d mu> = dD/dmu t
+  and <d mu^2> = 2 D t with D = (pi/4) (q-1) |Omega| (1 - mu^2) (v k_min |mu/Omega|)^(q-1) A2A, and a
+  distribution uniform in mu stays uniform (the drift term is D's mu derivative).
+"""
+import numpy as np
+import pytest
+
+import feature_setups as F
+from spectrum_amd import configs as cf
+from spectrum_amd.data_container import DataContainer as DC
+
+THREADS = 8
+BZ = 3.0
+
+
+def uniform_parker(model, params, p, T=200.0, n_e=1):
+    s = cf.Setup("uniform_region", cf.TRAJ_PARKER, cf.BACKWARD)
+    s.add(cf.BACKGROUND, "BackgroundUniform", cf.background_base(DC(), 0.0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, BZ), 1.0e4))
+    dc = DC().insert_int(params[0])
+    for v in params[1:]:
+        dc.insert_double(v)
+    s.add(cf.DIFFUSION, model, dc)
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((0.0, 0.0, 0.0)))
+    s.add(cf.INITIAL, "InitialMomentumShell", DC().insert_double(p))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", -T)))
+    d = F.dist((4, 0, 0), (0.0, 0.0, 0.0), (1.0, 0.0, 0.0), (0, 0, 0), (1, 0, 0))
+    d.insert_double(1.0).insert_double(0.0).insert_int(1)
+    s.add(cf.DISTRIBUTION, "DistributionTimeUniform", d)
+    return s
+
+
+def kpar(model, lam, R0, B0, p, Bmag):
+    v = cf.vel_from_mom(p)
+    R = p * cf.c_code
+    if model == "DiffusionStraussEtAl2013":
+        f = np.cbrt(R / R0) if R < R0 else R / R0
+        return lam * v / 3.0 * f * B0 / Bmag
+    f = 1.0 if R < R0 else np.sqrt((R / R0) ** 3)
+    return lam * (v / cf.c_code) * f * B0 / Bmag
+
+
+@pytest.mark.parametrize("model", ["DiffusionStraussEtAl2013", "DiffusionPotgieterEtAl2015"])
+@pytest.mark.parametrize("above", [False, True])
+def test_uniform_field_variances(oracle_mod, model, above):
+    p = cf.mom_from_ekin(100.0 * cf.MeV)
+    R = p * cf.c_code
+    R0 = R / 3.0 if above else 3.0 * R
+    lam = 0.05 if model == "DiffusionStraussEtAl2013" else 0.02
+    B0, kr = 6.0, 0.1
+    s = uniform_parker(model, (-1, lam, 10.0 * lam, R0, B0, kr, 0.5), p)
+    o = oracle_mod.Oracle(s)
+    n, T = 4000, 200.0
+    _, rec = o.run(n, seed=13, threads=THREADS)
+    assert (rec["status"] == 0).all()
+    dx = rec["pos"] - rec["pos0"]
+    K1 = kpar(model, lam, R0, B0, p, BZ)
+    z2, r2 = np.mean(dx[:, 2] ** 2), np.mean(dx[:, 0] ** 2 + dx[:, 1] ** 2)
+    print(f"[{model} above={above}] <dz^2>/2T {z2 / (2 * T):.5g} vs {K1:.5g}; <dr^2>/4T {r2 / (4 * T):.5g} vs {kr * K1:.5g}")
+    assert abs(z2 / (2 * T) / K1 - 1.0) < 5.0 * np.sqrt(2.0 / n)
+    assert abs(r2 / (4 * T) / (kr * K1) - 1.0) < 5.0 * np.sqrt(1.0 / n)
+
+
+def spiral_bmag(pos):
+    """|B| of the Parker spiral of cf.solarwind() at pos (code units), from its closed form."""
+    V = 400.0e5 / cf.unit_velocity
+    BE = 5.0e-5 / cf.unit_magnetic
+    r_ref = 3.0 * 6.957e10 / cf.unit_length
+    w = 2.87e-6 * cf.unit_time
+    Bref = BE * (cf.one_au / r_ref) ** 2
+    r = np.linalg.norm(pos)
+    st = np.hypot(pos[0], pos[1]) / r
+    Br = Bref * (r_ref / r) ** 2
+    return abs(Br) * np.sqrt(1.0 + ((r - r_ref) * w * st / V) ** 2)
+
+
+@pytest.mark.parametrize("model", ["DiffusionStraussEtAl2013", "DiffusionPotgieterEtAl2015"])
+def test_lism_switch_on_the_solar_wind(oracle_mod, model):
+    p = cf.mom_from_ekin(200.0 * cf.MeV)
+    lam_in, lam_out, R0, B0, kr_in, kr_out = 0.05, 0.8, 2.0 * p * cf.c_code, 15.0, 0.02, 0.3
+    for idx in (-1, 0, 1):
+        s = cf.Setup("sw_region", cf.TRAJ_PARKER, cf.BACKWARD)
+        s.add(cf.BACKGROUND, "BackgroundSolarWind", cf.solarwind())
+        dc = DC().insert_int(idx)
+        for v in (lam_in, lam_out, R0, B0, kr_in, kr_out):
+            dc.insert_double(v)
+        s.add(cf.DIFFUSION, model, dc)
+        o = oracle_mod.Oracle(s)
+        for r_au in (50.0, 116.9, 117.1, 150.0):
+            pos = np.array([0.6, 0.64, 0.48]) * r_au * cf.one_au
+            outer = idx == 1 or (idx == 0 and r_au > 117.0)
+            Bm = spiral_bmag(pos)
+            want1 = kpar(model, lam_out, R0, Bm, p, Bm) if outer else kpar(model, lam_in, R0, B0, p, Bm)
+            want0 = (kr_out if outer else kr_in) * want1
+            got1, got0 = o.kappa_at(1, pos, p), o.kappa_at(0, pos, p)
+            assert got1 == pytest.approx(want1, rel=1e-9), (idx, r_au)
+            assert got0 == pytest.approx(want0, rel=1e-9), (idx, r_au)
+
+
+def qlt_setup(A2A, l_max, q, initial):
+    s = cf.Setup("qlt_uniform", cf.TRAJ_GUIDING_SCATT, cf.FORWARD)
+    s.add(cf.BACKGROUND, "BackgroundUniform", cf.background_base(DC(), 0.0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, BZ), 1.0e4))
+    s.add(cf.DIFFUSION, "DiffusionQLTConstant", DC().insert_double(A2A).insert_double(l_max).insert_double(q))
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((0.0, 0.0, 0.0)))
+    s.add(cf.INITIAL, *initial)
+    return s
+
+
+def qlt_D(mu, p, A2A, l_max, q):
+    v = cf.vel_from_mom(p)
+    gamma = np.sqrt(1.0 + (p / cf.c_code) ** 2)          # code units: m = 1, charge 1
+    Om = BZ / (gamma * cf.c_code)
+    return 0.25 * np.pi * (q - 1.0) * Om * (1.0 - mu ** 2) * (v * (2.0 * np.pi / l_max) * np.abs(mu / Om)) ** (q - 1.0) * A2A
+
+
+def test_qlt_short_time_moments(oracle_mod):
+    p = cf.mom_from_ekin(10.0 * cf.MeV)
+    A2A, l_max, q, th0, T = 0.03, 100.0, 5.0 / 3.0, 0.6, 10.0
+    s = qlt_setup(A2A, l_max, q, ("InitialMomentumRing", DC().insert_double(p).insert_double(th0)))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", T)))
+    d = F.dist((4, 0, 0), (0.0, 0.0, 0.0), (T, 0.0, 0.0), (0, 0, 0), (1, 0, 0))
+    d.insert_double(1.0).insert_double(0.0).insert_int(1)
+    s.add(cf.DISTRIBUTION, "DistributionTimeUniform", d)
+    o = oracle_mod.Oracle(s)
+    n = 8000
+    _, rec = o.run(n, seed=17, threads=THREADS)
+    assert (rec["status"] == 0).all()
+    mu0 = np.cos(th0)
+    mu = rec["mom"][:, 2] / np.linalg.norm(rec["mom"], axis=1)
+    dmu = mu - mu0
+    D0 = qlt_D(mu0, p, A2A, l_max, q)
+    h = 1e-6
+    V0 = (qlt_D(mu0 + h, p, A2A, l_max, q) - qlt_D(mu0 - h, p, A2A, l_max, q)) / (2 * h)
+    var = np.mean((dmu - dmu.mean()) ** 2)
+    print(f"[qlt] <dmu> {dmu.mean():.5f} vs {V0 * T:.5f}; var {var:.5f} vs {2 * D0 * T:.5f}")
+    assert abs(dmu.mean() - V0 * T) < 5.0 * dmu.std() / np.sqrt(n) + 0.1 * abs(V0 * T)
+    assert abs(var / (2 * D0 * T) - 1.0) < 0.1
+
+
+def test_qlt_keeps_an_isotropic_distribution(oracle_mod):
+    p = cf.mom_from_ekin(10.0 * cf.MeV)
+    A2A, l_max, q, T = 0.3, 100.0, 5.0 / 3.0, 300.0
+    s = qlt_setup(A2A, l_max, q, ("InitialMomentumShell", DC().insert_double(p)))   # mu uniform in [-1, 1]
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", T)))
+    d = F.dist((4, 0, 0), (0.0, 0.0, 0.0), (T, 0.0, 0.0), (0, 0, 0), (1, 0, 0))
+    d.insert_double(1.0).insert_double(0.0).insert_int(1)
+    s.add(cf.DISTRIBUTION, "DistributionTimeUniform", d)
+    o = oracle_mod.Oracle(s)
+    n = 4000
+    _, rec = o.run(n, seed=19, threads=THREADS)
+    assert (rec["status"] == 0).all()
+    mu = np.sort(rec["mom"][:, 2] / np.linalg.norm(rec["mom"], axis=1))
+    u = (mu + 1.0) / 2.0
+    ks = np.max(np.abs(u - np.arange(1, n + 1) / n))
+    steps = rec["steps"].mean()
+    print(f"[qlt] KS {ks:.4f} (99% level {1.63 / np.sqrt(n):.4f}), {steps:.0f} steps per trajectory")
+    assert steps > 100
+    assert ks < 1.63 / np.sqrt(n)
