@@ -24,6 +24,10 @@
 #include <stdio.h>
 #ifdef _OPENMP
 #include <omp.h>
+#endif
+
+/* the four guiding-centre transports */
+#define ORC_IS_GC(t) ((t) >= ORC_TRAJ_GUIDING_DIFF_SCATT && (t) <= ORC_TRAJ_GUIDING_DIFF)
 
 #ifdef ORC_PERTURB_LIBM
 /* The last-bit-perturbed build of the power functions (a GPU's libm may round them one unit differently):
@@ -32,7 +36,6 @@ static double orc_pow_1ulp(double a, double b) { return pow(a, b) * (1.0 + 2.220
 static double orc_cbrt_1ulp(double a) { return cbrt(a) * (1.0 + 2.220446049250313e-16); }
 #define pow orc_pow_1ulp
 #define cbrt orc_cbrt_1ulp
-#endif
 #endif
 
 /* ------------------------------------------------------------------------------------------------
@@ -844,7 +847,7 @@ static double bnd_eval(const bnd_t* b, bst_t* st, int traj_type, double t, vec3 
       st->normal = b->norm;
       return vdot(vsub(pos, b->origin), b->norm);
    case BK_MOMENTUM:  /* src/boundary_momentum.cc:69-76 */
-      if (traj_type >= ORC_TRAJ_GUIDING_DIFF_SCATT) return vnorm(mom) - b->momentum;
+      if (ORC_IS_GC(traj_type)) return vnorm(mom) - b->momentum;
       return mom.v[0] - b->momentum;
    }
    return 0.0;
@@ -905,6 +908,7 @@ static void dist_event(const orc_sim* s, const dist_t* d, int action,
    case DS_MOMENTUM: {  /* :262-290 */
       vec3 m = (d->val_time == 0 ? mom1 : mom2);
       if (d->val_coord == 0 || parker) value = m;
+      else if (traj_type == ORC_TRAJ_FIELDLINE) value = V(m.v[2], 0.0, 0.0);
       else { value.v[0] = vnorm(m); value.v[1] = m.v[2] / value.v[0]; value.v[2] = 0.0; }
       break;
    }
@@ -1002,7 +1006,8 @@ static inline vec3 gc_convert(vec3 m) { return V(sqrt(Sqr(m.v[0]) + Sqr(m.v[2]))
 
 static int common_fields(const orc_sim* s, lane_t* L, traj_t* T, uint16_t mask)
 {
-   double pm = (s->traj_type >= ORC_TRAJ_GUIDING_DIFF_SCATT) ? gc_convert(T->mom).v[0] : T->mom.v[0];
+   /* ConvertMomentum: (|p|, mu) for the guiding centre, (|p_par|, 0, 0) on a field line (src/trajectory_fieldline.hh:86-89) */
+   double pm = ORC_IS_GC(s->traj_type) ? gc_convert(T->mom).v[0] : (s->traj_type == ORC_TRAJ_FIELDLINE ? fabs(T->mom.v[2]) : T->mom.v[0]);
    return get_fields(&s->bg, L, T->t, T->pos, pm, mask, &T->sp);
 }
 
@@ -1054,7 +1059,8 @@ static void handle_boundaries(const orc_sim* s, traj_t* T)
          active = b;
          if (st & BND_REFLECT) {
             T->pos = vsub(T->pos, vscl(2.0 * T->bs[b].delta, T->bs[b].normal));
-            if (s->traj_type >= ORC_TRAJ_GUIDING_DIFF_SCATT) T->mom.v[2] = -T->mom.v[2];  /* src/trajectory_guiding.hh:121-124 */
+            if (ORC_IS_GC(s->traj_type)) T->mom.v[2] = -T->mom.v[2];  /* src/trajectory_guiding.hh:121-124 */
+            else if (s->traj_type == ORC_TRAJ_FIELDLINE) T->mom = vneg(T->mom);  /* TrajectoryBase's, src/trajectory_base.cc:139-142 */
             T->n_refl++;
             bnd_compute(&s->bs[b], &T->bs[b], s->traj_type, T->t, T->pos, T->mom);
             T->bs[b].crossings_left--;
@@ -1419,10 +1425,33 @@ static int gcd_advance(const orc_sim* s, lane_t* L, traj_t* T)
    return gc_end_step(s, L, T);
 }
 
+/* TrajectoryFieldline::Advance = RKAdvance (src/trajectory_fieldline.cc:41-77, src/trajectory_base.cc:545-579):
+   Slopes follow the magnetic field at the velocity's third component, PhysicalStep 0.5 dmax / |v_z|,
+   the momentum is constant */
+static int fl_advance(const orc_sim* s, lane_t* L, traj_t* T)
+{
+   int fwd = (s->time_flow == ORC_FLOW_FORWARD);
+   spd_t* sp = &T->sp;
+   vec3 vel = VelCart(T->mom);
+   vec3 slope_pos = vscl(vel.v[2], sp->bhat);
+   T->dt_physical = 0.5 * sp->dmax / fabs(vel.v[2]);
+   T->dt = fmin(T->dt_physical, T->dt_adaptive);
+   time_boundary_proximity(s, T);
+   double dtv = T->dt * 1.0;
+   if (fwd) { T->t += T->dt; T->pos = vadd(T->pos, vscl(dtv, slope_pos)); }
+   else { T->t -= T->dt; T->pos = vsub(T->pos, vscl(dtv, slope_pos)); }
+   handle_boundaries(s, T);
+   if (!(T->status & TRAJ_FINISH)) {
+      if (common_fields(s, L, T, BG_B)) EX(ORC_EX_FIELD);
+   }
+   T->segs++;
+   return ORC_EX_NONE;
+}
+
 /* Initial momentum sample (src/initial_momentum.cc:283-410) */
 static vec3 mom_sample(const orc_sim* s, lane_t* L)
 {
-   int gc = (s->traj_type >= ORC_TRAJ_GUIDING_DIFF_SCATT);
+   int gc = ORC_IS_GC(s->traj_type);
    double p;
    switch (s->im_kind) {
    case IM_FIXED: return s->im_mom;
@@ -1443,6 +1472,7 @@ static vec3 mom_sample(const orc_sim* s, lane_t* L)
    }
    default: p = 0.0;
    }
+   if (s->traj_type == ORC_TRAJ_FIELDLINE) return V(0.0, 0.0, p);  /* src/initial_momentum.cc:289-292, :390-391 */
    if (!gc) return V(p, 0.0, 0.0);
    double mu = -1.0 + 2.0 * draw_uniform(L);
    double st = sqrt(1.0 - Sqr(mu));
@@ -1481,6 +1511,7 @@ static int integrate_one(const orc_sim* s, lane_t* L, traj_t* T)
       int ex;
       switch (s->traj_type) {
       case ORC_TRAJ_GUIDING_DIFF_SCATT: ex = gcds_advance(s, L, T); break;
+      case ORC_TRAJ_FIELDLINE: ex = fl_advance(s, L, T); break;
       case ORC_TRAJ_GUIDING: ex = gcg_advance(s, L, T); break;
       case ORC_TRAJ_GUIDING_SCATT: ex = gcs_advance(s, L, T); break;
       case ORC_TRAJ_GUIDING_DIFF: ex = gcd_advance(s, L, T); break;

@@ -202,6 +202,7 @@ static __device__ int bin_events(PlanPtr P, QueryPtr Q, LaunchPtr L, uint64_t sl
          case SPX_DS_TIME: v0 = first ? t1 : t2; break;                                                      // :143-148
          case SPX_DS_MOMENTUM:                                                                                // :262-290
             if (D.val_coord == 0 || parker) { v0 = ma; v1 = mb; v2 = mc; }
+            else if (P->traj_type == SPX_TRAJ_FIELDLINE) { v0 = mc; v1 = 0.0; v2 = 0.0; }
             else { v0 = norm(mk<double>(ma, mb, mc)); v1 = mc / v0; v2 = 0.0; }
             break;
          case SPX_DS_POSMOM:                                                                                  // :342-353

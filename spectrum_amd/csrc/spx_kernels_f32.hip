@@ -16,7 +16,7 @@ namespace spx {
 hipError_t launch_f32(const spx_plan_desc& hp, uint64_t n_lanes, const Launch* d_launch, hipStream_t st,
                       hipEvent_t mid)
 {
-   if (hp.traj_type == SPX_TRAJ_PARKER || hp.traj_type == SPX_TRAJ_PARKER_SOURCE) return dispatch_parker<float>(hp, n_lanes, d_launch, st, mid);
+   if (hp.traj_type == SPX_TRAJ_PARKER || hp.traj_type == SPX_TRAJ_PARKER_SOURCE || hp.traj_type == SPX_TRAJ_FIELDLINE) return dispatch_parker<float>(hp, n_lanes, d_launch, st, mid);
    if (hp.traj_type >= SPX_TRAJ_GUIDING_DIFF_SCATT && hp.traj_type <= SPX_TRAJ_GUIDING_DIFF) return dispatch_gc<float>(hp, n_lanes, d_launch, st, mid);
    return hipErrorInvalidValue;
 }

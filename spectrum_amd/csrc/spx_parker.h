@@ -162,7 +162,7 @@ __device__ __forceinline__ double parker_mom_sample(PlanPtr P, const spx_rng_des
                                                     uint64_t traj, uint32_t& nu, bool& ex)
 {
    switch (P->im_kind) {
-   case SPX_IM_FIXED: return P->im_mom[0];
+   case SPX_IM_FIXED: return P->traj_type == SPX_TRAJ_FIELDLINE ? P->im_mom[2] : P->im_mom[0];   // (0, 0, p_par) on a field line
    case SPX_IM_SHELL: case SPX_IM_RING: return P->im_p0;
    case SPX_IM_THICKSHELL: {
       const double u = next_uniform(rng, first, traj, nu, ex);
@@ -201,14 +201,16 @@ __device__ int parker_set_start(PlanPtr P, QueryPtr Q, LaunchPtr L, const spx_rn
    const double p = parker_mom_sample(P, rng, first, traj, nu, ex);
    if (ex) return 5;
    const uint64_t n = L->n_lanes;
-   L->t[slot] = t; L->x[slot] = x; L->y[slot] = y; L->z[slot] = z; L->m0[slot] = p; L->m1[slot] = 0.0; L->m2[slot] = 0.0;
+   // Parker keeps (p, 0, 0), the field line (0, 0, p_par) (src/initial_momentum.cc:289-292)
+   const double pa = (TT == 2) ? 0.0 : p, pc = (TT == 2) ? p : 0.0;
+   L->t[slot] = t; L->x[slot] = x; L->y[slot] = y; L->z[slot] = z; L->m0[slot] = pa; L->m1[slot] = 0.0; L->m2[slot] = pc;
    L->amp[slot] = 0.0; L->wgt[slot] = 0.0; L->dt_prev[slot] = 0.0;
    L->ds_prev[slot] = (double)ds; L->dmax[slot] = (double)F.dmax;
    L->pf[slot] = p1;   // the first step uses these fields, evaluated with the first sample's magnitude
    L->steps[slot] = 0; L->n_events[slot] = 0;   // the draw counters continue across attempts
    // the adaptive step starts large so that dt is the physical step (src/trajectory_base.cc:894-895)
    L->dt_adapt[slot] = sp_large * (double)F.dmax / c_code;
-   L->t0[slot] = t; L->x0[slot] = x; L->y0[slot] = y; L->z0[slot] = z; L->m00[slot] = p; L->m01[slot] = 0.0; L->m02[slot] = 0.0;
+   L->t0[slot] = t; L->x0[slot] = x; L->y0[slot] = y; L->z0[slot] = z; L->m00[slot] = pa; L->m01[slot] = 0.0; L->m02[slot] = pc;
    for (int d = 0; d < SPX_MAX_DIST; d++) L->ev_bin[(uint64_t)d * n + slot] = -1;
    // ResetAllBoundaries (src/trajectory_base.cc:67-81, src/boundary_base.cc:138-162)
    const double dmax = (double)F.dmax;
@@ -216,7 +218,7 @@ __device__ int parker_set_start(PlanPtr P, QueryPtr Q, LaunchPtr L, const spx_rn
       const SPX_CP spx_bnd_desc& B = P->bnd[b];
       const double scale = (B.cls == SPX_BC_TIME) ? fabs(dmax / c_code) : (B.cls == SPX_BC_SPACE ? fabs(dmax) : fabs(p));
       const int left = B.max_crossings;
-      const double d = bnd_eval(B, left, false, t, x, y, z, p, 0.0, 0.0, nullptr);
+      const double d = bnd_eval(B, left, false, t, x, y, z, pa, 0.0, pc, nullptr);
       L->bcross[b * n + slot] = left;
       L->bdelta[b * n + slot] = d;
       L->bdelta_old[b * n + slot] = (d == 0.0) ? -sp_tiny * scale : d;
@@ -246,11 +248,48 @@ struct PkHot {
    int bst[SPX_MAX_BND];                        // boundary status bits
    NormalStream<R> rs;
    int32_t steps;
+   bool fieldline;                              // TrajectoryFieldline: p is the signed p_par of (0, 0, p_par)
    typedef R real;
-   __device__ __forceinline__ double m0() const { return p; }
-   __device__ __forceinline__ double m2() const { return 0.0; }
-   __device__ __forceinline__ void reflect_mom() {}
+   __device__ __forceinline__ double m0() const { return fieldline ? 0.0 : p; }
+   __device__ __forceinline__ double m2() const { return fieldline ? p : 0.0; }
+   // TrajectoryParker::ReverseMomentum does nothing (src/trajectory_parker.hh:129-131); the field line
+   // keeps TrajectoryBase's, which reverses the momentum (src/trajectory_base.cc:139-142)
+   __device__ __forceinline__ void reflect_mom() { if (fieldline) p = -p; }
 };
+
+// One step of TrajectoryFieldline::Advance = RKAdvance with the Euler table (src/trajectory_fieldline.cc:41-77,
+// src/trajectory_base.cc:545-579): the centre follows the magnetic field, slope = v_par bhat with v_par the
+// third component of Particle::Vel(0, 0, p_par) (common/physics.hh:330-336), PhysicalStep
+// dt = 0.5 dmax / |v_par|; the momentum is constant and no random numbers are drawn.
+template <typename R, int BG, int FWD>
+__device__ __forceinline__ int fieldline_step(PlanPtr P, LaunchPtr L, uint64_t slot, PkHot<R>& H, ColdLDS& C, int tid, int& bflags)
+{
+   const SPX_CP spx_bg_t<R>& bg = PlanView<R>::bg(P);
+   const R p = R(H.p), pm = fabs(p);
+   const R v = p * (VelP(pm) / pm);
+   const R dtp = R(0.5) * H.F.dmax / fabs(v);
+   double dt = fmin((double)dtp, C.dt_adapt[tid]);
+   dt = time_proximity<FWD>(P, H, dt, (double)H.F.dmax);
+   const v3<R> slope_pos = v * H.F.bhat;
+   const R dtv = R(dt) * R(1);
+   if (FWD) {
+      H.t += dt;
+      H.x += (double)(dtv * slope_pos.x); H.y += (double)(dtv * slope_pos.y); H.z += (double)(dtv * slope_pos.z);
+   }
+   else {
+      H.t -= dt;
+      H.x -= (double)(dtv * slope_pos.x); H.y -= (double)(dtv * slope_pos.y); H.z -= (double)(dtv * slope_pos.z);
+   }
+   bflags = handle_boundaries<false>(P, L, slot, H, C, tid);
+   if (!(bflags & 1)) {
+      if (!parker_fields<R, BG>(bg, H.t, H.x, H.y, H.z, R(fabs(H.p)), H.ds_prev, H.F)) return 1;
+   }
+   H.steps++;
+   if (H.steps > P->max_steps) return 4;
+   if (dt < sp_tiny * (double)H.F.dmax / c_code) return 2;
+   if (!__builtin_isnormal(dt)) return 3;
+   return 0;
+}
 
 // One Parker step: TrajectoryParker::Advance (src/trajectory_parker.cc:255-306) with the Euler table
 // (common/rk_config.hh:72-80) and TrajectoryBase::Integrate's checks (src/trajectory_base.cc:964-992).
@@ -462,7 +501,8 @@ __device__ __forceinline__ void parker_step_body(const Launch* __restrict__ Lg)
    PkHot<R> H;
    if (run) {
       const uint64_t n = Lp->n_lanes;
-      H.t = Lp->t[slot]; H.x = Lp->x[slot]; H.y = Lp->y[slot]; H.z = Lp->z[slot]; H.p = Lp->m0[slot];
+      H.fieldline = (TT == 2);
+      H.t = Lp->t[slot]; H.x = Lp->x[slot]; H.y = Lp->y[slot]; H.z = Lp->z[slot]; H.p = (TT == 2) ? Lp->m2[slot] : Lp->m0[slot];
       H.steps = (int32_t)Lp->steps[slot];
       H.ds_prev = R(Lp->ds_prev[slot]);
       H.rs.traj = Lp->traj[slot]; H.rs.nn = Lp->nn[slot]; H.rs.spare_ok = false; H.rs.exhausted = false; H.rs.spare = R(0);
@@ -493,7 +533,8 @@ __device__ __forceinline__ void parker_step_body(const Launch* __restrict__ Lg)
       LaunchPtr Li = Lp;
       asm volatile("" : "+s"(Li));
       int bflags = 0;
-      const int e = parker_step<R, BG, FWD, TT, DF>(Pi, Li, rng, first, slot, H, C, tid, bflags);
+      const int e = (TT == 2) ? fieldline_step<R, BG, FWD>(Pi, Li, slot, H, C, tid, bflags)
+                              : parker_step<R, BG, FWD, TT, DF>(Pi, Li, rng, first, slot, H, C, tid, bflags);
       steps_here++;
       if (e != 0) { exc = e; st = SLOT_DISCARD; }
       else if (bflags & 1) st = SLOT_DONE;
@@ -508,9 +549,11 @@ __device__ __forceinline__ void parker_step_body(const Launch* __restrict__ Lg)
    const uint64_t n = Ls->n_lanes;
    Ls->active[slot] = st;
    Ls->ex_code[slot] = exc;
-   Ls->t[slot] = H.t; Ls->x[slot] = H.x; Ls->y[slot] = H.y; Ls->z[slot] = H.z; Ls->m0[slot] = H.p;
+   Ls->t[slot] = H.t; Ls->x[slot] = H.x; Ls->y[slot] = H.y; Ls->z[slot] = H.z;
+   if (TT == 2) Ls->m2[slot] = H.p;
+   else Ls->m0[slot] = H.p;
    Ls->steps[slot] = H.steps; Ls->nn[slot] = H.rs.nn;
-   if (steps_here) Ls->pf[slot] = H.p;   // every step ends with fields at the new momentum
+   if (steps_here) Ls->pf[slot] = (TT == 2) ? fabs(H.p) : H.p;   // every step ends with fields at the new momentum
    Ls->ds_prev[slot] = (double)H.ds_prev; Ls->dmax[slot] = (double)H.F.dmax;
    Ls->amp[slot] = C.amp[tid]; Ls->wgt[slot] = C.wgt[tid];
    Ls->dt_prev[slot] = C.dt_prev[tid]; Ls->src_old[slot] = C.src_old[tid];
@@ -575,6 +618,8 @@ static hipError_t dispatch_flow(const spx_plan_desc& hp, uint64_t n_lanes, const
 {
    const bool fwd = hp.time_flow == 0;
    if (hp.traj_type == 1) return fwd ? dispatch_df<R, BG, 1, 1>(hp, n_lanes, d_launch, st, mid) : dispatch_df<R, BG, 0, 1>(hp, n_lanes, d_launch, st, mid);
+   if (hp.traj_type == SPX_TRAJ_FIELDLINE)
+      return fwd ? launch_parker<R, BG, 1, 2, 0>(n_lanes, d_launch, st, mid) : launch_parker<R, BG, 0, 2, 0>(n_lanes, d_launch, st, mid);
    return fwd ? dispatch_df<R, BG, 1, 0>(hp, n_lanes, d_launch, st, mid) : dispatch_df<R, BG, 0, 0>(hp, n_lanes, d_launch, st, mid);
 }
 
@@ -585,6 +630,8 @@ static hipError_t dispatch_grid(const spx_plan_desc& hp, uint64_t n_lanes, const
    const bool fwd = hp.time_flow == 0;
    constexpr int G = SPX_BG_GRID;
    if (hp.traj_type == 1) return fwd ? launch_parker<R, G, 1, 1, 0>(n_lanes, d_launch, st, mid) : launch_parker<R, G, 0, 1, 0>(n_lanes, d_launch, st, mid);
+   if (hp.traj_type == SPX_TRAJ_FIELDLINE)
+      return fwd ? launch_parker<R, G, 1, 2, 0>(n_lanes, d_launch, st, mid) : launch_parker<R, G, 0, 2, 0>(n_lanes, d_launch, st, mid);
    return fwd ? launch_parker<R, G, 1, 0, 0>(n_lanes, d_launch, st, mid) : launch_parker<R, G, 0, 0, 0>(n_lanes, d_launch, st, mid);
 }
 

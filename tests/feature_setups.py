@@ -229,14 +229,50 @@ def gc_qlt_spiral():
     return s
 
 
+def fieldline_spiral():
+    """TrajectoryFieldline (forward) along the Parker spiral from 1 au off the ecliptic: no diffusion model, a
+    reflecting plane the line crosses (the momentum reverses and the centre runs back inward), absorbing
+    spheres at 0.5 and 3 au and a time distribution."""
+    s = cf.Setup("fieldline_spiral", cf.TRAJ_FIELDLINE, cf.FORWARD)
+    s.add(cf.BACKGROUND, "BackgroundSolarWind", cf.solarwind())
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((0.8 * cf.one_au, 0.5 * cf.one_au, 0.2 * cf.one_au)))
+    s.add(cf.INITIAL, "InitialMomentumThickShell", DC().insert_double(cf.mom_from_ekin(1.0 * cf.MeV)).insert_double(cf.mom_from_ekin(100.0 * cf.MeV)).insert_bool(True))
+    s.add(cf.BOUNDARY, "BoundaryPlaneReflect", cf.boundary(-1, [-1], ("vec", (0.0, 0.0, 0.35 * cf.one_au)), ("vec", (0.0, 0.0, 1.0))))
+    s.add(cf.BOUNDARY, "BoundarySphereAbsorb", cf.boundary(1, [0], ("vec", (0.0, 0.0, 0.0)), ("double", 3.0 * cf.one_au)))
+    s.add(cf.BOUNDARY, "BoundarySphereAbsorb", cf.boundary(1, [0], ("vec", (0.0, 0.0, 0.0)), ("double", 0.5 * cf.one_au)))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", 2.0 * cf.one_day)))
+    d = dist((12, 0, 0), (0.0, 0, 0), (4.0 * cf.one_day, 0, 0), (0, 0, 0), (1, 0, 0))
+    d.insert_double(1.0).insert_double(0.0).insert_int(1)
+    s.add(cf.DISTRIBUTION, "DistributionTimeUniform", d)
+    return s
+
+
+def fieldline_backward_shock():
+    """TrajectoryFieldline (backward) through a smooth shock with an oblique field."""
+    s = cf.Setup("fieldline_backward_shock", cf.TRAJ_FIELDLINE, cf.BACKWARD)
+    u1, w = 0.013343, 2.5
+    dc = cf.background_base(DC(), 0.0, (0.0, 0.0, 0.0), (u1, 0.0, 0.0), (8.0, 6.0, 0.0), 2.0 * w)
+    dc.insert_vec((-1.0, 0.0, 0.0)).insert_double(0.0).insert_double(4.0).insert_double(w).insert_double(0.1)
+    s.add(cf.BACKGROUND, "BackgroundSmoothShock", dc)
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((20.0, 0.0, 0.0)))
+    s.add(cf.INITIAL, "InitialMomentumShell", DC().insert_double(cf.mom_from_ekin(1.0 * cf.MeV)))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", -600.0)))
+    d = dist((10, 0, 0), (-600.0, 0, 0), (0.0, 0, 0), (0, 0, 0), (1, 0, 0))
+    d.insert_double(1.0).insert_double(0.0).insert_int(1)
+    s.add(cf.DISTRIBUTION, "DistributionTimeUniform", d)
+    return s
+
+
 ALL = [shock_flowmom_reflect, solarwind_kinrad_reflect_recurrent, smoothshock_perp_momentum, uniform_para_plane_pass,
        lambda: modulation_spectrum_type(0), lambda: modulation_spectrum_type(2), gc_rigidity_reflect, gc_isotropic_only,
        guiding_spiral, guiding_scatt_uniform, guiding_diff_rigidity, parker_strauss_heliopause, parker_potgieter_uniform,
-       gc_qlt_spiral]
+       gc_qlt_spiral, fieldline_spiral, fieldline_backward_shock]
 NAMES = ["shock_flowmom_reflect", "solarwind_kinrad_reflect_recurrent", "smoothshock_perp_momentum", "uniform_para_plane_pass",
          "modulation_type0", "modulation_type2", "gc_rigidity_reflect", "gc_isotropic_only",
          "guiding_spiral", "guiding_scatt_uniform", "guiding_diff_rigidity", "parker_strauss_heliopause", "parker_potgieter_uniform",
-         "gc_qlt_spiral"]
+         "gc_qlt_spiral", "fieldline_spiral", "fieldline_backward_shock"]
 
 
 # Setups whose endpoints amplify last-bit differences far beyond 1e-10: QLT's mu derivative is
