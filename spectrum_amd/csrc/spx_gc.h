@@ -376,8 +376,18 @@ __device__ __forceinline__ int gc_step(PlanPtr P, LaunchPtr L, const spx_rng_des
    return 0;
 }
 
+// SPX_GC_WAVES resident waves per SIMD for the step kernel (0: the compiler's own register budget).
+#ifndef SPX_GC_WAVES
+#define SPX_GC_WAVES 0
+#endif
+#if SPX_GC_WAVES > 0
+#define SPX_GC_ATTR __attribute__((amdgpu_waves_per_eu(SPX_GC_WAVES)))
+#else
+#define SPX_GC_ATTR
+#endif
+
 template <typename R, int BG>
-__global__ __launch_bounds__(BLOCK) void gc_step_kernel(const Launch* __restrict__ Lg)
+__global__ __launch_bounds__(BLOCK) SPX_GC_ATTR void gc_step_kernel(const Launch* __restrict__ Lg)
 {
    const LaunchPtr Lp = (LaunchPtr)Lg;
    __shared__ ColdLDS C;
