@@ -26,6 +26,9 @@
 #include <omp.h>
 #endif
 
+/* 2 pi (M_2PI of the reference) */
+#define M_2PI_ORC (2.0 * M_PI)
+
 /* the four guiding-centre transports */
 #define ORC_IS_GC(t) ((t) >= ORC_TRAJ_GUIDING_DIFF_SCATT && (t) <= ORC_TRAJ_GUIDING_DIFF)
 
@@ -847,7 +850,7 @@ static double bnd_eval(const bnd_t* b, bst_t* st, int traj_type, double t, vec3 
       st->normal = b->norm;
       return vdot(vsub(pos, b->origin), b->norm);
    case BK_MOMENTUM:  /* src/boundary_momentum.cc:69-76 */
-      if (ORC_IS_GC(traj_type)) return vnorm(mom) - b->momentum;
+      if (ORC_IS_GC(traj_type) || traj_type == ORC_TRAJ_LORENTZ) return vnorm(mom) - b->momentum;
       return mom.v[0] - b->momentum;
    }
    return 0.0;
@@ -989,6 +992,7 @@ typedef struct {
    double t0; vec3 pos0, mom0;
    spd_t sp;
    double dt, dt_adaptive, dt_physical;
+   vec3 U0;   /* full orbit: the flow the background keeps from SetStart's evaluation with U in the mask */
    uint16_t status;
    int64_t segs;
    bst_t bt[MAX_BND], bs[MAX_BND], bm[MAX_BND];
@@ -1060,7 +1064,7 @@ static void handle_boundaries(const orc_sim* s, traj_t* T)
          if (st & BND_REFLECT) {
             T->pos = vsub(T->pos, vscl(2.0 * T->bs[b].delta, T->bs[b].normal));
             if (ORC_IS_GC(s->traj_type)) T->mom.v[2] = -T->mom.v[2];  /* src/trajectory_guiding.hh:121-124 */
-            else if (s->traj_type == ORC_TRAJ_FIELDLINE) T->mom = vneg(T->mom);  /* TrajectoryBase's, src/trajectory_base.cc:139-142 */
+            else if (s->traj_type == ORC_TRAJ_FIELDLINE || s->traj_type == ORC_TRAJ_LORENTZ) T->mom = vneg(T->mom);  /* TrajectoryBase's, src/trajectory_base.cc:139-142 */
             T->n_refl++;
             bnd_compute(&s->bs[b], &T->bs[b], s->traj_type, T->t, T->pos, T->mom);
             T->bs[b].crossings_left--;
@@ -1448,6 +1452,42 @@ static int fl_advance(const orc_sim* s, lane_t* L, traj_t* T)
    return ORC_EX_NONE;
 }
 
+/* The full orbit's fields after SetStart: mask E | B (src/trajectory_lorentz.cc:36-37); the solar wind's E is
+   formed from the flow the background still holds, that of SetStart's evaluation (src/background_solarwind.cc:252,
+   src/background_base.cc:425-456) */
+static int lz_fields(const orc_sim* s, lane_t* L, traj_t* T)
+{
+   if (common_fields(s, L, T, BG_B)) return 1;
+   if (s->bg.kind == BG_UNIFORM) T->sp.E = s->bg.Eu;
+   else T->sp.E = vdiv(vneg(vcross(T->U0, T->sp.B)), c_code);
+   return 0;
+}
+
+/* TrajectoryLorentz::Advance = RKAdvance (src/trajectory_lorentz.cc:51-83, src/trajectory_base.cc:545-579):
+   Slopes pos' = v, p' = LorentzForce (common/physics.hh:409-412), PhysicalStep 2 pi / Omega / 100 and
+   0.1 dmax / |v|, the Euler step */
+static int lz_advance(const orc_sim* s, lane_t* L, traj_t* T)
+{
+   int fwd = (s->time_flow == ORC_FLOW_FORWARD);
+   spd_t* sp = &T->sp;
+   vec3 vel = VelCart(T->mom);
+   double Omega = fmax(CyclotronFrequency(vnorm(vel), sp->Bmag), sp_tiny);
+   T->dt_physical = M_2PI_ORC / Omega / 100;
+   T->dt_physical = fmin(T->dt_physical, 0.1 * sp->dmax / vnorm(vel));
+   T->dt = fmin(T->dt_physical, T->dt_adaptive);
+   time_boundary_proximity(s, T);
+   vec3 force = vscl(sp_charge, vadd(sp->E, vdiv(vcross(vel, sp->B), c_code)));
+   double dtv = T->dt * 1.0;
+   if (fwd) { T->t += T->dt; T->pos = vadd(T->pos, vscl(dtv, vel)); T->mom = vadd(T->mom, vscl(dtv, force)); }
+   else { T->t -= T->dt; T->pos = vsub(T->pos, vscl(dtv, vel)); T->mom = vsub(T->mom, vscl(dtv, force)); }
+   handle_boundaries(s, T);
+   if (!(T->status & TRAJ_FINISH)) {
+      if (lz_fields(s, L, T)) EX(ORC_EX_FIELD);
+   }
+   T->segs++;
+   return ORC_EX_NONE;
+}
+
 /* Initial momentum sample (src/initial_momentum.cc:283-410) */
 static vec3 mom_sample(const orc_sim* s, lane_t* L)
 {
@@ -1455,6 +1495,20 @@ static vec3 mom_sample(const orc_sim* s, lane_t* L)
    double p;
    switch (s->im_kind) {
    case IM_FIXED: return s->im_mom;
+   default: break;
+   }
+   if (s->traj_type == ORC_TRAJ_LORENTZ) {  /* src/initial_momentum.cc:300-325, :381-413 with TRAJ_LORENTZ */
+      if (s->im_kind == IM_THICKSHELL) {
+         if (s->im_log) p = pow(10.0, s->im_p1 + (s->im_p2 - s->im_p1) * draw_uniform(L));
+         else p = s->im_p1 + (s->im_p2 - s->im_p1) * draw_uniform(L);
+      }
+      else p = s->im_p0;
+      double mu = -1.0 + 2.0 * draw_uniform(L);
+      double st = sqrt(1.0 - Sqr(mu));
+      double phi = M_2PI_ORC * draw_uniform(L);
+      return V(p * st * cos(phi), p * st * sin(phi), p * mu);
+   }
+   switch (s->im_kind) {
    case IM_RING:  /* src/initial_momentum.cc:203-220: fixed (p sin theta0, 0, p cos theta0), no draw */
       if (gc) return V(s->im_p0 * s->im_st0, 0.0, s->im_p0 * s->im_mu0);
       return V(s->im_p0, 0.0, 0.0);
@@ -1494,6 +1548,7 @@ static int integrate_one(const orc_sim* s, lane_t* L, traj_t* T)
    if (s->bg.kind == BG_SMOOTHSHOCK)
       L->smooth_ds = (vdot(vsub(T->pos, s->bg.r0), s->bg.n_shock) - s->bg.v_shock * T->t) / s->bg.width_shock;
    if (common_fields(s, L, T, BG_ALL | BG_gradALL | BG_dALLdt)) EX(ORC_EX_FIELD);
+   T->U0 = T->sp.U;
    T->mom = mom_sample(s, L);
    if (L->exhausted) EX(ORC_EX_REPLAY_EXHAUSTED);
    T->dt_adaptive = sp_large * T->sp.dmax / c_code;
@@ -1512,6 +1567,7 @@ static int integrate_one(const orc_sim* s, lane_t* L, traj_t* T)
       switch (s->traj_type) {
       case ORC_TRAJ_GUIDING_DIFF_SCATT: ex = gcds_advance(s, L, T); break;
       case ORC_TRAJ_FIELDLINE: ex = fl_advance(s, L, T); break;
+      case ORC_TRAJ_LORENTZ: ex = lz_advance(s, L, T); break;
       case ORC_TRAJ_GUIDING: ex = gcg_advance(s, L, T); break;
       case ORC_TRAJ_GUIDING_SCATT: ex = gcs_advance(s, L, T); break;
       case ORC_TRAJ_GUIDING_DIFF: ex = gcd_advance(s, L, T); break;

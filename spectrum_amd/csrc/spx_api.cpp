@@ -517,7 +517,8 @@ static int check_ready(spx_sim* s)
    if (!s->have_it || !s->have_is || !s->have_im) return fail(s, SPX_ERR_NOT_READY, "initial condition missing");
    if (s->plan.n_bt < 1) return fail(s, SPX_ERR_NOT_READY, "at least one time boundary is required");
    // TrajectoryGuiding (no diffusion, src/trajectory_guiding.hh) integrates without a diffusion model
-   if (!s->have_df && s->plan.traj_type != SPX_TRAJ_GUIDING && s->plan.traj_type != SPX_TRAJ_FIELDLINE)
+   if (!s->have_df && s->plan.traj_type != SPX_TRAJ_GUIDING && s->plan.traj_type != SPX_TRAJ_FIELDLINE &&
+       s->plan.traj_type != SPX_TRAJ_LORENTZ)
       return fail(s, SPX_ERR_NOT_READY, "no diffusion model");
    if (s->plan.traj_type == SPX_TRAJ_PARKER_SOURCE && s->plan.src_kind == SPX_SRC_NONE) return fail(s, SPX_ERR_NOT_READY, "no source");
    if (s->plan.bg.kind == SPX_BG_GRID && !s->grid) return fail(s, SPX_ERR_NOT_READY, "BackgroundServerCartesian has no grid data");
@@ -527,7 +528,21 @@ static int check_ready(spx_sim* s)
 // Combinations the kernels implement (the reference rejects the others at compile time).
 static int check_transport(spx_sim* s)
 {
-   if (s->plan.traj_type == SPX_TRAJ_FIELDLINE) {
+   if (s->plan.traj_type == SPX_TRAJ_LORENTZ) {
+      if (s->plan.bg.kind != SPX_BG_SOLARWIND && s->plan.bg.kind != SPX_BG_UNIFORM)
+         return fail(s, SPX_ERR_ARG, "full orbit: SolarWind and Uniform backgrounds");
+      const int im = s->plan.im_kind;
+      if (im != SPX_IM_FIXED && im != SPX_IM_SHELL && im != SPX_IM_THICKSHELL)
+         return fail(s, SPX_ERR_ARG, "full orbit: initial momentum must be Fixed, Shell or ThickShell");
+      if (s->plan.src_kind != SPX_SRC_NONE) return fail(s, SPX_ERR_ARG, "full orbit: no source term");
+      // its pitch-angle values need bhat at the event (src/distribution_other.cc:284-287)
+      for (int d = 0; d < s->plan.n_dist; d++) {
+         const spx_ds_desc& D = s->plan.dist[d];
+         if (D.kind == SPX_DS_POSPITCH || (D.kind == SPX_DS_MOMENTUM && D.val_coord != 0))
+            return fail(s, SPX_ERR_ARG, "full orbit: pitch-angle distributions are not supported");
+      }
+   }
+   else if (s->plan.traj_type == SPX_TRAJ_FIELDLINE) {
       // BoundaryMomentum evaluates nothing for the field line (src/boundary_momentum.cc:69-76); its momentum is
       // (0, 0, p_par) (src/initial_momentum.cc:289-292, :390-391)
       for (int b = 0; b < s->plan.n_bnd; b++)
@@ -565,7 +580,7 @@ extern "C" {
 
 int spx_create(int traj_type, int time_flow, spx_sim** out)
 {
-   if (!out || traj_type < 0 || traj_type > SPX_TRAJ_FIELDLINE || (time_flow != 0 && time_flow != 1)) return SPX_ERR_ARG;
+   if (!out || traj_type < 0 || traj_type > SPX_TRAJ_LORENTZ || (time_flow != 0 && time_flow != 1)) return SPX_ERR_ARG;
    spx_sim* s = new spx_sim();
    std::memset(&s->plan, 0, sizeof(s->plan));
    s->plan.traj_type = traj_type;

@@ -357,8 +357,8 @@ __device__ __forceinline__ double time_proximity(PlanPtr P, const HOT& H, double
 // TrajectoryBase::HandleBoundaries (src/trajectory_base.cc:413-535) + UpdateAllBoundaries.  Events are
 // queued in the slot's event buffer for the service kernel.  Returns a bit set: 1 = TRAJ_FINISH,
 // 2 = an event was queued.  HOT is the transport's register state: t, x, y, z, the boundary arrays
-// bd / bdo / bst and the momentum accessors m0() / m2() (Parker keeps (p, 0, 0), the guiding centre
-// (p_perp, 0, p_par)) and reflect_mom() (a reflecting wall reverses p_par of a guiding centre,
+// bd / bdo / bst and the momentum accessors m0() / m1() / m2() (Parker keeps (p, 0, 0), the guiding centre
+// (p_perp, 0, p_par), the field line (0, 0, p_par), the full orbit (px, py, pz)) and reflect_mom() (a reflecting wall reverses p_par of a guiding centre,
 // src/trajectory_guiding.hh:121-124).
 template <bool GC, typename HOT>
 __device__ __forceinline__ int handle_boundaries(PlanPtr P, LaunchPtr L, uint64_t slot,
@@ -370,7 +370,7 @@ __device__ __forceinline__ int handle_boundaries(PlanPtr P, LaunchPtr L, uint64_
       if (b < P->n_bnd) {
          // ComputeBoundary (src/boundary_base.cc:105-127)
          const int left = (H.bst[b] & SPX_BND_RECURRENT) ? C.cross[b][tid] : 0;
-         const double d = bnd_eval<typename HOT::real>(P->bnd[b], left, GC, H.t, H.x, H.y, H.z, H.m0(), 0.0, H.m2(), nullptr);
+         const double d = bnd_eval<typename HOT::real>(P->bnd[b], left, GC, H.t, H.x, H.y, H.z, H.m0(), H.m1(), H.m2(), nullptr);
          const double prod = d * H.bdo[b];
          H.bd[b] = (prod == 0.0) ? -H.bdo[b] : d;
          H.bst[b] = (prod <= 0.0) ? (H.bst[b] | SPX_BND_CROSSED) : (H.bst[b] & ~SPX_BND_CROSSED);
@@ -393,12 +393,12 @@ __device__ __forceinline__ int handle_boundaries(PlanPtr P, LaunchPtr L, uint64_
          active = b;
          if (cls == SPX_BC_SPACE && (H.bst[b] & SPX_BND_REFLECT)) {
             double nrm[3];
-            (void)bnd_eval(P->bnd[b], 0, GC, H.t, H.x, H.y, H.z, H.m0(), 0.0, H.m2(), nrm);
+            (void)bnd_eval(P->bnd[b], 0, GC, H.t, H.x, H.y, H.z, H.m0(), H.m1(), H.m2(), nrm);
             H.x -= 2.0 * H.bd[b] * nrm[0];
             H.y -= 2.0 * H.bd[b] * nrm[1];
             H.z -= 2.0 * H.bd[b] * nrm[2];
             H.reflect_mom();
-            const double d = bnd_eval(P->bnd[b], 0, GC, H.t, H.x, H.y, H.z, H.m0(), 0.0, H.m2(), nullptr);
+            const double d = bnd_eval(P->bnd[b], 0, GC, H.t, H.x, H.y, H.z, H.m0(), H.m1(), H.m2(), nullptr);
             const double prod = d * H.bdo[b];
             H.bd[b] = (prod == 0.0) ? -H.bdo[b] : d;
             H.bst[b] = (prod <= 0.0) ? (H.bst[b] | SPX_BND_CROSSED) : (H.bst[b] & ~SPX_BND_CROSSED);
@@ -410,7 +410,7 @@ __device__ __forceinline__ int handle_boundaries(PlanPtr P, LaunchPtr L, uint64_
       }
       if (active >= 0 && bnd_has_action(P, active)) {
          ev_field(L, cls, EV_T, slot) = H.t; ev_field(L, cls, EV_X, slot) = H.x; ev_field(L, cls, EV_Y, slot) = H.y;
-         ev_field(L, cls, EV_Z, slot) = H.z; ev_field(L, cls, EV_M0, slot) = H.m0(); ev_field(L, cls, EV_M1, slot) = 0.0;
+         ev_field(L, cls, EV_Z, slot) = H.z; ev_field(L, cls, EV_M0, slot) = H.m0(); ev_field(L, cls, EV_M1, slot) = H.m1();
          ev_field(L, cls, EV_M2, slot) = H.m2(); ev_field(L, cls, EV_AMP, slot) = C.amp[tid]; ev_field(L, cls, EV_WGT, slot) = C.wgt[tid];
          L->ev_bnd[cls * L->n_lanes + slot] = active;
          L->ev_pending[slot] |= (1 << cls);
@@ -426,7 +426,7 @@ __device__ __forceinline__ int handle_boundaries(PlanPtr P, LaunchPtr L, uint64_
          const int left = C.cross[b][tid] - (crossed ? 1 : 0);
          C.cross[b][tid] = left;
          const double d_now = H.bd[b];
-         const double d_rec = recur ? bnd_eval(P->bnd[b], left, GC, H.t, H.x, H.y, H.z, H.m0(), 0.0, H.m2(), nullptr) : d_now;
+         const double d_rec = recur ? bnd_eval(P->bnd[b], left, GC, H.t, H.x, H.y, H.z, H.m0(), H.m1(), H.m2(), nullptr) : d_now;
          H.bdo[b] = recur ? H.bdo[b] : d_now;
          H.bd[b] = d_rec;
          const int s = H.bst[b] & ~(SPX_BND_CROSSED | SPX_BND_TERMINAL);

@@ -271,6 +271,7 @@ struct GcHot {
    typedef R real;
    __device__ __forceinline__ double m0() const { return pp; }
    __device__ __forceinline__ double m2() const { return pl; }
+   __device__ __forceinline__ double m1() const { return 0.0; }
    __device__ __forceinline__ void reflect_mom() { pl = -pl; }
 };
 

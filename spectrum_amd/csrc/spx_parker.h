@@ -251,6 +251,7 @@ struct PkHot {
    bool fieldline;                              // TrajectoryFieldline: p is the signed p_par of (0, 0, p_par)
    typedef R real;
    __device__ __forceinline__ double m0() const { return fieldline ? 0.0 : p; }
+   __device__ __forceinline__ double m1() const { return 0.0; }
    __device__ __forceinline__ double m2() const { return fieldline ? p : 0.0; }
    // TrajectoryParker::ReverseMomentum does nothing (src/trajectory_parker.hh:129-131); the field line
    // keeps TrajectoryBase's, which reverses the momentum (src/trajectory_base.cc:139-142)

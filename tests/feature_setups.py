@@ -265,14 +265,48 @@ def fieldline_backward_shock():
     return s
 
 
+def lorentz_uniform():
+    """TrajectoryLorentz (forward) in a uniform oblique field carried by a flow (E = -u x B / c): isotropic
+    momenta on a thick shell, a reflecting plane below the start, a pass plane above it and a time limit.
+    The time step is the gyro period / 100 for the slow particles and 0.1 dmax / v for the fast ones."""
+    s = cf.Setup("lorentz_uniform", cf.TRAJ_LORENTZ, cf.FORWARD)
+    s.add(cf.BACKGROUND, "BackgroundUniform", cf.background_base(DC(), 0.0, (0.0, 0.0, 0.0), (0.002, 0.001, 0.0), (3.0, 4.0, 12.0), 1.0))
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((0.0, 0.0, 0.0)))
+    s.add(cf.INITIAL, "InitialMomentumThickShell", DC().insert_double(cf.mom_from_ekin(1.0 * cf.MeV)).insert_double(cf.mom_from_ekin(100.0 * cf.MeV)).insert_bool(True))
+    s.add(cf.BOUNDARY, "BoundaryPlaneReflect", cf.boundary(-1, [-1], ("vec", (0.0, 0.0, -3.0)), ("vec", (0.0, 0.0, 1.0))))
+    s.add(cf.BOUNDARY, "BoundaryPlanePass", cf.boundary(-1, [0], ("vec", (0.0, 0.0, 6.0)), ("vec", (0.0, 0.0, 1.0))))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", 40.0)))
+    d = dist((10, 0, 0), (0.0, 0, 0), (44.0, 0, 0))
+    d.insert_double(1.0).insert_double(0.5).insert_int(1)
+    s.add(cf.DISTRIBUTION, "DistributionTimeUniform", d)
+    return s
+
+
+def lorentz_spiral_backward():
+    """TrajectoryLorentz (backward) in the solar wind near 1 au: protons on a shell, the motional field of
+    the flow at the start point, an absorbing sphere just inside the start and a time limit."""
+    s = cf.Setup("lorentz_spiral_backward", cf.TRAJ_LORENTZ, cf.BACKWARD)
+    s.add(cf.BACKGROUND, "BackgroundSolarWind", cf.solarwind())
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((0.8 * cf.one_au, 0.5 * cf.one_au, 0.2 * cf.one_au)))
+    s.add(cf.INITIAL, "InitialMomentumShell", DC().insert_double(cf.mom_from_ekin(10.0 * cf.MeV)))
+    s.add(cf.BOUNDARY, "BoundarySphereAbsorb", cf.boundary(1, [0], ("vec", (0.0, 0.0, 0.0)), ("double", 0.9635 * cf.one_au)))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", -30.0)))
+    d = dist((10, 0, 0), (-30.0, 0, 0), (0.0, 0, 0), (0, 0, 0), (1, 0, 0))
+    d.insert_double(1.0).insert_double(0.0).insert_int(1)
+    s.add(cf.DISTRIBUTION, "DistributionTimeUniform", d)
+    return s
+
+
 ALL = [shock_flowmom_reflect, solarwind_kinrad_reflect_recurrent, smoothshock_perp_momentum, uniform_para_plane_pass,
        lambda: modulation_spectrum_type(0), lambda: modulation_spectrum_type(2), gc_rigidity_reflect, gc_isotropic_only,
        guiding_spiral, guiding_scatt_uniform, guiding_diff_rigidity, parker_strauss_heliopause, parker_potgieter_uniform,
-       gc_qlt_spiral, fieldline_spiral, fieldline_backward_shock]
+       gc_qlt_spiral, fieldline_spiral, fieldline_backward_shock, lorentz_uniform, lorentz_spiral_backward]
 NAMES = ["shock_flowmom_reflect", "solarwind_kinrad_reflect_recurrent", "smoothshock_perp_momentum", "uniform_para_plane_pass",
          "modulation_type0", "modulation_type2", "gc_rigidity_reflect", "gc_isotropic_only",
          "guiding_spiral", "guiding_scatt_uniform", "guiding_diff_rigidity", "parker_strauss_heliopause", "parker_potgieter_uniform",
-         "gc_qlt_spiral", "fieldline_spiral", "fieldline_backward_shock"]
+         "gc_qlt_spiral", "fieldline_spiral", "fieldline_backward_shock", "lorentz_uniform", "lorentz_spiral_backward"]
 
 
 # Setups whose endpoints amplify last-bit differences far beyond 1e-10: QLT's mu derivative is
