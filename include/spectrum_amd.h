@@ -34,7 +34,8 @@ enum { SPX_BACKGROUND = 0, SPX_DIFFUSION = 1, SPX_SOURCE = 2, SPX_BOUNDARY = 3, 
 
 /* Transport (configure.ac:237-261 TRAJ_TYPE) and time flow (configure.ac:264-280 TRAJ_TIME_FLOW). */
 enum { SPX_TRAJ_PARKER = 0, SPX_TRAJ_PARKER_SOURCE = 1, SPX_TRAJ_GUIDING_DIFF_SCATT = 2, SPX_TRAJ_GUIDING = 3,
-       SPX_TRAJ_GUIDING_SCATT = 4, SPX_TRAJ_GUIDING_DIFF = 5, SPX_TRAJ_FIELDLINE = 6, SPX_TRAJ_LORENTZ = 7 };
+       SPX_TRAJ_GUIDING_SCATT = 4, SPX_TRAJ_GUIDING_DIFF = 5, SPX_TRAJ_FIELDLINE = 6, SPX_TRAJ_LORENTZ = 7,
+       SPX_TRAJ_FOCUSED = 8 };
 enum { SPX_FLOW_FORWARD = 0, SPX_FLOW_BACKWARD = 1 };
 
 /* Step precision: FP64 drop-in, or FP32 step arithmetic with FP64 accumulation. */

@@ -902,7 +902,7 @@ static void dist_event(const orc_sim* s, const dist_t* d, int action,
    vec3 value = V(0, 0, 0);
    double w[9] = {0};
    int traj_type = s->traj_type;
-   int parker = (traj_type < ORC_TRAJ_GUIDING_DIFF_SCATT);
+   int parker = (traj_type < ORC_TRAJ_GUIDING_DIFF_SCATT || traj_type == ORC_TRAJ_FOCUSED);
    ev->valid = 0;
    /* EvaluateValue */
    switch (d->kind) {
@@ -1065,6 +1065,7 @@ static void handle_boundaries(const orc_sim* s, traj_t* T)
             T->pos = vsub(T->pos, vscl(2.0 * T->bs[b].delta, T->bs[b].normal));
             if (ORC_IS_GC(s->traj_type)) T->mom.v[2] = -T->mom.v[2];  /* src/trajectory_guiding.hh:121-124 */
             else if (s->traj_type == ORC_TRAJ_FIELDLINE || s->traj_type == ORC_TRAJ_LORENTZ) T->mom = vneg(T->mom);  /* TrajectoryBase's, src/trajectory_base.cc:139-142 */
+            else if (s->traj_type == ORC_TRAJ_FOCUSED) T->mom.v[1] = -T->mom.v[1];  /* src/trajectory_focused.hh:126-129 */
             T->n_refl++;
             bnd_compute(&s->bs[b], &T->bs[b], s->traj_type, T->t, T->pos, T->mom);
             T->bs[b].crossings_left--;
@@ -1488,10 +1489,62 @@ static int lz_advance(const orc_sim* s, lane_t* L, traj_t* T)
    return ORC_EX_NONE;
 }
 
+/* TrajectoryFocused (src/trajectory_focused.cc:31-158, src/trajectory_focused.hh:17-32): momentum (p, mu, 0),
+   velocity (p (v(p) / p), mu, 0) (Particle::Vel with the SphericalRMP metric, common/physics.hh:330-336,
+   common/coordinates.hh:48,379-382); fields with mask ALL | gradB | gradU | dUdt (:36-37) */
+#define FC_MASK (BG_ALL | BG_gradB | BG_gradU | BG_dUdt)
+
+/* V * M with V a row vector: (V * M)_j = sum_i V_i M_ij (common/matrix.hh) */
+static inline vec3 vmmul(vec3 a, const mat3* m)
+{
+   return vadd(vadd(vscl(a.v[0], m->row[0]), vscl(a.v[1], m->row[1])), vscl(a.v[2], m->row[2]));
+}
+
+/* DriftCoeff with TRAJ_FOCUSED_USE_B_DRIFTS (:55-79), Slopes with PPERP_METHOD 1 (:98-125), PhysicalStep
+   (:87-91), the Euler RKStep, HandleBoundaries, then CommonFields and MomentumCorrection (:142-152) in
+   RKAdvance's order (src/trajectory_base.cc:545-579); no random numbers */
+static int fc_advance(const orc_sim* s, lane_t* L, traj_t* T)
+{
+   int fwd = (s->time_flow == ORC_FLOW_FORWARD);
+   const spd_t* sp = &T->sp;
+   double p = T->mom.v[0], mu = T->mom.v[1];
+   double v = p * (VelP(p) / p);
+   double ct2 = Sqr(mu), st2 = 1.0 - ct2;
+   vec3 drift = vdiv(vscl(0.5 * st2, vcross(sp->bhat, sp->gradBmag)), sp->Bmag);
+   vec3 BgB = vmmul(sp->B, &sp->gradB);
+   drift = vadd(drift, vdiv(vadd(vscl(vdot(sp->B, curlB(sp)), vscl(0.5 * st2, sp->bhat)), vscl(ct2, vcross(sp->bhat, BgB))), Sqr(sp->Bmag)));
+   drift = vscl(LarmorRadius(p, sp->Bmag) * v, drift);
+   drift = vadd(drift, vadd(sp->U, vscl(v * mu, sp->bhat)));
+   /* bb : grad U (common/matrix.hh:556-561), 2 b . (dU/dt + U . grad U) / v */
+   double bbgU = 0.0;
+   int i;
+   for (i = 0; i < 3; i++) bbgU += vdot(vscl(sp->bhat.v[i], sp->bhat), sp->gradU.row[i]);
+   vec3 cdU = vadd(sp->dUdt, vmmul(sp->U, &sp->gradU));
+   double bcdU = vdot(vscl(2.0, sp->bhat), cdU) / v;
+   double divu = divU(sp);
+   double divbhat = (mtrace(&sp->gradB) - vdot(sp->gradBmag, sp->bhat)) / sp->Bmag;   /* common/spatial_data.hh:304-307 */
+   vec3 slope_mom = V(0.5 * p * ((3.0 * st2 - 2.0) * bbgU - st2 * divu - mu * bcdU),
+                      0.5 * st2 * (v * divbhat - bcdU + mu * (divu - 3.0 * bbgU)), 0.0);
+   T->dt_physical = 0.5 * sp->dmax / (vnorm(drift) + 0.5 * v);
+   T->dt = fmin(T->dt_physical, T->dt_adaptive);
+   time_boundary_proximity(s, T);
+   double dtv = T->dt * 1.0;
+   if (fwd) { T->t += T->dt; T->pos = vadd(T->pos, vscl(dtv, drift)); T->mom = vadd(T->mom, vscl(dtv, slope_mom)); }
+   else { T->t -= T->dt; T->pos = vsub(T->pos, vscl(dtv, drift)); T->mom = vsub(T->mom, vscl(dtv, slope_mom)); }
+   handle_boundaries(s, T);
+   if (!(T->status & TRAJ_FINISH)) {
+      if (common_fields(s, L, T, FC_MASK)) EX(ORC_EX_FIELD);
+      if (T->mom.v[1] > 1.0) T->mom.v[1] = 1.0;
+      else if (T->mom.v[1] < -1.0) T->mom.v[1] = -1.0;
+   }
+   T->segs++;
+   return ORC_EX_NONE;
+}
+
 /* Initial momentum sample (src/initial_momentum.cc:283-410) */
 static vec3 mom_sample(const orc_sim* s, lane_t* L)
 {
-   int gc = ORC_IS_GC(s->traj_type);
+   int gc = ORC_IS_GC(s->traj_type), fc = (s->traj_type == ORC_TRAJ_FOCUSED);
    double p;
    switch (s->im_kind) {
    case IM_FIXED: return s->im_mom;
@@ -1511,6 +1564,7 @@ static vec3 mom_sample(const orc_sim* s, lane_t* L)
    switch (s->im_kind) {
    case IM_RING:  /* src/initial_momentum.cc:203-220: fixed (p sin theta0, 0, p cos theta0), no draw */
       if (gc) return V(s->im_p0 * s->im_st0, 0.0, s->im_p0 * s->im_mu0);
+      if (fc) return V(s->im_p0, s->im_mu0, 0.0);
       return V(s->im_p0, 0.0, 0.0);
    case IM_SHELL: p = s->im_p0; break;
    case IM_THICKSHELL:
@@ -1527,9 +1581,10 @@ static vec3 mom_sample(const orc_sim* s, lane_t* L)
    default: p = 0.0;
    }
    if (s->traj_type == ORC_TRAJ_FIELDLINE) return V(0.0, 0.0, p);  /* src/initial_momentum.cc:289-292, :390-391 */
-   if (!gc) return V(p, 0.0, 0.0);
+   if (!gc && !fc) return V(p, 0.0, 0.0);
    double mu = -1.0 + 2.0 * draw_uniform(L);
    double st = sqrt(1.0 - Sqr(mu));
+   if (fc) return V(p, mu, 0.0);  /* src/initial_momentum.cc:311-312, :399-400 */
    return V(p * st, 0.0, p * mu);
 }
 
@@ -1568,6 +1623,7 @@ static int integrate_one(const orc_sim* s, lane_t* L, traj_t* T)
       case ORC_TRAJ_GUIDING_DIFF_SCATT: ex = gcds_advance(s, L, T); break;
       case ORC_TRAJ_FIELDLINE: ex = fl_advance(s, L, T); break;
       case ORC_TRAJ_LORENTZ: ex = lz_advance(s, L, T); break;
+      case ORC_TRAJ_FOCUSED: ex = fc_advance(s, L, T); break;
       case ORC_TRAJ_GUIDING: ex = gcg_advance(s, L, T); break;
       case ORC_TRAJ_GUIDING_SCATT: ex = gcs_advance(s, L, T); break;
       case ORC_TRAJ_GUIDING_DIFF: ex = gcd_advance(s, L, T); break;

@@ -25,7 +25,7 @@ enum { ORC_BACKGROUND = 0, ORC_DIFFUSION = 1, ORC_SOURCE = 2, ORC_BOUNDARY = 3, 
 /* Transport families (configure.ac:237-261 TRAJ_TYPE) and time flow (configure.ac:264-280). */
 enum { ORC_TRAJ_PARKER = 0, ORC_TRAJ_PARKER_SOURCE = 1, ORC_TRAJ_GUIDING_DIFF_SCATT = 2, ORC_TRAJ_GUIDING = 3,
        ORC_TRAJ_GUIDING_SCATT = 4, ORC_TRAJ_GUIDING_DIFF = 5, ORC_TRAJ_FIELDLINE = 6,
-       ORC_TRAJ_LORENTZ = 7 };
+       ORC_TRAJ_LORENTZ = 7, ORC_TRAJ_FOCUSED = 8 };
 enum { ORC_FLOW_FORWARD = 0, ORC_FLOW_BACKWARD = 1 };
 
 /* RNG modes: counter-based Philox4x32-10 streams, or replay of supplied draws. */

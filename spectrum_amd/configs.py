@@ -63,7 +63,7 @@ BACKGROUND, DIFFUSION, SOURCE, BOUNDARY, INITIAL, DISTRIBUTION = range(6)
 ROLE_NAMES = ["background", "diffusion", "source", "boundary", "initial", "distribution"]
 
 # transports and time flows
-TRAJ_PARKER, TRAJ_PARKER_SOURCE, TRAJ_GUIDING_DIFF_SCATT, TRAJ_GUIDING, TRAJ_GUIDING_SCATT, TRAJ_GUIDING_DIFF, TRAJ_FIELDLINE, TRAJ_LORENTZ = 0, 1, 2, 3, 4, 5, 6, 7
+TRAJ_PARKER, TRAJ_PARKER_SOURCE, TRAJ_GUIDING_DIFF_SCATT, TRAJ_GUIDING, TRAJ_GUIDING_SCATT, TRAJ_GUIDING_DIFF, TRAJ_FIELDLINE, TRAJ_LORENTZ, TRAJ_FOCUSED = 0, 1, 2, 3, 4, 5, 6, 7, 8
 FORWARD, BACKWARD = 0, 1
 
 

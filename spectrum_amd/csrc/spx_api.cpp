@@ -518,7 +518,7 @@ static int check_ready(spx_sim* s)
    if (s->plan.n_bt < 1) return fail(s, SPX_ERR_NOT_READY, "at least one time boundary is required");
    // TrajectoryGuiding (no diffusion, src/trajectory_guiding.hh) integrates without a diffusion model
    if (!s->have_df && s->plan.traj_type != SPX_TRAJ_GUIDING && s->plan.traj_type != SPX_TRAJ_FIELDLINE &&
-       s->plan.traj_type != SPX_TRAJ_LORENTZ)
+       s->plan.traj_type != SPX_TRAJ_LORENTZ && s->plan.traj_type != SPX_TRAJ_FOCUSED)
       return fail(s, SPX_ERR_NOT_READY, "no diffusion model");
    if (s->plan.traj_type == SPX_TRAJ_PARKER_SOURCE && s->plan.src_kind == SPX_SRC_NONE) return fail(s, SPX_ERR_NOT_READY, "no source");
    if (s->plan.bg.kind == SPX_BG_GRID && !s->grid) return fail(s, SPX_ERR_NOT_READY, "BackgroundServerCartesian has no grid data");
@@ -541,6 +541,13 @@ static int check_transport(spx_sim* s)
          if (D.kind == SPX_DS_POSPITCH || (D.kind == SPX_DS_MOMENTUM && D.val_coord != 0))
             return fail(s, SPX_ERR_ARG, "full orbit: pitch-angle distributions are not supported");
       }
+   }
+   else if (s->plan.traj_type == SPX_TRAJ_FOCUSED) {
+      if (s->plan.bg.kind != SPX_BG_SOLARWIND && s->plan.bg.kind != SPX_BG_UNIFORM)
+         return fail(s, SPX_ERR_ARG, "focused transport: SolarWind and Uniform backgrounds");
+      if (s->plan.src_kind != SPX_SRC_NONE) return fail(s, SPX_ERR_ARG, "focused transport: no source term");
+      for (int d = 0; d < s->plan.n_dist; d++)
+         if (s->plan.dist[d].kind == SPX_DS_POSPITCH) return fail(s, SPX_ERR_ARG, "focused transport: its momentum is (p, mu, 0); use DistributionMomentumUniform");
    }
    else if (s->plan.traj_type == SPX_TRAJ_FIELDLINE) {
       // BoundaryMomentum evaluates nothing for the field line (src/boundary_momentum.cc:69-76); its momentum is
@@ -580,7 +587,7 @@ extern "C" {
 
 int spx_create(int traj_type, int time_flow, spx_sim** out)
 {
-   if (!out || traj_type < 0 || traj_type > SPX_TRAJ_LORENTZ || (time_flow != 0 && time_flow != 1)) return SPX_ERR_ARG;
+   if (!out || traj_type < 0 || traj_type > SPX_TRAJ_FOCUSED || (time_flow != 0 && time_flow != 1)) return SPX_ERR_ARG;
    spx_sim* s = new spx_sim();
    std::memset(&s->plan, 0, sizeof(s->plan));
    s->plan.traj_type = traj_type;

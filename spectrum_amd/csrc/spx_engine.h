@@ -176,7 +176,7 @@ __device__ __forceinline__ double wave_sum(double v)
 static __device__ int bin_events(PlanPtr P, QueryPtr Q, LaunchPtr L, uint64_t slot, bool has, int bnd,
                                  double t2, double x2, double y2, double z2, double m20, double m21, double m22, double amp, double logwgt)
 {
-   const bool parker = (P->traj_type < 2);
+   const bool parker = (P->traj_type < 2 || P->traj_type == SPX_TRAJ_FOCUSED);   // momentum (p, ., .)
    const double t1 = L->t0[slot], x1 = L->x0[slot], y1 = L->y0[slot], z1 = L->z0[slot];
    const double m10 = L->m00[slot], m11 = L->m01[slot], m12 = L->m02[slot];
    int n_ev = 0;

@@ -13,6 +13,7 @@
 #include "spx_parker.h"
 #include "spx_gc.h"
 #include "spx_lorentz.h"
+#include "spx_focused.h"
 
 namespace spx {
 
@@ -22,6 +23,7 @@ hipError_t launch_f32(const spx_plan_desc& hp, uint64_t n_lanes, const Launch* d
    if (hp.traj_type == SPX_TRAJ_PARKER || hp.traj_type == SPX_TRAJ_PARKER_SOURCE || hp.traj_type == SPX_TRAJ_FIELDLINE) return dispatch_parker<float>(hp, n_lanes, d_launch, st, mid);
    if (hp.traj_type >= SPX_TRAJ_GUIDING_DIFF_SCATT && hp.traj_type <= SPX_TRAJ_GUIDING_DIFF) return dispatch_gc<float>(hp, n_lanes, d_launch, st, mid);
    if (hp.traj_type == SPX_TRAJ_LORENTZ) return dispatch_lorentz<float>(hp, n_lanes, d_launch, st, mid);
+   if (hp.traj_type == SPX_TRAJ_FOCUSED) return dispatch_focused<float>(hp, n_lanes, d_launch, st, mid);
    return hipErrorInvalidValue;
 }
 

@@ -299,21 +299,64 @@ def lorentz_spiral_backward():
     return s
 
 
+def focused_spiral():
+    """TrajectoryFocused (forward) along the Parker spiral from 1 au off the ecliptic: adiabatic focusing,
+    adiabatic cooling in the expanding wind and the gradient/curvature drifts; a reflecting plane, absorbing
+    spheres at 0.5 and 3 au, a time limit, a momentum and a time distribution."""
+    s = cf.Setup("focused_spiral", cf.TRAJ_FOCUSED, cf.FORWARD)
+    s.add(cf.BACKGROUND, "BackgroundSolarWind", cf.solarwind())
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((0.8 * cf.one_au, 0.5 * cf.one_au, 0.2 * cf.one_au)))
+    s.add(cf.INITIAL, "InitialMomentumThickShell", DC().insert_double(cf.mom_from_ekin(1.0 * cf.MeV)).insert_double(cf.mom_from_ekin(100.0 * cf.MeV)).insert_bool(True))
+    s.add(cf.BOUNDARY, "BoundaryPlaneReflect", cf.boundary(-1, [-1, -1], ("vec", (0.0, 0.0, 0.35 * cf.one_au)), ("vec", (0.0, 0.0, 1.0))))
+    s.add(cf.BOUNDARY, "BoundarySphereAbsorb", cf.boundary(1, [0, 0], ("vec", (0.0, 0.0, 0.0)), ("double", 3.0 * cf.one_au)))
+    s.add(cf.BOUNDARY, "BoundarySphereAbsorb", cf.boundary(1, [0, 0], ("vec", (0.0, 0.0, 0.0)), ("double", 0.5 * cf.one_au)))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0, 0], ("double", 2.0 * cf.one_day)))
+    d = dist((12, 0, 0), (0.0, 0, 0), (4.0 * cf.one_day, 0, 0), (0, 0, 0), (1, 0, 0))
+    d.insert_double(1.0).insert_double(0.0).insert_int(1)
+    s.add(cf.DISTRIBUTION, "DistributionTimeUniform", d)
+    d = dist((10, 8, 1), (cf.mom_from_ekin(0.5 * cf.MeV), -1.0, -1.0), (cf.mom_from_ekin(120.0 * cf.MeV), 1.0, 1.0), (1, 0, 0), (1, 1, 1))
+    d.insert_double(1.0).insert_double(0.0).insert_int(1).insert_int(1)
+    s.add(cf.DISTRIBUTION, "DistributionMomentumUniform", d)
+    return s
+
+
+def focused_uniform_backward():
+    """TrajectoryFocused (backward) in a uniform oblique field carried by a flow: a ring of pitch cosine 0.6,
+    straight drift at U + v mu b through a pass plane, against a reflecting plane, to a time limit."""
+    s = cf.Setup("focused_uniform_backward", cf.TRAJ_FOCUSED, cf.BACKWARD)
+    s.add(cf.BACKGROUND, "BackgroundUniform", cf.background_base(DC(), 0.0, (0.0, 0.0, 0.0), (0.02, 0.01, 0.0), (3.0, 4.0, 12.0), 2.0))
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((0.0, 0.0, 0.0)))
+    s.add(cf.INITIAL, "InitialMomentumRing", DC().insert_double(cf.mom_from_ekin(10.0 * cf.MeV)).insert_double(0.6))
+    s.add(cf.BOUNDARY, "BoundaryPlanePass", cf.boundary(-1, [0], ("vec", (0.0, 0.0, -20.0)), ("vec", (0.0, 0.0, 1.0))))
+    s.add(cf.BOUNDARY, "BoundaryPlaneReflect", cf.boundary(-1, [-1], ("vec", (0.0, 0.0, -60.0)), ("vec", (0.0, 0.0, 1.0))))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", -200.0)))
+    d = dist((10, 0, 0), (-220.0, 0, 0), (0.0, 0, 0), (0, 0, 0), (1, 0, 0))
+    d.insert_double(1.0).insert_double(0.0).insert_int(1)
+    s.add(cf.DISTRIBUTION, "DistributionTimeUniform", d)
+    return s
+
+
 ALL = [shock_flowmom_reflect, solarwind_kinrad_reflect_recurrent, smoothshock_perp_momentum, uniform_para_plane_pass,
        lambda: modulation_spectrum_type(0), lambda: modulation_spectrum_type(2), gc_rigidity_reflect, gc_isotropic_only,
        guiding_spiral, guiding_scatt_uniform, guiding_diff_rigidity, parker_strauss_heliopause, parker_potgieter_uniform,
-       gc_qlt_spiral, fieldline_spiral, fieldline_backward_shock, lorentz_uniform, lorentz_spiral_backward]
+       gc_qlt_spiral, fieldline_spiral, fieldline_backward_shock, lorentz_uniform, lorentz_spiral_backward,
+       focused_spiral, focused_uniform_backward]
 NAMES = ["shock_flowmom_reflect", "solarwind_kinrad_reflect_recurrent", "smoothshock_perp_momentum", "uniform_para_plane_pass",
          "modulation_type0", "modulation_type2", "gc_rigidity_reflect", "gc_isotropic_only",
          "guiding_spiral", "guiding_scatt_uniform", "guiding_diff_rigidity", "parker_strauss_heliopause", "parker_potgieter_uniform",
-         "gc_qlt_spiral", "fieldline_spiral", "fieldline_backward_shock", "lorentz_uniform", "lorentz_spiral_backward"]
+         "gc_qlt_spiral", "fieldline_spiral", "fieldline_backward_shock", "lorentz_uniform", "lorentz_spiral_backward",
+         "focused_spiral", "focused_uniform_backward"]
 
 
 # Setups whose endpoints amplify last-bit differences far beyond 1e-10: QLT's mu derivative is
 # DiffusionBase::GetMuDerivative's forward difference with dmu = 1e-8 (src/diffusion_base.cc:238), which
 # multiplies rounding differences by ~1e8.  Their parity bound is scaled by the oracle's own disagreement
-# with its last-bit-perturbed build (test_gpu_parity.sensitivity).
-ILL_CONDITIONED = {"gc_qlt_spiral"}
+# with its last-bit-perturbed build (test_gpu_parity.sensitivity).  The focused transport along the spiral
+# differentiates the field over the Larmor radius (a difference of nearby field values divided by ~1e-3 of
+# the distance to the Sun), and its long trajectories reach a few 1e-10 of last-bit sensitivity.
+ILL_CONDITIONED = {"gc_qlt_spiral", "focused_spiral"}
 
 
 def weight_atol(name, rec, rtol):

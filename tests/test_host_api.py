@@ -73,7 +73,7 @@ def new_sim(traj=cf.TRAJ_PARKER, flow=cf.BACKWARD):
 def test_create_rejects_bad_arguments():
     L = engine.lib()
     h = C.c_void_p()
-    assert L.spx_create(8, 0, C.byref(h)) == -1
+    assert L.spx_create(9, 0, C.byref(h)) == -1
     assert L.spx_create(0, 3, C.byref(h)) == -1
     assert L.spx_create(0, 0, None) == -1
 
