@@ -56,6 +56,7 @@ static inline Flop log(Flop a) { flop::count[flop::LOG]++; return Flop(::log(a.v
 static inline Flop log10(Flop a) { flop::count[flop::LOG]++; return Flop(::log10(a.v)); }
 static inline Flop pow(Flop a, Flop b) { flop::count[flop::POW]++; return Flop(::pow(a.v, b.v)); }
 static inline Flop cbrt(Flop a) { flop::count[flop::POW]++; return Flop(::cbrt(a.v)); }
+static inline Flop tgamma(Flop a) { flop::count[flop::POW]++; return Flop(::tgamma(a.v)); }
 static inline Flop tanh(Flop a) { flop::count[flop::TANH]++; return Flop(::tanh(a.v)); }
 static inline Flop cos(Flop a) { flop::count[flop::TRIG]++; return Flop(::cos(a.v)); }
 static inline Flop sin(Flop a) { flop::count[flop::TRIG]++; return Flop(::sin(a.v)); }

@@ -66,6 +66,8 @@ def lib(path: str = LIB_PATH):
         L.orc_grid_fields.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_kappa_at.argtypes = [C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_double), C.c_double, C.c_double]
         L.orc_kappa_at.restype = C.c_double
+        L.orc_hyp2f1_11.argtypes = [C.c_double, C.c_double]
+        L.orc_hyp2f1_11.restype = C.c_double
         L.orc_kappa.argtypes = [C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double]
         L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.orc_counter_normal.restype = C.c_double
@@ -181,6 +183,11 @@ class Oracle:
     def kappa(self, comp, pos, p, mu=0.0, Bmag=1.0, t=0.0):
         P = (C.c_double * 3)(*pos)
         return self.L.orc_kappa(self.h, comp, t, P, p, mu, Bmag)
+
+
+def hyp2f1_11(c, z):
+    """The oracle's 2F1(1, 1; c; z)."""
+    return lib().orc_hyp2f1_11(c, z)
 
 
 def philox(ctr, key):

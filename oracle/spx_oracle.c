@@ -327,7 +327,8 @@ typedef struct {
    double dom_min[3], dom_max[3], block_length[3];
    double* data;
 } grid_t;
-enum { DF_NONE = 0, DF_ISO, DF_PARA, DF_PERP, DF_FULL, DF_FLOWMOM, DF_KINRAD, DF_RIGBMAG, DF_PERPISO, DF_STRAUSS, DF_POTGIETER, DF_QLT };
+enum { DF_NONE = 0, DF_ISO, DF_PARA, DF_PERP, DF_FULL, DF_FLOWMOM, DF_KINRAD, DF_RIGBMAG, DF_PERPISO, DF_STRAUSS, DF_POTGIETER, DF_QLT,
+       DF_WNLT, DF_WNLT_RAMP };
 enum { BK_TIME = 0, BK_SPHERE, BK_PLANE, BK_MOMENTUM };
 enum { IT_FIXED = 0 };
 enum { IS_FIXED = 0 };
@@ -358,6 +359,9 @@ typedef struct {
    double lam0, R0, B0, pow_law_R, pow_law_B;
    int LISM_idx; double lam_in, lam_out, kap_rat_in, kap_rat_out;   /* Strauss (lam) / Potgieter (kappa) */
    double A2A, l_max, ps_index, k_min, ps_minus;                     /* QLTConstant */
+   double A2T, A2L, ps_plus;                                         /* WNLTConstant */
+   double l_max_HP, z_nose, z_sheath, k_min_ref, A2A_ref, A2T_ref, A2L_ref, dl_max, dz;   /* WNLTRampVLISM */
+   double hyp_c, hyp_g, hyp_r;                                       /* 2F1(1, 1; c; z): c, Gamma(c) Gamma(2 - c), (c - 1) / (c - 2) */
 } df_t;
 
 typedef struct {
@@ -402,6 +406,7 @@ typedef struct {
    mat3 gradU, gradB; vec3 gradBmag;
    vec3 dUdt, dBdt; double dBmagdt;
    double region[3]; double dmax;
+   double dr[3];   /* the spatial derivative steps (2 r_g) of NumericalDerivatives */
 } spd_t;
 
 typedef struct {
@@ -537,6 +542,7 @@ static void numerical_derivatives(const bg_t* bg, lane_t* L, double t, vec3 pos,
          }
          if (m1 & BG_E) gE.row[xyz] = vdiv(vsub(f.E, b.E), dr);
       }
+      sp->dr[0] = sp->dr[1] = sp->dr[2] = 2.0 * r_g;
       mat3 rot = mtranspose((const mat3*)fa);
       if (m1 & BG_U) sp->gradU = mmmul(&rot, &gU);
       if (m1 & BG_B) { sp->gradB = mmmul(&rot, &gB); sp->gradBmag = mvmul(&rot, gBm); }
@@ -721,6 +727,70 @@ static inline vec3 dbhatdt(const spd_t* s) { return vdiv(vsub(s->dBdt, vscl(s->d
  * ---------------------------------------------------------------------------------------------- */
 typedef struct { double K[3]; } kap_t;
 
+/* Gauss's hypergeometric function 2F1(1, 1; c; z) for 0 <= z < 1, the function GSL's gsl_sf_hyperg_2F1 gives
+   DiffusionWNLTConstant (src/diffusion_other.cc:236-241).  Written from the series (Abramowitz & Stegun
+   15.1.1: term ratio (n + 1) z / (c + n) for a = b = 1) below z = 1/2, and above it from the linear
+   transformation to 1 - z (A&S 15.3.6 with a = b = 1, where 2F1(c-1, c-1; c-1; 1-z) = z^(1-c)):
+     2F1(1, 1; c; z) = (c-1)/(c-2) 2F1(1, 1; 3-c; 1-z) + Gamma(c) Gamma(2-c) (1-z)^(c-2) z^(1-c),
+   valid for c - 2 not an integer.  g = Gamma(c) Gamma(2 - c), r = (c - 1)/(c - 2). */
+static double hyp_series_11(double c, double z)
+{
+   double sum = 1.0, term = 1.0;
+   int n;
+   for (n = 0; n < 400; n++) {
+      term *= (n + 1.0) * z / (c + n);
+      sum += term;
+      if (fabs(term) < 1e-17 * fabs(sum)) break;
+   }
+   return sum;
+}
+
+static double hyp2f1_11(double c, double g, double r, double z)
+{
+   if (z < 0.5) return hyp_series_11(c, z);
+   double w = 1.0 - z;
+   return r * hyp_series_11(3.0 - c, w) + g * pow(w, c - 2.0) * pow(z, 1.0 - c);
+}
+
+double orc_hyp2f1_11(double c, double z)
+{
+   return hyp2f1_11(c, tgamma(c) * tgamma(2.0 - c), (c - 1.0) / (c - 2.0), z);
+}
+
+/* DiffusionWNLTConstant::EvaluateDiffusion (src/diffusion_other.cc:215-245) with DiffusionQLTConstant's
+   component 2 (:145-149), and DiffusionWNLTRampVLISM's position-dependent k_min and amplitudes
+   (:301-328): components 0 (and 2 when comp == 2) into kp. */
+static void wnlt_eval(const df_t* df, kap_t* kp, int comp, vec3 pos, double vmag, double mu, double Bmag)
+{
+   double k_min = df->k_min, A2A = df->A2A, A2T = df->A2T, A2L = df->A2L;
+   if (df->kind == DF_WNLT_RAMP) {
+      double r0 = vnorm(pos);
+      double z0 = sqrt(0.5 * r0 * (r0 + pos.v[2]));
+      if (z0 > df->z_sheath) { k_min = df->k_min_ref; A2A = df->A2A_ref; A2T = df->A2T_ref; A2L = df->A2L_ref; }
+      else {
+         k_min = M_2PI_ORC / (df->l_max_HP + df->dl_max * (z0 - df->z_nose) / df->dz);
+         double k_ratio = pow(df->k_min_ref / k_min, df->ps_minus);
+         A2A = df->A2A_ref * k_ratio; A2T = df->A2T_ref * k_ratio; A2L = df->A2L_ref * k_ratio;
+      }
+   }
+   double st2 = 1.0 - Sqr(mu);
+   double Omega = CyclotronFrequency(vmag, Bmag);
+   double CT = 0.5 * df->ps_minus / df->ps_plus * Sqr(mu / k_min) * A2T;
+   double CL = 0.125 * Sqr(vmag * st2 / Omega) * A2L;
+   kp->K[0] = vmag * sqrt(CT + CL);
+   if (comp == 0) return;
+   kp->K[2] = 0.25 * M_PI * df->ps_minus * fabs(Omega) * st2 * pow(vmag * k_min * fabs(mu / Omega), df->ps_minus) * A2A;
+   if (A2T > sp_tiny) {
+      double xi1 = vmag * k_min * sqrt(st2) / M_SQRT2 / fabs(Omega);
+      double F21 = hyp2f1_11(df->hyp_c, df->hyp_g, df->hyp_r, 1.0 / (1.0 + Sqr(xi1)));
+      double DT1 = 1.0 / (1.0 + Sqr(xi1)) * F21;
+      double xi2 = kp->K[0] * k_min * k_min / fabs(Omega);
+      F21 = hyp2f1_11(df->hyp_c, df->hyp_g, df->hyp_r, 1.0 / (1.0 + Sqr(xi2)));
+      double DT2 = 1.0 / (1.0 + Sqr(xi2)) * F21;
+      kp->K[2] += (df->ps_minus / df->ps_plus) * Omega * st2 * A2T * DT1 * DT2 / sqrt(Sqr(DT1) + Sqr(DT2));
+   }
+}
+
 static double df_component(const df_t* df, kap_t* kp, int comp, vec3 pos, double p, double mu, const spd_t* sp)
 {
    double vmag = VelP(p);
@@ -783,6 +853,10 @@ static double df_component(const df_t* df, kap_t* kp, int comp, vec3 pos, double
       }
       break;
    }
+   case DF_WNLT: case DF_WNLT_RAMP:
+      if (comp == 1) break;
+      wnlt_eval(df, kp, comp, pos, vmag, mu, sp->Bmag);
+      break;
    case DF_QLT: {  /* :145-149 */
       if (comp == 0 || comp == 1) break;
       double Omega = CyclotronFrequency(vmag, sp->Bmag);
@@ -795,9 +869,30 @@ static double df_component(const df_t* df, kap_t* kp, int comp, vec3 pos, double
 }
 
 /* GetDirectionalDerivative overrides (analytic per model) for xyz in 0..2 */
-static double df_dir_deriv(const df_t* df, const kap_t* kp, int comp, int xyz, vec3 pos, const spd_t* sp)
+/* DiffusionBase::GetDirectionalDerivative (src/diffusion_base.cc:130-170): the component at the position
+   moved by half the background's derivative step along the Cartesian axis xyz, B and |B| carried there by
+   their gradients, and back by the full step */
+static double df_generic_dir_deriv(const df_t* df, kap_t* kp, int comp, int xyz, vec3 pos, double p, double mu, const spd_t* sp)
+{
+   kap_t saved = *kp;
+   spd_t f = *sp;
+   double dr = 0.5 * sp->dr[xyz];
+   vec3 pf = pos;
+   pf.v[xyz] += dr;
+   f.Bmag += sp->gradBmag.v[xyz] * dr;
+   double Kf = df_component(df, kp, comp, pf, p, mu, &f);
+   dr *= 2.0;
+   pf.v[xyz] -= dr;
+   f.Bmag -= sp->gradBmag.v[xyz] * dr;
+   double Kb = df_component(df, kp, comp, pf, p, mu, &f);
+   *kp = saved;
+   return (Kf - Kb) / dr;
+}
+
+static double df_dir_deriv(const df_t* df, kap_t* kp, int comp, int xyz, vec3 pos, double p, double mu, const spd_t* sp)
 {
    switch (df->kind) {
+   case DF_WNLT: case DF_WNLT_RAMP: return df_generic_dir_deriv(df, kp, comp, xyz, pos, p, mu, sp);
    case DF_PARA: case DF_PERP: case DF_FULL: case DF_PERPISO: return 0.0;
    case DF_FLOWMOM: return kp->K[comp] * df->pow_law_U * vdot(sp->gradU.row[xyz], sp->U) / Sqr(vnorm(sp->U));  /* :637-642 */
    case DF_KINRAD: {  /* :732-742 */
@@ -817,7 +912,7 @@ static double df_mu_deriv(const df_t* df, kap_t* kp, vec3 pos, double p, double 
 {
    switch (df->kind) {
    case DF_ISO: case DF_PERPISO: return -2.0 * df->D0 * mu;  /* src/diffusion_other.cc:76-79 */
-   case DF_QLT: {
+   case DF_QLT: case DF_WNLT: case DF_WNLT_RAMP: {
       kap_t saved = *kp;
       double dmu = sp_small * (mu + sp_small < 1.0 ? 1.0 : -1.0);
       double derivative = (df_component(df, kp, 2, pos, p, mu + dmu, sp) - saved.K[2]) / dmu;
@@ -1220,7 +1315,7 @@ static void gcd_diffcoeff(const orc_sim* s, traj_t* T, kap_t* kp, vec3* fa, doub
    *Dperp = df_component(&s->df, kp, 0, T->pos, mc.v[0], mc.v[1], sp);
    vec3 g;
    int k;
-   for (k = 0; k < 3; k++) g.v[k] = df_dir_deriv(&s->df, kp, 0, k, T->pos, sp);
+   for (k = 0; k < 3; k++) g.v[k] = df_dir_deriv(&s->df, kp, 0, k, T->pos, mc.v[0], mc.v[1], sp);
    vec3 vp = V(vdot(g, fa[0]), vdot(g, fa[1]), 0.0);
    *Vperp = from_basis(vp, fa);
 }
@@ -1782,6 +1877,22 @@ static int setup_diffusion(orc_sim* s, const char* cls, dcread* r)
       df->A2A = rd_d(r); df->l_max = rd_d(r); df->ps_index = rd_d(r);
       df->k_min = 2.0 * M_PI / df->l_max;
       df->ps_minus = df->ps_index - 1.0;
+   }
+   else if (!strcmp(cls, "DiffusionWNLTConstant") || !strcmp(cls, "DiffusionWNLTRampVLISM")) {  /* :202-213, :281-296 */
+      df->kind = !strcmp(cls, "DiffusionWNLTConstant") ? DF_WNLT : DF_WNLT_RAMP;
+      df->A2A = rd_d(r); df->l_max = rd_d(r); df->ps_index = rd_d(r);
+      df->k_min = 2.0 * M_PI / df->l_max;
+      df->ps_minus = df->ps_index - 1.0;
+      df->A2T = rd_d(r); df->A2L = rd_d(r);
+      df->ps_plus = df->ps_index + 1.0;
+      if (df->kind == DF_WNLT_RAMP) {
+         df->l_max_HP = rd_d(r); df->z_nose = rd_d(r); df->z_sheath = rd_d(r);
+         df->k_min_ref = df->k_min; df->A2A_ref = df->A2A; df->A2T_ref = df->A2T; df->A2L_ref = df->A2L;
+         df->dl_max = df->l_max - df->l_max_HP; df->dz = df->z_sheath - df->z_nose;
+      }
+      df->hyp_c = (5.0 + df->ps_index) / 4.0;
+      df->hyp_g = tgamma(df->hyp_c) * tgamma(2.0 - df->hyp_c);
+      df->hyp_r = (df->hyp_c - 1.0) / (df->hyp_c - 2.0);
    }
    else return -1;
    s->have_df = 1;

@@ -85,6 +85,7 @@ int orc_set_grid(orc_sim* s, const int32_t nb[3], const double dom_min[3], const
    grad B (9), grad |B| (3), dmax, E(3).  Returns 0, 1 where the reference fails (field error), -1 without a grid. */
 int orc_grid_fields(orc_sim* s, const double pos[3], double out[32]);
 double orc_kappa_at(orc_sim* s, int comp, double t, const double pos[3], double p, double mu);
+double orc_hyp2f1_11(double c, double z);   /* 2F1(1, 1; c; z) of DiffusionWNLTConstant */
 
 /* Building blocks exposed for closed-form tests. */
 void orc_background_fields(orc_sim* s, double t, const double pos[3], double p, double out_U[3], double out_B[3], double* out_dmax, double* out_divU);

@@ -304,6 +304,23 @@ static int add_diffusion(spx_sim* s, const std::string& cls, Reader& r)
       d.k_min = (2.0 * M_PI) / l_max;
       d.ps_minus = ps_index - 1.0;
    }
+   else if (cls == "DiffusionWNLTConstant" || cls == "DiffusionWNLTRampVLISM") {   // :130-140, :202-213, :281-296
+      d.kind = cls == "DiffusionWNLTConstant" ? SPX_DF_WNLT : SPX_DF_WNLT_RAMP;
+      d.A2A = r.d();
+      const double l_max = r.d(), ps_index = r.d();
+      d.k_min = (2.0 * M_PI) / l_max;
+      d.ps_minus = ps_index - 1.0;
+      d.A2T = r.d(); d.A2L = r.d();
+      d.ps_plus = ps_index + 1.0;
+      if (d.kind == SPX_DF_WNLT_RAMP) {
+         d.l_max_HP = r.d(); d.z_nose = r.d(); d.z_sheath = r.d();
+         d.k_min_ref = d.k_min; d.A2A_ref = d.A2A; d.A2T_ref = d.A2T; d.A2L_ref = d.A2L;
+         d.dl_max = l_max - d.l_max_HP; d.dz = d.z_sheath - d.z_nose;
+      }
+      d.hyp_c = (5.0 + ps_index) / 4.0;
+      d.hyp_g = tgamma(d.hyp_c) * tgamma(2.0 - d.hyp_c);
+      d.hyp_r = (d.hyp_c - 1.0) / (d.hyp_c - 2.0);
+   }
    else return fail(s, SPX_ERR_CLASS, "unknown diffusion class " + cls);
    s->have_df = true;
    return SPX_OK;
@@ -503,6 +520,10 @@ static void finalize_plan(spx_sim* s)
    e.kind = d.kind; e.stream_dep_idx = d.stream_dep_idx; e.pos_dep = d.pos_dep; e.region_idx = d.region_idx;
    e.lam_in = (float)d.lam_in; e.lam_out = (float)d.lam_out; e.kap_rat_in = (float)d.kap_rat_in; e.kap_rat_out = (float)d.kap_rat_out;
    e.A2A = (float)d.A2A; e.k_min = (float)d.k_min; e.ps_minus = (float)d.ps_minus;
+   e.A2T = (float)d.A2T; e.A2L = (float)d.A2L; e.ps_plus = (float)d.ps_plus;
+   e.l_max_HP = (float)d.l_max_HP; e.z_nose = (float)d.z_nose; e.z_sheath = (float)d.z_sheath; e.k_min_ref = (float)d.k_min_ref;
+   e.A2A_ref = (float)d.A2A_ref; e.A2T_ref = (float)d.A2T_ref; e.A2L_ref = (float)d.A2L_ref; e.dl_max = (float)d.dl_max; e.dz = (float)d.dz;
+   e.hyp_c = (float)d.hyp_c; e.hyp_g = (float)d.hyp_g; e.hyp_r = (float)d.hyp_r;
    e.D0 = (float)d.D0; e.Dperp = (float)d.Dperp; e.Dpara = (float)d.Dpara; e.kap0 = (float)d.kap0; e.U0 = (float)d.U0;
    e.pow_law_U = (float)d.pow_law_U; e.p0 = (float)d.p0; e.pow_law_p = (float)d.pow_law_p; e.kap_rat = (float)d.kap_rat;
    e.T0 = (float)d.T0; e.r0 = (float)d.r0; e.pow_law_T = (float)d.pow_law_T; e.pow_law_r = (float)d.pow_law_r;
@@ -570,6 +591,19 @@ static int check_transport(spx_sim* s)
    }
    else if (s->plan.df.kind == SPX_DF_QLT)   // src/diffusion_other.cc:83 compiles it for the guiding-centre transports only
       return fail(s, SPX_ERR_ARG, "DiffusionQLTConstant needs a pitch-angle transport");
+   else if (s->plan.df.kind == SPX_DF_WNLT || s->plan.df.kind == SPX_DF_WNLT_RAMP)   // :153 likewise
+      return fail(s, SPX_ERR_ARG, "the WNLT models need a guiding-centre transport");
+   if (s->plan.df.kind == SPX_DF_WNLT || s->plan.df.kind == SPX_DF_WNLT_RAMP) {
+      const int tt = s->plan.traj_type;
+      if (s->plan.bg.kind != SPX_BG_SOLARWIND && s->plan.bg.kind != SPX_BG_UNIFORM)
+         return fail(s, SPX_ERR_ARG, "the WNLT models run on the SolarWind and Uniform backgrounds");
+      // the perpendicular drift differentiates kappa_perp over the background's numerical derivative steps
+      // (DiffusionBase::GetDirectionalDerivative, src/diffusion_base.cc:130-170), which only the solar wind sets
+      if ((tt == SPX_TRAJ_GUIDING_DIFF || tt == SPX_TRAJ_GUIDING_DIFF_SCATT) && s->plan.bg.kind != SPX_BG_SOLARWIND)
+         return fail(s, SPX_ERR_ARG, "the WNLT models with perpendicular diffusion need the SolarWind background");
+      if (fabs(s->plan.df.hyp_c - 2.0) < 1e-3)
+         return fail(s, SPX_ERR_ARG, "the WNLT models: (5 + ps_index) / 4 within 1e-3 of 2 (ps_index near 3) is not supported");
+   }
    if (s->plan.df.kind == SPX_DF_STRAUSS || s->plan.df.kind == SPX_DF_POTGIETER) {
       // the region indicators the backgrounds define: SolarWind sets region[0] (heliopause) and region[1] (-1) only
       // (src/background_solarwind.cc:144-148); the gridded server carries none here

@@ -19,6 +19,7 @@ template <typename R>
 struct GcFields {
    v3<R> E, B, bhat, gradBmag, curlbhat;
    R Bmag, dmax;
+   R rg;               // the solar wind's derivative step r_g (half of SpatialData::_dr)
 };
 
 // CommonFields with mask ALL|gradB|dBdt (src/trajectory_guiding.cc:51): BackgroundBase::GetFields
@@ -37,6 +38,7 @@ __device__ __forceinline__ bool gc_fields(const SPX_CP spx_bg_t<R>& bg, double t
       double zl_min;
       if (!grid_eval_gc<R>(bg.grid, x, y, z, U, F.B, F.Bmag, F.E, F.gradBmag, gB, zl_min)) return false;
       F.dmax = fmin(F.dmax, R(zl_min));
+      F.rg = R(0);
       if (F.Bmag < R(sp_tiny)) return false;
       F.bhat = F.B / F.Bmag;
       const v3<R> curlB = mk<R>(gB[3] - gB[5], gB[4] - gB[1], gB[0] - gB[2]);
@@ -54,6 +56,7 @@ __device__ __forceinline__ bool gc_fields(const SPX_CP spx_bg_t<R>& bg, double t
    else F.E = (-cross(U, B)) / R(c_code);                          // src/background_solarwind.cc:250-253
    if (BG == SPX_BG_SOLARWIND) {
       const R r_g = fmin(LarmorRadius(pm, F.Bmag), F.dmax);
+      F.rg = r_g;
       v3<R> fa[3];
       fa[2] = F.bhat;
       fa[0] = second_unit(F.bhat);
@@ -99,16 +102,84 @@ __device__ __forceinline__ bool gc_fields(const SPX_CP spx_bg_t<R>& bg, double t
    else {
       F.gradBmag = mk<R>(R(0), R(0), R(0));
       F.curlbhat = mk<R>(R(0), R(0), R(0));
+      F.rg = R(0);
    }
    return true;
 }
 
-// DiffusionBase::GetComponent(0) (src/diffusion_base.cc:105-121) of the models usable with the
-// guiding centre; |p| = pmag.
+// Gauss's 2F1(1, 1; c; z), 0 <= z < 1: the series (Abramowitz & Stegun 15.1.1) below z = 1/2, above it the
+// transformation to 1 - z (A&S 15.3.6 with a = b = 1):
+//   2F1(1, 1; c; z) = (c-1)/(c-2) 2F1(1, 1; 3-c; 1-z) + Gamma(c) Gamma(2-c) (1-z)^(c-2) z^(1-c),
+// with hyp_g = Gamma(c) Gamma(2-c) and hyp_r = (c-1)/(c-2) from the host.
 template <typename R>
-__device__ __forceinline__ R gc_kperp(const SPX_CP spx_df_t<R>& df, R pmag, R Bmag)
+__device__ __forceinline__ R hyp_series_11(R c, R z)
+{
+   const R tol = is_f32<R>() ? R(1e-8) : R(1e-17);
+   R sum = R(1), term = R(1);
+#pragma unroll 1
+   for (int n = 0; n < 400; n++) {
+      term *= (R(n) + R(1)) * z / (c + R(n));
+      sum += term;
+      if (fabs(term) < tol * fabs(sum)) break;
+   }
+   return sum;
+}
+
+template <typename R>
+__device__ __forceinline__ R hyp2f1_11(const SPX_CP spx_df_t<R>& df, R z)
+{
+   if (z < R(0.5)) return hyp_series_11(df.hyp_c, z);
+   const R w = R(1) - z;
+   return df.hyp_r * hyp_series_11(R(3) - df.hyp_c, w) + df.hyp_g * rpow(w, df.hyp_c - R(2)) * rpow(z, R(1) - df.hyp_c);
+}
+
+// DiffusionWNLTConstant::EvaluateDiffusion (src/diffusion_other.cc:215-245) over DiffusionQLTConstant's
+// component 2 (:145-149), with DiffusionWNLTRampVLISM's k_min and amplitudes at the position (:301-328):
+// kappa_perp into K0 and, when comp2, D_mumu into K2.
+template <typename R>
+__device__ __noinline__ void wnlt_k(const SPX_CP spx_df_t<R>& df, bool comp2, double x, double y, double z, R pmag, R mu, R Bmag, R& K0, R& K2)
+{
+   R k_min = df.k_min, A2A = df.A2A, A2T = df.A2T, A2L = df.A2L;
+   if (df.kind == SPX_DF_WNLT_RAMP) {
+      const double r0 = norm(mk<double>(x, y, z));
+      const R z0 = R(sqrt(0.5 * r0 * (r0 + z)));
+      if (z0 > df.z_sheath) { k_min = df.k_min_ref; A2A = df.A2A_ref; A2T = df.A2T_ref; A2L = df.A2L_ref; }
+      else {
+         k_min = R(2.0 * M_PI) / (df.l_max_HP + df.dl_max * (z0 - df.z_nose) / df.dz);
+         const R k_ratio = rpow(df.k_min_ref / k_min, df.ps_minus);
+         A2A = df.A2A_ref * k_ratio; A2T = df.A2T_ref * k_ratio; A2L = df.A2L_ref * k_ratio;
+      }
+   }
+   const R vmag = VelP(pmag);
+   const R st2 = R(1) - sqr(mu);
+   const R Omega = CyclotronFrequency(vmag, Bmag);
+   const R CT = R(0.5) * df.ps_minus / df.ps_plus * sqr(mu / k_min) * A2T;
+   const R CL = R(0.125) * sqr(vmag * st2 / Omega) * A2L;
+   K0 = vmag * rsqrt_(CT + CL);
+   if (!comp2) return;
+   K2 = R(0.25) * R(M_PI) * df.ps_minus * R(fabs(Omega)) * st2 * rpow(R(vmag * k_min * R(fabs(mu / Omega))), df.ps_minus) * A2A;
+   if (A2T > R(sp_tiny)) {
+      const R xi1 = vmag * k_min * rsqrt_(st2) / R(M_SQRT2) / R(fabs(Omega));
+      R F21 = hyp2f1_11(df, R(1) / (R(1) + sqr(xi1)));
+      const R DT1 = R(1) / (R(1) + sqr(xi1)) * F21;
+      const R xi2 = K0 * k_min * k_min / R(fabs(Omega));
+      F21 = hyp2f1_11(df, R(1) / (R(1) + sqr(xi2)));
+      const R DT2 = R(1) / (R(1) + sqr(xi2)) * F21;
+      K2 += (df.ps_minus / df.ps_plus) * Omega * st2 * A2T * DT1 * DT2 / rsqrt_(sqr(DT1) + sqr(DT2));
+   }
+}
+
+// DiffusionBase::GetComponent(0) (src/diffusion_base.cc:105-121) of the models usable with the
+// guiding centre; |p| = pmag, mu the pitch cosine, (x, y, z) the position.
+template <typename R>
+__device__ __forceinline__ R gc_kperp(const SPX_CP spx_df_t<R>& df, R pmag, R mu, R Bmag, double x, double y, double z)
 {
    switch (df.kind) {
+   case SPX_DF_WNLT: case SPX_DF_WNLT_RAMP: {
+      R K0, K2;
+      wnlt_k(df, false, x, y, z, pmag, mu, Bmag, K0, K2);
+      return K0;
+   }
    case SPX_DF_PERP: return df.D0;                                  // src/diffusion_other.cc:459-463
    case SPX_DF_FULL: return df.Dperp;                               // :538-543
    case SPX_DF_PERPISO: return df.Dperp;
@@ -126,8 +197,16 @@ __device__ __forceinline__ R gc_kperp(const SPX_CP spx_df_t<R>& df, R pmag, R Bm
 // difference of DiffusionBase::GetMuDerivative (src/diffusion_base.cc:226-255) for DiffusionQLTConstant
 // (src/diffusion_other.cc:145-149); the other models have no component 2.
 template <typename R>
-__device__ __forceinline__ void gc_dmumu(const SPX_CP spx_df_t<R>& df, R pmag, R Bmag, R mu, R& Dmumu, R& Vmu)
+__device__ __forceinline__ void gc_dmumu(const SPX_CP spx_df_t<R>& df, R pmag, R Bmag, R mu, double x, double y, double z, R& Dmumu, R& Vmu)
 {
+   if (df.kind == SPX_DF_WNLT || df.kind == SPX_DF_WNLT_RAMP) {
+      R K0, K2;
+      wnlt_k(df, true, x, y, z, pmag, mu, Bmag, K0, Dmumu);
+      const R dmu = R(sp_small) * ((mu + R(sp_small) < R(1)) ? R(1) : R(-1));
+      wnlt_k(df, true, x, y, z, pmag, mu + dmu, Bmag, K0, K2);
+      Vmu = (K2 - Dmumu) / dmu;
+      return;
+   }
    if (df.kind == SPX_DF_ISO || df.kind == SPX_DF_PERPISO) {
       Dmumu = df.D0 * (R(1) - sqr(mu));
       Vmu = R(-2) * df.D0 * mu;
@@ -149,9 +228,32 @@ __device__ __forceinline__ void gc_dmumu(const SPX_CP spx_df_t<R>& df, R pmag, R
 // drift V_perp = grad_perp D_perp in the field-aligned frame.  Only the rigidity power law has a
 // spatial derivative (src/diffusion_other.cc:818-823), and only through |B|.
 template <typename R>
-__device__ __forceinline__ void gc_perp(const SPX_CP spx_df_t<R>& df, const GcFields<R>& F, const v3<R>* fa, R pmag, R& Dperp, v3<R>& Vperp)
+__device__ __forceinline__ void gc_perp(const SPX_CP spx_df_t<R>& df, const GcFields<R>& F, const v3<R>* fa, R pmag, R mu,
+                                        double x, double y, double z, R& Dperp, v3<R>& Vperp)
 {
-   Dperp = gc_kperp(df, pmag, F.Bmag);
+   Dperp = gc_kperp(df, pmag, mu, F.Bmag, x, y, z);
+   if (df.kind == SPX_DF_WNLT || df.kind == SPX_DF_WNLT_RAMP) {
+      // DiffusionBase::GetDirectionalDerivative (src/diffusion_base.cc:130-170): along each Cartesian axis,
+      // kappa_perp at the position moved by r_g with |B| carried by its gradient, and back by 2 r_g
+      R g[3];
+#pragma unroll 1
+      for (int k = 0; k < 3; k++) {
+         R dr = F.rg;
+         const R gk = k == 0 ? F.gradBmag.x : (k == 1 ? F.gradBmag.y : F.gradBmag.z);
+         double xs[3] = {x, y, z};
+         xs[k] += (double)dr;
+         R Bm = F.Bmag + gk * dr;
+         const R Kf = gc_kperp(df, pmag, mu, Bm, xs[0], xs[1], xs[2]);
+         dr *= R(2);
+         xs[k] -= (double)dr;
+         Bm -= gk * dr;
+         const R Kb = gc_kperp(df, pmag, mu, Bm, xs[0], xs[1], xs[2]);
+         g[k] = (Kf - Kb) / dr;
+      }
+      const v3<R> gv = mk<R>(g[0], g[1], g[2]);
+      Vperp = from_basis(mk<R>(dot(gv, fa[0]), dot(gv, fa[1]), R(0)), fa);
+      return;
+   }
    if (df.kind == SPX_DF_RIGBMAG && df.pow_law_B != R(0)) {
       const R c = Dperp * df.pow_law_B;
       const v3<R> g = mk<R>(c * F.gradBmag.x / F.Bmag, c * F.gradBmag.y / F.Bmag, c * F.gradBmag.z / F.Bmag);
@@ -303,9 +405,9 @@ __device__ __forceinline__ int gc_step(PlanPtr P, LaunchPtr L, const spx_rng_des
    const R mu = pl / norm(mk<R>(pp, R(0), pl));
    R Dperp = R(0);
    v3<R> Vperp = mk<R>(R(0), R(0), R(0));
-   if (perp_step) gc_perp(df, H.F, fa, pmag, Dperp, Vperp);
+   if (perp_step) gc_perp(df, H.F, fa, pmag, mu, H.x, H.y, H.z, Dperp, Vperp);
    R Dmumu = R(0), Vmu = R(0);
-   if (pa_step) gc_dmumu(df, pmag, H.F.Bmag, mu, Dmumu, Vmu);
+   if (pa_step) gc_dmumu(df, pmag, H.F.Bmag, mu, H.x, H.y, H.z, Dmumu, Vmu);
    v3<R> drift, Es, Bs;
    gc_drift(H.F, vel, pp, pl, drift, Es, Bs);
    // PhysicalStep: TrajectoryGuiding's (src/trajectory_guiding.cc:96-100), GuidingDiff's with V_perp and
@@ -348,14 +450,15 @@ __device__ __forceinline__ int gc_step(PlanPtr P, LaunchPtr L, const spx_rng_des
    v3<R> slope_pos = drift;
    v3<R> dr = mk<R>(R(0), R(0), R(0));
    if (perp_step) {
-      gc_perp(df, H.F, fa, pmag, Dperp, Vperp);
+      const R mu2 = pl / norm(mk<R>(pp, R(0), pl));
+      gc_perp(df, H.F, fa, pmag, mu2, H.x, H.y, H.z, Dperp, Vperp);
       slope_pos = drift + Vperp;
       // EulerPerpDiffSlopes
       const R sdt = rsqrt_(dtr);
       const R dWx = sdt * next_normal<R>(rng, first, H.rs);
       const R dWy = sdt * next_normal<R>(rng, first, H.rs);
       if (H.rs.exhausted) return 5;
-      Dperp = gc_kperp(df, pmag, H.F.Bmag);
+      Dperp = gc_kperp(df, pmag, mu2, H.F.Bmag, H.x, H.y, H.z);
       dr = from_basis(mk<R>(rsqrt_(R(2) * Dperp) * dWx, rsqrt_(R(2) * Dperp) * dWy, R(0)), fa);
    }
    // RKStep (Euler), then the perpendicular displacement

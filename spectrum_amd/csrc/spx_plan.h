@@ -16,7 +16,7 @@
 enum spx_bg_kind { SPX_BG_NONE = 0, SPX_BG_UNIFORM = 1, SPX_BG_SOLARWIND = 2, SPX_BG_SHOCK = 3, SPX_BG_SMOOTHSHOCK = 4, SPX_BG_GRID = 5 };
 enum spx_df_kind {
    SPX_DF_NONE = 0, SPX_DF_ISO, SPX_DF_PARA, SPX_DF_PERP, SPX_DF_FULL, SPX_DF_FLOWMOM, SPX_DF_KINRAD, SPX_DF_RIGBMAG, SPX_DF_PERPISO,
-   SPX_DF_STRAUSS, SPX_DF_POTGIETER, SPX_DF_QLT
+   SPX_DF_STRAUSS, SPX_DF_POTGIETER, SPX_DF_QLT, SPX_DF_WNLT, SPX_DF_WNLT_RAMP
 };
 enum spx_bnd_class { SPX_BC_TIME = 0, SPX_BC_SPACE = 1, SPX_BC_MOMENTUM = 2 };
 enum spx_bnd_kind { SPX_BK_TIME = 0, SPX_BK_SPHERE = 1, SPX_BK_PLANE = 2, SPX_BK_MOMENTUM = 3 };
@@ -83,6 +83,12 @@ struct spx_df_t {
    T lam_in, lam_out, kap_rat_in, kap_rat_out;
    // DiffusionQLTConstant: A2A, k_min = 2 pi / l_max, ps_minus = ps_index - 1
    T A2A, k_min, ps_minus;
+   // DiffusionWNLTConstant: A2T, A2L, ps_plus = ps_index + 1; DiffusionWNLTRampVLISM: the ramp of l_max
+   // between z_nose and z_sheath and the reference k_min and amplitudes; 2F1(1, 1; c; z) with
+   // c = (5 + ps_index) / 4, hyp_g = Gamma(c) Gamma(2 - c), hyp_r = (c - 1) / (c - 2)
+   T A2T, A2L, ps_plus;
+   T l_max_HP, z_nose, z_sheath, k_min_ref, A2A_ref, A2T_ref, A2L_ref, dl_max, dz;
+   T hyp_c, hyp_g, hyp_r;
 };
 typedef spx_df_t<double> spx_df_desc;
 

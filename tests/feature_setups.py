@@ -338,25 +338,66 @@ def focused_uniform_backward():
     return s
 
 
+def wnlt(ramp=False):
+    """DiffusionWNLTConstant (A2A, l_max, ps_index, A2T, A2L) or DiffusionWNLTRampVLISM (+ l_max_HP, z_nose,
+    z_sheath) with a Kolmogorov spectrum."""
+    dc = DC().insert_double(0.05).insert_double(0.01 * cf.one_au).insert_double(5.0 / 3.0).insert_double(0.1).insert_double(0.2)
+    if ramp:
+        dc.insert_double(0.002 * cf.one_au).insert_double(0.5 * cf.one_au).insert_double(1.5 * cf.one_au)
+    return ("DiffusionWNLTRampVLISM" if ramp else "DiffusionWNLTConstant"), dc
+
+
+def gc_wnlt_scatt_spiral():
+    """GuidingScatt in the Parker spiral with WNLT pitch-angle scattering (QLT plus the nonlinear 2F1 term;
+    the mu derivative is DiffusionBase's forward difference)."""
+    s = cf.Setup("gc_wnlt_scatt_spiral", cf.TRAJ_GUIDING_SCATT, cf.FORWARD)
+    s.add(cf.BACKGROUND, "BackgroundSolarWind", cf.solarwind())
+    s.add(cf.DIFFUSION, *wnlt())
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((1.2 * cf.one_au, 0.4 * cf.one_au, 0.1 * cf.one_au)))
+    s.add(cf.INITIAL, "InitialMomentumShell", DC().insert_double(cf.mom_from_ekin(20.0 * cf.MeV)))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", 0.0001 * cf.one_day)))
+    d = dist((16, 16, 0), (1.0 * cf.one_au, -1.0, 0.0), (1.5 * cf.one_au, 1.0, 1.0), (0, 0, 0), (1, 0, 0))
+    d.insert_double(1.0).insert_double(0.0).insert_int(1)
+    s.add(cf.DISTRIBUTION, "DistributionPositionPitchUniform", d)
+    return s
+
+
+def gc_wnlt_ramp_diffscatt():
+    """GuidingDiffScatt in the Parker spiral with the WNLT ramp model: kappa_perp from WNLT and its
+    perpendicular drift by DiffusionBase's directional derivative over the solar wind's derivative steps."""
+    s = cf.Setup("gc_wnlt_ramp_diffscatt", cf.TRAJ_GUIDING_DIFF_SCATT, cf.FORWARD)
+    s.add(cf.BACKGROUND, "BackgroundSolarWind", cf.solarwind())
+    s.add(cf.DIFFUSION, *wnlt(ramp=True))
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((1.2 * cf.one_au, 0.4 * cf.one_au, 0.1 * cf.one_au)))
+    s.add(cf.INITIAL, "InitialMomentumThickShell", DC().insert_double(cf.mom_from_ekin(5.0 * cf.MeV)).insert_double(cf.mom_from_ekin(50.0 * cf.MeV)).insert_bool(True))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", 0.00015 * cf.one_day)))
+    d = dist((16, 16, 0), (1.0 * cf.one_au, -1.0, 0.0), (1.5 * cf.one_au, 1.0, 1.0), (0, 0, 0), (1, 0, 0))
+    d.insert_double(1.0).insert_double(0.0).insert_int(1)
+    s.add(cf.DISTRIBUTION, "DistributionPositionPitchUniform", d)
+    return s
+
+
 ALL = [shock_flowmom_reflect, solarwind_kinrad_reflect_recurrent, smoothshock_perp_momentum, uniform_para_plane_pass,
        lambda: modulation_spectrum_type(0), lambda: modulation_spectrum_type(2), gc_rigidity_reflect, gc_isotropic_only,
        guiding_spiral, guiding_scatt_uniform, guiding_diff_rigidity, parker_strauss_heliopause, parker_potgieter_uniform,
        gc_qlt_spiral, fieldline_spiral, fieldline_backward_shock, lorentz_uniform, lorentz_spiral_backward,
-       focused_spiral, focused_uniform_backward]
+       focused_spiral, focused_uniform_backward, gc_wnlt_scatt_spiral, gc_wnlt_ramp_diffscatt]
 NAMES = ["shock_flowmom_reflect", "solarwind_kinrad_reflect_recurrent", "smoothshock_perp_momentum", "uniform_para_plane_pass",
          "modulation_type0", "modulation_type2", "gc_rigidity_reflect", "gc_isotropic_only",
          "guiding_spiral", "guiding_scatt_uniform", "guiding_diff_rigidity", "parker_strauss_heliopause", "parker_potgieter_uniform",
          "gc_qlt_spiral", "fieldline_spiral", "fieldline_backward_shock", "lorentz_uniform", "lorentz_spiral_backward",
-         "focused_spiral", "focused_uniform_backward"]
+         "focused_spiral", "focused_uniform_backward", "gc_wnlt_scatt_spiral", "gc_wnlt_ramp_diffscatt"]
 
 
 # Setups whose endpoints amplify last-bit differences far beyond 1e-10: QLT's mu derivative is
 # DiffusionBase::GetMuDerivative's forward difference with dmu = 1e-8 (src/diffusion_base.cc:238), which
-# multiplies rounding differences by ~1e8.  Their parity bound is scaled by the oracle's own disagreement
+# multiplies rounding differences by ~1e8 (the WNLT models inherit it).  Their parity bound is scaled by the oracle's own disagreement
 # with its last-bit-perturbed build (test_gpu_parity.sensitivity).  The focused transport along the spiral
 # differentiates the field over the Larmor radius (a difference of nearby field values divided by ~1e-3 of
 # the distance to the Sun), and its long trajectories reach a few 1e-10 of last-bit sensitivity.
-ILL_CONDITIONED = {"gc_qlt_spiral", "focused_spiral"}
+ILL_CONDITIONED = {"gc_qlt_spiral", "focused_spiral", "gc_wnlt_scatt_spiral", "gc_wnlt_ramp_diffscatt"}
 
 
 def weight_atol(name, rec, rtol):
