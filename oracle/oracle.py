@@ -67,6 +67,8 @@ def lib(path: str = LIB_PATH):
         L.orc_kappa_at.argtypes = [C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_double), C.c_double, C.c_double]
         L.orc_kappa_at.restype = C.c_double
         L.orc_hyp2f1_11.argtypes = [C.c_double, C.c_double]
+        L.orc_gc_vperp.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_double), C.c_double, C.c_double, C.POINTER(C.c_double)]
+        L.orc_gc_vperp.restype = C.c_int
         L.orc_hyp2f1_11.restype = C.c_double
         L.orc_kappa.argtypes = [C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double]
         L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
@@ -175,6 +177,13 @@ class Oracle:
         a = np.frombuffer(out, dtype=np.float64).copy()
         return {"U": a[0:3], "B": a[3:6], "Bmag": a[6], "gradU": a[7:16].reshape(3, 3), "gradB": a[16:25].reshape(3, 3),
                 "gradBmag": a[25:28], "dmax": a[28], "E": a[29:32]}
+
+    def gc_vperp(self, pos, p, mu, t=0.0):
+        """(kappa_perp, V_perp) of the guiding centre at a point."""
+        out = (C.c_double * 4)()
+        if self.L.orc_gc_vperp(self.h, t, (C.c_double * 3)(*pos), p, mu, out):
+            raise RuntimeError("field error")
+        return out[0], np.array(out[1:4])
 
     def kappa_at(self, comp, pos, p, mu=0.0, t=0.0):
         """Diffusion component `comp` with the background's own fields and region indicators at pos."""

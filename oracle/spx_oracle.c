@@ -892,7 +892,7 @@ static double df_generic_dir_deriv(const df_t* df, kap_t* kp, int comp, int xyz,
 static double df_dir_deriv(const df_t* df, kap_t* kp, int comp, int xyz, vec3 pos, double p, double mu, const spd_t* sp)
 {
    switch (df->kind) {
-   case DF_WNLT: case DF_WNLT_RAMP: return df_generic_dir_deriv(df, kp, comp, xyz, pos, p, mu, sp);
+   case DF_WNLT: case DF_WNLT_RAMP: case DF_STRAUSS: case DF_POTGIETER: return df_generic_dir_deriv(df, kp, comp, xyz, pos, p, mu, sp);
    case DF_PARA: case DF_PERP: case DF_FULL: case DF_PERPISO: return 0.0;
    case DF_FLOWMOM: return kp->K[comp] * df->pow_law_U * vdot(sp->gradU.row[xyz], sp->U) / Sqr(vnorm(sp->U));  /* :637-642 */
    case DF_KINRAD: {  /* :732-742 */
@@ -2143,11 +2143,30 @@ double orc_kappa_at(orc_sim* s, int comp, double t, const double pos[3], double 
    return df_component(&s->df, &kp, comp, x, p, mu, &sp);
 }
 
+/* TrajectoryGuidingDiff::DiffusionCoeff at one point: kappa_perp and the drift V_perp = (grad kappa_perp)_perp
+   for momentum magnitude p and pitch cosine mu, with the guiding centre's fields (mask ALL | gradB | dBdt). */
+int orc_gc_vperp(orc_sim* s, double t, const double pos[3], double p, double mu, double out[4])
+{
+   traj_t T; memset(&T, 0, sizeof(T));
+   lane_t L; memset(&L, 0, sizeof(L));
+   kap_t kp = {{0, 0, 0}};
+   vec3 fa[3], Vperp;
+   double Dperp;
+   T.t = t; T.pos = V(pos[0], pos[1], pos[2]);
+   T.mom = V(p * sqrt(1.0 - Sqr(mu)), 0.0, p * mu);
+   if (get_fields(&s->bg, &L, t, T.pos, p, BG_ALL | BG_gradB | BG_dBdt, &T.sp)) return 1;
+   gcd_diffcoeff(s, &T, &kp, fa, &Dperp, &Vperp);
+   out[0] = Dperp; out[1] = Vperp.v[0]; out[2] = Vperp.v[1]; out[3] = Vperp.v[2];
+   return 0;
+}
+
 double orc_kappa(orc_sim* s, int comp, double t, const double pos[3], double p, double mu, double Bmag)
 {
-   (void)t;
    kap_t kp = {{0, 0, 0}};
    spd_t sp; memset(&sp, 0, sizeof(sp));
+   lane_t L; memset(&L, 0, sizeof(L));
+   /* the background at the point (its region indicators), with |B| given */
+   if (s->bg.kind != BG_GRID) bg_evaluate(&s->bg, &L, t, V(pos[0], pos[1], pos[2]), BG_ALL, &sp);
    sp.Bmag = Bmag;
    return df_component(&s->df, &kp, comp, V(pos[0], pos[1], pos[2]), p, mu, &sp);
 }

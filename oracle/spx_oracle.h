@@ -86,6 +86,8 @@ int orc_set_grid(orc_sim* s, const int32_t nb[3], const double dom_min[3], const
 int orc_grid_fields(orc_sim* s, const double pos[3], double out[32]);
 double orc_kappa_at(orc_sim* s, int comp, double t, const double pos[3], double p, double mu);
 double orc_hyp2f1_11(double c, double z);   /* 2F1(1, 1; c; z) of DiffusionWNLTConstant */
+/* guiding-centre kappa_perp and V_perp at a point: out = (kappa_perp, V_perp); 1 on a field error */
+int orc_gc_vperp(orc_sim* s, double t, const double pos[3], double p, double mu, double out[4]);
 
 /* Building blocks exposed for closed-form tests. */
 void orc_background_fields(orc_sim* s, double t, const double pos[3], double p, double out_U[3], double out_B[3], double* out_dmax, double* out_divU);

@@ -586,8 +586,12 @@ static int check_transport(spx_sim* s)
          return fail(s, SPX_ERR_ARG, "guiding-centre transport supports the static SolarWind, Uniform and Cartesian-grid backgrounds");
       if (s->plan.df.kind == SPX_DF_FLOWMOM || s->plan.df.kind == SPX_DF_KINRAD)
          return fail(s, SPX_ERR_ARG, "guiding-centre transport: diffusion model needs grad U, which its field mask does not compute");
-      if (s->plan.df.kind == SPX_DF_STRAUSS || s->plan.df.kind == SPX_DF_POTGIETER)
-         return fail(s, SPX_ERR_ARG, "guiding-centre transport: the Strauss / Potgieter models run with the Parker transport here");
+      // Strauss / Potgieter set no component 2 (src/diffusion_other.cc:886-888), so with pitch-angle scattering
+      // D_mumu would be whatever the class held before; their perpendicular drift steps over the solar wind's
+      // numerical derivative steps (DiffusionBase::GetDirectionalDerivative, src/diffusion_base.cc:130-170)
+      if ((s->plan.df.kind == SPX_DF_STRAUSS || s->plan.df.kind == SPX_DF_POTGIETER) &&
+          (s->plan.traj_type != SPX_TRAJ_GUIDING_DIFF || s->plan.bg.kind != SPX_BG_SOLARWIND))
+         return fail(s, SPX_ERR_ARG, "guiding-centre transport: the Strauss / Potgieter models run with GuidingDiff on the SolarWind background");
    }
    else if (s->plan.df.kind == SPX_DF_QLT)   // src/diffusion_other.cc:83 compiles it for the guiding-centre transports only
       return fail(s, SPX_ERR_ARG, "DiffusionQLTConstant needs a pitch-angle transport");

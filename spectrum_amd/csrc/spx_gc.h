@@ -170,11 +170,16 @@ __device__ __noinline__ void wnlt_k(const SPX_CP spx_df_t<R>& df, bool comp2, do
 }
 
 // DiffusionBase::GetComponent(0) (src/diffusion_base.cc:105-121) of the models usable with the
-// guiding centre; |p| = pmag, mu the pitch cosine, (x, y, z) the position.
+// guiding centre; |p| = pmag, mu the pitch cosine, (x, y, z) the position, outer the LISM indicator the
+// Strauss / Potgieter models read (src/diffusion_other.cc:886-906, :968-988).
 template <typename R>
-__device__ __forceinline__ R gc_kperp(const SPX_CP spx_df_t<R>& df, R pmag, R mu, R Bmag, double x, double y, double z)
+__device__ __forceinline__ R gc_kperp(const SPX_CP spx_df_t<R>& df, R pmag, R mu, R Bmag, double x, double y, double z, bool outer)
 {
    switch (df.kind) {
+   case SPX_DF_STRAUSS: case SPX_DF_POTGIETER: {
+      const R B0_eff = outer ? Bmag : df.B0;
+      return (outer ? df.kap_rat_out : df.kap_rat_in) * (kappa_mom_region(df, pmag, outer) * (B0_eff / Bmag));
+   }
    case SPX_DF_WNLT: case SPX_DF_WNLT_RAMP: {
       R K0, K2;
       wnlt_k(df, false, x, y, z, pmag, mu, Bmag, K0, K2);
@@ -229,12 +234,13 @@ __device__ __forceinline__ void gc_dmumu(const SPX_CP spx_df_t<R>& df, R pmag, R
 // spatial derivative (src/diffusion_other.cc:818-823), and only through |B|.
 template <typename R>
 __device__ __forceinline__ void gc_perp(const SPX_CP spx_df_t<R>& df, const GcFields<R>& F, const v3<R>* fa, R pmag, R mu,
-                                        double x, double y, double z, R& Dperp, v3<R>& Vperp)
+                                        double x, double y, double z, bool outer, R& Dperp, v3<R>& Vperp)
 {
-   Dperp = gc_kperp(df, pmag, mu, F.Bmag, x, y, z);
-   if (df.kind == SPX_DF_WNLT || df.kind == SPX_DF_WNLT_RAMP) {
+   Dperp = gc_kperp(df, pmag, mu, F.Bmag, x, y, z, outer);
+   if (df.kind == SPX_DF_WNLT || df.kind == SPX_DF_WNLT_RAMP || df.kind == SPX_DF_STRAUSS || df.kind == SPX_DF_POTGIETER) {
       // DiffusionBase::GetDirectionalDerivative (src/diffusion_base.cc:130-170): along each Cartesian axis,
-      // kappa_perp at the position moved by r_g with |B| carried by its gradient, and back by 2 r_g
+      // kappa_perp at the position moved by r_g with |B| carried by its gradient, and back by 2 r_g; the
+      // region indicator stays the point's (the background is not evaluated again)
       R g[3];
 #pragma unroll 1
       for (int k = 0; k < 3; k++) {
@@ -243,11 +249,11 @@ __device__ __forceinline__ void gc_perp(const SPX_CP spx_df_t<R>& df, const GcFi
          double xs[3] = {x, y, z};
          xs[k] += (double)dr;
          R Bm = F.Bmag + gk * dr;
-         const R Kf = gc_kperp(df, pmag, mu, Bm, xs[0], xs[1], xs[2]);
+         const R Kf = gc_kperp(df, pmag, mu, Bm, xs[0], xs[1], xs[2], outer);
          dr *= R(2);
          xs[k] -= (double)dr;
          Bm -= gk * dr;
-         const R Kb = gc_kperp(df, pmag, mu, Bm, xs[0], xs[1], xs[2]);
+         const R Kb = gc_kperp(df, pmag, mu, Bm, xs[0], xs[1], xs[2], outer);
          g[k] = (Kf - Kb) / dr;
       }
       const v3<R> gv = mk<R>(g[0], g[1], g[2]);
@@ -405,7 +411,9 @@ __device__ __forceinline__ int gc_step(PlanPtr P, LaunchPtr L, const spx_rng_des
    const R mu = pl / norm(mk<R>(pp, R(0), pl));
    R Dperp = R(0);
    v3<R> Vperp = mk<R>(R(0), R(0), R(0));
-   if (perp_step) gc_perp(df, H.F, fa, pmag, mu, H.x, H.y, H.z, Dperp, Vperp);
+   // the LISM indicator of the Strauss / Potgieter models at the point of the last field evaluation
+   const bool outer = (df.kind == SPX_DF_STRAUSS || df.kind == SPX_DF_POTGIETER) && lism_outer<R, BG>(bg, df, H.x, H.y, H.z);
+   if (perp_step) gc_perp(df, H.F, fa, pmag, mu, H.x, H.y, H.z, outer, Dperp, Vperp);
    R Dmumu = R(0), Vmu = R(0);
    if (pa_step) gc_dmumu(df, pmag, H.F.Bmag, mu, H.x, H.y, H.z, Dmumu, Vmu);
    v3<R> drift, Es, Bs;
@@ -451,14 +459,14 @@ __device__ __forceinline__ int gc_step(PlanPtr P, LaunchPtr L, const spx_rng_des
    v3<R> dr = mk<R>(R(0), R(0), R(0));
    if (perp_step) {
       const R mu2 = pl / norm(mk<R>(pp, R(0), pl));
-      gc_perp(df, H.F, fa, pmag, mu2, H.x, H.y, H.z, Dperp, Vperp);
+      gc_perp(df, H.F, fa, pmag, mu2, H.x, H.y, H.z, outer, Dperp, Vperp);
       slope_pos = drift + Vperp;
       // EulerPerpDiffSlopes
       const R sdt = rsqrt_(dtr);
       const R dWx = sdt * next_normal<R>(rng, first, H.rs);
       const R dWy = sdt * next_normal<R>(rng, first, H.rs);
       if (H.rs.exhausted) return 5;
-      Dperp = gc_kperp(df, pmag, mu2, H.F.Bmag, H.x, H.y, H.z);
+      Dperp = gc_kperp(df, pmag, mu2, H.F.Bmag, H.x, H.y, H.z, outer);
       dr = from_basis(mk<R>(rsqrt_(R(2) * Dperp) * dWx, rsqrt_(R(2) * Dperp) * dWy, R(0)), fa);
    }
    // RKStep (Euler), then the perpendicular displacement

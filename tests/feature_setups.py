@@ -379,16 +379,39 @@ def gc_wnlt_ramp_diffscatt():
     return s
 
 
+def gc_strauss_diff():
+    """GuidingDiff in the Parker spiral with the Strauss et al. 2013 model (inner parameters by region[0]):
+    kappa_perp ~ 1/|B| and its perpendicular drift by DiffusionBase's directional derivative."""
+    s = cf.Setup("gc_strauss_diff", cf.TRAJ_GUIDING_DIFF, cf.FORWARD)
+    s.add(cf.BACKGROUND, "BackgroundSolarWind", cf.solarwind())
+    p0 = cf.mom_from_ekin(1.0 * cf.GeV)
+    dc = DC().insert_int(0).insert_double(0.002 * cf.one_au).insert_double(2.0 * cf.one_au).insert_double(p0 * cf.c_code)
+    dc.insert_double(5.0e-6 / cf.unit_magnetic).insert_double(0.05).insert_double(0.2)
+    s.add(cf.DIFFUSION, "DiffusionStraussEtAl2013", dc)
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((1.2 * cf.one_au, 0.4 * cf.one_au, 0.1 * cf.one_au)))
+    s.add(cf.INITIAL, "InitialMomentumThickShell", DC().insert_double(cf.mom_from_ekin(5.0 * cf.MeV)).insert_double(cf.mom_from_ekin(500.0 * cf.MeV)).insert_bool(True))
+    s.add(cf.BOUNDARY, "BoundarySphereAbsorb", cf.boundary(1, [0], ("vec", (0.0, 0.0, 0.0)), ("double", 1.4 * cf.one_au)))
+    s.add(cf.BOUNDARY, "BoundarySphereAbsorb", cf.boundary(1, [0], ("vec", (0.0, 0.0, 0.0)), ("double", 0.3 * cf.one_au)))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", 0.1 * cf.one_day)))
+    d = dist((16, 16, 0), (1.0 * cf.one_au, -1.0, 0.0), (1.5 * cf.one_au, 1.0, 1.0), (0, 0, 0), (1, 0, 0))
+    d.insert_double(1.0).insert_double(0.0).insert_int(1)
+    s.add(cf.DISTRIBUTION, "DistributionPositionPitchUniform", d)
+    return s
+
+
 ALL = [shock_flowmom_reflect, solarwind_kinrad_reflect_recurrent, smoothshock_perp_momentum, uniform_para_plane_pass,
        lambda: modulation_spectrum_type(0), lambda: modulation_spectrum_type(2), gc_rigidity_reflect, gc_isotropic_only,
        guiding_spiral, guiding_scatt_uniform, guiding_diff_rigidity, parker_strauss_heliopause, parker_potgieter_uniform,
        gc_qlt_spiral, fieldline_spiral, fieldline_backward_shock, lorentz_uniform, lorentz_spiral_backward,
-       focused_spiral, focused_uniform_backward, gc_wnlt_scatt_spiral, gc_wnlt_ramp_diffscatt]
+       focused_spiral, focused_uniform_backward, gc_wnlt_scatt_spiral, gc_wnlt_ramp_diffscatt,
+       gc_strauss_diff]
 NAMES = ["shock_flowmom_reflect", "solarwind_kinrad_reflect_recurrent", "smoothshock_perp_momentum", "uniform_para_plane_pass",
          "modulation_type0", "modulation_type2", "gc_rigidity_reflect", "gc_isotropic_only",
          "guiding_spiral", "guiding_scatt_uniform", "guiding_diff_rigidity", "parker_strauss_heliopause", "parker_potgieter_uniform",
          "gc_qlt_spiral", "fieldline_spiral", "fieldline_backward_shock", "lorentz_uniform", "lorentz_spiral_backward",
-         "focused_spiral", "focused_uniform_backward", "gc_wnlt_scatt_spiral", "gc_wnlt_ramp_diffscatt"]
+         "focused_spiral", "focused_uniform_backward", "gc_wnlt_scatt_spiral", "gc_wnlt_ramp_diffscatt",
+         "gc_strauss_diff"]
 
 
 # Setups whose endpoints amplify last-bit differences far beyond 1e-10: QLT's mu derivative is
