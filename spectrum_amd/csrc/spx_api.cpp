@@ -564,8 +564,8 @@ static int check_transport(spx_sim* s)
       }
    }
    else if (s->plan.traj_type == SPX_TRAJ_FOCUSED) {
-      if (s->plan.bg.kind != SPX_BG_SOLARWIND && s->plan.bg.kind != SPX_BG_UNIFORM)
-         return fail(s, SPX_ERR_ARG, "focused transport: SolarWind and Uniform backgrounds");
+      if (s->plan.bg.kind != SPX_BG_SOLARWIND && s->plan.bg.kind != SPX_BG_UNIFORM && s->plan.bg.kind != SPX_BG_SMOOTHSHOCK)
+         return fail(s, SPX_ERR_ARG, "focused transport: SolarWind, Uniform and SmoothShock backgrounds");
       if (s->plan.src_kind != SPX_SRC_NONE) return fail(s, SPX_ERR_ARG, "focused transport: no source term");
       for (int d = 0; d < s->plan.n_dist; d++)
          if (s->plan.dist[d].kind == SPX_DS_POSPITCH) return fail(s, SPX_ERR_ARG, "focused transport: its momentum is (p, mu, 0); use DistributionMomentumUniform");

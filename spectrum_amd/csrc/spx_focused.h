@@ -8,7 +8,10 @@
 // The fields carry the mask ALL | gradB | gradU | dUdt (src/trajectory_focused.cc:36-37).  The solar wind
 // differentiates numerically (BackgroundBase::NumericalDerivatives, src/background_base.cc:228-312) in the
 // field-aligned frame at r_g = min(r_L(p), dmax), p the momentum magnitude of the evaluation; it is static,
-// so its time derivative is exactly zero, as is every derivative of the uniform background.
+// so its time derivative is exactly zero, as is every derivative of the uniform background.  The smooth
+// shock's derivatives are analytic (BackgroundSmoothShock::EvaluateBackgroundDerivatives,
+// src/background_smooth_shock.cc:159-187), dU/dt included when the shock moves; its dmax reads the shock
+// coordinate of the previous evaluation (:192-196), which each slot carries.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -28,20 +31,52 @@ struct FcFields {
    R divbhat;                      // (div B - grad|B| . b) / |B| (common/spatial_data.hh:304-307)
 };
 
-// GetFields with the focused mask at a position for momentum magnitude pm.  Returns false on
+// The contractions the slopes read, from grad U (rows gU), grad B (rows gB), dU/dt and F.gradBmag.
+template <typename R>
+__device__ __forceinline__ void fc_contract(FcFields<R>& F, v3<R> U, v3<R> B, const v3<R> (&gU)[3], const v3<R> (&gB)[3], v3<R> dUdt)
+{
+   const v3<R> b = F.bhat;
+   F.divU = (gU[0].x + gU[1].y) + gU[2].z;
+   F.bbgU = ((R(0) + dot(b.x * b, gU[0])) + dot(b.y * b, gU[1])) + dot(b.z * b, gU[2]);
+   const v3<R> UgU = (U.x * gU[0] + U.y * gU[1]) + U.z * gU[2];
+   F.bcdU2 = dot(R(2) * b, dUdt + UgU);
+   const v3<R> curlB = mk<R>(gB[1].z - gB[2].y, gB[2].x - gB[0].z, gB[0].y - gB[1].x);
+   F.BcB = dot(B, curlB);
+   F.BgB = (B.x * gB[0] + B.y * gB[1]) + B.z * gB[2];
+   const R divB = (gB[0].x + gB[1].y) + gB[2].z;
+   F.divbhat = (divB - dot(F.gradBmag, b)) / F.Bmag;
+}
+
+// GetFields with the focused mask at a position for momentum magnitude pm; ds_prev is the smooth shock's
+// hidden coordinate (read for dmax, then set to this evaluation's).  Returns false on
 // ExFieldError (|B| below sp_tiny).
 template <typename R, int BG>
-__device__ __forceinline__ bool fc_fields(const SPX_CP spx_bg_t<R>& bg, double t, double x, double y, double z, R pm, FcFields<R>& F)
+__device__ __forceinline__ bool fc_fields(const SPX_CP spx_bg_t<R>& bg, double t, double x, double y, double z, R pm, R& ds_prev, FcFields<R>& F)
 {
-   F.dmax = bg_dmax<R, BG>(bg, x, y, z, R(0));
+   F.dmax = bg_dmax<R, BG>(bg, x, y, z, ds_prev);
    v3<R> U, B;
-   R ds = R(0);
+   R ds = ds_prev;
    bg_eval<R, BG>(bg, t, x, y, z, 3, U, B, ds);
+   if (BG == SPX_BG_SMOOTHSHOCK) ds_prev = ds;
    F.U = U;
    F.Bmag = norm(B);
    if (F.Bmag < R(sp_tiny)) return false;
    F.bhat = B / F.Bmag;
    const R z0 = R(0);
+   if (BG == SPX_BG_SMOOTHSHOCK) {
+      // grad X = n (X_up - X_down) f and dX/dt = -v_sh (X_up - X_down) f with f = tanh-transition' / width
+      const R f = shock_transition_deriv(ds) / bg.width_shock;
+      const v3<R> n = v3of(bg.n_shock);
+      const v3<R> du = v3of(bg.u0) - v3of(bg.u1), db = v3of(bg.B0) - v3of(bg.B1);
+      v3<R> gU[3], gB[3];
+      gU[0] = f * (n.x * du); gU[1] = f * (n.y * du); gU[2] = f * (n.z * du);
+      gB[0] = f * (n.x * db); gB[1] = f * (n.y * db); gB[2] = f * (n.z * db);
+      const v3<R> b = F.bhat;
+      F.gradBmag = mk<R>(dot(gB[0], b), dot(gB[1], b), dot(gB[2], b));
+      const v3<R> dUdt = (shock_transition_deriv(ds) * bg.v_shock / bg.width_shock) * (v3of(bg.u1) - v3of(bg.u0));
+      fc_contract(F, U, B, gU, gB, dUdt);
+      return true;
+   }
    if (BG != SPX_BG_SOLARWIND) {
       F.gradBmag = mk<R>(z0, z0, z0); F.BgB = F.gradBmag;
       F.BcB = z0; F.divU = z0; F.bbgU = z0; F.bcdU2 = z0; F.divbhat = z0;
@@ -90,17 +125,8 @@ __device__ __forceinline__ bool fc_fields(const SPX_CP spx_bg_t<R>& bg, double t
       }
    }
    F.gradBmag = gbm;
-   F.divU = (gU[0].x + gU[1].y) + gU[2].z;
-   const v3<R> b = F.bhat;
-   F.bbgU = ((R(0) + dot(b.x * b, gU[0])) + dot(b.y * b, gU[1])) + dot(b.z * b, gU[2]);
-   // dU/dt is exactly zero: the sum is (U . grad) U
-   const v3<R> UgU = (U.x * gU[0] + U.y * gU[1]) + U.z * gU[2];
-   F.bcdU2 = dot(R(2) * b, UgU);
-   const v3<R> curlB = mk<R>(gB[1].z - gB[2].y, gB[2].x - gB[0].z, gB[0].y - gB[1].x);
-   F.BcB = dot(B, curlB);
-   F.BgB = (B.x * gB[0] + B.y * gB[1]) + B.z * gB[2];
-   const R divB = (gB[0].x + gB[1].y) + gB[2].z;
-   F.divbhat = (divB - dot(gbm, b)) / F.Bmag;
+   // the static wind's dU/dt is exactly zero
+   fc_contract(F, U, B, gU, gB, mk<R>(z0, z0, z0));
    return true;
 }
 
@@ -142,12 +168,17 @@ __device__ int fc_set_start(PlanPtr P, QueryPtr Q, LaunchPtr L, const spx_rng_de
    double p1, mu1, p, mu;
    fc_mom_sample(P, rng, first, traj, nu, ex, p1, mu1);
    FcFields<R> F;
-   if (!fc_fields<R, BG>(bg, t, x, y, z, R(p1), F)) return 1;
+   // the smooth shock's hidden coordinate starts at the starting point's
+   R ds = R(0);
+   if (BG == SPX_BG_SMOOTHSHOCK) ds = (dot(rel(bg, x, y, z), v3of(bg.n_shock)) - bg.v_shock * R(t)) / bg.width_shock;
+   const R ds_dm = ds;
+   if (!fc_fields<R, BG>(bg, t, x, y, z, R(p1), ds, F)) return 1;
    fc_mom_sample(P, rng, first, traj, nu, ex, p, mu);
    if (ex) return 5;
    const uint64_t n = L->n_lanes;
    L->t[slot] = t; L->x[slot] = x; L->y[slot] = y; L->z[slot] = z; L->m0[slot] = p; L->m1[slot] = mu; L->m2[slot] = 0.0;
-   L->amp[slot] = 0.0; L->wgt[slot] = 0.0; L->dt_prev[slot] = 0.0; L->src_old[slot] = 0.0; L->ds_prev[slot] = 0.0;
+   L->amp[slot] = 0.0; L->wgt[slot] = 0.0; L->dt_prev[slot] = 0.0; L->src_old[slot] = 0.0;
+   L->ds_prev[slot] = (double)ds_dm;   // the coordinate the current fields' dmax read
    L->dmax[slot] = (double)F.dmax;
    L->pf[slot] = p1;
    L->steps[slot] = 0; L->n_events[slot] = 0;   // the draw counters continue across attempts
@@ -183,6 +214,7 @@ template <typename R>
 struct FcHot {
    double t, x, y, z, p, mu;
    double pf;                                   // p of the last field evaluation
+   R ds, ds_dm;                                 // smooth shock: coordinate of the last evaluation, and the one its dmax read
    FcFields<R> F;
    double bd[SPX_MAX_BND], bdo[SPX_MAX_BND];
    int bst[SPX_MAX_BND];
@@ -229,7 +261,8 @@ __device__ __forceinline__ int fc_step(PlanPtr P, LaunchPtr L, uint64_t slot, Fc
    }
    bflags = handle_boundaries<false>(P, L, slot, H, C, tid);
    if (!(bflags & 1)) {
-      if (!fc_fields<R, BG>(bg, H.t, H.x, H.y, H.z, R(H.p), H.F)) return 1;
+      H.ds_dm = H.ds;
+      if (!fc_fields<R, BG>(bg, H.t, H.x, H.y, H.z, R(H.p), H.ds, H.F)) return 1;
       H.pf = H.p;
       if (H.mu > 1.0) H.mu = 1.0;
       else if (H.mu < -1.0) H.mu = -1.0;
@@ -266,8 +299,10 @@ __global__ __launch_bounds__(BLOCK) void fc_step_kernel(const Launch* __restrict
          H.bd[b] = Lp->bdelta[b * n + slot]; H.bdo[b] = Lp->bdelta_old[b * n + slot];
          C.cross[b][tid] = Lp->bcross[b * n + slot]; H.bst[b] = Lp->bstat[b * n + slot];
       }
-      // the fields are a function of (t, x, p_f): recomputed bit for bit
-      (void)fc_fields<R, BG>(PlanView<R>::bg(P), H.t, H.x, H.y, H.z, R(H.pf), H.F);
+      // the fields are a function of (t, x, p_f, the dmax coordinate): recomputed bit for bit
+      H.ds_dm = R(Lp->ds_prev[slot]);
+      H.ds = H.ds_dm;
+      (void)fc_fields<R, BG>(PlanView<R>::bg(P), H.t, H.x, H.y, H.z, R(H.pf), H.ds, H.F);
    }
    int st = run ? SLOT_RUN : SLOT_IDLE;
    int exc = 0;
@@ -298,6 +333,7 @@ __global__ __launch_bounds__(BLOCK) void fc_step_kernel(const Launch* __restrict
    Ls->t[slot] = H.t; Ls->x[slot] = H.x; Ls->y[slot] = H.y; Ls->z[slot] = H.z;
    Ls->m0[slot] = H.p; Ls->m1[slot] = H.mu; Ls->m2[slot] = 0.0;
    Ls->pf[slot] = H.pf;
+   Ls->ds_prev[slot] = (double)H.ds_dm;
    Ls->steps[slot] = H.steps;
    Ls->dmax[slot] = (double)H.F.dmax;
    Ls->amp[slot] = C.amp[tid]; Ls->wgt[slot] = C.wgt[tid];
@@ -330,6 +366,8 @@ hipError_t dispatch_focused(const spx_plan_desc& hp, uint64_t n_lanes, const Lau
       return fwd ? launch_fc_bg<R, SPX_BG_SOLARWIND, 1>(n_lanes, d_launch, st, mid) : launch_fc_bg<R, SPX_BG_SOLARWIND, 0>(n_lanes, d_launch, st, mid);
    case SPX_BG_UNIFORM:
       return fwd ? launch_fc_bg<R, SPX_BG_UNIFORM, 1>(n_lanes, d_launch, st, mid) : launch_fc_bg<R, SPX_BG_UNIFORM, 0>(n_lanes, d_launch, st, mid);
+   case SPX_BG_SMOOTHSHOCK:
+      return fwd ? launch_fc_bg<R, SPX_BG_SMOOTHSHOCK, 1>(n_lanes, d_launch, st, mid) : launch_fc_bg<R, SPX_BG_SMOOTHSHOCK, 0>(n_lanes, d_launch, st, mid);
    default: return hipErrorInvalidValue;
    }
 }

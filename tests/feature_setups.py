@@ -400,18 +400,38 @@ def gc_strauss_diff():
     return s
 
 
+def focused_smoothshock():
+    """TrajectoryFocused (forward) through a moving smooth shock with an oblique field: analytic grad U and
+    grad B across the transition and dU/dt of the moving shock; the dmax of the shock's hidden coordinate."""
+    s = cf.Setup("focused_smoothshock", cf.TRAJ_FOCUSED, cf.FORWARD)
+    u1, w = 0.013343, 2.5
+    dc = cf.background_base(DC(), 0.0, (0.0, 0.0, 0.0), (u1, 0.0, 0.0), (8.0, 6.0, 0.0), 2.0 * w)
+    dc.insert_vec((-1.0, 0.0, 0.0)).insert_double(0.002).insert_double(4.0).insert_double(w).insert_double(0.1)
+    s.add(cf.BACKGROUND, "BackgroundSmoothShock", dc)
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((-4.0, 0.0, 0.0)))
+    s.add(cf.INITIAL, "InitialMomentumThickShell", DC().insert_double(cf.mom_from_ekin(0.1 * cf.MeV)).insert_double(cf.mom_from_ekin(1.0 * cf.MeV)).insert_bool(True))
+    s.add(cf.BOUNDARY, "BoundaryPlaneAbsorb", cf.boundary(1, [0], ("vec", (-40.0, 0.0, 0.0)), ("vec", (1.0, 0.0, 0.0))))
+    s.add(cf.BOUNDARY, "BoundaryPlaneAbsorb", cf.boundary(1, [0], ("vec", (40.0, 0.0, 0.0)), ("vec", (1.0, 0.0, 0.0))))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", 300.0)))
+    d = dist((10, 0, 0), (0.0, 0, 0), (330.0, 0, 0), (0, 0, 0), (1, 0, 0))
+    d.insert_double(1.0).insert_double(0.0).insert_int(1)
+    s.add(cf.DISTRIBUTION, "DistributionTimeUniform", d)
+    return s
+
+
 ALL = [shock_flowmom_reflect, solarwind_kinrad_reflect_recurrent, smoothshock_perp_momentum, uniform_para_plane_pass,
        lambda: modulation_spectrum_type(0), lambda: modulation_spectrum_type(2), gc_rigidity_reflect, gc_isotropic_only,
        guiding_spiral, guiding_scatt_uniform, guiding_diff_rigidity, parker_strauss_heliopause, parker_potgieter_uniform,
        gc_qlt_spiral, fieldline_spiral, fieldline_backward_shock, lorentz_uniform, lorentz_spiral_backward,
        focused_spiral, focused_uniform_backward, gc_wnlt_scatt_spiral, gc_wnlt_ramp_diffscatt,
-       gc_strauss_diff]
+       gc_strauss_diff, focused_smoothshock]
 NAMES = ["shock_flowmom_reflect", "solarwind_kinrad_reflect_recurrent", "smoothshock_perp_momentum", "uniform_para_plane_pass",
          "modulation_type0", "modulation_type2", "gc_rigidity_reflect", "gc_isotropic_only",
          "guiding_spiral", "guiding_scatt_uniform", "guiding_diff_rigidity", "parker_strauss_heliopause", "parker_potgieter_uniform",
          "gc_qlt_spiral", "fieldline_spiral", "fieldline_backward_shock", "lorentz_uniform", "lorentz_spiral_backward",
          "focused_spiral", "focused_uniform_backward", "gc_wnlt_scatt_spiral", "gc_wnlt_ramp_diffscatt",
-         "gc_strauss_diff"]
+         "gc_strauss_diff", "focused_smoothshock"]
 
 
 # Setups whose endpoints amplify last-bit differences far beyond 1e-10: QLT's mu derivative is
@@ -420,7 +440,7 @@ NAMES = ["shock_flowmom_reflect", "solarwind_kinrad_reflect_recurrent", "smooths
 # with its last-bit-perturbed build (test_gpu_parity.sensitivity).  The focused transport along the spiral
 # differentiates the field over the Larmor radius (a difference of nearby field values divided by ~1e-3 of
 # the distance to the Sun), and its long trajectories reach a few 1e-10 of last-bit sensitivity.
-ILL_CONDITIONED = {"gc_qlt_spiral", "focused_spiral", "gc_wnlt_scatt_spiral", "gc_wnlt_ramp_diffscatt"}
+ILL_CONDITIONED = {"gc_qlt_spiral", "focused_spiral", "gc_wnlt_scatt_spiral", "gc_wnlt_ramp_diffscatt", "focused_smoothshock"}
 
 
 def weight_atol(name, rec, rtol):

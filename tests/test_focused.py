@@ -2,7 +2,7 @@
 
 * a uniform field carried by a uniform flow: no gradients, so p and mu never change and the centre moves
   at U + v mu b: pos - pos0 = (U + v mu b) (t - t0);
-* one step in the solar wind against the focused transport equation in its textbook form (Skilling 1971;
+* one step in the solar wind and across a moving smooth shock against the focused transport equation in its textbook form (Skilling 1971;
   le Roux & Webb 2009), with the gradients taken here by Cartesian central differences of the field and
   the guiding-centre drift written with the curvature (b . grad) b and b . curl b rather than B . grad B
   and B . curl B:
@@ -57,9 +57,16 @@ def test_uniform_field_drift(oracle_mod, flow):
     np.testing.assert_allclose(rec["pos"] - rec["pos0"], want, rtol=1e-11, atol=1e-11 * np.abs(want).max())
 
 
-def fields(o, x):
-    U, Bv, _, _ = o.background_fields(0.0, x, 1.0)
+def fields(o, x, t=0.0):
+    U, Bv, _, _ = o.background_fields(t, x, 1.0)
     return np.array(U), np.array(Bv)
+
+
+def smooth_shock():
+    """A smooth shock moving at 0.002 along -x, oblique field (BackgroundSmoothShock)."""
+    dc = cf.background_base(DC(), 0.0, (0.0, 0.0, 0.0), (0.013343, 0.0, 0.0), (8.0, 6.0, 0.0), 5.0)
+    dc.insert_vec((-1.0, 0.0, 0.0)).insert_double(0.002).insert_double(4.0).insert_double(2.5).insert_double(0.1)
+    return ("BackgroundSmoothShock", dc)
 
 
 def jacobians(o, x, h):
@@ -75,11 +82,17 @@ def jacobians(o, x, h):
     return JU, Jb, gB
 
 
-def test_one_step_against_the_focused_transport_equation(oracle_mod):
-    x0 = np.array([0.8, 0.5, 0.2]) * cf.one_au
-    s = setup(("BackgroundSolarWind", cf.solarwind()), tuple(x0),
+@pytest.mark.parametrize("case", ["solarwind", "smoothshock"])
+def test_one_step_against_the_focused_transport_equation(oracle_mod, case):
+    """In the solar wind the oracle's gradients are its field-aligned stencil; across the smooth shock they
+    are the background's analytic ones, and the moving shock adds dU/dt, here a central difference in t."""
+    if case == "solarwind":
+        bg, x0, t_end, h = ("BackgroundSolarWind", cf.solarwind()), np.array([0.8, 0.5, 0.2]) * cf.one_au, 2.0, 1e-3
+    else:
+        bg, x0, t_end, h = smooth_shock(), np.array([-0.2, 0.3, 0.1]), 0.01, 1e-4
+    s = setup(bg, tuple(x0),
               ("InitialMomentumThickShell", DC().insert_double(cf.mom_from_ekin(1.0 * cf.MeV)).insert_double(cf.mom_from_ekin(100.0 * cf.MeV)).insert_bool(True)),
-              [("BoundaryTimeExpire", cf.boundary(1, [0], ("double", 2.0)))])
+              [("BoundaryTimeExpire", cf.boundary(1, [0], ("double", t_end)))])
     o = oracle_mod.Oracle(s)
     n = 24
     _, rec = o.run(n, seed=9, threads=THREADS)
@@ -87,10 +100,11 @@ def test_one_step_against_the_focused_transport_equation(oracle_mod):
     U, Bv = fields(o, x0)
     Bm = np.linalg.norm(Bv)
     b = Bv / Bm
-    JU, Jb, gB = jacobians(o, x0, 1e-3)
+    JU, Jb, gB = jacobians(o, x0, h)
     divU = np.trace(JU)
     bbgU = b @ JU @ b
-    UgU = U @ JU                            # (U . grad) U
+    dUdt = (fields(o, x0, 1e-4)[0] - fields(o, x0, -1e-4)[0]) / 2e-4
+    UgU = dUdt + U @ JU                     # dU/dt + (U . grad) U
     divb = np.trace(Jb)
     kappa = b @ Jb                          # (b . grad) b
     curlb = np.array([Jb[1, 2] - Jb[2, 1], Jb[2, 0] - Jb[0, 2], Jb[0, 1] - Jb[1, 0]])
@@ -141,10 +155,10 @@ def test_reflection_reverses_mu(oracle_mod):
 
 def test_focused_host_checks():
     dc = cf.background_base(DC(), 0.0, (0.0, 0.0, 0.0), (0.01, 0.0, 0.0), (0.0, 3.0, 0.0), 10.0)
-    dc.insert_vec((-1.0, 0.0, 0.0)).insert_double(0.0).insert_double(4.0).insert_double(1.0).insert_double(0.1)
-    s = setup(("BackgroundSmoothShock", dc), (0.0, 0.0, 0.0), ("InitialMomentumShell", DC().insert_double(1.0)),
+    dc.insert_vec((-1.0, 0.0, 0.0)).insert_double(0.0).insert_double(4.0)
+    s = setup(("BackgroundShock", dc), (0.0, 0.0, 0.0), ("InitialMomentumShell", DC().insert_double(1.0)),
               [("BoundaryTimeExpire", cf.boundary(1, [0], ("double", 10.0)))])
-    with pytest.raises(E.SpxError, match="SolarWind and Uniform"):
+    with pytest.raises(E.SpxError, match="SolarWind, Uniform and SmoothShock"):
         E.Simulation.from_setup(s).run(4)
     bg = ("BackgroundUniform", cf.background_base(DC(), 0.0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 1.0))
     s = setup(bg, (0.0, 0.0, 0.0), ("InitialMomentumShell", DC().insert_double(1.0)), [("BoundaryTimeExpire", cf.boundary(1, [0], ("double", 10.0)))])
