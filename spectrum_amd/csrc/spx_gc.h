@@ -172,19 +172,23 @@ __device__ __noinline__ void wnlt_k(const SPX_CP spx_df_t<R>& df, bool comp2, do
 // DiffusionBase::GetComponent(0) (src/diffusion_base.cc:105-121) of the models usable with the
 // guiding centre; |p| = pmag, mu the pitch cosine, (x, y, z) the position, outer the LISM indicator the
 // Strauss / Potgieter models read (src/diffusion_other.cc:886-906, :968-988).
-template <typename R>
+// XD compiles in the models that depend on mu and position (WNLT) or on the region (Strauss / Potgieter),
+// kept out of the kernels of the other models.
+template <typename R, bool XD>
 __device__ __forceinline__ R gc_kperp(const SPX_CP spx_df_t<R>& df, R pmag, R mu, R Bmag, double x, double y, double z, bool outer)
 {
+   if constexpr (XD) {
+      if (df.kind == SPX_DF_STRAUSS || df.kind == SPX_DF_POTGIETER) {
+         const R B0_eff = outer ? Bmag : df.B0;
+         return (outer ? df.kap_rat_out : df.kap_rat_in) * (kappa_mom_region(df, pmag, outer) * (B0_eff / Bmag));
+      }
+      if (df.kind == SPX_DF_WNLT || df.kind == SPX_DF_WNLT_RAMP) {
+         R K0, K2;
+         wnlt_k(df, false, x, y, z, pmag, mu, Bmag, K0, K2);
+         return K0;
+      }
+   }
    switch (df.kind) {
-   case SPX_DF_STRAUSS: case SPX_DF_POTGIETER: {
-      const R B0_eff = outer ? Bmag : df.B0;
-      return (outer ? df.kap_rat_out : df.kap_rat_in) * (kappa_mom_region(df, pmag, outer) * (B0_eff / Bmag));
-   }
-   case SPX_DF_WNLT: case SPX_DF_WNLT_RAMP: {
-      R K0, K2;
-      wnlt_k(df, false, x, y, z, pmag, mu, Bmag, K0, K2);
-      return K0;
-   }
    case SPX_DF_PERP: return df.D0;                                  // src/diffusion_other.cc:459-463
    case SPX_DF_FULL: return df.Dperp;                               // :538-543
    case SPX_DF_PERPISO: return df.Dperp;
@@ -201,10 +205,10 @@ __device__ __forceinline__ R gc_kperp(const SPX_CP spx_df_t<R>& df, R pmag, R mu
 // and its mu derivative, analytic for the isotropic model (src/diffusion_other.cc:64-79) and the forward
 // difference of DiffusionBase::GetMuDerivative (src/diffusion_base.cc:226-255) for DiffusionQLTConstant
 // (src/diffusion_other.cc:145-149); the other models have no component 2.
-template <typename R>
+template <typename R, bool XD>
 __device__ __forceinline__ void gc_dmumu(const SPX_CP spx_df_t<R>& df, R pmag, R Bmag, R mu, double x, double y, double z, R& Dmumu, R& Vmu)
 {
-   if (df.kind == SPX_DF_WNLT || df.kind == SPX_DF_WNLT_RAMP) {
+   if (XD && (df.kind == SPX_DF_WNLT || df.kind == SPX_DF_WNLT_RAMP)) {
       R K0, K2;
       wnlt_k(df, true, x, y, z, pmag, mu, Bmag, K0, Dmumu);
       const R dmu = R(sp_small) * ((mu + R(sp_small) < R(1)) ? R(1) : R(-1));
@@ -232,12 +236,12 @@ __device__ __forceinline__ void gc_dmumu(const SPX_CP spx_df_t<R>& df, R pmag, R
 // TrajectoryGuidingDiff::DiffusionCoeff (src/trajectory_guiding_diff.cc:72-98): D_perp and the
 // drift V_perp = grad_perp D_perp in the field-aligned frame.  Only the rigidity power law has a
 // spatial derivative (src/diffusion_other.cc:818-823), and only through |B|.
-template <typename R>
+template <typename R, bool XD>
 __device__ __forceinline__ void gc_perp(const SPX_CP spx_df_t<R>& df, const GcFields<R>& F, const v3<R>* fa, R pmag, R mu,
                                         double x, double y, double z, bool outer, R& Dperp, v3<R>& Vperp)
 {
-   Dperp = gc_kperp(df, pmag, mu, F.Bmag, x, y, z, outer);
-   if (df.kind == SPX_DF_WNLT || df.kind == SPX_DF_WNLT_RAMP || df.kind == SPX_DF_STRAUSS || df.kind == SPX_DF_POTGIETER) {
+   Dperp = gc_kperp<R, XD>(df, pmag, mu, F.Bmag, x, y, z, outer);
+   if (XD && (df.kind == SPX_DF_WNLT || df.kind == SPX_DF_WNLT_RAMP || df.kind == SPX_DF_STRAUSS || df.kind == SPX_DF_POTGIETER)) {
       // DiffusionBase::GetDirectionalDerivative (src/diffusion_base.cc:130-170): along each Cartesian axis,
       // kappa_perp at the position moved by r_g with |B| carried by its gradient, and back by 2 r_g; the
       // region indicator stays the point's (the background is not evaluated again)
@@ -249,11 +253,11 @@ __device__ __forceinline__ void gc_perp(const SPX_CP spx_df_t<R>& df, const GcFi
          double xs[3] = {x, y, z};
          xs[k] += (double)dr;
          R Bm = F.Bmag + gk * dr;
-         const R Kf = gc_kperp(df, pmag, mu, Bm, xs[0], xs[1], xs[2], outer);
+         const R Kf = gc_kperp<R, XD>(df, pmag, mu, Bm, xs[0], xs[1], xs[2], outer);
          dr *= R(2);
          xs[k] -= (double)dr;
          Bm -= gk * dr;
-         const R Kb = gc_kperp(df, pmag, mu, Bm, xs[0], xs[1], xs[2], outer);
+         const R Kb = gc_kperp<R, XD>(df, pmag, mu, Bm, xs[0], xs[1], xs[2], outer);
          g[k] = (Kf - Kb) / dr;
       }
       const v3<R> gv = mk<R>(g[0], g[1], g[2]);
@@ -387,7 +391,7 @@ struct GcHot {
 // DiffusionCoeff, DriftCoeff, PhysicalStep (src/trajectory_guiding_diff.cc:324-332 then
 // src/trajectory_guiding_scatt.cc:255-262), EulerPitchAngleScatt (:81-106), Slopes, EulerPerpDiffSlopes
 // (src/trajectory_guiding_diff.cc:123-140), the Euler RKStep, boundaries and the end-of-step fields.
-template <typename R, int BG>
+template <typename R, int BG, bool XD>
 __device__ __forceinline__ int gc_step(PlanPtr P, LaunchPtr L, const spx_rng_desc& rng,
                                        uint64_t first, uint64_t slot, GcHot<R>& H, ColdLDS& C, int tid, int& bflags)
 {
@@ -412,10 +416,10 @@ __device__ __forceinline__ int gc_step(PlanPtr P, LaunchPtr L, const spx_rng_des
    R Dperp = R(0);
    v3<R> Vperp = mk<R>(R(0), R(0), R(0));
    // the LISM indicator of the Strauss / Potgieter models at the point of the last field evaluation
-   const bool outer = (df.kind == SPX_DF_STRAUSS || df.kind == SPX_DF_POTGIETER) && lism_outer<R, BG>(bg, df, H.x, H.y, H.z);
-   if (perp_step) gc_perp(df, H.F, fa, pmag, mu, H.x, H.y, H.z, outer, Dperp, Vperp);
+   const bool outer = XD && (df.kind == SPX_DF_STRAUSS || df.kind == SPX_DF_POTGIETER) && lism_outer<R, BG>(bg, df, H.x, H.y, H.z);
+   if (perp_step) gc_perp<R, XD>(df, H.F, fa, pmag, mu, H.x, H.y, H.z, outer, Dperp, Vperp);
    R Dmumu = R(0), Vmu = R(0);
-   if (pa_step) gc_dmumu(df, pmag, H.F.Bmag, mu, H.x, H.y, H.z, Dmumu, Vmu);
+   if (pa_step) gc_dmumu<R, XD>(df, pmag, H.F.Bmag, mu, H.x, H.y, H.z, Dmumu, Vmu);
    v3<R> drift, Es, Bs;
    gc_drift(H.F, vel, pp, pl, drift, Es, Bs);
    // PhysicalStep: TrajectoryGuiding's (src/trajectory_guiding.cc:96-100), GuidingDiff's with V_perp and
@@ -459,14 +463,14 @@ __device__ __forceinline__ int gc_step(PlanPtr P, LaunchPtr L, const spx_rng_des
    v3<R> dr = mk<R>(R(0), R(0), R(0));
    if (perp_step) {
       const R mu2 = pl / norm(mk<R>(pp, R(0), pl));
-      gc_perp(df, H.F, fa, pmag, mu2, H.x, H.y, H.z, outer, Dperp, Vperp);
+      gc_perp<R, XD>(df, H.F, fa, pmag, mu2, H.x, H.y, H.z, outer, Dperp, Vperp);
       slope_pos = drift + Vperp;
       // EulerPerpDiffSlopes
       const R sdt = rsqrt_(dtr);
       const R dWx = sdt * next_normal<R>(rng, first, H.rs);
       const R dWy = sdt * next_normal<R>(rng, first, H.rs);
       if (H.rs.exhausted) return 5;
-      Dperp = gc_kperp(df, pmag, mu2, H.F.Bmag, H.x, H.y, H.z, outer);
+      Dperp = gc_kperp<R, XD>(df, pmag, mu2, H.F.Bmag, H.x, H.y, H.z, outer);
       dr = from_basis(mk<R>(rsqrt_(R(2) * Dperp) * dWx, rsqrt_(R(2) * Dperp) * dWy, R(0)), fa);
    }
    // RKStep (Euler), then the perpendicular displacement
@@ -498,7 +502,7 @@ __device__ __forceinline__ int gc_step(PlanPtr P, LaunchPtr L, const spx_rng_des
 #define SPX_GC_ATTR
 #endif
 
-template <typename R, int BG>
+template <typename R, int BG, bool XD>
 __global__ __launch_bounds__(BLOCK) SPX_GC_ATTR void gc_step_kernel(const Launch* __restrict__ Lg)
 {
    const LaunchPtr Lp = (LaunchPtr)Lg;
@@ -540,7 +544,7 @@ __global__ __launch_bounds__(BLOCK) SPX_GC_ATTR void gc_step_kernel(const Launch
       LaunchPtr Li = Lp;
       asm volatile("" : "+s"(Li));
       int bflags = 0;
-      const int e = gc_step<R, BG>(Pi, Li, rng, first, slot, H, C, tid, bflags);
+      const int e = gc_step<R, BG, XD>(Pi, Li, rng, first, slot, H, C, tid, bflags);
       steps_here++;
       if (e != 0) { exc = e; st = SLOT_DISCARD; }
       else if (bflags & 1) st = SLOT_DONE;
@@ -567,7 +571,7 @@ __global__ __launch_bounds__(BLOCK) SPX_GC_ATTR void gc_step_kernel(const Launch
    }
 }
 
-template <typename R, int BG>
+template <typename R, int BG, bool XD>
 static hipError_t launch_gc_bg(uint64_t n_lanes, const Launch* d_launch, hipStream_t st, hipEvent_t mid)
 {
    const unsigned grid = (unsigned)((n_lanes + BLOCK - 1) / BLOCK);
@@ -576,7 +580,7 @@ static hipError_t launch_gc_bg(uint64_t n_lanes, const Launch* d_launch, hipStre
       hipError_t e = hipEventRecord(mid, st);
       if (e != hipSuccess) return e;
    }
-   hipLaunchKernelGGL((gc_step_kernel<R, BG>), dim3(grid), dim3(BLOCK), 0, st, d_launch);
+   hipLaunchKernelGGL((gc_step_kernel<R, BG, XD>), dim3(grid), dim3(BLOCK), 0, st, d_launch);
    return hipGetLastError();
 }
 
@@ -584,10 +588,15 @@ template <typename R>
 hipError_t dispatch_gc(const spx_plan_desc& hp, uint64_t n_lanes, const Launch* d_launch, hipStream_t st, hipEvent_t mid)
 {
    if (hp.time_flow != 0) return hipErrorInvalidValue;
+   // the WNLT and Strauss / Potgieter models run in kernels of their own (SolarWind, Uniform)
+   const int k = hp.df.kind;
+   const bool xd = (k == SPX_DF_WNLT || k == SPX_DF_WNLT_RAMP || k == SPX_DF_STRAUSS || k == SPX_DF_POTGIETER);
    switch (hp.bg.kind) {
-   case SPX_BG_SOLARWIND: return launch_gc_bg<R, SPX_BG_SOLARWIND>(n_lanes, d_launch, st, mid);
-   case SPX_BG_UNIFORM: return launch_gc_bg<R, SPX_BG_UNIFORM>(n_lanes, d_launch, st, mid);
-   case SPX_BG_GRID: return launch_gc_bg<R, SPX_BG_GRID>(n_lanes, d_launch, st, mid);
+   case SPX_BG_SOLARWIND:
+      return xd ? launch_gc_bg<R, SPX_BG_SOLARWIND, true>(n_lanes, d_launch, st, mid) : launch_gc_bg<R, SPX_BG_SOLARWIND, false>(n_lanes, d_launch, st, mid);
+   case SPX_BG_UNIFORM:
+      return xd ? launch_gc_bg<R, SPX_BG_UNIFORM, true>(n_lanes, d_launch, st, mid) : launch_gc_bg<R, SPX_BG_UNIFORM, false>(n_lanes, d_launch, st, mid);
+   case SPX_BG_GRID: return xd ? hipErrorInvalidValue : launch_gc_bg<R, SPX_BG_GRID, false>(n_lanes, d_launch, st, mid);
    default: return hipErrorInvalidValue;
    }
 }
