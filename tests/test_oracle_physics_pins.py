@@ -1,0 +1,403 @@
+"""Pins of the oracle's physics functions against mathematics written independently of it.
+
+Each check is built so that a plausible slip in the oracle (a dropped term, a wrong sign or index, p for
+p^2, initial for final momentum) fails it:
+
+* Parker div(kappa) (TrajectoryParker::DiffusionCoeff, src/trajectory_parker.cc:100-119): one forward
+  step with zero Wiener increments moves the particle by dt (U + div kappa); div kappa is recomputed here
+  as the divergence of kappa_perp I + (kappa_par - kappa_perp) b b by a fourth-order Cartesian stencil of
+  the closed-form Parker spiral, with a kappa that depends on |B| so that grad kappa and grad b both count.
+* Guiding centre (TrajectoryGuiding::ModifiedFields / DriftCoeff / Slopes, src/trajectory_guiding.cc:65-122):
+  one step in the spiral against the guiding-centre equations of Tao, Chan & Brizard (2007) with b, |B|,
+  grad |B| and curl b differentiated here, and against the first-order drifts (parallel streaming,
+  E x B, grad-B, curvature) and the mirror force, which it must reproduce to O(r_L / L).
+* BackgroundShock's jump (src/background_shock.cc:58-86): Rankine-Hugoniot conditions of a moving oblique
+  shock (normal flow in the shock frame divided by r, tangential flow and normal field continuous,
+  tangential field times r), upstream where (x - r0) . n > v_sh t.
+* DiffusionFlowMomentumPowerLaw and DiffusionKineticEnergyRadialDistancePowerLaw
+  (src/diffusion_other.cc:622-627, 713-723): the scaling in |U|, p, T and r, the normalisation at the
+  reference point and the stream-dependent branches.
+* DistributionSpectrumKineticEnergyPowerLaw (src/distribution_other.cc:561-594): the bin of the initial
+  kinetic energy and, per trajectory, the weight J0 [v] (T/T0)^g / [p^2] of the final momentum, TYPE 0/1/2.
+* SourceMomentumInjection (src/source_other.cc:113-119, src/trajectory_parker_source.cc:58-65): in a
+  radial wind with no diffusion the steps form a closed recurrence, and the amplitude added on the step
+  after the crossing is rate * dt_now / dt_before (the source divides by the previous step's dt).
+* WNLT pitch-angle scattering (src/diffusion_other.cc:215-240): the nonlinear term with its
+  hypergeometric factors computed from Euler's integral by quadrature.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate
+
+from spectrum_amd import configs as cf
+from spectrum_amd.data_container import DataContainer as DC
+
+C = 10.0            # speed of light in code units (0.1 c)
+M = 1.0             # proton mass in code units
+Q = 1.0             # proton charge in code units
+AU = cf.one_au
+V_SW = 400.0e5 / cf.unit_velocity
+B1AU = 5.0e-5 / cf.unit_magnetic
+R_REF = 3.0 * 6.957e10 / cf.unit_length
+OMEGA = 2.87e-6 * cf.unit_time
+
+
+# --- textbook kinematics and the Parker spiral, written here --------------------------------------------
+def energy(p):
+    return math.sqrt((p * C) ** 2 + (M * C * C) ** 2)
+
+
+def ekin(p):
+    return energy(p) - M * C * C
+
+
+def speed(p):
+    return p * C * C / energy(p)
+
+
+def mom_of_ekin(T):
+    return math.sqrt(T * (T + 2.0 * M * C * C)) / C
+
+
+def spiral(x):
+    """U and B of the Parker spiral (rotation about z), and the induced E = -U x B / c."""
+    x = np.asarray(x, dtype=float)
+    r = np.linalg.norm(x)
+    rhat = x / r
+    st = math.hypot(x[0], x[1]) / r
+    Br = B1AU * (AU / r) ** 2
+    Bphi = -Br * st * (r - R_REF) * OMEGA / V_SW
+    phihat = np.array([-x[1], x[0], 0.0]) / math.hypot(x[0], x[1])
+    U = V_SW * rhat
+    B = Br * rhat + Bphi * phihat
+    return U, B, -np.cross(U, B) / C
+
+
+def d4(f, x, j, h):
+    """Fourth-order central difference of f along axis j."""
+    e = np.zeros(3)
+    e[j] = h
+    return (-f(x + 2 * e) + 8 * f(x + e) - 8 * f(x - e) + f(x - 2 * e)) / (12 * h)
+
+
+def one_step(oracle_mod, s, n=1, uniforms=None, n_normals=3):
+    normals = np.zeros((n, n_normals))
+    u = uniforms if uniforms is not None else np.full((n, 4), 0.5)
+    _, rec = oracle_mod.Oracle(s).run(n, normals=normals, uniforms=u, threads=1)
+    return rec
+
+
+def time_dist(t_hi):
+    d = cf.distribution_base(DC(), (4, 0, 0), (0.0, 0, 0), (t_hi, 0, 0), (0, 0, 0), (0, 0, 0))
+    return d.insert_double(1.0).insert_double(0.0).insert_int(1)
+
+
+# --- Parker div(kappa) --------------------------------------------------------------------------------
+@pytest.mark.parametrize("pos_au,T_GeV,b_exp,ratio", [((0.8, 0.5, 0.2), 1.0, -0.7, 0.1), ((2.0, -1.5, 0.6), 0.3, 0.5, 0.3),
+                                                      ((-5.0, 3.0, -4.0), 3.0, -1.0, 0.02)])
+def test_parker_div_kappa_one_step(oracle_mod, pos_au, T_GeV, b_exp, ratio):
+    x0 = np.array(pos_au) * AU
+    p = mom_of_ekin(T_GeV * cf.GeV)
+    lam0, R0, a = 150.0, 106.58, 0.5
+    s = cf.Setup("divk", cf.TRAJ_PARKER, cf.FORWARD)
+    s.add(cf.BACKGROUND, "BackgroundSolarWind", cf.solarwind())
+    dif = DC().insert_double(lam0).insert_double(R0).insert_double(B1AU).insert_double(a).insert_double(b_exp).insert_double(ratio)
+    s.add(cf.DIFFUSION, "DiffusionRigidityMagneticFieldPowerLaw", dif)
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec(tuple(x0)))
+    s.add(cf.INITIAL, "InitialMomentumShell", DC().insert_double(p))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", 0.01)))
+    s.add(cf.DISTRIBUTION, "DistributionTimeUniform", time_dist(1.0))
+    rec = one_step(oracle_mod, s)
+    assert rec["status"][0] == 0 and rec["steps"][0] == 1
+    dt = rec["t"][0] - rec["t0"][0]
+    U0, B0, _ = spiral(x0)
+    got = (rec["pos"][0] - x0) / dt - U0
+
+    v = speed(p)
+    rig = p * C / Q
+
+    def kappa_tensor(x):
+        _, B, _ = spiral(x)
+        Bm = np.linalg.norm(B)
+        kpar = lam0 * v / 3.0 * (rig / R0) ** a * (Bm / B1AU) ** b_exp
+        kperp = ratio * kpar
+        b = B / Bm
+        return kperp * np.eye(3) + (kpar - kperp) * np.outer(b, b)
+
+    h = 1e-3 * np.linalg.norm(x0)
+    want = sum(d4(kappa_tensor, x0, j, h)[:, j] for j in range(3))
+    # the oracle differentiates over delta = min(r_L, dmax) to second order: truncation ~ (delta / r)^2
+    r = np.linalg.norm(x0)
+    delta = min(p * C / (Q * np.linalg.norm(B0)), min(0.1 * r, AU))
+    assert np.linalg.norm(got - want) < (4.0 * (delta / r) ** 2 + 1e-6) * np.linalg.norm(want), (got, want, delta / r)
+    # the momentum slope -p div U / 3 with div U = 2 V / r
+    dp = (rec["mom"][0, 0] - p) / dt
+    assert dp == pytest.approx(-p * 2.0 * V_SW / np.linalg.norm(x0) / 3.0, rel=1e-4)
+
+
+# --- guiding centre ------------------------------------------------------------------------------------
+@pytest.mark.parametrize("pos_au,T_MeV,theta", [((0.8, 0.5, 0.2), 100.0, 1.0), ((3.0, -2.0, 1.5), 500.0, 2.3), ((-0.3, 0.4, -0.2), 30.0, 0.4)])
+def test_guiding_centre_one_step(oracle_mod, pos_au, T_MeV, theta):
+    x0 = np.array(pos_au) * AU
+    p = mom_of_ekin(T_MeV * cf.MeV)
+    s = cf.Setup("gc_step", cf.TRAJ_GUIDING, cf.FORWARD)
+    s.add(cf.BACKGROUND, "BackgroundSolarWind", cf.solarwind())
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec(tuple(x0)))
+    s.add(cf.INITIAL, "InitialMomentumRing", DC().insert_double(p).insert_double(theta))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", 0.002)))
+    s.add(cf.DISTRIBUTION, "DistributionTimeUniform", time_dist(1.0))
+    rec = one_step(oracle_mod, s, n_normals=1)
+    assert rec["status"][0] == 0 and rec["steps"][0] == 1
+    dt = rec["t"][0] - rec["t0"][0]
+    pperp, ppar = p * math.sin(theta), p * math.cos(theta)
+    assert rec["mom0"][0, 0] == pytest.approx(pperp, rel=1e-14) and rec["mom0"][0, 2] == pytest.approx(ppar, rel=1e-14)
+
+    U, B, E = spiral(x0)
+    Bm = np.linalg.norm(B)
+    b = B / Bm
+    h = 1e-4 * np.linalg.norm(x0)
+    bmag = lambda x: np.array([np.linalg.norm(spiral(x)[1])])
+    bhat = lambda x: spiral(x)[1] / np.linalg.norm(spiral(x)[1])
+    gradB = np.array([d4(bmag, x0, j, h)[0] for j in range(3)])
+    J = np.array([d4(bhat, x0, j, h) for j in range(3)])            # J[j, i] = d b_i / d x_j
+    curlb = np.array([J[1, 2] - J[2, 1], J[2, 0] - J[0, 2], J[0, 1] - J[1, 0]])
+    curv = b @ J                                                    # (b . grad) b
+    v = speed(p)
+    vperp, vpar = v * math.sin(theta), v * math.cos(theta)
+    rL, rR = pperp * C / (Q * Bm), ppar * C / (Q * Bm)
+    # Tao, Chan & Brizard (2007): E* = E - (p_perp v_perp / 2qB) grad B, B* = B + (p_par c / q) curl b
+    Es = E - rL * vperp * gradB / (2.0 * C)
+    Bs = B + rR * Bm * curlb
+    Bspar = Bs @ b
+    vd = (vpar * Bs + C * np.cross(Es, b)) / Bspar
+    dppar = Q * (Es @ Bs) / Bspar
+    dpperp = 0.5 * pperp / Bm * (vd @ gradB)
+
+    got_vd = (rec["pos"][0] - x0) / dt
+    got_ppar = (rec["mom"][0, 2] - ppar) / dt
+    got_pperp = (rec["mom"][0, 0] - pperp) / dt
+    vperp_part = vd - (vd @ b) * b
+    assert np.linalg.norm(got_vd - vd) < 1e-4 * np.linalg.norm(vperp_part), (got_vd, vd)
+    assert got_ppar == pytest.approx(dppar, rel=1e-4, abs=1e-6 * abs(dppar))
+    assert got_pperp == pytest.approx(dpperp, rel=1e-4)
+
+    # first-order drifts and the mirror force, independent of the B*/E* form: agree to O(r_L / L)
+    vE = C * np.cross(E, B) / Bm ** 2
+    vgrad = 0.5 * rL * vperp * np.cross(b, gradB) / Bm
+    vcurv = rR * vpar * np.cross(b, curv)
+    first = vpar * b + vE + vgrad + vcurv
+    eps = max(rL, abs(rR)) / np.linalg.norm(x0)
+    assert np.linalg.norm(got_vd - first) < 20 * eps * np.linalg.norm(vE + vgrad + vcurv) + 1e-9 * v
+    mirror = -0.5 * pperp * vperp / Bm * (b @ gradB)
+    assert got_ppar == pytest.approx(mirror, rel=0.05)
+
+
+# --- BackgroundShock jump -----------------------------------------------------------------------------
+def test_shock_rankine_hugoniot(oracle_mod):
+    n = np.array([1.0, 0.3, -0.2])
+    n /= np.linalg.norm(n)
+    u0 = np.array([-0.02, 0.004, 0.003])
+    B0 = np.array([2.0, -1.0, 3.0])
+    v_sh, r, r0 = -0.004, 3.2, np.array([0.5, -0.2, 0.1])
+    dc = cf.background_base(DC(), 0.0, tuple(r0), tuple(u0), tuple(B0), 10.0)
+    dc.insert_vec(tuple(n * 2.0)).insert_double(v_sh).insert_double(r)        # the normal is normalised by the class
+    s = cf.Setup("shock", cf.TRAJ_PARKER, cf.FORWARD)
+    s.add(cf.BACKGROUND, "BackgroundShock", dc)
+    s.add(cf.DIFFUSION, "DiffusionFullConstant", DC().insert_double(1.0).insert_double(1.0))
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((0.0, 0.0, 0.0)))
+    s.add(cf.INITIAL, "InitialMomentumShell", DC().insert_double(1.0))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", 10.0)))
+    o = oracle_mod.Oracle(s)
+    t = 7.0
+    front = r0 + v_sh * t * n                                      # the shock plane has moved by v_sh t
+    tang = np.cross(n, [0.0, 0.0, 1.0])
+    up = front + 0.5 * n + 0.3 * tang
+    dn = front - 0.5 * n + 0.3 * tang
+    Uu, Bu, _, _ = o.background_fields(t, up, 1.0)
+    Ud, Bd, _, _ = o.background_fields(t, dn, 1.0)
+    np.testing.assert_allclose(Uu, u0, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(Bu, B0, rtol=0, atol=1e-15)
+    assert (Ud @ n - v_sh) == pytest.approx((u0 @ n - v_sh) / r, rel=1e-13)     # normal flow in the shock frame
+    np.testing.assert_allclose(Ud - (Ud @ n) * n, u0 - (u0 @ n) * n, rtol=0, atol=1e-15)
+    assert Bd @ n == pytest.approx(B0 @ n, rel=1e-13)
+    np.testing.assert_allclose(Bd - (Bd @ n) * n, r * (B0 - (B0 @ n) * n), rtol=1e-13, atol=1e-13)
+    # mass flux through the moving shock: rho1 (u1n - v_sh) = rho2 (u2n - v_sh) with rho2 = r rho1
+    assert (u0 @ n - v_sh) == pytest.approx(r * (Ud @ n - v_sh), rel=1e-13)
+    # at t = 0 the same downstream point is upstream of the (not yet moved) shock only if v_sh moved it
+    U_then, _, _, _ = o.background_fields(0.0, front - 0.5 * n * abs(v_sh) * t / 2.0 + 0.3 * tang, 1.0)
+    assert np.allclose(U_then, u0) == ((front - 0.5 * n * abs(v_sh) * t / 2.0 - r0) @ n > 0)
+
+
+# --- FlowMomentumPowerLaw / KineticEnergyRadialDistancePowerLaw -----------------------------------------
+def smooth_shock_setup(diffusion):
+    s = cf.dsa_shock()
+    s.plugins = [x for x in s.plugins if x[0] != cf.DIFFUSION]
+    s.add(cf.DIFFUSION, *diffusion)
+    return s
+
+
+def test_flow_momentum_power_law(oracle_mod):
+    kap0, U0, aU, p0, bp, rat = 3.0, 0.01, 0.7, 0.05, 1.3, 0.25
+    dc = DC().insert_double(kap0).insert_double(U0).insert_double(aU).insert_double(p0).insert_double(bp).insert_double(rat)
+    o = oracle_mod.Oracle(smooth_shock_setup(("DiffusionFlowMomentumPowerLaw", dc)))
+    pts = [np.array([x, 0.2, -0.1]) for x in (-10.0, -0.5, 0.0, 0.3, 10.0)]   # across the smooth shock: |U| from u1 to u1/4
+    for x in pts:
+        Um = np.linalg.norm(o.background_fields(0.0, x, p0)[0])
+        for p in (0.5 * p0, p0, 3.0 * p0):
+            kpar = o.kappa_at(1, x, p)
+            assert kpar == pytest.approx(kap0 * (Um / U0) ** aU * (p / p0) ** bp, rel=1e-13)
+            assert o.kappa_at(0, x, p) == pytest.approx(rat * kpar, rel=1e-14)
+    # scaling checks, independent of the normalisation
+    x1, x2 = pts[0], pts[-1]
+    U1, U2 = (np.linalg.norm(o.background_fields(0.0, x, p0)[0]) for x in (x1, x2))
+    assert U1 / U2 == pytest.approx(4.0, rel=1e-6)
+    assert o.kappa_at(1, x1, p0) / o.kappa_at(1, x2, p0) == pytest.approx(4.0 ** aU, rel=1e-6)
+    assert o.kappa_at(1, x1, 2.0 * p0) / o.kappa_at(1, x1, p0) == pytest.approx(2.0 ** bp, rel=1e-13)
+
+
+@pytest.mark.parametrize("stream", [0, 1])
+def test_kinetic_energy_radial_distance_power_law(oracle_mod, stream):
+    kap0, T0, r0, aT, br, rat = 500.0, cf.GeV, 2.0 * AU, 1.0, 2.0, 0.5
+    u_up, w_sh, s_sh = 0.02, 3.0 * AU, 2.5
+    dc = DC().insert_double(kap0).insert_double(T0).insert_double(r0).insert_double(aT).insert_double(br).insert_double(rat)
+    dc.insert_int(stream).insert_double(u_up).insert_double(w_sh).insert_double(s_sh)
+    s = cf.parker_modulation()
+    s.plugins = [x for x in s.plugins if x[0] != cf.DIFFUSION]
+    s.add(cf.DIFFUSION, "DiffusionKineticEnergyRadialDistancePowerLaw", dc)
+    o = oracle_mod.Oracle(s)
+    for r_au in (0.5, 1.9, 2.5, 4.9, 5.1, 20.0):
+        x = np.array([0.6, 0.8, 0.0]) * r_au * AU
+        for T in (0.1 * T0, T0, 7.0 * T0):
+            p = mom_of_ekin(T)
+            kmom = kap0 * (T / T0) ** aT
+            r = r_au * AU
+            if stream == 0 or r < r0:
+                want = kmom * (r / r0) ** br
+            elif r < r0 + w_sh:
+                want = kmom * (V_SW / u_up) ** 2          # |U| of the solar wind
+            else:
+                want = kmom / s_sh ** 2
+            kpar = o.kappa_at(1, x, p)
+            assert kpar == pytest.approx(want, rel=1e-12), (r_au, T)
+            assert o.kappa_at(0, x, p) == pytest.approx(rat * want, rel=1e-12)
+    # kappa(T0, r0) is kap0: the normalisation point
+    assert o.kappa_at(1, np.array([r0, 0.0, 0.0]) * (1 - 1e-15), mom_of_ekin(T0)) == pytest.approx(kap0, rel=1e-12)
+
+
+# --- spectrum weights ------------------------------------------------------------------------------------
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_spectrum_weights_per_trajectory(oracle_mod, kind):
+    """Each trajectory's event: bin of T(p at the start), weight from p at the end (they differ: the
+    wind decelerates the particles), J0 [v] (T/T0)^gamma / [p^2] by DISTRO_KINETIC_ENERGY_POWER_LAW_TYPE."""
+    s = cf.parker_modulation(n_energies=8, t_limit_years=3.0 / 365.0, time_action=0, spectrum_type=kind)
+    o = oracle_mod.Oracle(s)
+    _, rec = o.run(96, seed=21, threads=4)
+    assert (rec["status"] == 0).all()
+    p0, p1 = rec["mom0"][:, 0], rec["mom"][:, 0]
+    assert (np.abs(p1 / p0 - 1.0) > 1e-6).all()               # the momentum changed along every trajectory
+    J0, T0, g = 1.0, cf.GeV, -2.7
+    lo, hi = cf.spectrum_bins(0.1 * cf.GeV, 10.0 * cf.GeV, 8)
+    inner = rec["n_events"] == 0                               # absorbed by the inner sphere (action -1): no event
+    assert (np.linalg.norm(rec["pos"][inner], axis=1) < 0.01 * AU).all() and inner.sum() < len(rec) // 4
+    for k in np.flatnonzero(~inner):
+        T1 = ekin(p1[k])
+        w = J0 * (T1 / T0) ** g
+        if kind == 0:
+            w *= speed(p1[k]) / p1[k] ** 2
+        elif kind == 1:
+            w /= p1[k] ** 2
+        assert rec["ev_weight"][k, 0] == pytest.approx(w, rel=1e-13)
+        Tstart = ekin(p0[k])
+        want_bin = int((math.log10(Tstart) - math.log10(lo)) / ((math.log10(hi) - math.log10(lo)) / 8))
+        assert rec["ev_bin"][k, 0] == want_bin
+    # the distribution's sums are the weights of the events in each bin
+    c, sums, _ = o.distro(0)
+    for b in range(8):
+        sel = rec["ev_bin"][:, 0] == b
+        assert c[b] == sel.sum()
+        assert sums[b] == pytest.approx(math.fsum(rec["ev_weight"][sel, 0]), rel=1e-13, abs=0)
+
+
+# --- source amplitude -----------------------------------------------------------------------------------
+def test_source_amplitude_uses_the_previous_step(oracle_mod):
+    """Radial wind, no diffusion, dmax = r / 10: every step is limited by dlnp_max, dt_k = 0.015 / div U(r_k),
+    so p_{k+1} = 0.995 p_k and r_{k+1} = r_k + V dt_k.  The injection momentum lies between p_3 and p_4; the
+    source sees the sign change at the start of step 4 and adds rate * dt_4 / dt_3 = rate * r_4 / r_3."""
+    x0 = AU
+    p0 = mom_of_ekin(10.0 * cf.MeV)
+    p_inj = p0 * 0.995 ** 3.5
+    rate = 2.5
+    s = cf.Setup("source", cf.TRAJ_PARKER_SOURCE, cf.FORWARD)
+    s.add(cf.BACKGROUND, "BackgroundSolarWind", cf.solarwind(dmax0=1.0e9))
+    s.add(cf.DIFFUSION, "DiffusionFullConstant", DC().insert_double(0.0).insert_double(0.0))
+    s.add(cf.SOURCE, "SourceMomentumInjection", DC().insert_double(p_inj).insert_double(rate))
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((x0, 0.0, 0.0)))
+    s.add(cf.INITIAL, "InitialMomentumShell", DC().insert_double(p0))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [-1], ("double", 1.0e9)))
+    s.add(cf.BOUNDARY, "BoundarySphereAbsorb", cf.boundary(1, [0], ("vec", (0.0, 0.0, 0.0)), ("double", 1.05 * x0)))
+    s.add(cf.DISTRIBUTION, "DistributionTimeUniform", time_dist(1.0e9))
+    rec = one_step(oracle_mod, s, n_normals=64)
+    assert rec["status"][0] == 0
+    n = int(rec["steps"][0])
+    r = [x0]
+    for _ in range(n):
+        r.append(r[-1] + V_SW * 0.015 / (2.0 * V_SW / r[-1]))
+    assert n == next(k for k in range(1, 20) if r[k] > 1.05 * x0)
+    assert rec["pos"][0, 0] == pytest.approx(r[n], rel=1e-6)
+    assert rec["mom"][0, 0] == pytest.approx(p0 * 0.995 ** n, rel=1e-6)
+    assert rec["amp"][0] == pytest.approx(rate * r[4] / r[3], rel=1e-6)
+    assert abs(rec["amp"][0] / rate - 1.0) > 5e-3                   # not rate * dt_now / dt_now
+
+
+# --- WNLT pitch-angle scattering ---------------------------------------------------------------------
+def euler_2f1_over(c, xi):
+    """2F1(1, 1; c; 1/(1 + xi^2)) / (1 + xi^2) from Euler's integral: (c - 1) int_0^1 (1 - t)^(c - 2) / (1 + xi^2 - t) dt."""
+    val, _ = integrate.quad(lambda t: 1.0 / (1.0 + xi * xi - t), 0.0, 1.0, weight="alg", wvar=(0.0, c - 2.0), epsabs=0, epsrel=1e-13, limit=200)
+    return (c - 1.0) * val
+
+
+@pytest.mark.parametrize("q", [5.0 / 3.0, 1.4, 2.5])
+def test_wnlt_dmumu_by_quadrature(oracle_mod, q):
+    A2A, l_max, A2T, A2L = 0.05, 0.01 * AU, 0.1, 0.2
+    Bm = 15.0
+    cls, base = "DiffusionQLTConstant", DC().insert_double(A2A).insert_double(l_max).insert_double(q)
+    full = DC().insert_double(A2A).insert_double(l_max).insert_double(q).insert_double(A2T).insert_double(A2L)
+
+    def oracle(cls, dc):
+        s = cf.Setup("wnlt", cf.TRAJ_GUIDING_SCATT, cf.FORWARD)
+        s.add(cf.BACKGROUND, "BackgroundUniform", cf.background_base(DC(), 0.0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, Bm), 1.0))
+        s.add(cf.DIFFUSION, cls, dc)
+        s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+        s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((0.0, 0.0, 0.0)))
+        s.add(cf.INITIAL, "InitialMomentumShell", DC().insert_double(1.0))
+        s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", 1.0)))
+        s.add(cf.DISTRIBUTION, "DistributionTimeUniform", time_dist(1.0))
+        return oracle_mod.Oracle(s)
+
+    qlt, wnlt = oracle(cls, base), oracle("DiffusionWNLTConstant", full)
+    k_min = 2.0 * math.pi / l_max
+    c = (5.0 + q) / 4.0
+    for T_MeV in (2.0, 20.0, 300.0):
+        p = mom_of_ekin(T_MeV * cf.MeV)
+        v = speed(p)
+        gamma = energy(p) / (M * C * C)
+        Om = Q * Bm / (gamma * M * C)
+        for mu in (-0.95, -0.4, 0.1, 0.6, 0.99):
+            st2 = 1.0 - mu * mu
+            d_qlt = 0.25 * math.pi * (q - 1) * Om * st2 * (v * k_min * abs(mu / Om)) ** (q - 1) * A2A
+            assert qlt.kappa_at(2, (0.0, 0.0, 0.0), p, mu) == pytest.approx(d_qlt, rel=1e-12)
+            kperp = v * math.sqrt(0.5 * (q - 1) / (q + 1) * (mu / k_min) ** 2 * A2T + 0.125 * (v * st2 / Om) ** 2 * A2L)
+            assert wnlt.kappa_at(0, (0.0, 0.0, 0.0), p, mu) == pytest.approx(kperp, rel=1e-12)
+            xi1 = v * k_min * math.sqrt(st2) / math.sqrt(2.0) / Om
+            xi2 = kperp * k_min ** 2 / Om
+            DT1, DT2 = euler_2f1_over(c, xi1), euler_2f1_over(c, xi2)
+            extra = (q - 1) / (q + 1) * Om * st2 * A2T * DT1 * DT2 / math.hypot(DT1, DT2)
+            got = wnlt.kappa_at(2, (0.0, 0.0, 0.0), p, mu)
+            assert got == pytest.approx(d_qlt + extra, rel=1e-9), (T_MeV, mu)
+            assert extra > 1e-3 * d_qlt                              # the nonlinear term is not negligible here
