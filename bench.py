@@ -56,9 +56,14 @@ def gpu_leg(E, cf, prec, args, setup=None, n_traj=N_ENERGIES * PER_ENERGY, autot
     tuned = None
     if autotune:
         table = sim.autotune(n_probe=n_traj, seed=2)
-        best = max(table, key=lambda e: e[2])
-        tuned = {"lanes": best[0], "steps_per_launch": best[1], "candidates": len(table)}
-        log(f"{s.name} {prec}: autotuned lanes {best[0]} steps/launch {best[1]} ({best[2]:.3e} steps/s in the probe)")
+        plain = [e for e in table if e[3] == 0]
+        best = max(plain, key=lambda e: e[2])
+        graph = next((e for e in table if e[3] > 0), None)
+        use_graph = bool(graph and graph[2] > 1.02 * best[2])      # the autotuner's own rule
+        tuned = {"lanes": best[0], "steps_per_launch": best[1], "candidates": len(plain),
+                 "graph_vs_plain": round(graph[2] / best[2], 4) if graph else None, "graph": use_graph}
+        log(f"{s.name} {prec}: autotuned lanes {best[0]} steps/launch {best[1]} ({best[2]:.3e} steps/s in the probe); "
+            f"as a HIP graph {tuned['graph_vs_plain']}x -> graph {'on' if use_graph else 'off'}")
         spl = best[1]
     # a shorter launch gets proportionally more timed launches, so every leg times about as many steps per slot
     launches = max(args.steps, (args.steps * args.steps_per_launch) // spl)
@@ -70,7 +75,9 @@ def gpu_leg(E, cf, prec, args, setup=None, n_traj=N_ENERGIES * PER_ENERGY, autot
     steps = st.steps_executed - w.steps_executed
     log(f"{s.name} {prec}: lanes {st.lanes} steps/launch {spl} launches {launches} trajectory-steps {steps} wall {wall:.3f}s "
         f"step-kernel {st.step_kernel_ms:.1f} ms service {st.service_kernel_ms:.1f} ms -> {steps / wall:.4e} steps/s")
-    return dict(steps=steps, wall=wall, lanes=st.lanes, step_ms=st.step_kernel_ms, service_ms=st.service_kernel_ms,
+    # under graph replay the kernels are not timed one by one: the launch time (service + step) stands in
+    step_ms = st.step_kernel_ms if st.step_kernel_ms > 0 else st.kernel_ms
+    return dict(steps=steps, wall=wall, lanes=st.lanes, step_ms=step_ms, service_ms=st.service_kernel_ms,
                 done=st.done, failed=st.failed, tuned=tuned, launches=launches, steps_per_launch=spl)
 
 
@@ -132,7 +139,7 @@ def grid_workloads(E, cf, args, traffic_all):
         name = f"parker_modulation_cartesian_{nb}^3_blocks"
         res = {"zones": 4 * nb, "zone_records_bytes": {"fp64": 64 * (4 * nb) ** 3, "mixed": 32 * (4 * nb) ** 3}}
         for p in (("fp64", "mixed") if args.precision == "both" else (args.precision,)):
-            leg = gpu_leg(E, cf, p, args, setup, 10_000_000, autotune=args.autotune and not args.lanes,
+            leg = gpu_leg(E, cf, p, args, setup, 10_000_000, autotune=False,
                           lanes=grid_lanes(E, args), steps_per_launch=args.grid_steps_per_launch)
             per_launch_s = leg["step_ms"] * 1e-3 / leg["launches"]
             steps_per_launch = leg["steps"] / leg["launches"]
@@ -154,6 +161,23 @@ def grid_workloads(E, cf, args, traffic_all):
     return out
 
 
+def host_cpus():
+    """CPU share of this process (affinity mask, capped by OMP_NUM_THREADS when the environment sets it), the
+    machine's logical CPU count (nproc) and the processor model (lscpu)."""
+    share = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        share = min(share, int(omp))
+    model = ""
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        model = next((l.split(":", 1)[1].strip() for l in out.splitlines() if l.startswith("Model name")), "")
+    except Exception:
+        pass
+    return share, os.cpu_count() or 0, model
+
+
 def cpu_leg(cf, seconds, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -168,16 +192,27 @@ def cpu_leg(cf, seconds, threads):
         n += batch
         first += batch
     wall = time.perf_counter() - t0
+    share, nproc, model = host_cpus()
     log(f"cpu: {n} trajectories, {steps} steps in {wall:.2f}s on {threads} threads -> {steps / wall:.4e} steps/s")
     return {"value": steps / wall, "unit": "trajectory-steps/s", "cores": threads, "kind": "port",
             "sample": f"{n} whole trajectories (ids 0..{n - 1} of the same workload, seed 1), FP64 oracle, "
-                      f"OpenMP over trajectories, {wall:.1f} s"}
+                      f"OpenMP over trajectories, {wall:.1f} s",
+            "host": {"cpu_share": share, "nproc": nproc, "model": model}}
+
+
+def pct(a, q):
+    import numpy as np
+    return round(float(np.percentile(a, q)), 4) if len(a) else None
 
 
 def serve_leg(E, cf, args):
-    """A served stream: Poisson arrivals of modulation-spectrum queries (radius U[1, 80] au, 8-64
-    energies, log-uniform trajectories per energy) with 20% diffusive-shock-acceleration queries,
-    submitted at their arrival times to a query server running the mixed kernel."""
+    """The served mixed stream: Poisson arrivals (mean rate --serve-rate) of modulation-spectrum queries
+    (start radius U[1, 80] au, 8-64 energies, log-uniform 1e3-1e5 trajectories per energy times
+    --serve-scale) with 20% diffusive-shock-acceleration queries, submitted at their arrival times to a query
+    server running the mixed kernel.  Each query's latency is split into queue wait (submission to its first
+    trajectory on the device) and service (first trajectory to the last).  The offered load is the arrival
+    rate times the measured work per query over the engine's measured rate: above 1 the queue only grows, and
+    latency is set by the query's place in it."""
     import numpy as np
     from spectrum_amd.server import Server
     rng = np.random.default_rng(args.serve_seed)
@@ -204,21 +239,72 @@ def serve_leg(E, cf, args):
             if delay > 0:
                 time.sleep(delay)
             ids.append(srv.submit(proto, n, seed=seed))
-        for q in ids:
+        for k, q in enumerate(ids):
             srv.wait(q)
+            log(f"serve: query {k + 1}/{len(ids)} done")
         makespan = time.perf_counter() - t0
         info = [srv.info(q) for q in ids]
         st = srv.stats()
     lat = np.array([i["latency_s"] for i in info])
-    steps = sum(i["steps_executed"] for i in info)
+    wait = np.array([i["t_start"] - i["t_submit"] for i in info])
+    service = np.array([i["t_done"] - i["t_start"] for i in info])
+    steps = np.array([i["steps_executed"] for i in info], dtype=float)
     n_traj = sum(p[2] for p in plan)
-    log(f"serve: {len(ids)} queries ({sum(p[4] == 'dsa' for p in plan)} DSA), {n_traj} trajectories, {steps} steps in {makespan:.2f}s; "
-        f"latency p50 {np.percentile(lat, 50):.3f}s p99 {np.percentile(lat, 99):.3f}s; engine busy {st['busy_s']:.2f}s")
+    engine_rate = steps.sum() / max(st["busy_s"], 1e-9)
+    capacity_qps = engine_rate / steps.mean()
+    offered = args.serve_rate / capacity_qps
+    log(f"serve: {len(ids)} queries ({sum(p[4] == 'dsa' for p in plan)} DSA), {n_traj} trajectories, {int(steps.sum())} steps in {makespan:.2f}s; "
+        f"latency p50 {np.percentile(lat, 50):.3f}s p99 {np.percentile(lat, 99):.3f}s (wait p50 {np.percentile(wait, 50):.3f}s, "
+        f"service p50 {np.percentile(service, 50):.3f}s); engine busy {st['busy_s']:.2f}s; capacity {capacity_qps:.3f} queries/s, "
+        f"offered load {offered:.1f}x capacity")
     return {"queries": len(ids), "dsa_queries": int(sum(p[4] == "dsa" for p in plan)), "arrival_rate_hz": args.serve_rate,
             "trajectories": int(n_traj), "trajectories_per_energy": [round(1e3 * args.serve_scale, 1), round(1e5 * args.serve_scale, 1)],
-            "p50_latency_s": round(float(np.percentile(lat, 50)), 4), "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
-            "mean_latency_s": round(float(lat.mean()), 4), "makespan_s": round(makespan, 3),
-            "trajectory_steps_per_s": steps / makespan, "precision": "mixed", "steps_per_launch": args.serve_steps_per_launch}
+            "p50_latency_s": pct(lat, 50), "p99_latency_s": pct(lat, 99), "max_latency_s": round(float(lat.max()), 4),
+            "p99_is_the_max": len(lat) < 100, "mean_latency_s": round(float(lat.mean()), 4),
+            "queue_wait_s": {"p50": pct(wait, 50), "p99": pct(wait, 99)}, "service_s": {"p50": pct(service, 50), "p99": pct(service, 99)},
+            "mean_steps_per_query": float(steps.mean()), "engine_steps_per_s_busy": engine_rate, "capacity_queries_per_s": capacity_qps,
+            "offered_load_vs_capacity": round(offered, 2), "makespan_s": round(makespan, 3), "engine_busy_s": round(st["busy_s"], 3),
+            "trajectory_steps_per_s": float(steps.sum()) / makespan, "precision": "mixed", "steps_per_launch": args.serve_steps_per_launch}
+
+
+def drain_leg(E, cf, args):
+    """Time to solution: --drain-traj trajectories of the headline workload queued and run to completion with
+    the mixed kernel, one engine launch at a time.  The lane occupancy of each launch (steps executed over
+    slots x steps per launch) shows the steady state while the queue feeds the slots and the drain once it is
+    empty, when the longest trajectories run with most slots idle."""
+    import numpy as np
+    spl = args.steps_per_launch
+    sim = E.Simulation.from_setup(cf.parker_modulation(n_energies=N_ENERGIES), precision=PREC_ID["mixed"], steps_per_launch=spl)
+    n = args.drain_traj
+    sim.begin(n, seed=3)
+    t0 = time.perf_counter()
+    trace, last = [], 0
+    done = 0
+    while True:
+        st = sim.advance(1)
+        now = time.perf_counter() - t0
+        trace.append((now, st.done, st.steps_executed - last, st.lanes))
+        last = st.steps_executed
+        done = st.done
+        if done >= n or now > args.drain_seconds:
+            break
+    wall = time.perf_counter() - t0
+    occ = np.array([k / (l * spl) for _, _, k, l in trace])
+    tt = np.array([x[0] for x in trace])
+    dn = np.array([x[1] for x in trace])
+    queue_empty = next((i for i, x in enumerate(trace) if x[1] >= n - st.lanes), len(trace) - 1)
+    t_at = lambda f: float(tt[np.argmax(dn >= f * n)]) if (dn >= f * n).any() else None
+    res = {"trajectories": n, "completed": int(done), "wall_s": round(wall, 3), "launches": len(trace),
+           "trajectory_steps_per_s": float(last) / wall, "steps_per_launch": spl, "lanes": int(trace[-1][3]),
+           "time_to_50pct_s": t_at(0.5), "time_to_99pct_s": t_at(0.99), "time_to_all_s": round(wall, 3) if done >= n else None,
+           "occupancy_mean_until_queue_empty": round(float(occ[:queue_empty + 1].mean()), 4),
+           "occupancy_mean_drain": round(float(occ[queue_empty + 1:].mean()), 4) if queue_empty + 1 < len(occ) else None,
+           "drain_s": round(float(tt[-1] - tt[queue_empty]), 3),
+           "occupancy_trace": [[round(float(a), 3), round(float(b), 4)] for a, b in zip(tt[:: max(1, len(tt) // 40)], occ[:: max(1, len(tt) // 40)])]}
+    log(f"drain: {done}/{n} trajectories in {wall:.2f}s, {res['trajectory_steps_per_s']:.4e} steps/s overall; occupancy "
+        f"{res['occupancy_mean_until_queue_empty']} while the queue feeds the slots, {res['occupancy_mean_drain']} over the last "
+        f"{res['drain_s']} s")
+    return res
 
 
 def copy_probe(mib):
@@ -240,7 +326,7 @@ def main():
     ap.add_argument("--steps-per-launch", type=int, default=2000)
     ap.add_argument("--lanes", type=int, default=0, help="resident trajectory slots (0 = engine default)")
     ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=30.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-other-workloads", action="store_true", help="skip the DSA and guiding-centre throughput legs")
     ap.add_argument("--no-grid-workloads", action="store_true", help="skip the gridded-background (Cartesian) legs")
@@ -252,12 +338,14 @@ def main():
                     help="steps per slot per launch of the gridded legs (the autotuner's choice for them)")
     ap.add_argument("--grid-only", action="store_true",
                     help="run only the gridded-background legs and print their results (for profiling their kernels)")
-    ap.add_argument("--autotune", action="store_true",
-                    help="autotune the modulation legs' launch shape first (the kernel trace then also holds the probe launches)")
-    ap.add_argument("--serve-queries", type=int, default=40, help="served-stream leg: number of queries (0 = skip)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the autotuner on the modulation legs (for profiling: the trace then holds the timed launches only)")
+    ap.add_argument("--serve-queries", type=int, default=16, help="served-stream leg: number of queries (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="mean query arrival rate (1/s)")
-    ap.add_argument("--serve-scale", type=float, default=0.01,
-                    help="trajectories per energy are log-uniform in [1e3, 1e5] times this factor")
+    ap.add_argument("--serve-scale", type=float, default=1.0,
+                    help="trajectories per energy are log-uniform in [1e3, 1e5] times this factor (1: the workload's sizes)")
+    ap.add_argument("--drain-traj", type=int, default=1_000_000, help="time-to-solution leg: trajectories run to completion (0 = skip)")
+    ap.add_argument("--drain-seconds", type=float, default=90.0, help="time-to-solution leg: stop after this long")
     ap.add_argument("--serve-steps-per-launch", type=int, default=500)
     ap.add_argument("--serve-seed", type=int, default=7)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -289,7 +377,7 @@ def main():
     precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
     legs = {}
     for p in precs:
-        legs[p] = gpu_leg(E, cf, p, args, autotune=args.autotune and not args.lanes)
+        legs[p] = gpu_leg(E, cf, p, args, autotune=not args.no_autotune and not args.lanes)
     main_prec = "mixed" if "mixed" in legs else "fp64"
     m = legs[main_prec]
     value = m["steps"] / m["wall"]
@@ -322,12 +410,13 @@ def main():
         out["other_workloads"] = other_workloads(E, cf, args, flops_all)
     if not args.no_grid_workloads:
         out["grid_workloads"] = grid_workloads(E, cf, args, traffic_grid)
+    if args.drain_traj > 0:
+        out["time_to_solution"] = drain_leg(E, cf, args)
     if args.serve_queries > 0:
         out["serve"] = serve_leg(E, cf, args)
         out["p99_query_latency_s"] = out["serve"]["p99_latency_s"]
     if not args.no_cpu:
-        threads = max(1, min(16, len(os.sched_getaffinity(0))))
-        out["cpu_baseline"] = cpu_leg(cf, args.cpu_seconds, threads)
+        out["cpu_baseline"] = cpu_leg(cf, args.cpu_seconds, max(1, host_cpus()[0]))
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -111,7 +111,10 @@ int spx_set_background_grid(spx_sim* sim, const int32_t n_blocks[3], const doubl
 /* Run-time replacement of compile-time switches and engine knobs:
    "DISTRO_KINETIC_ENERGY_POWER_LAW_TYPE" (src/distribution_other.hh:325), "max_trajectory_steps"
    (src/trajectory_base.hh:38), "max_attempts", "precision" (SPX_PREC_*), "block", "lanes",
-   "steps_per_launch", "graph_launches" (0 = no graph capture). */
+   "steps_per_launch", "graph_launches" (0 = no graph capture), and "step_scale" in (0, 1]: a factor on
+   the physical time step of the Parker transports (TrajectoryParker::PhysicalStep's CFL and dlnpmax
+   constants, src/trajectory_parker.hh:32-41; 1 = the reference's step), for time-step convergence
+   studies of the Euler-Maruyama estimate. */
 int spx_set_option(spx_sim* sim, const char* name, double value);
 
 /* SetTasks + MainLoop (src/simulation.cc:509-519, :818-835): integrate trajectories
@@ -127,11 +130,13 @@ int spx_finish(spx_sim* sim, spx_traj_record* records, spx_run_stats* stats);
 /* Persistent-engine autotuner (no counterpart in the reference, whose MPI workers have no launch
    shape): times a short probe run of `n_probe` trajectories of this configuration for resident slots of
    1, 2, 3, 4, 6 and 8 waves per SIMD and 250 / 1000 / 4000 steps per launch, keeps the fastest as the
-   simulation's "lanes" and "steps_per_launch" options, and resets the distributions.  `table` (may be
-   NULL) receives up to `max_entries` measurements. */
+   simulation's "lanes" and "steps_per_launch" options, then times that shape replayed as a HIP graph and
+   sets "graph_launches" to 2 when the graph is more than 2% faster (0 otherwise); resets the
+   distributions.  `table` (may be NULL) receives up to `max_entries` measurements (19 are made). */
 typedef struct {
    int64_t lanes;
-   int32_t steps_per_launch, pad_;
+   int32_t steps_per_launch;
+   int32_t graph_launches;     /* 0: plain launches; 2: the shape replayed as a HIP graph of two launches */
    double steps_per_s;
 } spx_tune_entry;
 int spx_autotune(spx_sim* sim, uint64_t n_probe, uint64_t seed, spx_tune_entry* table, int max_entries, int* n_entries);
@@ -206,6 +211,7 @@ typedef struct {
    int32_t pad_;
    uint64_t n_traj, done, failed, steps_done, steps_executed;
    double t_submit, t_admit, t_done;   /* seconds since the server started (-1: not yet) */
+   double t_start;             /* end of the launch round in which its first trajectory started (-1: not yet) */
    double latency_s;           /* t_done - t_submit (-1 until done) */
    int64_t footprint_bytes;    /* device memory charged against the budget */
 } spx_query_info;

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 FLOPS_LIB_PATH = os.path.join(HERE, "liboracle_flops.so")   # same source, operation-counting scalar
 LIBM_LIB_PATH = os.path.join(HERE, "liboracle_libm.so")     # same source, pow / cbrt one ulp larger
 FMA_LIB_PATH = os.path.join(HERE, "liboracle_fma.so")       # same source, multiply-adds contracted (last-bit perturbation)
+COMP_LIB_PATH = os.path.join(HERE, "liboracle_comp.so")     # same source, Kahan-compensated sums of t, x, p in the Parker step
 
 RNG_COUNTER, RNG_REPLAY = 0, 1
 EX_NAMES = {0: "none", 1: "field", 2: "step_too_small", 3: "step_nan", 4: "max_steps", 5: "replay_exhausted", 6: "boundary"}

@@ -331,7 +331,7 @@ enum { DF_NONE = 0, DF_ISO, DF_PARA, DF_PERP, DF_FULL, DF_FLOWMOM, DF_KINRAD, DF
        DF_WNLT, DF_WNLT_RAMP };
 enum { BK_TIME = 0, BK_SPHERE, BK_PLANE, BK_MOMENTUM };
 enum { IT_FIXED = 0 };
-enum { IS_FIXED = 0 };
+enum { IS_FIXED = 0, IS_SPHERE = 1, IS_SECTOR = 2 };
 enum { IM_FIXED = 0, IM_SHELL, IM_THICKSHELL, IM_SHELLGRID, IM_RING };
 enum { DS_SPECTRUM = 0, DS_TIME, DS_MOMENTUM, DS_POSMOM, DS_CUMO1, DS_CUMO2, DS_POSPITCH };
 enum { SRC_NONE = 0, SRC_MOMINJ };
@@ -386,12 +386,13 @@ struct orc_sim {
    df_t df; int have_df;
    int src_kind; double src_p_inj, src_rate;
    int it_kind; double it_t;
-   int is_kind; vec3 is_pos;
+   int is_kind; vec3 is_pos; double is_radius, is_ct1, is_ct2, is_phi1, is_phi2;
    int im_kind; double im_p0, im_p1, im_p2; int im_log; vec3 im_mom; double im_T1, im_T2; int im_n; double im_mu0, im_st0;
    int n_t, n_s, n_m; bnd_t bt[MAX_BND], bs[MAX_BND], bm[MAX_BND];
    int n_dist; dist_t dist[MAX_DIST];
    int spectrum_type;          /* DISTRO_KINETIC_ENERGY_POWER_LAW_TYPE, src/distribution_other.hh:325 */
    int64_t max_steps;          /* max_trajectory_steps, src/trajectory_base.hh:38 */
+   double dt_scale;            /* "step_scale": factor on the Parker physical step (1 = the reference) */
    int max_attempts;
    int64_t discards[8];
    grid_t grid; int have_grid;
@@ -1096,7 +1097,33 @@ typedef struct {
    int n_events; int ev_bin[MAX_DIST]; double ev_w[MAX_DIST][9]; int ev_has[MAX_DIST];
    /* histogram sinks of the calling thread: events are binned when they happen */
    int64_t** hc; double** hs; int64_t* hev;
+   double ct, cp; vec3 cpos;   /* ORC_COMPENSATED: low-order bits lost by the sums of t, p and x */
 } traj_t;
+
+/* x += inc.  The reference adds in plain FP64 (src/trajectory_base.cc:378-389); the ORC_COMPENSATED
+   build (liboracle_comp.so) keeps Kahan's compensation term instead, which measures what the plain sums
+   lose over a whole trajectory. */
+static inline void acc_add(double* x, double* c, double inc)
+{
+#ifdef ORC_COMPENSATED
+   double y = inc - *c, t = *x + y;
+   *c = (t - *x) - y;
+   *x = t;
+#else
+   (void)c;
+   *x += inc;
+#endif
+}
+/* x -= dec, likewise */
+static inline void acc_sub(double* x, double* c, double dec)
+{
+#ifdef ORC_COMPENSATED
+   acc_add(x, c, -dec);
+#else
+   (void)c;
+   *x -= dec;
+#endif
+}
 
 #define EX(code) do { return (code); } while (0)
 
@@ -1252,7 +1279,7 @@ static int parker_advance(const orc_sim* s, lane_t* L, traj_t* T)
    T->dt_physical = 0.5 * sp->dmax / (fwd ? vnorm(vadd(drift, divK)) : vnorm(vsub(drift, divK)));
    T->dt_physical = fmin(T->dt_physical, 0.5 * Sqr(sp->dmax) / fmax(Kperp, Kpara));
    T->dt_physical = fmin(T->dt_physical, 0.5 * 3.0 * 0.01 / fabs(divU(sp)));
-   T->dt = fmin(T->dt_physical, T->dt_adaptive);
+   T->dt = fmin(s->dt_scale == 1.0 ? T->dt_physical : T->dt_physical * s->dt_scale, T->dt_adaptive);
    time_boundary_proximity(s, T);
 
    /* EulerDiffSlopes (src/trajectory_parker.cc:159-180) */
@@ -1272,18 +1299,18 @@ static int parker_advance(const orc_sim* s, lane_t* L, traj_t* T)
    /* RKStep, Euler (src/trajectory_base.cc:362-389) */
    double dtv = T->dt * 1.0;
    if (fwd) {
-      T->t += T->dt;
-      T->pos = vadd(T->pos, vscl(dtv, slope_pos));
-      T->mom.v[0] += dtv * slope_mom;
+      acc_add(&T->t, &T->ct, T->dt);
+      for (i = 0; i < 3; i++) acc_add(&T->pos.v[i], &T->cpos.v[i], dtv * slope_pos.v[i]);
+      acc_add(&T->mom.v[0], &T->cp, dtv * slope_mom);
    }
    else {
-      T->t -= T->dt;
-      T->pos = vsub(T->pos, vscl(dtv, slope_pos));
-      T->mom.v[0] -= dtv * slope_mom;
+      acc_sub(&T->t, &T->ct, T->dt);
+      for (i = 0; i < 3; i++) acc_sub(&T->pos.v[i], &T->cpos.v[i], dtv * slope_pos.v[i]);
+      acc_sub(&T->mom.v[0], &T->cp, dtv * slope_mom);
    }
    T->amp += dtv * slope_amp * exp(T->wgt);
    T->wgt += dtv * 0.0;
-   T->pos = vadd(T->pos, dr);
+   for (i = 0; i < 3; i++) acc_add(&T->pos.v[i], &T->cpos.v[i], dr.v[i]);
 
    handle_boundaries(s, T);
    if (!(T->status & TRAJ_FINISH)) {
@@ -1637,6 +1664,21 @@ static int fc_advance(const orc_sim* s, lane_t* L, traj_t* T)
 }
 
 /* Initial momentum sample (src/initial_momentum.cc:283-410) */
+/* InitialSpaceFixed / InitialSpaceSphere / InitialSpaceSphereSector::EvaluateInitial (src/initial_space.cc:49-66,
+   :308-316, :384-392): the polar cosine uniform between cos(theta1) and cos(theta2), then the azimuth uniform
+   between phi1 and phi2 (the full sphere: -1 + 2 u and 2 pi u) */
+static vec3 pos_sample(const orc_sim* s, lane_t* L)
+{
+   if (s->is_kind == IS_FIXED) return s->is_pos;
+   double ct, st, phi;
+   if (s->is_kind == IS_SPHERE) ct = -1.0 + 2.0 * draw_uniform(L);
+   else ct = s->is_ct1 + (s->is_ct2 - s->is_ct1) * draw_uniform(L);
+   st = sqrt(1.0 - Sqr(ct));
+   if (s->is_kind == IS_SPHERE) phi = M_2PI_ORC * draw_uniform(L);
+   else phi = s->is_phi1 + (s->is_phi2 - s->is_phi1) * draw_uniform(L);
+   return vadd(s->is_pos, V(s->is_radius * st * cos(phi), s->is_radius * st * sin(phi), s->is_radius * ct));
+}
+
 static vec3 mom_sample(const orc_sim* s, lane_t* L)
 {
    int gc = ORC_IS_GC(s->traj_type), fc = (s->traj_type == ORC_TRAJ_FOCUSED);
@@ -1692,7 +1734,7 @@ static int integrate_one(const orc_sim* s, lane_t* L, traj_t* T)
    memset(T, 0, sizeof(*T));
    T->hc = hc; T->hs = hs; T->hev = hev;
    T->t = s->it_t;
-   T->pos = s->is_pos;
+   T->pos = pos_sample(s, L);
    T->mom = mom_sample(s, L);
    /* hidden state of BackgroundSmoothShock::ds_shock: taken at the starting point */
    if (s->bg.kind == BG_SMOOTHSHOCK)
@@ -1742,6 +1784,7 @@ orc_sim* orc_create(int traj_type, int time_flow)
    s->time_flow = time_flow;
    s->spectrum_type = 0;
    s->max_steps = 10000000;
+   s->dt_scale = 1.0;
    s->max_attempts = 16;
    return s;
 }
@@ -1796,6 +1839,7 @@ int orc_set_option(orc_sim* s, const char* name, double value)
 {
    if (!strcmp(name, "DISTRO_KINETIC_ENERGY_POWER_LAW_TYPE")) { s->spectrum_type = (int)value; return 0; }
    if (!strcmp(name, "max_trajectory_steps")) { s->max_steps = (int64_t)value; return 0; }
+   if (!strcmp(name, "step_scale")) { if (!(value > 0.0 && value <= 1.0)) return -1; s->dt_scale = value; return 0; }
    if (!strcmp(name, "max_attempts")) { s->max_attempts = (int)value; return 0; }
    return -1;
 }
@@ -1928,6 +1972,19 @@ static int setup_initial(orc_sim* s, const char* cls, dcread* r)
 {
    if (!strcmp(cls, "InitialTimeFixed")) { s->it_kind = IT_FIXED; s->it_t = rd_d(r); }
    else if (!strcmp(cls, "InitialSpaceFixed")) { s->is_kind = IS_FIXED; s->is_pos = rd_v(r); }
+   else if (!strcmp(cls, "InitialSpaceSphere")) {   /* src/initial_space.cc:297-316 */
+      s->is_kind = IS_SPHERE; s->is_pos = rd_v(r); s->is_radius = rd_d(r);
+      s->is_ct1 = 1.0; s->is_ct2 = -1.0; s->is_phi1 = 0.0; s->is_phi2 = M_2PI_ORC;
+   }
+   else if (!strcmp(cls, "InitialSpaceSphereSector")) {   /* src/initial_space.cc:353-378 */
+      double th1, th2, ph1, ph2;
+      s->is_kind = IS_SECTOR; s->is_pos = rd_v(r); s->is_radius = rd_d(r);
+      th1 = rd_d(r); if (th1 < 0.0) th1 = 0.0;
+      th2 = rd_d(r); if (th2 > M_PI) th2 = M_PI; if (th2 < th1) th2 = th1;
+      ph1 = rd_d(r);
+      ph2 = rd_d(r); if (ph2 > M_2PI_ORC) ph2 = M_2PI_ORC; if (ph2 < ph1) ph2 = ph1;
+      s->is_ct1 = cos(th1); s->is_ct2 = cos(th2); s->is_phi1 = ph1; s->is_phi2 = ph2;
+   }
    else if (!strcmp(cls, "InitialMomentumFixed")) { s->im_kind = IM_FIXED; s->im_mom = rd_v(r); }
    else if (!strcmp(cls, "InitialMomentumShell")) { s->im_kind = IM_SHELL; s->im_p0 = rd_d(r); }
    else if (!strcmp(cls, "InitialMomentumRing")) {

@@ -298,6 +298,37 @@ def parker_modulation_cartesian(grid, n_bins=100, T1_MeV=10.0, T2_MeV=5000.0, in
     return s
 
 
+def parker_modulation_radial(n_energies=8, T1_MeV=10.0, T2_MeV=5000.0, inner_au=0.05, outer_au=80.0, t_limit_years=100.0,
+                             spectrum_type=0, name="parker_modulation_radial") -> Setup:
+    """The physics of the reference's Cartesian modulation benchmark (benchmarks/main_test_modulation_cartesian_parker.cc)
+    on the analytic solar wind instead of its sampled grid: backward Parker, radial wind V = 400 km/s,
+    isotropic kappa = 1.5e22 cm^2/s (T / 1 GeV) (r / 1 au)^2, absorbing spheres at 0.05 au (cold, 0) and
+    80 au (hot, J0 [v] (T / 1 GeV)^-1.8 [/ p^2]), from 1 au at n log-spaced kinetic energies, and a time limit
+    long enough that no trajectory reaches it (not binned).  With kappa isotropic the magnetic field drops
+    out, so the spectrum at 1 au is the solution of the spherically symmetric Parker equation."""
+    s = Setup(name, TRAJ_PARKER, BACKWARD, options={"DISTRO_KINETIC_ENERGY_POWER_LAW_TYPE": spectrum_type})
+    s.add(BACKGROUND, "BackgroundSolarWind", solarwind())
+    dif = DataContainer().insert_double(1.5e22 / unit_diffusion).insert_double(GeV).insert_double(one_au)
+    dif.insert_double(1.0).insert_double(2.0).insert_double(1.0).insert_int(0)
+    dif.insert_double(1.0).insert_double(0.0).insert_double(1.0)
+    s.add(DIFFUSION, "DiffusionKineticEnergyRadialDistancePowerLaw", dif)
+    s.add(INITIAL, "InitialTimeFixed", DataContainer().insert_double(0.0))
+    s.add(INITIAL, "InitialSpaceFixed", DataContainer().insert_vec((one_au, 0.0, 0.0)))
+    T1, T2 = T1_MeV * MeV, T2_MeV * MeV
+    s.add(INITIAL, "InitialMomentumShellGrid", DataContainer().insert_double(T1).insert_double(T2).insert_int(n_energies))
+    s.add(BOUNDARY, "BoundarySphereAbsorb", boundary(1, [1], ("vec", (0.0, 0.0, 0.0)), ("double", inner_au * one_au)))
+    s.add(BOUNDARY, "BoundarySphereAbsorb", boundary(1, [0], ("vec", (0.0, 0.0, 0.0)), ("double", outer_au * one_au)))
+    s.add(BOUNDARY, "BoundaryTimeExpire", boundary(1, [-1], ("double", -t_limit_years * one_year)))
+    lo, hi = spectrum_bins(T1, T2, n_energies)
+    dst = distribution_base(DataContainer(), (n_energies, 0, 0), (lo, 0.0, 0.0), (hi, 0.0, 0.0), (1, 0, 0), (0, 0, 0),
+                            1.0, (unit_energy, 1.0, 1.0), False)
+    dst.insert_double(1.0).insert_double(GeV).insert_double(-1.8).insert_double(0.0)
+    s.add(DISTRIBUTION, "DistributionSpectrumKineticEnergyPowerLaw", dst)
+    s.meta.update(energies=energy_grid(T1, T2, n_energies), n_energies=n_energies, inner=inner_au * one_au, outer=outer_au * one_au,
+                  V=400.0e5 / unit_velocity, kap0=1.5e22 / unit_diffusion, T0=GeV, r0=one_au, gamma=-1.8)
+    return s
+
+
 def gc_cartesian(grid, T_MeV=100.0, D0=1.668e-4, kperp_cgs=8.6e19, t_limit_days=2.0, r_start_au=1.0, name="gc_cartesian") -> Setup:
     """Guiding centre with perpendicular diffusion and pitch-angle scattering on a gridded solar wind
     (BackgroundServerCartesian, dmax 1 au): the gc_diff_scatt workload with the analytic spiral replaced

@@ -124,31 +124,31 @@ inline void grid_units(spx_grid_desc& g)
 struct GridData {
    spx_grid_desc desc{};
    std::vector<double> rec;            // [nz2][nz1][nz0][SPX_GRID_REC]
-   DevBuf<double> d64;
-   DevBuf<float> d32;
+   DevBuf<double> c64;                 // cell records, 8 corners each (spx_grid_desc::c64 / c32)
+   DevBuf<float> c32;
    std::mutex mu;
    bool uploaded = false;
-   size_t bytes() const { return rec.size() * (sizeof(double) + sizeof(float)); }
+   size_t bytes() const { return rec.size() * 8 * (sizeof(double) + sizeof(float)); }
+   // the zone records go up once and are expanded into cell records on the device
    hipError_t upload()
    {
       std::lock_guard<std::mutex> lk(mu);
       if (uploaded) return hipSuccess;
-      hipError_t e = d64.alloc(rec.size());
+      DevBuf<double> zones;
+      hipError_t e = zones.alloc(rec.size());
+      if (e == hipSuccess) e = c64.alloc(rec.size() * 8);
+      if (e == hipSuccess) e = c32.alloc(rec.size() * 8);
+      if (e == hipSuccess) e = hipMemcpy(zones.p, rec.data(), sizeof(double) * rec.size(), hipMemcpyHostToDevice);
+      if (e == hipSuccess) e = grid_expand_cells(zones.p, desc.nz, c64.p, c32.p, nullptr);
+      if (e == hipSuccess) e = hipDeviceSynchronize();
+      zones.release();
       if (e != hipSuccess) return e;
-      e = d32.alloc(rec.size());
-      if (e != hipSuccess) return e;
-      e = hipMemcpy(d64.p, rec.data(), sizeof(double) * rec.size(), hipMemcpyHostToDevice);
-      if (e != hipSuccess) return e;
-      std::vector<float> f(rec.size());
-      for (size_t i = 0; i < rec.size(); i++) f[i] = (float)rec[i];
-      e = hipMemcpy(d32.p, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice);
-      if (e != hipSuccess) return e;
-      desc.z64 = d64.p;
-      desc.z32 = d32.p;
+      desc.c64 = c64.p;
+      desc.c32 = c32.p;
       uploaded = true;
       return hipSuccess;
    }
-   ~GridData() { d64.release(); d32.release(); }
+   ~GridData() { c64.release(); c32.release(); }
 };
 
 // What the distribution file formats need beyond the plan: the class name, the configuration bytes
@@ -178,8 +178,7 @@ struct spx_sim {
    bool dev_ready = false;
    Pool pool;
    QueryRes q;
-   DevBuf<unsigned long long> hist_counts, hist_events;
-   DevBuf<double> hist_sums;
+   DevBuf<unsigned long long> hist_counts, hist_events, hist_acc;   // counts, event totals, exact sums (spx_acc.h)
    std::shared_ptr<GridData> grid;     // BackgroundServerCartesian data
    std::vector<DistMeta> dmeta;        // per distribution, for the file formats
    // run state
@@ -199,6 +198,8 @@ static int fail(spx_sim* s, int code, const std::string& msg)
    if (s) s->err = msg;
    return code;
 }
+
+static int ensure_hist(spx_sim* s);
 
 #define HIPCHK(sim, expr)                                                                   \
    do {                                                                                     \
@@ -362,7 +363,23 @@ static int add_initial(spx_sim* s, const std::string& cls, Reader& r)
 {
    spx_plan_desc& P = s->plan;
    if (cls == "InitialTimeFixed") { P.it_t = r.d(); s->have_it = true; }                        // src/initial_time.cc:48-58
-   else if (cls == "InitialSpaceFixed") { r.v(P.is_pos); s->have_is = true; }                    // src/initial_space.cc:49-59
+   else if (cls == "InitialSpaceFixed") { P.is_kind = SPX_IS_FIXED; r.v(P.is_pos); s->have_is = true; }   // src/initial_space.cc:49-59
+   else if (cls == "InitialSpaceSphere") {                                                        // src/initial_space.cc:297-303
+      P.is_kind = SPX_IS_SPHERE; r.v(P.is_pos); P.is_radius = r.d();
+      P.is_ct1 = 1.0; P.is_ct2 = -1.0; P.is_phi1 = 0.0; P.is_phi2 = 2.0 * M_PI;
+      s->have_is = true;
+   }
+   else if (cls == "InitialSpaceSphereSector") {                                                  // src/initial_space.cc:353-378
+      P.is_kind = SPX_IS_SECTOR; r.v(P.is_pos); P.is_radius = r.d();
+      double th1 = r.d(), th2 = r.d(), ph1 = r.d(), ph2 = r.d();
+      if (th1 < 0.0) th1 = 0.0;
+      if (th2 > M_PI) th2 = M_PI;
+      if (th2 < th1) th2 = th1;
+      if (ph2 > 2.0 * M_PI) ph2 = 2.0 * M_PI;
+      if (ph2 < ph1) ph2 = ph1;
+      P.is_ct1 = std::cos(th1); P.is_ct2 = std::cos(th2); P.is_phi1 = ph1; P.is_phi2 = ph2;
+      s->have_is = true;
+   }
    else if (cls == "InitialMomentumFixed") { P.im_kind = SPX_IM_FIXED; r.v(P.im_mom); s->have_im = true; }
    else if (cls == "InitialMomentumShell") { P.im_kind = SPX_IM_SHELL; P.im_p0 = r.d(); s->have_im = true; }   // src/initial_momentum.cc:283-295
    else if (cls == "InitialMomentumRing") {                                                       // src/initial_momentum.cc:203-220
@@ -430,6 +447,7 @@ static int add_distribution(spx_sim* s, const std::string& cls, Reader& r)
    D.nb = D.n_bins[0] * D.n_bins[1] * D.n_bins[2];
    if (D.nb <= 0 || D.nb > (1 << 24)) return fail(s, SPX_ERR_LIMIT, "distribution size out of range");
    P.hist_off[P.n_dist] = s->hist_bins;
+   P.acc_off[P.n_dist] = acc_components(P);
    s->hist_bins += D.nb;
    P.n_dist++;
    M.cls = cls;
@@ -632,6 +650,7 @@ int spx_create(int traj_type, int time_flow, spx_sim** out)
    s->plan.time_flow = time_flow;
    s->plan.max_steps = 10000000;
    s->plan.max_attempts = 16;
+   s->plan.dt_scale = 1.0;
    *out = s;
    return SPX_OK;
 }
@@ -641,7 +660,7 @@ void spx_destroy(spx_sim* s)
    if (!s) return;
    s->q.release();
    s->pool.release();
-   s->hist_counts.release(); s->hist_events.release(); s->hist_sums.release();
+   s->hist_counts.release(); s->hist_events.release(); s->hist_acc.release();
    delete s;
 }
 
@@ -710,7 +729,8 @@ int spx_set_background_grid(spx_sim* s, const int32_t n_blocks[3], const double 
       g.inner_lo[a] = lo + margin;
       g.inner_hi[a] = hi - margin;
    }
-   if (nzone > ((int64_t)1 << 34) / SPX_GRID_REC) return fail(s, SPX_ERR_LIMIT, "grid too large");
+   // the device holds 8 corner records per zone in both precisions (768 B per zone): 2^28 zones are 206 GB
+   if (nzone > ((int64_t)1 << 28)) return fail(s, SPX_ERR_LIMIT, "grid too large (more than 2^28 zones)");
    G->rec.assign((size_t)nzone * SPX_GRID_REC, 0.0);
    // ReadCartesianData's reordering (src/reader_cartesian.cc:103-140): blocks x fastest, zones x fastest
    const int B = SPX_GRID_BLOCK;
@@ -769,6 +789,7 @@ int spx_set_option(spx_sim* s, const char* name, double v)
    if (n == "DISTRO_KINETIC_ENERGY_POWER_LAW_TYPE") { if (v < 0 || v > 2) return SPX_ERR_ARG; s->plan.spectrum_type = (int)v; }
    else if (n == "max_trajectory_steps") { if (v < 1) return SPX_ERR_ARG; s->plan.max_steps = (int64_t)v; }
    else if (n == "max_attempts") { if (v < 1) return SPX_ERR_ARG; s->plan.max_attempts = (int)v; }
+   else if (n == "step_scale") { if (!(v > 0.0 && v <= 1.0)) return SPX_ERR_ARG; s->plan.dt_scale = v; }
    else if (n == "precision") { if (v != 0 && v != 1) return SPX_ERR_ARG; s->precision = (int)v; }
    else if (n == "block") { if (v < 64 || v > 1024 || ((int)v % 64)) return SPX_ERR_ARG; s->block = (int)v; }
    else if (n == "lanes") { if (v < 0) return SPX_ERR_ARG; s->lanes_opt = (int64_t)v; }
@@ -802,18 +823,11 @@ int spx_begin(spx_sim* s, uint64_t first, uint64_t n_traj, const spx_rng* rng, i
    }
    else HIPCHK(s, P.reset_slots());
    P.sig = s->plan;
-   if (s->hist_counts.n < (size_t)std::max<int64_t>(s->hist_bins, 1)) {
-      const size_t nb = (size_t)std::max<int64_t>(s->hist_bins, 1);
-      HIPCHK(s, s->hist_counts.alloc(nb));
-      HIPCHK(s, s->hist_sums.alloc(nb * 9));
-      HIPCHK(s, s->hist_events.alloc(SPX_MAX_DIST));
-      HIPCHK(s, hipMemsetAsync(s->hist_counts.p, 0, sizeof(unsigned long long) * nb, P.stream));
-      HIPCHK(s, hipMemsetAsync(s->hist_sums.p, 0, sizeof(double) * nb * 9, P.stream));
-      HIPCHK(s, hipMemsetAsync(s->hist_events.p, 0, sizeof(unsigned long long) * SPX_MAX_DIST, P.stream));
-   }
+   rc = ensure_hist(s);
+   if (rc) return rc;
    s->q.release();
    s->keep_records = keep_records != 0;
-   HIPCHK(s, s->q.create(s->plan, s->hist_bins, first, n_traj, rng, s->keep_records, s->hist_counts.p, s->hist_sums.p,
+   HIPCHK(s, s->q.create(s->plan, s->hist_bins, first, n_traj, rng, s->keep_records, s->hist_counts.p, s->hist_acc.p,
                          s->hist_events.p, P.stream));
    const int qi = P.alloc_index();
    if (qi < 0) return fail(s, SPX_ERR_STATE, "query table full");
@@ -946,6 +960,7 @@ int spx_autotune(spx_sim* s, uint64_t n_probe, uint64_t seed, spx_tune_entry* ta
          if (table && n < max_entries) {
             table[n].lanes = s->lanes_opt;
             table[n].steps_per_launch = spl;
+            table[n].graph_launches = 0;
             table[n].steps_per_s = rate;
          }
          n++;
@@ -953,15 +968,41 @@ int spx_autotune(spx_sim* s, uint64_t n_probe, uint64_t seed, spx_tune_entry* ta
       }
       if (rc) break;
    }
+   // the best shape replayed as a HIP graph of two launches: kept when it runs more than 2% faster
+   int use_graph = 0;
+   if (!rc) {
+      s->lanes_opt = best_lanes;
+      s->steps_per_launch = best_spl;
+      s->graph_launches = 2;
+      rc = spx_begin(s, 0, n_probe, &rng, 0);
+      if (!rc) {
+         spx_run_stats w0, w1;
+         rc = spx_advance(s, 2, &w0);   // captures the graph and replays it once
+         if (!rc) rc = spx_advance(s, 2, &w1);
+         HIPCHK(s, s->pool.remove_queue(0));
+         s->pool.free_index(0);
+         s->running = false;
+         if (!rc) {
+            const double rate = (double)(w1.steps_executed - w0.steps_executed) / std::max(w1.wall_s, 1e-9);
+            if (table && n < max_entries) {
+               table[n].lanes = best_lanes;
+               table[n].steps_per_launch = best_spl;
+               table[n].graph_launches = 2;
+               table[n].steps_per_s = rate;
+            }
+            n++;
+            use_graph = rate > 1.02 * best;
+         }
+      }
+   }
    if (n_entries) *n_entries = n;
-   s->graph_launches = keep_graph;
+   s->graph_launches = use_graph ? 2 : 0;
    if (rc) {
       s->lanes_opt = keep_lanes;
       s->steps_per_launch = keep_spl;
+      s->graph_launches = keep_graph;
       return rc;
    }
-   s->lanes_opt = best_lanes;
-   s->steps_per_launch = best_spl;
    return spx_reset_distros(s);
 }
 
@@ -984,14 +1025,7 @@ int spx_get_distro(const spx_sim* sc, int d, int64_t* counts, double* sums, int6
       if (n_events) *n_events = 0;
       return SPX_OK;
    }
-   QueryDesc qd = s->q.desc;
-   qd.hist_counts = s->hist_counts.p;
-   qd.hist_sums = s->hist_sums.p;
-   qd.hist_events = s->hist_events.p;
-   QueryRes view;
-   view.desc = qd;
-   HIPCHK(s, view.read_distro(s->plan, d, counts, sums, n_events, s->pool.stream));
-   view.desc = QueryDesc{};
+   HIPCHK(s, read_hist(s->plan, d, s->hist_counts.p, s->hist_acc.p, s->hist_events.p, counts, sums, n_events, s->pool.stream));
    return SPX_OK;
 }
 
@@ -1000,7 +1034,7 @@ int spx_reset_distros(spx_sim* s)
    if (!s) return SPX_ERR_ARG;
    if (!s->hist_counts.p) return SPX_OK;
    HIPCHK(s, hipMemsetAsync(s->hist_counts.p, 0, sizeof(unsigned long long) * s->hist_counts.n, s->pool.stream));
-   HIPCHK(s, hipMemsetAsync(s->hist_sums.p, 0, sizeof(double) * s->hist_sums.n, s->pool.stream));
+   HIPCHK(s, hipMemsetAsync(s->hist_acc.p, 0, sizeof(unsigned long long) * s->hist_acc.n, s->pool.stream));
    HIPCHK(s, hipMemsetAsync(s->hist_events.p, 0, sizeof(unsigned long long) * SPX_MAX_DIST, s->pool.stream));
    HIPCHK(s, hipStreamSynchronize(s->pool.stream));
    return SPX_OK;
@@ -1009,16 +1043,17 @@ int spx_reset_distros(spx_sim* s)
 // ---- distribution files (src/distribution_templated.cc:394-632) --------------------------------------
 static int ensure_hist(spx_sim* s)
 {
-   if (s->hist_counts.p) return SPX_OK;
+   const size_t nb = (size_t)std::max<int64_t>(s->hist_bins, 1);
+   const size_t nl = (size_t)std::max<int64_t>(acc_components(s->plan), 1) * SPX_ACC_LIMBS;
+   if (s->hist_counts.p && s->hist_counts.n >= nb && s->hist_acc.n >= nl) return SPX_OK;
    int rc = ensure_device(s);
    if (rc) return rc;
-   if (!s->pool.stream) HIPCHK(s, hipStreamCreate(&s->pool.stream));
-   const size_t nb = (size_t)std::max<int64_t>(s->hist_bins, 1);
+   if (!s->pool.stream) HIPCHK(s, hipStreamCreateWithFlags(&s->pool.stream, hipStreamNonBlocking));
    HIPCHK(s, s->hist_counts.alloc(nb));
-   HIPCHK(s, s->hist_sums.alloc(nb * 9));
+   HIPCHK(s, s->hist_acc.alloc(nl));
    HIPCHK(s, s->hist_events.alloc(SPX_MAX_DIST));
    HIPCHK(s, hipMemsetAsync(s->hist_counts.p, 0, sizeof(unsigned long long) * nb, s->pool.stream));
-   HIPCHK(s, hipMemsetAsync(s->hist_sums.p, 0, sizeof(double) * nb * 9, s->pool.stream));
+   HIPCHK(s, hipMemsetAsync(s->hist_acc.p, 0, sizeof(unsigned long long) * nl, s->pool.stream));
    HIPCHK(s, hipMemsetAsync(s->hist_events.p, 0, sizeof(unsigned long long) * SPX_MAX_DIST, s->pool.stream));
    HIPCHK(s, hipStreamSynchronize(s->pool.stream));
    return SPX_OK;
@@ -1100,16 +1135,14 @@ int spx_restore_distro(spx_sim* s, int d, const char* file_name)
    if (!f) return fail(s, SPX_ERR_ARG, "distribution file is short");
    int rc = ensure_hist(s);
    if (rc) return rc;
-   std::vector<unsigned long long> c64((size_t)D.nb);
-   std::vector<double> s9((size_t)D.nb * 9, 0.0);
-   for (int64_t b = 0; b < D.nb; b++) {
-      c64[b] = (unsigned long long)std::max(c32[b], 0);
-      for (int k = 0; k < D.width; k++) s9[b * 9 + k] = sm[b * D.width + k];
-   }
+   std::vector<unsigned long long> c64((size_t)D.nb), acc(sm.size() * SPX_ACC_LIMBS);
+   for (int64_t b = 0; b < D.nb; b++) c64[b] = (unsigned long long)std::max(c32[b], 0);
+   for (size_t i = 0; i < sm.size(); i++) acc_from_double(sm[i], acc.data() + i * SPX_ACC_LIMBS);
    const unsigned long long ev = (unsigned long long)std::max(nev32, 0);
    const int64_t off = s->plan.hist_off[d];
    HIPCHK(s, hipMemcpyAsync(s->hist_counts.p + off, c64.data(), sizeof(unsigned long long) * D.nb, hipMemcpyHostToDevice, s->pool.stream));
-   HIPCHK(s, hipMemcpyAsync(s->hist_sums.p + off * 9, s9.data(), sizeof(double) * D.nb * 9, hipMemcpyHostToDevice, s->pool.stream));
+   HIPCHK(s, hipMemcpyAsync(s->hist_acc.p + (size_t)s->plan.acc_off[d] * SPX_ACC_LIMBS, acc.data(), sizeof(unsigned long long) * acc.size(),
+                            hipMemcpyHostToDevice, s->pool.stream));
    HIPCHK(s, hipMemcpyAsync(s->hist_events.p + d, &ev, sizeof(ev), hipMemcpyHostToDevice, s->pool.stream));
    HIPCHK(s, hipStreamSynchronize(s->pool.stream));
    return SPX_OK;
@@ -1373,7 +1406,7 @@ struct SQuery {
    Pool* pool = nullptr;
    int qidx = -1;
    QueryRes res;
-   double t_submit = 0.0, t_admit = -1.0, t_done = -1.0;
+   double t_submit = 0.0, t_admit = -1.0, t_start = -1.0, t_done = -1.0;
    RunCtl ctl{};
    std::vector<std::vector<int64_t>> counts;
    std::vector<std::vector<double>> sums;
@@ -1517,6 +1550,8 @@ static void server_loop(spx_server* S)
             }
             std::lock_guard<std::mutex> lk(S->mu);
             q->ctl = c;
+            // the round in which the query's first trajectory was claimed (its queue wait ends)
+            if (q->t_start < 0.0 && c.next > q->first) q->t_start = S->now();
             if (finished) {
                q->state = SPX_Q_DONE;
                q->t_done = S->now();
@@ -1606,6 +1641,7 @@ int spx_server_info(spx_server* S, int64_t id, spx_query_info* info)
    info->steps_executed = q->ctl.steps_executed;
    info->t_submit = q->t_submit;
    info->t_admit = q->t_admit;
+   info->t_start = q->t_start;
    info->t_done = q->t_done;
    info->latency_s = q->t_done >= 0.0 ? q->t_done - q->t_submit : -1.0;
    info->footprint_bytes = (int64_t)q->footprint;

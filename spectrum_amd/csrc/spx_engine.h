@@ -71,11 +71,16 @@ template <typename R>
 struct NormalStream {
    uint64_t traj;
    uint32_t nn;          // normal draws consumed
-   R spare;              // sine branch of the last even draw
+   R s0, s1, s2;         // the next normals of the last Philox block, in order (FP64: its sine branch in s0)
    bool spare_ok;
    bool exhausted;
 };
 
+// Normal draw nn of the trajectory's stream.  FP64: block nn/2, cosine branch for even nn, sine for odd
+// (normal_pair_f64); FP32: block nn/4, the four normals of normal_quad_f32 in order, the ones not yet
+// drawn kept as a queue (popped by moves: a select between the fields would make the compiler index the
+// lane's state in memory).  The spares are valid only within one launch (spare_ok is cleared when a
+// launch loads the slot), so a stream resumed in the middle of a block regenerates it and picks the same value.
 template <typename R>
 __device__ __forceinline__ R next_normal(const spx_rng_desc& rng, uint64_t first, NormalStream<R>& S)
 {
@@ -84,13 +89,31 @@ __device__ __forceinline__ R next_normal(const spx_rng_desc& rng, uint64_t first
       if ((int64_t)k >= rng.normal_stride) { S.exhausted = true; return R(0); }
       return R(rng.normals[(S.traj - first) * (uint64_t)rng.normal_stride + k]);
    }
-   if ((k & 1u) && S.spare_ok) { S.spare_ok = false; return S.spare; }
-   R c, s;
-   normal_pair<R>(rng.seed, S.traj, k >> 1, c, s);
-   if (k & 1u) return s;
-   S.spare = s;
-   S.spare_ok = true;
-   return c;
+   if constexpr (is_f32<R>()) {
+      const uint32_t j = k & 3u;
+      if (j != 0u && S.spare_ok) {
+         const float v = S.s0;
+         S.s0 = S.s1;
+         S.s1 = S.s2;
+         return v;
+      }
+      float n0, n1, n2, n3;
+      normal_quad_f32(rng.seed, S.traj, k >> 2, n0, n1, n2, n3);
+      S.s0 = j == 0u ? n1 : (j == 1u ? n2 : n3);
+      S.s1 = j == 0u ? n2 : n3;
+      S.s2 = n3;
+      S.spare_ok = true;
+      return j == 0u ? n0 : (j == 1u ? n1 : (j == 2u ? n2 : n3));
+   }
+   else {
+      if ((k & 1u) && S.spare_ok) { S.spare_ok = false; return S.s0; }
+      double c, s;
+      normal_pair_f64(rng.seed, S.traj, k >> 1, c, s);
+      if (k & 1u) return s;
+      S.s0 = s;
+      S.spare_ok = true;
+      return c;
+   }
 }
 
 __device__ __forceinline__ double next_uniform(const spx_rng_desc& rng, uint64_t first, uint64_t traj, uint32_t& nu, bool& exhausted)
@@ -101,6 +124,27 @@ __device__ __forceinline__ double next_uniform(const spx_rng_desc& rng, uint64_t
       return rng.uniforms[(traj - first) * (uint64_t)rng.uniform_stride + k];
    }
    return counter_uniform(rng.seed, traj, k);
+}
+
+// Initial position: InitialSpaceFixed, InitialSpaceSphere and InitialSpaceSphereSector::EvaluateInitial
+// (src/initial_space.cc:49-66, :308-316, :384-392).  The polar cosine is uniform between cos(theta1) and
+// cos(theta2) (the sphere: -1 + 2 u), then the azimuth between phi1 and phi2 (the sphere: 2 pi u).
+__device__ __forceinline__ void pos_sample(PlanPtr P, const spx_rng_desc& rng, uint64_t first, uint64_t traj, uint32_t& nu, bool& ex,
+                                           double& x, double& y, double& z)
+{
+   x = P->is_pos[0]; y = P->is_pos[1]; z = P->is_pos[2];
+   if (P->is_kind == SPX_IS_FIXED) return;
+   const bool sphere = P->is_kind == SPX_IS_SPHERE;
+   const double u1 = next_uniform(rng, first, traj, nu, ex);
+   const double ct = sphere ? -1.0 + 2.0 * u1 : P->is_ct1 + (P->is_ct2 - P->is_ct1) * u1;
+   const double st = sqrt(1.0 - ct * ct);
+   const double u2 = next_uniform(rng, first, traj, nu, ex);
+   const double phi = sphere ? 6.283185307179586 * u2 : P->is_phi1 + (P->is_phi2 - P->is_phi1) * u2;
+   double sn, cs;
+   sincos(phi, &sn, &cs);
+   x = x + P->is_radius * st * cs;
+   y = y + P->is_radius * st * sn;
+   z = z + P->is_radius * ct;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -163,13 +207,20 @@ __device__ __forceinline__ double& ev_field(LaunchPtr L, int cls, int f, uint64_
 // Distribution events (src/distribution_base.cc:174-194, src/distribution_templated.cc:145-180)
 // Called by every lane of a wave together; `has` marks the lanes with an event at boundary `bnd`.
 // Each lane forms the value and weight of every distribution and its bin; the wave then merges the
-// lanes that hit the same bin (a shuffle reduction of the weights) and issues one atomic per bin:
-// exact integer counts and FP64 sums, plus the per-slot last-event record.
+// lanes that hit the same bin and issues one count atomic per bin; the weights go to the bin's exact
+// fixed-point sum (spx_acc.h), digits of lanes with the same bin and limb added as integers first.
+// Counts and sums are therefore independent of event order; also the per-slot last-event record.
 // ---------------------------------------------------------------------------------------------
-__device__ __forceinline__ double wave_sum(double v)
+__device__ __forceinline__ long long wave_sum_i64(long long v)
 {
 #pragma unroll
    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+   return v;
+}
+__device__ __forceinline__ int wave_or(int v)
+{
+#pragma unroll
+   for (int off = 32; off > 0; off >>= 1) v |= __shfl_xor(v, off);
    return v;
 }
 
@@ -270,25 +321,42 @@ static __device__ int bin_events(PlanPtr P, QueryPtr Q, LaunchPtr L, uint64_t sl
          }
          else L->ev_bin[(uint64_t)d * L->n_lanes + slot] = -1;
       }
-      // one atomic per distinct bin of the wave
-      uint64_t todo = __ballot(keep);
-      if (todo == 0) continue;
-      if (lane_id() == __ffsll((unsigned long long)todo) - 1) atomicAdd(&Q->hist_events[d], (unsigned long long)__popcll(todo));
+      // one count atomic per distinct bin of the wave
+      const uint64_t kept = __ballot(keep);
+      if (kept == 0) continue;
+      if (lane_id() == __ffsll((unsigned long long)kept) - 1) atomicAdd(&Q->hist_events[d], (unsigned long long)__popcll(kept));
+      uint64_t todo = kept;
       while (todo) {
          const int leader = __ffsll((unsigned long long)todo) - 1;
          const int key = __shfl(lin, leader);
-         const bool mine = keep && lin == key;
-         const uint64_t grp = __ballot(mine);
-         const int64_t off = Q->hist_off[d] + key;
-#pragma unroll
-         for (int k = 0; k < 9; k++) {
-            if (k < D.width) {
-               const double v = wave_sum(mine ? w[k] : 0.0);
-               if (lane_id() == leader) atomicAdd(&Q->hist_sums[off * 9 + k], v);
-            }
-         }
-         if (lane_id() == leader) atomicAdd(&Q->hist_counts[off], (unsigned long long)__popcll(grp));
+         const uint64_t grp = __ballot(keep && lin == key);
+         if (lane_id() == leader) atomicAdd(&Q->hist_counts[Q->hist_off[d] + key], (unsigned long long)__popcll(grp));
          todo &= ~grp;
+      }
+      // exact sums: lanes with the same bin and the same lowest limb add their digits as integers
+      // (associative), then one atomic per touched limb
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+         if (k >= D.width) continue;
+         const AccParts a = acc_split(keep ? w[k] : 0.0);
+         const bool act = keep && (a.special || a.c0 || a.c1 || a.c2);
+         todo = __ballot(act);
+         while (todo) {
+            const int leader = __ffsll((unsigned long long)todo) - 1;
+            const int kb = __shfl(lin, leader), ki = __shfl(a.idx, leader);
+            const bool mine = act && lin == kb && a.idx == ki;
+            const uint64_t grp = __ballot(mine);
+            const long long s0 = wave_sum_i64(mine ? a.c0 : 0ll), s1 = wave_sum_i64(mine ? a.c1 : 0ll), s2 = wave_sum_i64(mine ? a.c2 : 0ll);
+            const int sp = wave_or(mine ? a.special : 0);
+            if (lane_id() == leader) {
+               unsigned long long* limb = Q->hist_acc + (uint64_t)(Q->acc_off[d] + (int64_t)kb * D.width + k) * SPX_ACC_LIMBS;
+               if (s0) atomicAdd(limb + ki, (unsigned long long)s0);
+               if (s1) atomicAdd(limb + ki + 1, (unsigned long long)s1);
+               if (s2) atomicAdd(limb + ki + 2, (unsigned long long)s2);
+               if (sp) atomicOr(limb + SPX_ACC_FLAGS, (unsigned long long)sp);
+            }
+            todo &= ~grp;
+         }
       }
    }
    return n_ev;

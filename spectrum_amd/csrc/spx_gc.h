@@ -328,8 +328,10 @@ __device__ int gc_set_start(PlanPtr P, QueryPtr Q, LaunchPtr L, const spx_rng_de
 {
    const SPX_CP spx_bg_t<R>& bg = PlanView<R>::bg(P);
    const uint64_t first = Q->first;
-   const double t = P->it_t, x = P->is_pos[0], y = P->is_pos[1], z = P->is_pos[2];
+   const double t = P->it_t;
    bool ex = false;
+   double x, y, z;
+   pos_sample(P, rng, first, traj, nu, ex, x, y, z);
    double pp1, pl1, pp, pl;
    gc_mom_sample(P, rng, first, traj, nu, ex, pp1, pl1);
    const double pm1 = sqrt(pp1 * pp1 + pl1 * pl1);
@@ -523,7 +525,7 @@ __global__ __launch_bounds__(BLOCK) SPX_GC_ATTR void gc_step_kernel(const Launch
       H.t = Lp->t[slot]; H.x = Lp->x[slot]; H.y = Lp->y[slot]; H.z = Lp->z[slot]; H.pp = Lp->m0[slot]; H.pl = Lp->m2[slot];
       H.pf = Lp->pf[slot];
       H.steps = (int32_t)Lp->steps[slot];
-      H.rs.traj = Lp->traj[slot]; H.rs.nn = Lp->nn[slot]; H.rs.spare_ok = false; H.rs.exhausted = false; H.rs.spare = R(0);
+      H.rs.traj = Lp->traj[slot]; H.rs.nn = Lp->nn[slot]; H.rs.spare_ok = false; H.rs.exhausted = false; H.rs.s0 = H.rs.s1 = H.rs.s2 = R(0);
       C.amp[tid] = Lp->amp[slot]; C.wgt[tid] = Lp->wgt[slot]; C.dt_adapt[tid] = Lp->dt_adapt[slot];
 #pragma unroll
       for (int b = 0; b < SPX_MAX_BND; b++) {

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include "spx_plan.h"
+#include "spx_acc.h"
 #include "../../include/spectrum_amd.h"
 
 namespace spx {
@@ -28,9 +29,10 @@ struct QueryDesc {
    spx_rng_desc rng;                    // replay pointers are device pointers here
    uint64_t first, end;                 // trajectory id range
    int32_t n_dist, pad_;
-   int64_t hist_off[SPX_MAX_DIST];      // offsets of each distribution in this query's histograms
+   int64_t hist_off[SPX_MAX_DIST];      // offsets of each distribution in this query's counts (bins)
+   int64_t acc_off[SPX_MAX_DIST];       // and in its exact sums (bin components)
    unsigned long long* hist_counts;
-   double* hist_sums;                   // 9 slots per bin
+   unsigned long long* hist_acc;        // SPX_ACC_LIMBS limbs per bin component (spx_acc.h)
    unsigned long long* hist_events;     // per distribution
    spx_traj_record* records;            // optional per-trajectory records
    uint64_t rec_first, rec_n;
@@ -77,6 +79,11 @@ hipError_t launch_f64(const spx_plan_desc& host_plan, uint64_t n_lanes, const La
                       hipStream_t stream, hipEvent_t mid);
 hipError_t launch_f32(const spx_plan_desc& host_plan, uint64_t n_lanes, const Launch* dev_launch,
                       hipStream_t stream, hipEvent_t mid);
+
+// Cell records of a Cartesian grid (spx_grid_desc::c64 / c32) from its zone records (nz[0] x nz[1] x nz[2],
+// x fastest, SPX_GRID_REC doubles each): for every zone, the 8 zones of the stencil whose lowest corner it
+// is, in corner order i + 2j + 4k (corners beyond the last zone repeat it; no stencil reads them).
+hipError_t grid_expand_cells(const double* zones, const int32_t nz[3], double* c64, float* c32, hipStream_t st);
 
 inline hipError_t launch_transport(const spx_plan_desc& host_plan, uint64_t n_lanes, const Launch* dev_launch,
                                    int precision, hipStream_t stream, hipEvent_t mid)

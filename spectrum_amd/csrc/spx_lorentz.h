@@ -67,8 +67,10 @@ __device__ int lz_set_start(PlanPtr P, QueryPtr Q, LaunchPtr L, const spx_rng_de
 {
    const SPX_CP spx_bg_t<R>& bg = PlanView<R>::bg(P);
    const uint64_t first = Q->first;
-   const double t = P->it_t, x = P->is_pos[0], y = P->is_pos[1], z = P->is_pos[2];
+   const double t = P->it_t;
    bool ex = false;
+   double x, y, z;
+   pos_sample(P, rng, first, traj, nu, ex, x, y, z);
    double m1[3], m[3];
    lz_mom_sample(P, rng, first, traj, nu, ex, m1);
    v3<R> U0, B0;

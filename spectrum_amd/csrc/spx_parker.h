@@ -187,8 +187,10 @@ __device__ int parker_set_start(PlanPtr P, QueryPtr Q, LaunchPtr L, const spx_rn
 {
    const SPX_CP spx_bg_t<R>& bg = PlanView<R>::bg(P);
    const uint64_t first = Q->first;
-   const double t = P->it_t, x = P->is_pos[0], y = P->is_pos[1], z = P->is_pos[2];
+   const double t = P->it_t;
    bool ex = false;
+   double x, y, z;
+   pos_sample(P, rng, first, traj, nu, ex, x, y, z);
    // first sample along an arbitrary axis: only its magnitude enters the first field evaluation
    const double p1 = parker_mom_sample(P, rng, first, traj, nu, ex);
    R ds = R(0);
@@ -441,7 +443,7 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
    R dtp = R(0.5) * H.F.dmax / norm(slope_pos);
    dtp = fmin(dtp, R(0.5) * sqr(H.F.dmax) / fmax(K0, K1));
    dtp = fmin(dtp, R(0.5) * R(3) * R(0.01) / fabs(H.F.divU));
-   double dt = fmin((double)dtp, C.dt_adapt[tid]);
+   double dt = fmin(P->dt_scale == 1.0 ? (double)dtp : (double)dtp * P->dt_scale, C.dt_adapt[tid]);   // 1: the reference's step
    dt = time_proximity<FWD>(P, H, dt, (double)H.F.dmax);
    // EulerDiffSlopes (src/trajectory_parker.cc:159-180)
    v3<R> fa[3];
@@ -506,7 +508,7 @@ __device__ __forceinline__ void parker_step_body(const Launch* __restrict__ Lg)
       H.t = Lp->t[slot]; H.x = Lp->x[slot]; H.y = Lp->y[slot]; H.z = Lp->z[slot]; H.p = (TT == 2) ? Lp->m2[slot] : Lp->m0[slot];
       H.steps = (int32_t)Lp->steps[slot];
       H.ds_prev = R(Lp->ds_prev[slot]);
-      H.rs.traj = Lp->traj[slot]; H.rs.nn = Lp->nn[slot]; H.rs.spare_ok = false; H.rs.exhausted = false; H.rs.spare = R(0);
+      H.rs.traj = Lp->traj[slot]; H.rs.nn = Lp->nn[slot]; H.rs.spare_ok = false; H.rs.exhausted = false; H.rs.s0 = H.rs.s1 = H.rs.s2 = R(0);
       C.amp[tid] = Lp->amp[slot]; C.wgt[tid] = Lp->wgt[slot]; C.dt_adapt[tid] = Lp->dt_adapt[slot];
       C.dt_prev[tid] = Lp->dt_prev[slot]; C.src_old[tid] = Lp->src_old[slot];
 #pragma unroll

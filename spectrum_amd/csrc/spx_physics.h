@@ -280,17 +280,18 @@ struct GridFields {
 typedef double grid_d2 __attribute__((ext_vector_type(2)));
 typedef float grid_f4 __attribute__((ext_vector_type(4)));
 
-// One zone record: U, B, |B| (fluid units).
-template <typename R> __device__ __forceinline__ void grid_rec(const SPX_CP spx_grid_desc& g, int64_t zone, R v[7]);
-template <> __device__ __forceinline__ void grid_rec<double>(const SPX_CP spx_grid_desc& g, int64_t zone, double v[7])
+// Corner c = i + 2j + 4k of the stencil whose lowest zone is `cell`: that zone record (U, B, |B|, fluid
+// units), read from the cell's contiguous copy of its 8 corners.
+template <typename R> __device__ __forceinline__ void grid_corner(const SPX_CP spx_grid_desc& g, int64_t cell, int c, R v[7]);
+template <> __device__ __forceinline__ void grid_corner<double>(const SPX_CP spx_grid_desc& g, int64_t cell, int c, double v[7])
 {
-   const grid_d2* r = reinterpret_cast<const grid_d2*>(g.z64) + zone * (SPX_GRID_REC / 2);
-   const grid_d2 a = r[0], b = r[1], c = r[2], d = r[3];
-   v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y; v[4] = c.x; v[5] = c.y; v[6] = d.x;
+   const grid_d2* r = reinterpret_cast<const grid_d2*>(g.c64) + (cell * 8 + c) * (SPX_GRID_REC / 2);
+   const grid_d2 a = r[0], b = r[1], d = r[2], e = r[3];
+   v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y; v[4] = d.x; v[5] = d.y; v[6] = e.x;
 }
-template <> __device__ __forceinline__ void grid_rec<float>(const SPX_CP spx_grid_desc& g, int64_t zone, float v[7])
+template <> __device__ __forceinline__ void grid_corner<float>(const SPX_CP spx_grid_desc& g, int64_t cell, int c, float v[7])
 {
-   const grid_f4* r = reinterpret_cast<const grid_f4*>(g.z32) + zone * (SPX_GRID_REC / 4);
+   const grid_f4* r = reinterpret_cast<const grid_f4*>(g.c32) + (cell * 8 + c) * (SPX_GRID_REC / 4);
    const grid_f4 a = r[0], b = r[1];
    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z;
 }
@@ -348,7 +349,6 @@ __device__ __forceinline__ bool grid_eval(const SPX_CP spx_grid_desc& g, double 
    int64_t zone0;
    double zl_min;
    if (!grid_locate<R>(g, x, y, z, lo, hi, dl, zone0, zl_min)) return false;
-   const int64_t sy = g.nz[0], sz = (int64_t)g.nz[0] * g.nz[1];
    // corners in the reference's order c = i + 2j + 4k, one row (j, k) of two x-neighbours at a time: the
    // sums start from zero as the reference's do, and a rolled loop keeps two records in flight instead
    // of eight, which leaves registers for more resident waves to hide the gather latency
@@ -358,10 +358,9 @@ __device__ __forceinline__ bool grid_eval(const SPX_CP spx_grid_desc& g, double 
 #pragma unroll 1
    for (int r = 0; r < 4; r++) {
       const int j = r & 1, k = r >> 1;
-      const int64_t zr = zone0 + j * sy + k * sz;
       R v0[7], v1[7];
-      grid_rec<R>(g, zr, v0);
-      grid_rec<R>(g, zr + 1, v1);
+      grid_corner<R>(g, zone0, 2 * r, v0);
+      grid_corner<R>(g, zone0, 2 * r + 1, v1);
       const R fy = j ? lo[1] : hi[1], fz = k ? lo[2] : hi[2];
 #pragma unroll
       for (int i = 0; i < 2; i++) {
@@ -399,7 +398,6 @@ __device__ __forceinline__ bool grid_eval_gc(const SPX_CP spx_grid_desc& g, doub
    R lo[3], hi[3], dl[3];
    int64_t zone0;
    if (!grid_locate<R>(g, x, y, z, lo, hi, dl, zone0, zl_min)) return false;
-   const int64_t sy = g.nz[0], sz = (int64_t)g.nz[0] * g.nz[1];
    R acc[7], gm[3], g6[6];
 #pragma unroll
    for (int m = 0; m < 7; m++) acc[m] = R(0);
@@ -410,10 +408,9 @@ __device__ __forceinline__ bool grid_eval_gc(const SPX_CP spx_grid_desc& g, doub
 #pragma unroll 1
    for (int r = 0; r < 4; r++) {
       const int j = r & 1, k = r >> 1;
-      const int64_t zr = zone0 + j * sy + k * sz;
       R v0[7], v1[7];
-      grid_rec<R>(g, zr, v0);
-      grid_rec<R>(g, zr + 1, v1);
+      grid_corner<R>(g, zone0, 2 * r, v0);
+      grid_corner<R>(g, zone0, 2 * r + 1, v1);
       const R fy = j ? lo[1] : hi[1], fz = k ? lo[2] : hi[2];
 #pragma unroll
       for (int i = 0; i < 2; i++) {
@@ -541,8 +538,8 @@ SPX_HD double counter_uniform(uint64_t seed, uint64_t traj, uint64_t draw)
    return u53(o[0], o[1]);
 }
 
-// Box-Muller pairs: normal draw k comes from Philox block k/2 of the normal stream; even k takes the
-// cosine branch, odd k the sine branch of the same radius and angle.
+// FP64 step: Box-Muller pairs, normal draw k comes from Philox block k/2 of the normal stream; even k
+// takes the cosine branch, odd k the sine branch of the same radius and angle (53-bit uniforms).
 __device__ __forceinline__ void normal_pair_f64(uint64_t seed, uint64_t traj, uint64_t block, double& c, double& s)
 {
    uint32_t o[4];
@@ -554,19 +551,25 @@ __device__ __forceinline__ void normal_pair_f64(uint64_t seed, uint64_t traj, ui
    c = r * cs;
    s = r * sn;
 }
-__device__ __forceinline__ void normal_pair_f32(uint64_t seed, uint64_t traj, uint64_t block, float& c, float& s)
+// FP32 step: one Philox block gives four normals, two Box-Muller pairs (o[0], o[1]) and (o[2], o[3]).
+// The radius uniform is the whole 32-bit word, (o + 1/2) 2^-32 in (0, 1]: its smallest value 2^-33 cuts
+// the normal tails at sqrt(66 ln 2) = 6.76 sigma (probability 1.4e-11 per draw); the angle uses 24 bits.
+__device__ __forceinline__ void box_muller_f32(uint32_t a, uint32_t b, float& c, float& s)
 {
-   uint32_t o[4];
-   philox_block(seed, traj, block, 2u, o);
-   const float u1 = u24(o[0]), u2 = u24(o[2]);
+   const float u1 = __builtin_fmaf((float)a, 0x1.0p-32f, 0x1.0p-33f);
+   const float u2 = u24(b);
    const float r = __builtin_sqrtf(-2.0f * __logf(u1));
    float sn, cs;
    __sincosf(6.2831853071795864f * u2, &sn, &cs);
    c = r * cs;
    s = r * sn;
 }
-template <typename R> __device__ __forceinline__ void normal_pair(uint64_t seed, uint64_t traj, uint64_t block, R& c, R& s);
-template <> __device__ __forceinline__ void normal_pair<double>(uint64_t seed, uint64_t traj, uint64_t block, double& c, double& s) { normal_pair_f64(seed, traj, block, c, s); }
-template <> __device__ __forceinline__ void normal_pair<float>(uint64_t seed, uint64_t traj, uint64_t block, float& c, float& s) { normal_pair_f32(seed, traj, block, c, s); }
+__device__ __forceinline__ void normal_quad_f32(uint64_t seed, uint64_t traj, uint64_t block, float& n0, float& n1, float& n2, float& n3)
+{
+   uint32_t o[4];
+   philox_block(seed, traj, block, 2u, o);
+   box_muller_f32(o[0], o[1], n0, n1);
+   box_muller_f32(o[2], o[3], n2, n3);
+}
 
 }  // namespace spx

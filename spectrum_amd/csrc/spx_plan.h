@@ -20,6 +20,7 @@ enum spx_df_kind {
 };
 enum spx_bnd_class { SPX_BC_TIME = 0, SPX_BC_SPACE = 1, SPX_BC_MOMENTUM = 2 };
 enum spx_bnd_kind { SPX_BK_TIME = 0, SPX_BK_SPHERE = 1, SPX_BK_PLANE = 2, SPX_BK_MOMENTUM = 3 };
+enum spx_is_kind { SPX_IS_FIXED = 0, SPX_IS_SPHERE = 1, SPX_IS_SECTOR = 2 };
 enum spx_im_kind { SPX_IM_FIXED = 0, SPX_IM_SHELL = 1, SPX_IM_THICKSHELL = 2, SPX_IM_SHELLGRID = 3, SPX_IM_RING = 4 };
 enum spx_ds_kind { SPX_DS_SPECTRUM = 0, SPX_DS_TIME, SPX_DS_MOMENTUM, SPX_DS_POSMOM, SPX_DS_CUMO1, SPX_DS_CUMO2, SPX_DS_POSPITCH };
 enum spx_src_kind { SPX_SRC_NONE = 0, SPX_SRC_MOMINJ = 1 };
@@ -47,8 +48,11 @@ typedef struct {
    // 0 when the derivative points of div(kappa) need no field values: kappa_perp == kappa_par
    // identically, kappa reads neither U nor |B|, and no interpolated |B| can fall below sp_tiny
    int32_t fd_fields, pad_;
-   const double* z64;                   // device zone records (double)
-   const float* z32;                    // the same records rounded to float
+   // device cell records: for each zone, the 8 zone records of the trilinear stencil whose lowest corner it
+   // is, in the interpolation's corner order c = i + 2j + 4k (512 B in double, 256 B in float: one stencil
+   // is 4 or 2 whole 128-B lines instead of four scattered rows of the zone array)
+   const double* c64;
+   const float* c32;
 } spx_grid_desc;
 
 // Background parameters in the step type T (double for the drop-in kernel, float for the mixed one).
@@ -115,6 +119,7 @@ typedef struct {
    int32_t spectrum_type;               // DISTRO_KINETIC_ENERGY_POWER_LAW_TYPE
    int32_t max_attempts;
    int64_t max_steps;                   // max_trajectory_steps
+   double dt_scale;                     // Parker transports: factor on the physical time step (1: the reference's)
    spx_bg_desc bg;
    spx_df_desc df;
    spx_bg_t<float> bgf;                 // the same parameters rounded to float (mixed-precision kernel)
@@ -124,10 +129,13 @@ typedef struct {
    // initial conditions (src/initial_{time,space,momentum}.cc)
    int32_t im_kind, im_log, im_n, pad2_;
    double it_t, is_pos[3], im_mom[3], im_p0, im_p1, im_p2, im_T1, im_T2, im_mu0, im_st0;
+   int32_t is_kind, pad4_;              // spx_is_kind; is_pos is the fixed point or the sphere's origin
+   double is_radius, is_ct1, is_ct2, is_phi1, is_phi2;
    spx_bnd_desc bnd[SPX_MAX_BND];       // ordered: all time, then space, then momentum (ComputeAllBoundaries order)
    int32_t n_bt, n_bs, n_bm, pad3_;
    spx_ds_desc dist[SPX_MAX_DIST];
-   int64_t hist_off[SPX_MAX_DIST];      // offsets of each distribution in the count / sum arrays
+   int64_t hist_off[SPX_MAX_DIST];      // offsets of each distribution in the count array (bins)
+   int64_t acc_off[SPX_MAX_DIST];       // offsets in the exact-sum array (bin components: bins x width)
 } spx_plan_desc;
 
 // RNG description passed to a run (counter-based streams or replayed draws).

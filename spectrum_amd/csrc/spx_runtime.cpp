@@ -16,16 +16,23 @@ namespace spx {
 size_t QueryRes::footprint(const spx_plan_desc& plan, int64_t hist_bins, uint64_t n_traj, const spx_rng* rng, bool keep_records)
 {
    size_t b = sizeof(spx_plan_desc) + sizeof(RunCtl);
-   b += (size_t)std::max<int64_t>(hist_bins, 1) * (sizeof(unsigned long long) + 9 * sizeof(double)) + SPX_MAX_DIST * sizeof(unsigned long long);
+   b += (size_t)std::max<int64_t>(hist_bins, 1) * sizeof(unsigned long long) + SPX_MAX_DIST * sizeof(unsigned long long);
+   b += (size_t)std::max<int64_t>(acc_components(plan), 1) * SPX_ACC_LIMBS * sizeof(unsigned long long);
    if (keep_records) b += (size_t)n_traj * sizeof(spx_traj_record);
    if (rng && rng->mode == 1)
       b += (size_t)n_traj * sizeof(double) * ((size_t)std::max<int64_t>(rng->normal_stride, 0) + (size_t)std::max<int64_t>(rng->uniform_stride, 1));
-   (void)plan;
    return b;
 }
 
+int64_t acc_components(const spx_plan_desc& p)
+{
+   int64_t n = 0;
+   for (int d = 0; d < p.n_dist; d++) n += (int64_t)p.dist[d].nb * p.dist[d].width;
+   return n;
+}
+
 hipError_t QueryRes::create(const spx_plan_desc& p, int64_t bins, uint64_t first, uint64_t n, const spx_rng* rng, bool keep_records,
-                            unsigned long long* hist_counts, double* hist_sums, unsigned long long* hist_events, hipStream_t st)
+                            unsigned long long* hist_counts, unsigned long long* hist_acc, unsigned long long* hist_events, hipStream_t st)
 {
    n_traj = n;
    hist_bins = bins;
@@ -40,14 +47,15 @@ hipError_t QueryRes::create(const spx_plan_desc& p, int64_t bins, uint64_t first
    TRY(hipMemcpyAsync(ctl.p, &c, sizeof(c), hipMemcpyHostToDevice, st));
    if (!hist_counts) {
       const size_t nb = (size_t)std::max<int64_t>(bins, 1);
+      const size_t nl = (size_t)std::max<int64_t>(acc_components(p), 1) * SPX_ACC_LIMBS;
       TRY(own_counts.alloc(nb));
-      TRY(own_sums.alloc(nb * 9));
+      TRY(own_acc.alloc(nl));
       TRY(own_events.alloc(SPX_MAX_DIST));
       TRY(hipMemsetAsync(own_counts.p, 0, sizeof(unsigned long long) * nb, st));
-      TRY(hipMemsetAsync(own_sums.p, 0, sizeof(double) * nb * 9, st));
+      TRY(hipMemsetAsync(own_acc.p, 0, sizeof(unsigned long long) * nl, st));
       TRY(hipMemsetAsync(own_events.p, 0, sizeof(unsigned long long) * SPX_MAX_DIST, st));
       hist_counts = own_counts.p;
-      hist_sums = own_sums.p;
+      hist_acc = own_acc.p;
       hist_events = own_events.p;
    }
    desc.plan = plan.p;
@@ -71,9 +79,9 @@ hipError_t QueryRes::create(const spx_plan_desc& p, int64_t bins, uint64_t first
    desc.first = first;
    desc.end = first + n;
    desc.n_dist = p.n_dist;
-   for (int d = 0; d < SPX_MAX_DIST; d++) desc.hist_off[d] = p.hist_off[d];
+   for (int d = 0; d < SPX_MAX_DIST; d++) { desc.hist_off[d] = p.hist_off[d]; desc.acc_off[d] = p.acc_off[d]; }
    desc.hist_counts = hist_counts;
-   desc.hist_sums = hist_sums;
+   desc.hist_acc = hist_acc;
    desc.hist_events = hist_events;
    if (keep_records) {
       TRY(records.alloc(n ? n : 1));
@@ -91,28 +99,32 @@ hipError_t QueryRes::read_ctl(RunCtl& out, hipStream_t st) const
    return hipStreamSynchronize(st);
 }
 
-hipError_t QueryRes::read_distro(const spx_plan_desc& p, int d, int64_t* counts, double* sums, int64_t* n_events, hipStream_t st) const
+hipError_t read_hist(const spx_plan_desc& p, int d, const unsigned long long* dev_counts, const unsigned long long* dev_acc,
+                     const unsigned long long* dev_events, int64_t* counts, double* sums, int64_t* n_events, hipStream_t st)
 {
    const spx_ds_desc& D = p.dist[d];
-   const int64_t off = p.hist_off[d];
-   std::vector<unsigned long long> c((size_t)D.nb);
-   std::vector<double> sm((size_t)D.nb * 9);
+   const size_t nc = (size_t)D.nb * D.width;
+   std::vector<unsigned long long> c((size_t)D.nb), acc(nc * SPX_ACC_LIMBS);
    unsigned long long ev[SPX_MAX_DIST];
-   TRY(hipMemcpyAsync(c.data(), desc.hist_counts + off, sizeof(unsigned long long) * D.nb, hipMemcpyDeviceToHost, st));
-   TRY(hipMemcpyAsync(sm.data(), desc.hist_sums + off * 9, sizeof(double) * D.nb * 9, hipMemcpyDeviceToHost, st));
-   TRY(hipMemcpyAsync(ev, desc.hist_events, sizeof(ev), hipMemcpyDeviceToHost, st));
+   TRY(hipMemcpyAsync(c.data(), dev_counts + p.hist_off[d], sizeof(unsigned long long) * D.nb, hipMemcpyDeviceToHost, st));
+   TRY(hipMemcpyAsync(acc.data(), dev_acc + (size_t)p.acc_off[d] * SPX_ACC_LIMBS, sizeof(unsigned long long) * acc.size(),
+                      hipMemcpyDeviceToHost, st));
+   TRY(hipMemcpyAsync(ev, dev_events, sizeof(ev), hipMemcpyDeviceToHost, st));
    TRY(hipStreamSynchronize(st));
    if (counts) for (int b = 0; b < D.nb; b++) counts[b] = (int64_t)c[b];
-   if (sums)
-      for (int b = 0; b < D.nb; b++)
-         for (int k = 0; k < D.width; k++) sums[(size_t)b * D.width + k] = sm[(size_t)b * 9 + k];
+   if (sums) for (size_t i = 0; i < nc; i++) sums[i] = acc_value(acc.data() + i * SPX_ACC_LIMBS);
    if (n_events) *n_events = (int64_t)ev[d];
    return hipSuccess;
 }
 
+hipError_t QueryRes::read_distro(const spx_plan_desc& p, int d, int64_t* counts, double* sums, int64_t* n_events, hipStream_t st) const
+{
+   return read_hist(p, d, desc.hist_counts, desc.hist_acc, desc.hist_events, counts, sums, n_events, st);
+}
+
 void QueryRes::release()
 {
-   plan.release(); ctl.release(); own_counts.release(); own_events.release(); own_sums.release();
+   plan.release(); ctl.release(); own_counts.release(); own_events.release(); own_acc.release();
    rep_normals.release(); rep_uniforms.release(); records.release();
    std::memset(&desc, 0, sizeof(desc));
 }

@@ -39,8 +39,8 @@ struct DevBuf {
 struct QueryRes {
    DevBuf<spx_plan_desc> plan;
    DevBuf<RunCtl> ctl;
-   DevBuf<unsigned long long> own_counts, own_events;
-   DevBuf<double> own_sums, rep_normals, rep_uniforms;
+   DevBuf<unsigned long long> own_counts, own_events, own_acc;
+   DevBuf<double> rep_normals, rep_uniforms;
    DevBuf<spx_traj_record> records;
    QueryDesc desc{};
    uint64_t n_traj = 0;
@@ -49,16 +49,23 @@ struct QueryRes {
    // Bytes of device memory the query holds (histograms, records, replay buffers, plan, control block).
    static size_t footprint(const spx_plan_desc& plan, int64_t hist_bins, uint64_t n_traj, const spx_rng* rng, bool keep_records);
 
-   // Upload the plan, zero the control block (next = first) and, when `hist` is null, allocate and
-   // zero the query's own histograms; otherwise bin into the given (persistent) arrays.
+   // Upload the plan, zero the control block (next = first) and, when `hist_counts` is null, allocate
+   // and zero the query's own histograms; otherwise bin into the given (persistent) arrays.
    hipError_t create(const spx_plan_desc& plan, int64_t hist_bins, uint64_t first, uint64_t n_traj, const spx_rng* rng,
-                     bool keep_records, unsigned long long* hist_counts, double* hist_sums, unsigned long long* hist_events,
+                     bool keep_records, unsigned long long* hist_counts, unsigned long long* hist_acc, unsigned long long* hist_events,
                      hipStream_t st);
    hipError_t read_ctl(RunCtl& out, hipStream_t st) const;
    // counts (n_bins) and sums (n_bins * width) of distribution d, plus its event count
    hipError_t read_distro(const spx_plan_desc& plan, int d, int64_t* counts, double* sums, int64_t* n_events, hipStream_t st) const;
    void release();
 };
+
+// Bin components (bins x width, all distributions) of a plan's exact sums; SPX_ACC_LIMBS limbs each.
+int64_t acc_components(const spx_plan_desc& plan);
+// Host copy of distribution d of a query's histograms: counts, sums (n_bins x width, each the exact
+// sum rounded once) and the event count.
+hipError_t read_hist(const spx_plan_desc& p, int d, const unsigned long long* dev_counts, const unsigned long long* dev_acc,
+                     const unsigned long long* dev_events, int64_t* counts, double* sums, int64_t* n_events, hipStream_t st);
 
 // Resident slots running one kernel instantiation.
 struct Pool {

@@ -42,7 +42,7 @@ class SpxRunStats(C.Structure):
 
 
 class SpxTuneEntry(C.Structure):
-    _fields_ = [("lanes", C.c_int64), ("steps_per_launch", C.c_int32), ("pad_", C.c_int32), ("steps_per_s", C.c_double)]
+    _fields_ = [("lanes", C.c_int64), ("steps_per_launch", C.c_int32), ("graph_launches", C.c_int32), ("steps_per_s", C.c_double)]
 
 
 RECORD_DTYPE = np.dtype([("t", "f8"), ("pos", "f8", 3), ("mom", "f8", 3), ("amp", "f8"), ("wgt", "f8"),
@@ -261,7 +261,7 @@ class Simulation:
         table = (SpxTuneEntry * 64)()
         n = C.c_int()
         self._check(lib().spx_autotune(self.h, n_probe, seed, table, 64, C.byref(n)), "autotune")
-        return [(table[i].lanes, table[i].steps_per_launch, table[i].steps_per_s) for i in range(min(n.value, 64))]
+        return [(table[i].lanes, table[i].steps_per_launch, table[i].steps_per_s, table[i].graph_launches) for i in range(min(n.value, 64))]
 
     # --- distributions ----------------------------------------------------------------------
     def distro(self, d: int):

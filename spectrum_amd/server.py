@@ -26,7 +26,7 @@ class ServerOpts(C.Structure):
 class QueryInfo(C.Structure):
     _fields_ = [("state", C.c_int32), ("pad_", C.c_int32), ("n_traj", C.c_uint64), ("done", C.c_uint64), ("failed", C.c_uint64),
                 ("steps_done", C.c_uint64), ("steps_executed", C.c_uint64), ("t_submit", C.c_double), ("t_admit", C.c_double),
-                ("t_done", C.c_double), ("latency_s", C.c_double), ("footprint_bytes", C.c_int64)]
+                ("t_done", C.c_double), ("t_start", C.c_double), ("latency_s", C.c_double), ("footprint_bytes", C.c_int64)]
 
 
 class ServerStats(C.Structure):
@@ -96,7 +96,7 @@ class Server:
         q = QueryInfo()
         self._check(lib().spx_server_info(self.h, qid, C.byref(q)), "info")
         return {"state": STATES[q.state], "n_traj": q.n_traj, "done": q.done, "failed": q.failed, "steps_done": q.steps_done,
-                "steps_executed": q.steps_executed, "t_submit": q.t_submit, "t_admit": q.t_admit, "t_done": q.t_done,
+                "steps_executed": q.steps_executed, "t_submit": q.t_submit, "t_admit": q.t_admit, "t_start": q.t_start, "t_done": q.t_done,
                 "latency_s": q.latency_s, "footprint_bytes": q.footprint_bytes}
 
     def wait(self, qid: int, timeout: float = -1.0) -> bool:

@@ -420,18 +420,52 @@ def focused_smoothshock():
     return s
 
 
+def uniform_sphere_start():
+    """Forward Parker in a uniform flow and field started on a sphere (InitialSpaceSphere, two uniforms
+    before the momentum's), an absorbing sphere further out and a time limit; (r, p) map."""
+    s = cf.Setup("uniform_sphere_start", cf.TRAJ_PARKER, cf.FORWARD)
+    s.add(cf.BACKGROUND, "BackgroundUniform", cf.background_base(DC(), 0.0, (0.0, 0.0, 0.0), (0.01, -0.005, 0.002), (2.0, 1.0, 3.0), 0.4))
+    s.add(cf.DIFFUSION, "DiffusionFullConstant", DC().insert_double(0.5).insert_double(2.0))
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceSphere", DC().insert_vec((1.0, 2.0, 3.0)).insert_double(5.0))
+    s.add(cf.INITIAL, "InitialMomentumThickShell", DC().insert_double(0.05).insert_double(0.5).insert_bool(True))
+    s.add(cf.BOUNDARY, "BoundarySphereAbsorb", cf.boundary(1, [0], ("vec", (1.0, 2.0, 3.0)), ("double", 12.0)))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [1], ("double", 60.0)))
+    d = dist((10, 0, 8), (0.0, 0.0, 0.05), (15.0, 0.0, 0.5), (0, 0, 1), (1, 0, 1))
+    d.insert_double(1.0).insert_double(0.5).insert_int(1).insert_int(3).insert_int(0)
+    s.add(cf.DISTRIBUTION, "DistributionPositionMomentumUniform", d)
+    return s
+
+
+def gc_sector_start():
+    """Guiding centre with perpendicular diffusion and pitch-angle scattering in the Parker spiral, started on
+    a sector of the 1-au sphere (InitialSpaceSphereSector: theta in [1.2, 1.9], phi in [-0.3, 0.4])."""
+    s = cf.Setup("gc_sector_start", cf.TRAJ_GUIDING_DIFF_SCATT, cf.FORWARD)
+    s.add(cf.BACKGROUND, "BackgroundSolarWind", cf.solarwind())
+    s.add(cf.DIFFUSION, "DiffusionPerpConstantPitchIsotropic", DC().insert_double(8.6e19 / cf.unit_diffusion).insert_double(0.02))
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceSphereSector", DC().insert_vec((0.0, 0.0, 0.0)).insert_double(cf.one_au).insert_double(1.2)
+          .insert_double(1.9).insert_double(-0.3).insert_double(0.4))
+    s.add(cf.INITIAL, "InitialMomentumShell", DC().insert_double(cf.mom_from_ekin(30.0 * cf.MeV)))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", 0.02 * cf.one_day)))
+    d = dist((16, 16, 0), (0.9 * cf.one_au, -1.0, 0.0), (1.2 * cf.one_au, 1.0, 1.0), (0, 0, 0), (1, 0, 0))
+    d.insert_double(1.0).insert_double(0.0).insert_int(1)
+    s.add(cf.DISTRIBUTION, "DistributionPositionPitchUniform", d)
+    return s
+
+
 ALL = [shock_flowmom_reflect, solarwind_kinrad_reflect_recurrent, smoothshock_perp_momentum, uniform_para_plane_pass,
        lambda: modulation_spectrum_type(0), lambda: modulation_spectrum_type(2), gc_rigidity_reflect, gc_isotropic_only,
        guiding_spiral, guiding_scatt_uniform, guiding_diff_rigidity, parker_strauss_heliopause, parker_potgieter_uniform,
        gc_qlt_spiral, fieldline_spiral, fieldline_backward_shock, lorentz_uniform, lorentz_spiral_backward,
        focused_spiral, focused_uniform_backward, gc_wnlt_scatt_spiral, gc_wnlt_ramp_diffscatt,
-       gc_strauss_diff, focused_smoothshock]
+       gc_strauss_diff, focused_smoothshock, uniform_sphere_start, gc_sector_start]
 NAMES = ["shock_flowmom_reflect", "solarwind_kinrad_reflect_recurrent", "smoothshock_perp_momentum", "uniform_para_plane_pass",
          "modulation_type0", "modulation_type2", "gc_rigidity_reflect", "gc_isotropic_only",
          "guiding_spiral", "guiding_scatt_uniform", "guiding_diff_rigidity", "parker_strauss_heliopause", "parker_potgieter_uniform",
          "gc_qlt_spiral", "fieldline_spiral", "fieldline_backward_shock", "lorentz_uniform", "lorentz_spiral_backward",
          "focused_spiral", "focused_uniform_backward", "gc_wnlt_scatt_spiral", "gc_wnlt_ramp_diffscatt",
-         "gc_strauss_diff", "focused_smoothshock"]
+         "gc_strauss_diff", "focused_smoothshock", "uniform_sphere_start", "gc_sector_start"]
 
 
 # Setups whose endpoints amplify last-bit differences far beyond 1e-10: QLT's mu derivative is

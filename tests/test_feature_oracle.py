@@ -6,7 +6,9 @@ mathematics of that class fixes, independently of the kernels.
 * parallel-only diffusion: the displacement minus advection is parallel to B;
 * perpendicular-only diffusion with B along y and U along x: no displacement along y;
 * InitialMomentumThickShell with log spacing: log|p0| uniform between the shell radii;
-* momentum-inject boundary: every trajectory ends at or above the injection momentum or at the time limit.
+* momentum-inject boundary: every trajectory ends at or above the injection momentum or at the time limit;
+* InitialSpaceSphere / InitialSpaceSphereSector: starts on the sphere, uniform in area (the polar cosine and
+  the azimuth uniform, Kolmogorov-Smirnov), inside the sector's angles.
 """
 import numpy as np
 import pytest
@@ -76,3 +78,29 @@ def test_perpendicular_only_diffusion_and_momentum_boundaries(oracle_mod):
     at_limit = np.isclose(rec["t"], 20000.0, rtol=1e-7)
     assert ((p >= 1.6 * p0 * (1 - 1e-12)) | at_limit).all()
     assert (~at_limit).any()                               # some reach the injection momentum
+
+
+def test_sphere_and_sector_starts(oracle_mod):
+    from scipy import stats
+    _, _, rec = run(oracle_mod, F.uniform_sphere_start, 4000)
+    d = rec["pos0"] - np.array([1.0, 2.0, 3.0])
+    r = np.linalg.norm(d, axis=1)
+    np.testing.assert_allclose(r, 5.0, rtol=1e-14)
+    ct, phi = d[:, 2] / r, np.arctan2(d[:, 1], d[:, 0])
+    assert stats.kstest(ct, stats.uniform(loc=-1.0, scale=2.0).cdf).pvalue > 1e-3
+    assert stats.kstest(phi, stats.uniform(loc=-np.pi, scale=2 * np.pi).cdf).pvalue > 1e-3
+    # the momentum still takes its own uniforms: log p uniform over the shell
+    assert stats.kstest(np.log10(rec["mom0"][:, 0]), stats.uniform(loc=np.log10(0.05), scale=1.0).cdf).pvalue > 1e-3
+    assert abs(np.corrcoef(ct, np.log10(rec["mom0"][:, 0]))[0, 1]) < 4.0 / np.sqrt(len(ct))
+    s = F.gc_sector_start()
+    s.plugins = [x for x in s.plugins if not (x[0] == cf.BOUNDARY)]
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", 1e-6)))
+    _, rec = oracle_mod.Oracle(s).run(3000, seed=5, threads=THREADS)
+    d = rec["pos0"]
+    r = np.linalg.norm(d, axis=1)
+    np.testing.assert_allclose(r, cf.one_au, rtol=1e-14)
+    ct, phi = d[:, 2] / r, np.arctan2(d[:, 1], d[:, 0])
+    lo, hi = np.cos(1.9), np.cos(1.2)
+    assert ct.min() >= lo - 1e-12 and ct.max() <= hi + 1e-12 and phi.min() >= -0.3 - 1e-12 and phi.max() <= 0.4 + 1e-12
+    assert stats.kstest(ct, stats.uniform(loc=lo, scale=hi - lo).cdf).pvalue > 1e-3
+    assert stats.kstest(phi, stats.uniform(loc=-0.3, scale=0.7).cdf).pvalue > 1e-3

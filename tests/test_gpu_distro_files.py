@@ -82,4 +82,6 @@ def test_checkpoint_and_resume(tmp_path):
     cr, sr, nr = resumed.distro(0)
     np.testing.assert_array_equal(cw, cr)
     assert nw == nr
-    np.testing.assert_allclose(sr, sw, rtol=1e-12, atol=0)
+    # the checkpoint file holds each sum rounded to a double (the reference's format): resuming adds that
+    # rounding once, so the two differ by at most an ulp or two
+    np.testing.assert_allclose(sr, sw, rtol=5e-16, atol=0)

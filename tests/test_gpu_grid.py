@@ -113,7 +113,7 @@ def test_served_grid_query_equals_single_run(spiral):
     c1, s1, n1 = single.distro(0)
     np.testing.assert_array_equal(c, c1)
     assert ne == n1
-    np.testing.assert_allclose(sm, s1, rtol=1e-12, atol=0)
+    np.testing.assert_array_equal(sm, s1)
 
 
 def test_mixed_matches_fp64_within_mc_error(spiral):

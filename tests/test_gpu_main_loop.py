@@ -55,7 +55,7 @@ def test_main_loop_progress_and_final_dump(tmp_path, capfd):
     nev_f, c_f, s_f = _counts_sums(tmp_path / "m_0.out", c.size)
     assert nev_f == nev
     np.testing.assert_array_equal(c_f, c)
-    np.testing.assert_allclose(s_f, s, rtol=1e-12, atol=0)
+    np.testing.assert_array_equal(s_f, s)
 
 
 def test_main_loops_continue_ids_and_resume(tmp_path, capfd):
@@ -71,7 +71,7 @@ def test_main_loops_continue_ids_and_resume(tmp_path, capfd):
     c, s, nev = ref.distro(0)
     assert nev2 == nev
     np.testing.assert_array_equal(c2, c)
-    np.testing.assert_allclose(s2, s, rtol=1e-12, atol=0)
+    np.testing.assert_array_equal(s2, s)
     # RestoreDistro: the third loop starts from the second's file and adds trajectories 12000..17999
     sim.restore_distro_at_start(0)
     sim.main_loop(seed=5)
@@ -80,5 +80,5 @@ def test_main_loops_continue_ids_and_resume(tmp_path, capfd):
     c, s, nev = ref.distro(0)
     assert nev3 == nev
     np.testing.assert_array_equal(c3, c)
-    np.testing.assert_allclose(s3, s, rtol=1e-12, atol=0)
+    np.testing.assert_allclose(s3, s, rtol=5e-16, atol=0)   # restored from a file of rounded sums
     capfd.readouterr()

@@ -55,7 +55,7 @@ def test_parker_uniform_walk_msd(prec, flow):
 def test_gc_pitch_angle_isotropization(prec):
     mu0 = 0.8
     s = cf.gc_uniform(theta0=math.acos(mu0), t_limit=500.0)
-    n = 200000
+    n = 1000000
     _, st, rec = run(s, n, prec)
     assert st.failed == 0
     p = np.hypot(rec["mom"][:, 0], rec["mom"][:, 2])
@@ -81,43 +81,11 @@ def test_gc_isotropic_parallel_spread(prec):
     within(z2.mean(), z2.std() / math.sqrt(n), expect, "<z^2>")
 
 
-def dsa_index(counts, p_edges):
-    """Least-squares slope of log(counts per log bin) against log p, weighted by Poisson errors, over the
-    bins with at least 400 counts; returns (q, error of q) with counts per log bin ~ p^(3 - q)."""
-    pc = np.sqrt(p_edges[:-1] * p_edges[1:])
-    m = counts >= 400
-    x = np.log(pc[m])
-    y = np.log(counts[m])
-    w = counts[m].astype(float)      # 1 / var(log N) = N
-    A = np.vstack([x, np.ones_like(x)]).T
-    cov = np.linalg.inv(A.T @ (A * w[:, None]))
-    coef = cov @ (A.T @ (w * y))
-    return 3.0 - coef[0], math.sqrt(cov[0, 0]), int(m.sum())
-
-
-@pytest.mark.parametrize("prec", PRECS)
-def test_dsa_power_law_index(prec):
-    """Forward ParkerSource at a smooth shock with compression r = 4: counts per logarithmic momentum bin
-    ~ p^(3 - q) with q = 3 r / (r - 1) = 4 (benchmarks/main_postprocess_diff_shock_acc.cc:9-34).  The
-    shock width 0.01 kappa/u1 and the absorbing plane 4 kappa/u2 downstream steepen the spectrum by a few
-    percent of q; the bound allows 0.15 beyond the fit's statistical error."""
-    s = cf.dsa_shock()
-    n = 200000
-    sim, st, _ = run(s, n, prec, records=False)
-    counts, _, nev = sim.distro(0)
-    p0 = s.meta["p0"]
-    edges = p0 * np.logspace(0, 4, 129)
-    q, dq, nb = dsa_index(counts, edges)
-    print(f"[check] DSA index q = {q:.4f} +- {dq:.4f} over {nb} bins (expect 4)", flush=True)
-    assert nb >= 20
-    assert abs(q - 4.0) <= 0.15 + NSIG * dq
-
-
 def test_mixed_matches_fp64_modulation_spectrum():
-    """The reference problem (backward modulation to 90 au, 64 energies): the intensity estimate of every
-    energy bin from the mixed kernel agrees with the FP64 kernel's within the Monte Carlo error of the
-    difference, and the mean endpoint differs by far less than the spread of the endpoints."""
-    n_e, per = 64, 400
+    """The reference problem at the benchmark's per-energy scale (backward modulation to 90 au, 64 energies,
+    1e6 trajectories per precision): the intensity estimate of every energy bin from the mixed kernel agrees
+    with the FP64 kernel's within the Monte Carlo error of the difference, and the mean step count too."""
+    n_e, per = 64, 15625
     s = cf.parker_modulation(n_energies=n_e)
     n = n_e * per
     out = {}

@@ -1,8 +1,8 @@
 """The query server: continuous batching of different queries on shared slots, admission by device
 memory, and results that are the same alone, in a mix, and through the single-run interface.
 
-Counts and event totals must be identical; FP64 bin sums may differ only in the order of their
-floating-point atomic additions (tolerance 1e-12 relative).
+Counts, event totals and bin sums must be identical: sums are exact fixed-point accumulations, rounded
+once when read, so the order in which the waves of a mix bin their events cannot change them.
 """
 import time
 
@@ -14,7 +14,6 @@ from spectrum_amd import engine as E
 from spectrum_amd.server import Server
 
 pytestmark = pytest.mark.gpu
-SUM_RTOL = 1e-12
 
 
 def queries():
@@ -37,7 +36,7 @@ def same(a, b, what):
     for (c1, s1, n1), (c2, s2, n2) in zip(a, b):
         np.testing.assert_array_equal(c1, c2, err_msg=what)
         assert n1 == n2, what
-        np.testing.assert_allclose(s1, s2, rtol=SUM_RTOL, atol=0, err_msg=what)
+        np.testing.assert_array_equal(s1, s2, err_msg=what)
 
 
 def served(srv, setup, qid):
