@@ -328,7 +328,7 @@ typedef struct {
    double* data;
 } grid_t;
 enum { DF_NONE = 0, DF_ISO, DF_PARA, DF_PERP, DF_FULL, DF_FLOWMOM, DF_KINRAD, DF_RIGBMAG, DF_PERPISO, DF_STRAUSS, DF_POTGIETER, DF_QLT,
-       DF_WNLT, DF_WNLT_RAMP };
+       DF_WNLT, DF_WNLT_RAMP, DF_SOQLT };
 enum { BK_TIME = 0, BK_SPHERE, BK_PLANE, BK_MOMENTUM };
 enum { IT_FIXED = 0 };
 enum { IS_FIXED = 0, IS_SPHERE = 1, IS_SECTOR = 2 };
@@ -362,6 +362,8 @@ typedef struct {
    double A2T, A2L, ps_plus;                                         /* WNLTConstant */
    double l_max_HP, z_nose, z_sheath, k_min_ref, A2A_ref, A2T_ref, A2L_ref, dl_max, dz;   /* WNLTRampVLISM */
    double hyp_c, hyp_g, hyp_r;                                       /* 2F1(1, 1; c; z): c, Gamma(c) Gamma(2 - c), (c - 1) / (c - 2) */
+   double lam_para, lam_perp, kap_rat_red, radial_limit_perp_red, solar_cycle_effect;   /* EmpiricalSOQLTandUNLT */
+   int Bmix_idx, solar_cycle_idx;
 } df_t;
 
 typedef struct {
@@ -854,6 +856,24 @@ static double df_component(const df_t* df, kap_t* kp, int comp, vec3 pos, double
       }
       break;
    }
+   case DF_SOQLT: {  /* src/diffusion_other.cc:1053-1072: bent power laws in rigidity, one component per call */
+      if (comp == 2) break;
+      double lam = 0.0, rig_dep = 0.0;
+      double rig = Rigidity(p);
+      if (comp == 1) {
+         rig_dep = cbrt((rig / df->R0) * (1.0 + Sqr(Sqr(rig / df->R0))));
+         lam = df->lam_para;
+      }
+      else {
+         rig_dep = cbrt((rig / df->R0) * (1.0 + Sqr(rig / df->R0)));
+         double Bmix_ind = (sp->region[df->Bmix_idx] < 0.0 ? 0.0 : 1.0);
+         if (vnorm(pos) < df->radial_limit_perp_red) lam = df->lam_perp * (Bmix_ind + (1.0 - Bmix_ind) * df->kap_rat_red);
+         else lam = df->lam_perp;
+      }
+      kp->K[comp] = (lam * vmag / 3.0) * rig_dep * (df->B0 / sp->Bmag);
+      kp->K[comp] /= 1.0 + df->solar_cycle_effect * Sqr(cos(0.5 * sp->region[df->solar_cycle_idx]));
+      break;
+   }
    case DF_WNLT: case DF_WNLT_RAMP:
       if (comp == 1) break;
       wnlt_eval(df, kp, comp, pos, vmag, mu, sp->Bmag);
@@ -893,7 +913,8 @@ static double df_generic_dir_deriv(const df_t* df, kap_t* kp, int comp, int xyz,
 static double df_dir_deriv(const df_t* df, kap_t* kp, int comp, int xyz, vec3 pos, double p, double mu, const spd_t* sp)
 {
    switch (df->kind) {
-   case DF_WNLT: case DF_WNLT_RAMP: case DF_STRAUSS: case DF_POTGIETER: return df_generic_dir_deriv(df, kp, comp, xyz, pos, p, mu, sp);
+   case DF_WNLT: case DF_WNLT_RAMP: case DF_STRAUSS: case DF_POTGIETER: case DF_SOQLT:
+      return df_generic_dir_deriv(df, kp, comp, xyz, pos, p, mu, sp);
    case DF_PARA: case DF_PERP: case DF_FULL: case DF_PERPISO: return 0.0;
    case DF_FLOWMOM: return kp->K[comp] * df->pow_law_U * vdot(sp->gradU.row[xyz], sp->U) / Sqr(vnorm(sp->U));  /* :637-642 */
    case DF_KINRAD: {  /* :732-742 */
@@ -1915,6 +1936,12 @@ static int setup_diffusion(orc_sim* s, const char* cls, dcread* r)
       df->kind = !strcmp(cls, "DiffusionStraussEtAl2013") ? DF_STRAUSS : DF_POTGIETER;
       df->LISM_idx = rd_i(r); df->lam_in = rd_d(r); df->lam_out = rd_d(r); df->R0 = rd_d(r); df->B0 = rd_d(r);
       df->kap_rat_in = rd_d(r); df->kap_rat_out = rd_d(r);
+   }
+   else if (!strcmp(cls, "DiffusionEmpiricalSOQLTandUNLT")) {  /* src/diffusion_other.cc:1034-1046 */
+      df->kind = DF_SOQLT;
+      df->lam_para = rd_d(r); df->lam_perp = rd_d(r); df->R0 = rd_d(r); df->B0 = rd_d(r);
+      df->Bmix_idx = rd_i(r); df->kap_rat_red = rd_d(r); df->radial_limit_perp_red = rd_d(r);
+      df->solar_cycle_idx = rd_i(r); df->solar_cycle_effect = rd_d(r);
    }
    else if (!strcmp(cls, "DiffusionQLTConstant")) {  /* src/diffusion_other.cc:130-140 */
       df->kind = DF_QLT;

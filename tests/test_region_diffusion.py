@@ -1,5 +1,6 @@
-"""The oracle's DiffusionStraussEtAl2013, DiffusionPotgieterEtAl2015 (src/diffusion_other.cc:869-993) and
-DiffusionQLTConstant (:130-149) checked against what their published forms and the transports fix:
+"""The oracle's DiffusionStraussEtAl2013, DiffusionPotgieterEtAl2015 (src/diffusion_other.cc:869-993),
+DiffusionEmpiricalSOQLTandUNLT (:999-1082) and DiffusionQLTConstant (:130-149) checked against what their
+published forms and the transports fix:
 
 * Strauss / Potgieter in a uniform field with the Parker transport: the displacement variances
   <dz^2> = 2 kappa_par T along B and <dx^2 + dy^2> = 4 kappa_perp T across it, with
@@ -8,6 +9,9 @@ DiffusionQLTConstant (:130-149) checked against what their published forms and t
 * the LISM switch on the SolarWind background: inside the 117 au heliopause the inner parameters and
   B0 / |B| with |B| of the Parker spiral, beyond it the outer ones with no |B| factor; LISM_idx 1 (a
   unipolar-field indicator, always -1 here) selects the outer ones everywhere, -1 none;
+* the empirical SOQLT / UNLT model: both components at points of the solar wind against its bent power
+  laws, with the perpendicular reduction inside the reduction radius where the mixing indicator is
+  negative and the solar-cycle divisor; the displacement variances in a uniform field;
 * QLT pitch-angle scattering with GuidingScatt in a uniform field: over a short time <d mu> = dD/dmu t
   and <d mu^2> = 2 D t with D = (pi/4) (q-1) |Omega| (1 - mu^2) (v k_min |mu/Omega|)^(q-1) A2A, and a
   distribution uniform in mu stays uniform (the drift term is D's mu derivative).
@@ -23,12 +27,13 @@ THREADS = 8
 BZ = 3.0
 
 
-def uniform_parker(model, params, p, T=200.0, n_e=1):
+def uniform_parker(model, params, p, T=200.0, n_e=1, dc=None):
     s = cf.Setup("uniform_region", cf.TRAJ_PARKER, cf.BACKWARD)
     s.add(cf.BACKGROUND, "BackgroundUniform", cf.background_base(DC(), 0.0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, BZ), 1.0e4))
-    dc = DC().insert_int(params[0])
-    for v in params[1:]:
-        dc.insert_double(v)
+    if dc is None:
+        dc = DC().insert_int(params[0])
+        for v in params[1:]:
+            dc.insert_double(v)
     s.add(cf.DIFFUSION, model, dc)
     s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
     s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((0.0, 0.0, 0.0)))
@@ -167,3 +172,63 @@ def test_qlt_keeps_an_isotropic_distribution(oracle_mod):
     print(f"[qlt] KS {ks:.4f} (99% level {1.63 / np.sqrt(n):.4f}), {steps:.0f} steps per trajectory")
     assert steps > 100
     assert ks < 1.63 / np.sqrt(n)
+
+
+# DiffusionEmpiricalSOQLTandUNLT (src/diffusion_other.cc:1053-1072), written here from its published form:
+# kappa_par = (lam_par v / 3) (x (1 + x^4))^(1/3) (B0 / |B|) / S, kappa_perp = (lam_perp' v / 3) (x (1 + x^2))^(1/3) (B0 / |B|) / S
+# with x = R / R0, S = 1 + effect cos^2(region[solar_cycle_idx] / 2), and lam_perp' = lam_perp kap_rat_red
+# inside radial_limit_perp_red where the mixing indicator region[Bmix_idx] is negative.
+def soqlt_container(lam_par, lam_perp, R0, B0, bmix_idx, kr_red, r_lim, sc_idx, sc_eff):
+    return (DC().insert_double(lam_par).insert_double(lam_perp).insert_double(R0).insert_double(B0).insert_int(bmix_idx)
+            .insert_double(kr_red).insert_double(r_lim).insert_int(sc_idx).insert_double(sc_eff))
+
+
+def soqlt_want(p, Bmag, pos, region, lam_par, lam_perp, R0, B0, bmix_idx, kr_red, r_lim, sc_idx, sc_eff):
+    v = cf.vel_from_mom(p)
+    x = p * cf.c_code / R0
+    S = 1.0 + sc_eff * np.cos(region[sc_idx] / 2.0) ** 2
+    k1 = lam_par * v / 3.0 * (x + x ** 5) ** (1.0 / 3.0) * B0 / Bmag / S
+    reduced = np.linalg.norm(pos) < r_lim and region[bmix_idx] < 0.0
+    k0 = (lam_perp * kr_red if reduced else lam_perp) * v / 3.0 * (x + x ** 3) ** (1.0 / 3.0) * B0 / Bmag / S
+    return k0, k1
+
+
+@pytest.mark.parametrize("bmix_idx", [0, 1])
+@pytest.mark.parametrize("sc_idx", [0, 1])
+def test_soqlt_components_on_the_solar_wind(oracle_mod, bmix_idx, sc_idx):
+    """The solar wind's region indicators: region[0] = +1 inside the 117 au heliopause, -1 beyond;
+    region[1] = -1 (src/background_solarwind.cc:144-146).  Both rigidity branches of the bent power laws,
+    inside and outside the reduction radius and the heliopause."""
+    B0 = 5.0e-6 / cf.unit_magnetic
+    args = (0.3 * cf.one_au, 0.01 * cf.one_au, None, B0, bmix_idx, 0.25, 60.0 * cf.one_au, sc_idx, 0.4)
+    for T in (30.0 * cf.MeV, 3.0 * cf.GeV):
+        p = cf.mom_from_ekin(T)
+        a = list(args)
+        a[2] = cf.mom_from_ekin(300.0 * cf.MeV) * cf.c_code   # R0 between the two energies
+        s = cf.Setup("sw_soqlt", cf.TRAJ_PARKER, cf.BACKWARD)
+        s.add(cf.BACKGROUND, "BackgroundSolarWind", cf.solarwind())
+        s.add(cf.DIFFUSION, "DiffusionEmpiricalSOQLTandUNLT", soqlt_container(*a))
+        o = oracle_mod.Oracle(s)
+        for r_au in (5.0, 59.0, 61.0, 116.9, 117.1):
+            pos = np.array([0.6, 0.64, 0.48]) * r_au * cf.one_au
+            region = (1.0 if r_au < 117.0 else -1.0, -1.0, 0.0)
+            k0, k1 = soqlt_want(p, spiral_bmag(pos), pos, region, *a)
+            assert o.kappa_at(1, pos, p) == pytest.approx(k1, rel=1e-12), (T, r_au)
+            assert o.kappa_at(0, pos, p) == pytest.approx(k0, rel=1e-12), (T, r_au)
+
+
+def test_soqlt_uniform_field_variances(oracle_mod):
+    """Uniform background (every region indicator +1): <dz^2> = 2 kappa_par T, <dx^2 + dy^2> = 4 kappa_perp T."""
+    p = cf.mom_from_ekin(100.0 * cf.MeV)
+    a = (0.05, 0.004, 0.5 * p * cf.c_code, 6.0, 0, 0.25, 1.0e9, 2, 0.4)
+    s = uniform_parker("DiffusionEmpiricalSOQLTandUNLT", None, p, dc=soqlt_container(*a))
+    o = oracle_mod.Oracle(s)
+    n, T = 4000, 200.0
+    _, rec = o.run(n, seed=23, threads=THREADS)
+    assert (rec["status"] == 0).all()
+    dx = rec["pos"] - rec["pos0"]
+    k0, k1 = soqlt_want(p, BZ, np.zeros(3), (1.0, 1.0, 1.0), *a)
+    z2, r2 = np.mean(dx[:, 2] ** 2), np.mean(dx[:, 0] ** 2 + dx[:, 1] ** 2)
+    print(f"[soqlt] <dz^2>/2T {z2 / (2 * T):.5g} vs {k1:.5g}; <dr^2>/4T {r2 / (4 * T):.5g} vs {k0:.5g}")
+    assert abs(z2 / (2 * T) / k1 - 1.0) < 5.0 * np.sqrt(2.0 / n)
+    assert abs(r2 / (4 * T) / k0 - 1.0) < 5.0 * np.sqrt(1.0 / n)
