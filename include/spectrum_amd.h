@@ -182,6 +182,25 @@ int spx_distro_file_name(spx_sim* sim, const char* base);
 int spx_restore_distro_at_start(spx_sim* sim, int d);
 int spx_main_loop(spx_sim* sim, const spx_rng* rng, spx_run_stats* stats);
 
+/* Run without allocating (no counterpart in the reference): for callers that cannot allocate device
+   memory -- code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.  The
+   same integration as spx_run (counter streams keyed by `seed`, the sim's precision and steps per
+   launch), but every piece of device state -- launch block, query entry, plan, control block,
+   histograms and SPX_STATIC_LANES resident slots, 39.4 MiB in all -- lives in one static device array per GPU,
+   held until the process ends; no hipMalloc, hipFree or other allocating call is made.  Results go to
+   the caller's host arrays, all distributions in order: `counts` (n_bins of each, spx_distro_info),
+   `sums` (n_bins x width of each) and `n_events` (one per distribution); any may be NULL.  They equal
+   spx_get_distro's after spx_reset_distros + spx_run of the same trajectories, bit for bit; the sim's
+   own distributions are untouched.  Rejected: a gridded background (SPX_ERR_ARG; its grid is device
+   memory the library allocates), histograms beyond SPX_STATIC_BINS bins or SPX_STATIC_COMPONENTS
+   bin components (SPX_ERR_LIMIT).  Every call on one GPU shares the array and clears what must start
+   at zero; calls that overlap on one GPU race. */
+#define SPX_STATIC_LANES 65536
+#define SPX_STATIC_BINS 4096
+#define SPX_STATIC_COMPONENTS 4096
+int spx_run_static(spx_sim* sim, uint64_t first_traj, uint64_t n_traj, uint64_t seed, int64_t* counts, double* sums,
+                   int64_t* n_events, spx_run_stats* stats);
+
 const char* spx_last_error(const spx_sim* sim);
 /* Name, compute units and architecture of device 0 ("" and SPX_ERR_DEVICE without a GPU). */
 int spx_device_info(char* buf, size_t len);

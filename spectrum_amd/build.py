@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libspectrum_amd.so")
-SOURCES = ["spx_kernels_f64.hip", "spx_kernels_f32.hip", "spx_runtime.cpp", "spx_api.cpp"]
+SOURCES = ["spx_kernels_f64.hip", "spx_kernels_f32.hip", "spx_runtime.cpp", "spx_api.cpp", "spx_static.hip"]
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # FP64 kernels keep the reference's rounding: no contraction into multiply-adds

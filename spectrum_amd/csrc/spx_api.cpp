@@ -26,6 +26,7 @@
 
 #include "spx_kernels.h"
 #include "spx_runtime.h"
+#include "spx_static.h"
 
 using namespace spx;
 
@@ -919,6 +920,40 @@ int spx_run(spx_sim* s, uint64_t first, uint64_t n_traj, const spx_rng* rng, spx
    int rc = spx_begin(s, first, n_traj, rng, records != nullptr);
    if (rc) return rc;
    return spx_finish(s, records, st);
+}
+
+int spx_run_static(spx_sim* s, uint64_t first, uint64_t n_traj, uint64_t seed, int64_t* counts, double* sums, int64_t* n_events,
+                   spx_run_stats* st)
+{
+   if (!s) return SPX_ERR_ARG;
+   if (s->running) return fail(s, SPX_ERR_STATE, "static run during a run");
+   int rc = check_ready(s);
+   if (rc) return rc;
+   finalize_plan(s);
+   rc = check_transport(s);
+   if (rc) return rc;
+   if (s->plan.bg.kind == SPX_BG_GRID) return fail(s, SPX_ERR_ARG, "static run: a gridded background needs spx_run (its grid is allocated device memory)");
+   if (s->hist_bins > SPX_STATIC_BINS || acc_components(s->plan) > SPX_STATIC_COMPONENTS)
+      return fail(s, SPX_ERR_LIMIT, "static run: histograms beyond SPX_STATIC_BINS bins or SPX_STATIC_COMPONENTS bin components");
+   rc = ensure_device(s);
+   if (rc) return rc;
+   StaticResult res;
+   HIPCHK(s, static_run(s->plan, s->precision, s->steps_per_launch, first, n_traj, seed, counts, sums, n_events, res));
+   if (st) {
+      std::memset(st, 0, sizeof(*st));
+      st->n_traj = n_traj;
+      st->done = res.ctl.done;
+      st->failed = res.ctl.failed;
+      st->steps_done = res.ctl.steps_done;
+      st->steps_executed = res.ctl.steps_executed;
+      for (int k = 0; k < 8; k++) st->discards[k] = res.ctl.discards[k];
+      std::memcpy(&st->t_min, &res.ctl.tmin_bits, 8);
+      std::memcpy(&st->t_max, &res.ctl.tmax_bits, 8);
+      st->wall_s = res.wall_s;
+      st->launches = res.launches;
+      st->lanes = (int32_t)res.lanes;
+   }
+   return SPX_OK;
 }
 
 int spx_autotune(spx_sim* s, uint64_t n_probe, uint64_t seed, spx_tune_entry* table, int max_entries, int* n_entries)

@@ -66,6 +66,7 @@ def lib() -> C.CDLL:
         L.spx_set_option.argtypes = [C.c_void_p, C.c_char_p, C.c_double]
         L.spx_run.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(SpxRng), C.c_void_p, C.POINTER(SpxRunStats)]
         L.spx_begin.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(SpxRng), C.c_int]
+        L.spx_run_static.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(SpxRunStats)]
         L.spx_advance.argtypes = [C.c_void_p, C.c_int, C.POINTER(SpxRunStats)]
         L.spx_finish.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(SpxRunStats)]
         L.spx_distro_info.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
