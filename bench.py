@@ -103,7 +103,9 @@ def other_workloads(E, cf, args, flops_all):
                                       ("gc_diff_scatt_1e7", cf.gc_diff_scatt(), 10_000_000, "gc_diff_scatt", False)):
         res = {}
         for p in ("fp64", "mixed"):
-            leg = gpu_leg(E, cf, p, args, setup, n, autotune=tune)
+            leg = gpu_leg(E, cf, p, args, setup, n, autotune=tune and not args.no_autotune,
+                          lanes=args.dsa_lanes if (name.startswith("dsa") and args.no_autotune) else 0,
+                          steps_per_launch=args.dsa_steps_per_launch if (name.startswith("dsa") and args.no_autotune) else 0)
             rf = roofline(leg, p, flops_all[key], None)
             res[p] = {"value": leg["steps"] / leg["wall"], "avg_step_kernel_ms": rf["avg_launch_ms"], "roofline_frac": rf["frac"],
                       "achieved_tflops": rf["achieved"], "lanes": leg["lanes"]}
@@ -338,6 +340,10 @@ def main():
                     help="steps per slot per launch of the gridded legs (the autotuner's choice for them)")
     ap.add_argument("--grid-only", action="store_true",
                     help="run only the gridded-background legs and print their results (for profiling their kernels)")
+    ap.add_argument("--other-only", action="store_true",
+                    help="run only the DSA and guiding-centre legs and print their results (for profiling their kernels)")
+    ap.add_argument("--dsa-lanes", type=int, default=524288, help="DSA leg's slots without the autotuner (its usual choice)")
+    ap.add_argument("--dsa-steps-per-launch", type=int, default=4000, help="DSA leg's steps per launch without the autotuner")
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the autotuner on the modulation legs (for profiling: the trace then holds the timed launches only)")
     ap.add_argument("--serve-queries", type=int, default=16, help="served-stream leg: number of queries (0 = skip)")
@@ -372,6 +378,9 @@ def main():
     traffic_grid = load_json(os.path.join(ROOT, "profiles", "traffic_grid_step_kernel.json"))
     if args.grid_only:
         print(json.dumps({"grid_workloads": grid_workloads(E, cf, args, traffic_grid)}), flush=True)
+        return
+    if args.other_only:
+        print(json.dumps({"other_workloads": other_workloads(E, cf, args, flops_all)}), flush=True)
         return
 
     precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]

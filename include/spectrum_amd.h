@@ -112,9 +112,10 @@ int spx_set_background_grid(spx_sim* sim, const int32_t n_blocks[3], const doubl
    "DISTRO_KINETIC_ENERGY_POWER_LAW_TYPE" (src/distribution_other.hh:325), "max_trajectory_steps"
    (src/trajectory_base.hh:38), "max_attempts", "precision" (SPX_PREC_*), "block", "lanes",
    "steps_per_launch", "graph_launches" (0 = no graph capture), and "step_scale" in (0, 1]: a factor on
-   the physical time step of the Parker transports (TrajectoryParker::PhysicalStep's CFL and dlnpmax
-   constants, src/trajectory_parker.hh:32-41; 1 = the reference's step), for time-step convergence
-   studies of the Euler-Maruyama estimate. */
+   the physical time step of the Parker and guiding-centre transports (TrajectoryParker::PhysicalStep's
+   CFL and dlnpmax constants, src/trajectory_parker.hh:32-41; the guiding-centre, perpendicular and
+   pitch-angle limits of src/trajectory_guiding*.cc; 1 = the reference's step), for time-step
+   convergence studies of the Euler-Maruyama estimate. */
 int spx_set_option(spx_sim* sim, const char* name, double value);
 
 /* SetTasks + MainLoop (src/simulation.cc:509-519, :818-835): integrate trajectories

@@ -394,7 +394,7 @@ struct orc_sim {
    int n_dist; dist_t dist[MAX_DIST];
    int spectrum_type;          /* DISTRO_KINETIC_ENERGY_POWER_LAW_TYPE, src/distribution_other.hh:325 */
    int64_t max_steps;          /* max_trajectory_steps, src/trajectory_base.hh:38 */
-   double dt_scale;            /* "step_scale": factor on the Parker physical step (1 = the reference) */
+   double dt_scale;            /* "step_scale": factor on the physical step of the Parker and guiding-centre transports (1 = the reference) */
    int max_attempts;
    int64_t discards[8];
    grid_t grid; int have_grid;
@@ -1395,7 +1395,7 @@ static int gcds_advance(const orc_sim* s, lane_t* L, traj_t* T)
       T->dt_physical = fmin(T->dt_physical, 0.5 * Sqr(dmumax) / Dmumu);
       T->dt_physical = fmin(T->dt_physical, 0.5 * dmumax / fabs(Vmu));
    }
-   T->dt = fmin(T->dt_physical, T->dt_adaptive);
+   T->dt = fmin(s->dt_scale == 1.0 ? T->dt_physical : T->dt_physical * s->dt_scale, T->dt_adaptive);
    time_boundary_proximity(s, T);
 
    /* EulerPitchAngleScatt (src/trajectory_guiding_scatt.cc:81-106) */
@@ -1497,7 +1497,7 @@ static int gcg_advance(const orc_sim* s, lane_t* L, traj_t* T)
    vec3 drift, slope_mom;
    gcg_slopes(T, vel, &drift, &slope_mom);
    T->dt_physical = 0.5 * sp->dmax / (vnorm(drift) + 0.5 * vnorm(vel));
-   T->dt = fmin(T->dt_physical, T->dt_adaptive);
+   T->dt = fmin(s->dt_scale == 1.0 ? T->dt_physical : T->dt_physical * s->dt_scale, T->dt_adaptive);
    time_boundary_proximity(s, T);
    gc_finish_step(s, L, T, drift, slope_mom);
    return gc_end_step(s, L, T);
@@ -1524,7 +1524,7 @@ static int gcs_advance(const orc_sim* s, lane_t* L, traj_t* T)
       T->dt_physical = fmin(T->dt_physical, 0.5 * Sqr(dmumax) / Dmumu);
       T->dt_physical = fmin(T->dt_physical, 0.5 * dmumax / fabs(Vmu));
    }
-   T->dt = fmin(T->dt_physical, T->dt_adaptive);
+   T->dt = fmin(s->dt_scale == 1.0 ? T->dt_physical : T->dt_physical * s->dt_scale, T->dt_adaptive);
    time_boundary_proximity(s, T);
    {
       double mu_new;
@@ -1560,7 +1560,7 @@ static int gcd_advance(const orc_sim* s, lane_t* L, traj_t* T)
    vec3 slope_pos = vadd(drift, Vperp);
    T->dt_physical = 0.5 * sp->dmax / (vnorm(vadd(drift, Vperp)) + 0.5 * vnorm(vel));
    T->dt_physical = fmin(T->dt_physical, 0.5 * Sqr(sp->dmax) / Dperp);
-   T->dt = fmin(T->dt_physical, T->dt_adaptive);
+   T->dt = fmin(s->dt_scale == 1.0 ? T->dt_physical : T->dt_physical * s->dt_scale, T->dt_adaptive);
    time_boundary_proximity(s, T);
    vec3 mconv = gc_convert(T->mom);
    double dWx = sqrt(T->dt) * draw_normal(L);

@@ -120,36 +120,34 @@ inline void grid_units(spx_grid_desc& g)
    g.f_mag = fluid_magnetic / unit_magnetic;
 }
 
-// Zone records of a grid on the host and, once a run needs them, in device memory (FP64 and FP32
-// copies).  Shared by a simulation and the queries served from it, so it lives as long as any of them.
+// Zone records of a grid on the host and, once a run needs them, in device memory (the FP64 zone array and
+// the FP32 cell records, spx_grid_desc::z64 / c32).  Shared by a simulation and the queries served from it,
+// so it lives as long as any of them.
 struct GridData {
    spx_grid_desc desc{};
    std::vector<double> rec;            // [nz2][nz1][nz0][SPX_GRID_REC]
-   DevBuf<double> c64;                 // cell records, 8 corners each (spx_grid_desc::c64 / c32)
-   DevBuf<float> c32;
+   DevBuf<double> z64;
+   DevBuf<float> c32;                  // 8 corners per zone
    std::mutex mu;
    bool uploaded = false;
-   size_t bytes() const { return rec.size() * 8 * (sizeof(double) + sizeof(float)); }
-   // the zone records go up once and are expanded into cell records on the device
+   size_t bytes() const { return rec.size() * (sizeof(double) + 8 * sizeof(float)); }
+   // the zone records go up once and are expanded into the FP32 cell records on the device
    hipError_t upload()
    {
       std::lock_guard<std::mutex> lk(mu);
       if (uploaded) return hipSuccess;
-      DevBuf<double> zones;
-      hipError_t e = zones.alloc(rec.size());
-      if (e == hipSuccess) e = c64.alloc(rec.size() * 8);
+      hipError_t e = z64.alloc(rec.size());
       if (e == hipSuccess) e = c32.alloc(rec.size() * 8);
-      if (e == hipSuccess) e = hipMemcpy(zones.p, rec.data(), sizeof(double) * rec.size(), hipMemcpyHostToDevice);
-      if (e == hipSuccess) e = grid_expand_cells(zones.p, desc.nz, c64.p, c32.p, nullptr);
+      if (e == hipSuccess) e = hipMemcpy(z64.p, rec.data(), sizeof(double) * rec.size(), hipMemcpyHostToDevice);
+      if (e == hipSuccess) e = grid_expand_cells(z64.p, desc.nz, c32.p, nullptr);
       if (e == hipSuccess) e = hipDeviceSynchronize();
-      zones.release();
       if (e != hipSuccess) return e;
-      desc.c64 = c64.p;
+      desc.z64 = z64.p;
       desc.c32 = c32.p;
       uploaded = true;
       return hipSuccess;
    }
-   ~GridData() { c64.release(); c32.release(); }
+   ~GridData() { z64.release(); c32.release(); }
 };
 
 // What the distribution file formats need beyond the plan: the class name, the configuration bytes
@@ -166,6 +164,7 @@ struct DistMeta {
 
 struct spx_sim {
    spx_plan_desc plan;
+   int32_t soqlt_sc_idx = 0;            // DiffusionEmpiricalSOQLTandUNLT's solar_cycle_idx (checked against the background)
    bool have_bg = false, have_df = false, have_it = false, have_is = false, have_im = false;
    int precision = SPX_PREC_FP64;
    int block = 256;
@@ -298,6 +297,17 @@ static int add_diffusion(spx_sim* s, const std::string& cls, Reader& r)
       d.kind = cls == "DiffusionStraussEtAl2013" ? SPX_DF_STRAUSS : SPX_DF_POTGIETER;
       d.region_idx = r.i(); d.lam_in = r.d(); d.lam_out = r.d(); d.R0 = r.d(); d.B0 = r.d();
       d.kap_rat_in = r.d(); d.kap_rat_out = r.d();
+   }
+   else if (cls == "DiffusionEmpiricalSOQLTandUNLT") {   // src/diffusion_other.cc:1034-1046
+      d.kind = SPX_DF_SOQLT;
+      d.lam_para = r.d(); d.lam_perp = r.d(); d.R0 = r.d(); d.B0 = r.d();
+      d.region_idx = r.i(); d.kap_rat_red = r.d(); d.r_lim_red = r.d();
+      s->soqlt_sc_idx = r.i();
+      const double effect = r.d();
+      // Kappa /= 1 + effect Sqr(cos(0.5 region[solar_cycle_idx])) (:1071): the indicator is +1 or -1 on the
+      // backgrounds this model runs on and cos is even, so the divisor is one number
+      const double c = std::cos(0.5 * 1.0);
+      d.sc_den = 1.0 + effect * (c * c);
    }
    else if (cls == "DiffusionQLTConstant") {   // src/diffusion_other.cc:130-140
       d.kind = SPX_DF_QLT;
@@ -534,7 +544,7 @@ static void finalize_plan(spx_sim* s)
    if (d.kind == SPX_DF_FLOWMOM) d.pos_dep = (d.pow_law_U != 0.0);
    if (d.kind == SPX_DF_KINRAD) d.pos_dep = 1;
    if (d.kind == SPX_DF_RIGBMAG) d.pos_dep = (d.pow_law_B != 0.0);
-   if (d.kind == SPX_DF_STRAUSS || d.kind == SPX_DF_POTGIETER) d.pos_dep = 1;   // through |B| and the region
+   if (d.kind == SPX_DF_STRAUSS || d.kind == SPX_DF_POTGIETER || d.kind == SPX_DF_SOQLT) d.pos_dep = 1;   // through |B| and the region
    spx_df_t<float>& e = P.dff;
    e.kind = d.kind; e.stream_dep_idx = d.stream_dep_idx; e.pos_dep = d.pos_dep; e.region_idx = d.region_idx;
    e.lam_in = (float)d.lam_in; e.lam_out = (float)d.lam_out; e.kap_rat_in = (float)d.kap_rat_in; e.kap_rat_out = (float)d.kap_rat_out;
@@ -543,6 +553,8 @@ static void finalize_plan(spx_sim* s)
    e.l_max_HP = (float)d.l_max_HP; e.z_nose = (float)d.z_nose; e.z_sheath = (float)d.z_sheath; e.k_min_ref = (float)d.k_min_ref;
    e.A2A_ref = (float)d.A2A_ref; e.A2T_ref = (float)d.A2T_ref; e.A2L_ref = (float)d.A2L_ref; e.dl_max = (float)d.dl_max; e.dz = (float)d.dz;
    e.hyp_c = (float)d.hyp_c; e.hyp_g = (float)d.hyp_g; e.hyp_r = (float)d.hyp_r;
+   e.lam_para = (float)d.lam_para; e.lam_perp = (float)d.lam_perp; e.kap_rat_red = (float)d.kap_rat_red;
+   e.r_lim_red = (float)d.r_lim_red; e.sc_den = (float)d.sc_den;
    e.D0 = (float)d.D0; e.Dperp = (float)d.Dperp; e.Dpara = (float)d.Dpara; e.kap0 = (float)d.kap0; e.U0 = (float)d.U0;
    e.pow_law_U = (float)d.pow_law_U; e.p0 = (float)d.p0; e.pow_law_p = (float)d.pow_law_p; e.kap_rat = (float)d.kap_rat;
    e.T0 = (float)d.T0; e.r0 = (float)d.r0; e.pow_law_T = (float)d.pow_law_T; e.pow_law_r = (float)d.pow_law_r;
@@ -626,6 +638,21 @@ static int check_transport(spx_sim* s)
          return fail(s, SPX_ERR_ARG, "the WNLT models with perpendicular diffusion need the SolarWind background");
       if (fabs(s->plan.df.hyp_c - 2.0) < 1e-3)
          return fail(s, SPX_ERR_ARG, "the WNLT models: (5 + ps_index) / 4 within 1e-3 of 2 (ps_index near 3) is not supported");
+   }
+   if (s->plan.df.kind == SPX_DF_SOQLT) {
+      // the region indicators it reads (src/diffusion_other.cc:1062-1071): SolarWind sets region[0] (+1 inside
+      // the heliopause, -1 beyond) and region[1] (-1) only (src/background_solarwind.cc:144-148), Uniform sets
+      // all three to +1 (src/background_uniform.cc:68); a negative Bmix_idx reads region[-1] in the reference
+      const int tt = s->plan.traj_type, bk = s->plan.bg.kind, bi = s->plan.df.region_idx, si = s->soqlt_sc_idx;
+      if (bk != SPX_BG_SOLARWIND && bk != SPX_BG_UNIFORM)
+         return fail(s, SPX_ERR_ARG, "DiffusionEmpiricalSOQLTandUNLT runs on the SolarWind and Uniform backgrounds");
+      const int max_idx = bk == SPX_BG_SOLARWIND ? 1 : 2;
+      if (bi < 0 || bi > max_idx || si < 0 || si > max_idx)
+         return fail(s, SPX_ERR_ARG, "DiffusionEmpiricalSOQLTandUNLT: Bmix_idx / solar_cycle_idx name a region indicator this background does not set");
+      // it sets no component 2 (:1055), so no pitch-angle transport; GuidingDiff's drift steps over the
+      // background's numerical derivative steps, which only the solar wind sets
+      if (tt != SPX_TRAJ_PARKER && tt != SPX_TRAJ_PARKER_SOURCE && !(tt == SPX_TRAJ_GUIDING_DIFF && bk == SPX_BG_SOLARWIND))
+         return fail(s, SPX_ERR_ARG, "DiffusionEmpiricalSOQLTandUNLT runs with the Parker transports, or GuidingDiff on the SolarWind background");
    }
    if (s->plan.df.kind == SPX_DF_STRAUSS || s->plan.df.kind == SPX_DF_POTGIETER) {
       // the region indicators the backgrounds define: SolarWind sets region[0] (heliopause) and region[1] (-1) only
@@ -730,7 +757,7 @@ int spx_set_background_grid(spx_sim* s, const int32_t n_blocks[3], const double 
       g.inner_lo[a] = lo + margin;
       g.inner_hi[a] = hi - margin;
    }
-   // the device holds 8 corner records per zone in both precisions (768 B per zone): 2^28 zones are 206 GB
+   // the device holds the FP64 zone record and 8 FP32 corner records per zone (320 B): 2^28 zones are 86 GB
    if (nzone > ((int64_t)1 << 28)) return fail(s, SPX_ERR_LIMIT, "grid too large (more than 2^28 zones)");
    G->rec.assign((size_t)nzone * SPX_GRID_REC, 0.0);
    // ReadCartesianData's reordering (src/reader_cartesian.cc:103-140): blocks x fastest, zones x fastest

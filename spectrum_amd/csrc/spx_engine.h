@@ -401,6 +401,7 @@ static __device__ void finish_record(QueryPtr Q, LaunchPtr Lp, uint64_t slot, ui
 struct ColdLDS {
    double amp[BLOCK], wgt[BLOCK], dt_adapt[BLOCK], dt_prev[BLOCK], src_old[BLOCK];
    int32_t cross[SPX_MAX_BND][BLOCK];
+   float kc[SPX_KC][BLOCK];             // FP32 step: compensation terms of the FP64 sums (kahan_add)
 };
 
 
@@ -621,7 +622,11 @@ __device__ __forceinline__ void service_slot(LaunchPtr L, StartFn&& set_start)
       }
       L->nu[slot] = nu;
       L->attempt[slot] = att;
-      if (exc == 0) st = SLOT_RUN;
+      if (exc == 0) {
+         st = SLOT_RUN;
+#pragma unroll
+         for (int k = 0; k < SPX_KC; k++) L->kc[k * L->n_lanes + slot] = 0.0f;
+      }
       else {
          finish_record(Q, L, slot, traj, exc, att, 0, 0, P->it_t, P->is_pos[0], P->is_pos[1], P->is_pos[2], 0.0, 0.0, 0.0, 0.0, 0.0);
          st = SLOT_IDLE;

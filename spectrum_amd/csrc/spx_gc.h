@@ -172,8 +172,8 @@ __device__ __noinline__ void wnlt_k(const SPX_CP spx_df_t<R>& df, bool comp2, do
 // DiffusionBase::GetComponent(0) (src/diffusion_base.cc:105-121) of the models usable with the
 // guiding centre; |p| = pmag, mu the pitch cosine, (x, y, z) the position, outer the LISM indicator the
 // Strauss / Potgieter models read (src/diffusion_other.cc:886-906, :968-988).
-// XD compiles in the models that depend on mu and position (WNLT) or on the region (Strauss / Potgieter),
-// kept out of the kernels of the other models.
+// XD compiles in the models that depend on mu and position (WNLT) or on the region (Strauss / Potgieter,
+// SOQLT), kept out of the kernels of the other models.
 template <typename R, bool XD>
 __device__ __forceinline__ R gc_kperp(const SPX_CP spx_df_t<R>& df, R pmag, R mu, R Bmag, double x, double y, double z, bool outer)
 {
@@ -185,6 +185,12 @@ __device__ __forceinline__ R gc_kperp(const SPX_CP spx_df_t<R>& df, R pmag, R mu
       if (df.kind == SPX_DF_WNLT || df.kind == SPX_DF_WNLT_RAMP) {
          R K0, K2;
          wnlt_k(df, false, x, y, z, pmag, mu, Bmag, K0, K2);
+         return K0;
+      }
+      if (df.kind == SPX_DF_SOQLT) {   // :1053-1072; `outer` is region[Bmix_idx] < 0, the radius is this point's
+         const bool reduced = outer && norm(mk<double>(x, y, z)) < (double)df.r_lim_red;
+         R K0 = kappa_mom_soqlt(df, pmag, 0, reduced) * (df.B0 / Bmag);
+         K0 = K0 / df.sc_den;
          return K0;
       }
    }
@@ -241,7 +247,8 @@ __device__ __forceinline__ void gc_perp(const SPX_CP spx_df_t<R>& df, const GcFi
                                         double x, double y, double z, bool outer, R& Dperp, v3<R>& Vperp)
 {
    Dperp = gc_kperp<R, XD>(df, pmag, mu, F.Bmag, x, y, z, outer);
-   if (XD && (df.kind == SPX_DF_WNLT || df.kind == SPX_DF_WNLT_RAMP || df.kind == SPX_DF_STRAUSS || df.kind == SPX_DF_POTGIETER)) {
+   if (XD && (df.kind == SPX_DF_WNLT || df.kind == SPX_DF_WNLT_RAMP || df.kind == SPX_DF_STRAUSS || df.kind == SPX_DF_POTGIETER ||
+              df.kind == SPX_DF_SOQLT)) {
       // DiffusionBase::GetDirectionalDerivative (src/diffusion_base.cc:130-170): along each Cartesian axis,
       // kappa_perp at the position moved by r_g with |B| carried by its gradient, and back by 2 r_g; the
       // region indicator stays the point's (the background is not evaluated again)
@@ -417,8 +424,10 @@ __device__ __forceinline__ int gc_step(PlanPtr P, LaunchPtr L, const spx_rng_des
    const R mu = pl / norm(mk<R>(pp, R(0), pl));
    R Dperp = R(0);
    v3<R> Vperp = mk<R>(R(0), R(0), R(0));
-   // the LISM indicator of the Strauss / Potgieter models at the point of the last field evaluation
-   const bool outer = XD && (df.kind == SPX_DF_STRAUSS || df.kind == SPX_DF_POTGIETER) && lism_outer<R, BG>(bg, df, H.x, H.y, H.z);
+   // the LISM indicator of the Strauss / Potgieter models (the mixing indicator of the SOQLT model) at the
+   // point of the last field evaluation
+   const bool outer = XD && (df.kind == SPX_DF_STRAUSS || df.kind == SPX_DF_POTGIETER || df.kind == SPX_DF_SOQLT) &&
+                      lism_outer<R, BG>(bg, df, H.x, H.y, H.z);
    if (perp_step) gc_perp<R, XD>(df, H.F, fa, pmag, mu, H.x, H.y, H.z, outer, Dperp, Vperp);
    R Dmumu = R(0), Vmu = R(0);
    if (pa_step) gc_dmumu<R, XD>(df, pmag, H.F.Bmag, mu, H.x, H.y, H.z, Dmumu, Vmu);
@@ -435,7 +444,7 @@ __device__ __forceinline__ int gc_step(PlanPtr P, LaunchPtr L, const spx_rng_des
       dtp = fmin(dtp, R(0.5) * sqr(dmumax) / Dmumu);
       dtp = fmin(dtp, R(0.5) * dmumax / fabs(Vmu));
    }
-   double dt = fmin((double)dtp, C.dt_adapt[tid]);
+   double dt = fmin(P->dt_scale == 1.0 ? (double)dtp : (double)dtp * P->dt_scale, C.dt_adapt[tid]);   // 1: the reference's step
    dt = time_proximity<1>(P, H, dt, (double)H.F.dmax);
    const R dtr = R(dt);
    // EulerPitchAngleScatt: redraw until |mu'| <= 1
@@ -477,11 +486,20 @@ __device__ __forceinline__ int gc_step(PlanPtr P, LaunchPtr L, const spx_rng_des
    }
    // RKStep (Euler), then the perpendicular displacement
    const R dtv = dtr * R(1);
-   H.t += dt;
-   H.x += (double)(dtv * slope_pos.x); H.y += (double)(dtv * slope_pos.y); H.z += (double)(dtv * slope_pos.z);
-   H.pp = ppd + (double)(dtv * slope_pp);
-   H.pl = pld + (double)(dtv * slope_pl);
-   if (perp_step) { H.x += (double)dr.x; H.y += (double)dr.y; H.z += (double)dr.z; }
+   if constexpr (is_f32<R>()) {   // compensated FP64 sums; drift and perpendicular displacement added in FP32 first
+      const v3<R> d = perp_step ? dtv * slope_pos + dr : dtv * slope_pos;
+      kahan_add(H.t, C.kc[0][tid], dt);
+      kahan_add(H.x, C.kc[1][tid], (double)d.x); kahan_add(H.y, C.kc[2][tid], (double)d.y); kahan_add(H.z, C.kc[3][tid], (double)d.z);
+      H.pp = ppd + (double)(dtv * slope_pp);
+      H.pl = pld + (double)(dtv * slope_pl);
+   }
+   else {
+      H.t += dt;
+      H.x += (double)(dtv * slope_pos.x); H.y += (double)(dtv * slope_pos.y); H.z += (double)(dtv * slope_pos.z);
+      H.pp = ppd + (double)(dtv * slope_pp);
+      H.pl = pld + (double)(dtv * slope_pl);
+      if (perp_step) { H.x += (double)dr.x; H.y += (double)dr.y; H.z += (double)dr.z; }
+   }
    bflags = handle_boundaries<true>(P, L, slot, H, C, tid);
    if (!(bflags & 1)) {
       H.pf = sqrt(H.pp * H.pp + H.pl * H.pl);
@@ -523,6 +541,10 @@ __global__ __launch_bounds__(BLOCK) SPX_GC_ATTR void gc_step_kernel(const Launch
    if (run) {
       const uint64_t n = Lp->n_lanes;
       H.t = Lp->t[slot]; H.x = Lp->x[slot]; H.y = Lp->y[slot]; H.z = Lp->z[slot]; H.pp = Lp->m0[slot]; H.pl = Lp->m2[slot];
+      if constexpr (is_f32<R>()) {
+#pragma unroll
+         for (int k = 0; k < 4; k++) C.kc[k][tid] = Lp->kc[k * n + slot];
+      }
       H.pf = Lp->pf[slot];
       H.steps = (int32_t)Lp->steps[slot];
       H.rs.traj = Lp->traj[slot]; H.rs.nn = Lp->nn[slot]; H.rs.spare_ok = false; H.rs.exhausted = false; H.rs.s0 = H.rs.s1 = H.rs.s2 = R(0);
@@ -562,6 +584,10 @@ __global__ __launch_bounds__(BLOCK) SPX_GC_ATTR void gc_step_kernel(const Launch
    Ls->active[slot] = st;
    Ls->ex_code[slot] = exc;
    Ls->t[slot] = H.t; Ls->x[slot] = H.x; Ls->y[slot] = H.y; Ls->z[slot] = H.z; Ls->m0[slot] = H.pp; Ls->m2[slot] = H.pl;
+   if constexpr (is_f32<R>()) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) Ls->kc[k * n + slot] = C.kc[k][tid];
+   }
    Ls->pf[slot] = H.pf;
    Ls->steps[slot] = H.steps; Ls->nn[slot] = H.rs.nn;
    Ls->dmax[slot] = (double)H.F.dmax;
@@ -590,9 +616,9 @@ template <typename R>
 hipError_t dispatch_gc(const spx_plan_desc& hp, uint64_t n_lanes, const Launch* d_launch, hipStream_t st, hipEvent_t mid)
 {
    if (hp.time_flow != 0) return hipErrorInvalidValue;
-   // the WNLT and Strauss / Potgieter models run in kernels of their own (SolarWind, Uniform)
+   // the WNLT, Strauss / Potgieter and SOQLT models run in kernels of their own (SolarWind, Uniform)
    const int k = hp.df.kind;
-   const bool xd = (k == SPX_DF_WNLT || k == SPX_DF_WNLT_RAMP || k == SPX_DF_STRAUSS || k == SPX_DF_POTGIETER);
+   const bool xd = (k == SPX_DF_WNLT || k == SPX_DF_WNLT_RAMP || k == SPX_DF_STRAUSS || k == SPX_DF_POTGIETER || k == SPX_DF_SOQLT);
    switch (hp.bg.kind) {
    case SPX_BG_SOLARWIND:
       return xd ? launch_gc_bg<R, SPX_BG_SOLARWIND, true>(n_lanes, d_launch, st, mid) : launch_gc_bg<R, SPX_BG_SOLARWIND, false>(n_lanes, d_launch, st, mid);

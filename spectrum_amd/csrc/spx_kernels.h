@@ -52,6 +52,7 @@ struct Launch {
    int32_t* active;
    double *t, *x, *y, *z, *m0, *m1, *m2, *amp, *wgt, *dt_prev, *src_old, *ds_prev, *dmax, *dt_adapt;
    double* pf;                          // momentum magnitude of the slot's last field evaluation (guiding centre)
+   float* kc;                           // SPX_KC * n_lanes: compensation terms of the FP32-step kernels' FP64 sums
    double *bdelta, *bdelta_old;
    int32_t *bcross, *bstat;
    uint64_t* traj;
@@ -71,6 +72,10 @@ struct Launch {
    int32_t* ex_code;                    // discard reason of the last step kernel
 };
 
+// Compensated FP64 accumulation of the FP32-step kernels: the sums t, x, y, z and p (Parker) or
+// t, x, y, z (guiding centre) each carry a Kahan compensation term, kept per slot across launches.
+#define SPX_KC 5
+
 // Host-side dispatch.  One engine launch is the service kernel followed by the step kernel on
 // `stream`; when `mid` is not null it is recorded between the two, so the caller can time each
 // kernel with events.  launch_f64 / launch_f32 pick the instantiation for (transport, background,
@@ -80,10 +85,10 @@ hipError_t launch_f64(const spx_plan_desc& host_plan, uint64_t n_lanes, const La
 hipError_t launch_f32(const spx_plan_desc& host_plan, uint64_t n_lanes, const Launch* dev_launch,
                       hipStream_t stream, hipEvent_t mid);
 
-// Cell records of a Cartesian grid (spx_grid_desc::c64 / c32) from its zone records (nz[0] x nz[1] x nz[2],
-// x fastest, SPX_GRID_REC doubles each): for every zone, the 8 zones of the stencil whose lowest corner it
-// is, in corner order i + 2j + 4k (corners beyond the last zone repeat it; no stencil reads them).
-hipError_t grid_expand_cells(const double* zones, const int32_t nz[3], double* c64, float* c32, hipStream_t st);
+// FP32 cell records of a Cartesian grid (spx_grid_desc::c32) from its device zone records (nz[0] x nz[1] x
+// nz[2], x fastest, SPX_GRID_REC doubles each): for every zone, the 8 zones of the stencil whose lowest
+// corner it is, in corner order i + 2j + 4k (corners beyond the last zone repeat it; no stencil reads them).
+hipError_t grid_expand_cells(const double* zones, const int32_t nz[3], float* c32, hipStream_t st);
 
 inline hipError_t launch_transport(const spx_plan_desc& host_plan, uint64_t n_lanes, const Launch* dev_launch,
                                    int precision, hipStream_t stream, hipEvent_t mid)

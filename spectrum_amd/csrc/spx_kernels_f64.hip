@@ -10,7 +10,7 @@
 namespace spx {
 
 __global__ __launch_bounds__(256) void grid_cells_kernel(const double* __restrict__ zones, int32_t n0, int32_t n1, int32_t n2,
-                                                         double* __restrict__ c64, float* __restrict__ c32)
+                                                         float* __restrict__ c32)
 {
    const int64_t total = (int64_t)n0 * n1 * n2 * 8;
    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
@@ -22,19 +22,16 @@ __global__ __launch_bounds__(256) void grid_cells_kernel(const double* __restric
       const int64_t z = gz + (c >> 2) < n2 ? gz + (c >> 2) : n2 - 1;
       const double* src = zones + (x + n0 * (y + (int64_t)n1 * z)) * SPX_GRID_REC;
 #pragma unroll
-      for (int m = 0; m < SPX_GRID_REC; m++) {
-         c64[t * SPX_GRID_REC + m] = src[m];
-         c32[t * SPX_GRID_REC + m] = (float)src[m];
-      }
+      for (int m = 0; m < SPX_GRID_REC; m++) c32[t * SPX_GRID_REC + m] = (float)src[m];
    }
 }
 
-hipError_t grid_expand_cells(const double* zones, const int32_t nz[3], double* c64, float* c32, hipStream_t st)
+hipError_t grid_expand_cells(const double* zones, const int32_t nz[3], float* c32, hipStream_t st)
 {
    const int64_t total = (int64_t)nz[0] * nz[1] * nz[2] * 8;
    const int64_t blocks = (total + 255) / 256;
    hipLaunchKernelGGL(grid_cells_kernel, dim3((unsigned)(blocks < 65536 ? blocks : 65536)), dim3(256), 0, st, zones, nz[0], nz[1], nz[2],
-                      c64, c32);
+                      c32);
    return hipGetLastError();
 }
 

@@ -133,9 +133,10 @@ __device__ __forceinline__ bool lism_outer(const SPX_CP spx_bg_t<R>& bg, const S
 }
 
 // DiffusionBase::GetComponent for components 0 and 1 at a position.  DF is the diffusion class when it
-// is known at compile time (0 = any, dispatched at run time); km2 is the LISM factor of the region models.
+// is known at compile time (0 = any, dispatched at run time); km2 is the LISM factor of the region models,
+// or with km3 the two perpendicular factors of the empirical SOQLT model (lam_perp, lam_perp kap_rat_red).
 template <typename R, int BG, int DF>
-__device__ __forceinline__ void parker_kappa(const SPX_CP spx_bg_t<R>& bg, const SPX_CP spx_df_t<R>& df, R km, R km2, R Bmag,
+__device__ __forceinline__ void parker_kappa(const SPX_CP spx_bg_t<R>& bg, const SPX_CP spx_df_t<R>& df, R km, R km2, R km3, R Bmag,
                                              double x, double y, double z, v3<R> U, R& K0, R& K1)
 {
    if (DF == SPX_DF_RIGBMAG) {   // src/diffusion_other.cc:804-809; pow(x, 0) == 1 exactly
@@ -149,6 +150,16 @@ __device__ __forceinline__ void parker_kappa(const SPX_CP spx_bg_t<R>& bg, const
       const bool outer = lism_outer<R, BG>(bg, df, x, y, z);
       K1 = (outer ? km2 : km) * ((outer ? Bmag : df.B0) / Bmag);
       K0 = (outer ? df.kap_rat_out : df.kap_rat_in) * K1;
+      return;
+   }
+   if (df.kind == SPX_DF_SOQLT) {                                         // :1053-1072
+      // the reduced perpendicular path inside r_lim_red where region[Bmix_idx] < 0 (lism_outer reads it)
+      const bool reduced = lism_outer<R, BG>(bg, df, x, y, z) && norm(mk<double>(x, y, z)) < (double)df.r_lim_red;
+      const R bf = df.B0 / Bmag;
+      K1 = km * bf;
+      K1 = K1 / df.sc_den;
+      K0 = (reduced ? km3 : km2) * bf;
+      K0 = K0 / df.sc_den;
       return;
    }
    R r = R(0), Umag = R(0);
@@ -275,7 +286,14 @@ __device__ __forceinline__ int fieldline_step(PlanPtr P, LaunchPtr L, uint64_t s
    dt = time_proximity<FWD>(P, H, dt, (double)H.F.dmax);
    const v3<R> slope_pos = v * H.F.bhat;
    const R dtv = R(dt) * R(1);
-   if (FWD) {
+   if constexpr (is_f32<R>()) {   // compensated FP64 sums of the FP32 increments
+      const R sg = FWD ? R(1) : R(-1);
+      kahan_add(H.t, C.kc[0][tid], FWD ? dt : -dt);
+      kahan_add(H.x, C.kc[1][tid], (double)(sg * (dtv * slope_pos.x)));
+      kahan_add(H.y, C.kc[2][tid], (double)(sg * (dtv * slope_pos.y)));
+      kahan_add(H.z, C.kc[3][tid], (double)(sg * (dtv * slope_pos.z)));
+   }
+   else if (FWD) {
       H.t += dt;
       H.x += (double)(dtv * slope_pos.x); H.y += (double)(dtv * slope_pos.y); H.z += (double)(dtv * slope_pos.z);
    }
@@ -308,9 +326,12 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
    R K0 = R(0), K1 = R(0);
    const R km = (DF == SPX_DF_FULL) ? R(0) : kappa_mom(df, p);
    // the LISM parameters of the Strauss / Potgieter models, when a region indicator can select them
-   const R km2 = (DF == 0 && (df.kind == SPX_DF_STRAUSS || df.kind == SPX_DF_POTGIETER) && df.region_idx >= 0)
-                    ? kappa_mom_region(df, p, true) : R(0);
-   parker_kappa<R, BG, DF>(bg, df, km, km2, H.F.Bmag, H.x, H.y, H.z, H.F.U, K0, K1);
+   R km2 = R(0), km3 = R(0);
+   if (DF == 0) {
+      if ((df.kind == SPX_DF_STRAUSS || df.kind == SPX_DF_POTGIETER) && df.region_idx >= 0) km2 = kappa_mom_region(df, p, true);
+      else if (df.kind == SPX_DF_SOQLT) { km2 = kappa_mom_soqlt(df, p, 0, false); km3 = kappa_mom_soqlt(df, p, 0, true); }
+   }
+   parker_kappa<R, BG, DF>(bg, df, km, km2, km3, H.F.Bmag, H.x, H.y, H.z, H.F.U, K0, K1);
    const R delta = fmin(LarmorRadius(p, H.F.Bmag), H.F.dmax);
    const R hdi = is_f32<R>() ? R(0.5) / delta : R(0);
    v3<R> divK = mk<R>(R(0), R(0), R(0));
@@ -331,7 +352,7 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
          if (loads && Gf.Bmag < R(sp_tiny)) return 1;
          const v3<R> bf = !loads ? Gf.B : (is_f32<R>() ? (R(1) / Gf.Bmag) * Gf.B : Gf.B / Gf.Bmag);
          R K0f = K0, K1f = K1;
-         if (pos_dep) parker_kappa<R, BG, DF>(bg, df, km, km2, Gf.Bmag, xf, yf, zf, Gf.U, K0f, K1f);
+         if (pos_dep) parker_kappa<R, BG, DF>(bg, df, km, km2, km3, Gf.Bmag, xf, yf, zf, Gf.U, K0f, K1f);
          xf -= 2.0 * (double)delta * ex;
          yf -= 2.0 * (double)delta * ey;
          zf -= 2.0 * (double)delta * ez;
@@ -339,7 +360,7 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
          if (loads && Gb.Bmag < R(sp_tiny)) return 1;
          const v3<R> bb = !loads ? Gb.B : (is_f32<R>() ? (R(1) / Gb.Bmag) * Gb.B : Gb.B / Gb.Bmag);
          R K0b = K0, K1b = K1;
-         if (pos_dep) parker_kappa<R, BG, DF>(bg, df, km, km2, Gb.Bmag, xf, yf, zf, Gb.U, K0b, K1b);
+         if (pos_dep) parker_kappa<R, BG, DF>(bg, df, km, km2, km3, Gb.Bmag, xf, yf, zf, Gb.U, K0b, K1b);
          const R bfj = j == 0 ? bf.x : (j == 1 ? bf.y : bf.z);
          const R bbj = j == 0 ? bb.x : (j == 1 ? bb.y : bb.z);
          const R cf = (K1f - K0f) * bfj, cb = (K1b - K0b) * bbj;
@@ -396,7 +417,7 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
          else { Bmf = norm(B2); bf = B2 / Bmf; }
          if (Bmf < R(sp_tiny)) return 1;
          R K0f = K0, K1f = K1;
-         if (pos_dep) parker_kappa<R, BG, DF>(bg, df, km, km2, Bmf, xf, yf, zf, U2, K0f, K1f);
+         if (pos_dep) parker_kappa<R, BG, DF>(bg, df, km, km2, km3, Bmf, xf, yf, zf, U2, K0f, K1f);
          xf -= 2.0 * (double)delta * ex;
          yf -= 2.0 * (double)delta * ey;
          zf -= 2.0 * (double)delta * ez;
@@ -409,7 +430,7 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
          else { Bmb = norm(B2); bb = B2 / Bmb; }
          if (Bmb < R(sp_tiny)) return 1;
          R K0b = K0, K1b = K1;
-         if (pos_dep) parker_kappa<R, BG, DF>(bg, df, km, km2, Bmb, xf, yf, zf, U2, K0b, K1b);
+         if (pos_dep) parker_kappa<R, BG, DF>(bg, df, km, km2, km3, Bmb, xf, yf, zf, U2, K0b, K1b);
          const R bfj = j == 0 ? bf.x : (j == 1 ? bf.y : bf.z);
          const R bbj = j == 0 ? bb.x : (j == 1 ? bb.y : bb.z);
          const R cf = (K1f - K0f) * bfj, cb = (K1b - K0b) * bbj;
@@ -458,21 +479,33 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
    const v3<R> drp = from_basis(mk<R>(rsqrt_(R(2) * K0) * dWx, rsqrt_(R(2) * K0) * dWy, rsqrt_(R(2) * K1) * dWz), fa);
    // RKStep, Euler (src/trajectory_base.cc:362-389), then the stochastic displacement
    const R dtv = R(dt) * R(1);
-   if (FWD) {
-      H.t += dt;
-      H.x += (double)(dtv * slope_pos.x); H.y += (double)(dtv * slope_pos.y); H.z += (double)(dtv * slope_pos.z);
-      H.p += (double)(dtv * slope_mom);
+   if constexpr (is_f32<R>()) {
+      // compensated FP64 sums; the drift and the stochastic displacement of a coordinate are added in FP32
+      // first (their sum carries FP32 rounding either way)
+      const R sg = FWD ? R(1) : R(-1);
+      kahan_add(H.t, C.kc[0][tid], FWD ? dt : -dt);
+      kahan_add(H.x, C.kc[1][tid], (double)(sg * (dtv * slope_pos.x) + drp.x));
+      kahan_add(H.y, C.kc[2][tid], (double)(sg * (dtv * slope_pos.y) + drp.y));
+      kahan_add(H.z, C.kc[3][tid], (double)(sg * (dtv * slope_pos.z) + drp.z));
+      kahan_add(H.p, C.kc[4][tid], (double)(sg * (dtv * slope_mom)));
    }
    else {
-      H.t -= dt;
-      H.x -= (double)(dtv * slope_pos.x); H.y -= (double)(dtv * slope_pos.y); H.z -= (double)(dtv * slope_pos.z);
-      H.p -= (double)(dtv * slope_mom);
+      if (FWD) {
+         H.t += dt;
+         H.x += (double)(dtv * slope_pos.x); H.y += (double)(dtv * slope_pos.y); H.z += (double)(dtv * slope_pos.z);
+         H.p += (double)(dtv * slope_mom);
+      }
+      else {
+         H.t -= dt;
+         H.x -= (double)(dtv * slope_pos.x); H.y -= (double)(dtv * slope_pos.y); H.z -= (double)(dtv * slope_pos.z);
+         H.p -= (double)(dtv * slope_mom);
+      }
+      H.x += (double)drp.x; H.y += (double)drp.y; H.z += (double)drp.z;
    }
    if (TT == 1) {
       if (slope_amp != 0.0) C.amp[tid] += dt * 1.0 * slope_amp * exp(C.wgt[tid]);
       C.dt_prev[tid] = dt;
    }
-   H.x += (double)drp.x; H.y += (double)drp.y; H.z += (double)drp.z;
    // HandleBoundaries and the end-of-step fields
    bflags = handle_boundaries<false>(P, L, slot, H, C, tid);
    if (!(bflags & 1)) {
@@ -506,6 +539,10 @@ __device__ __forceinline__ void parker_step_body(const Launch* __restrict__ Lg)
       const uint64_t n = Lp->n_lanes;
       H.fieldline = (TT == 2);
       H.t = Lp->t[slot]; H.x = Lp->x[slot]; H.y = Lp->y[slot]; H.z = Lp->z[slot]; H.p = (TT == 2) ? Lp->m2[slot] : Lp->m0[slot];
+      if constexpr (is_f32<R>()) {
+#pragma unroll
+         for (int k = 0; k < SPX_KC; k++) C.kc[k][tid] = Lp->kc[k * n + slot];
+      }
       H.steps = (int32_t)Lp->steps[slot];
       H.ds_prev = R(Lp->ds_prev[slot]);
       H.rs.traj = Lp->traj[slot]; H.rs.nn = Lp->nn[slot]; H.rs.spare_ok = false; H.rs.exhausted = false; H.rs.s0 = H.rs.s1 = H.rs.s2 = R(0);
@@ -553,6 +590,10 @@ __device__ __forceinline__ void parker_step_body(const Launch* __restrict__ Lg)
    Ls->active[slot] = st;
    Ls->ex_code[slot] = exc;
    Ls->t[slot] = H.t; Ls->x[slot] = H.x; Ls->y[slot] = H.y; Ls->z[slot] = H.z;
+   if constexpr (is_f32<R>()) {
+#pragma unroll
+      for (int k = 0; k < SPX_KC; k++) Ls->kc[k * n + slot] = C.kc[k][tid];
+   }
    if (TT == 2) Ls->m2[slot] = H.p;
    else Ls->m0[slot] = H.p;
    Ls->steps[slot] = H.steps; Ls->nn[slot] = H.rs.nn;

@@ -61,6 +61,17 @@ SPX_HD float rtanh(float a) { return tanhf(a); }
 template <typename R> constexpr bool is_f32() { return std::is_same<R, float>::value; }
 __device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
+// Compensated (Kahan) addition of an increment to an FP64 sum, for the FP32-step kernels: c carries what
+// the rounding of s dropped (less than an ulp of s, so FP32 holds it to 24 bits) into the next addition.
+// No reassociation is enabled for the FP32 unit, so (t - s) - y is evaluated as written.
+__device__ __forceinline__ void kahan_add(double& s, float& c, double inc)
+{
+   const double y = inc - (double)c;
+   const double t = s + y;
+   c = (float)((t - s) - y);
+   s = t;
+}
+
 // ---- relativistic kinematics (common/physics.hh:246-456) ---------------------------------------
 template <typename R> SPX_HD R EnrTot(R p) { return rsqrt_(sqr(p) + sqr(R(sp_mass * c_code))) * R(c_code); }
 template <typename R> SPX_HD R EnrKin(R p) { return EnrTot(p) - R(sp_mass * (c_code * c_code)); }
@@ -281,11 +292,12 @@ typedef double grid_d2 __attribute__((ext_vector_type(2)));
 typedef float grid_f4 __attribute__((ext_vector_type(4)));
 
 // Corner c = i + 2j + 4k of the stencil whose lowest zone is `cell`: that zone record (U, B, |B|, fluid
-// units), read from the cell's contiguous copy of its 8 corners.
+// units).  FP64 reads it from the zone array, FP32 from the cell's contiguous copy of its 8 corners.
 template <typename R> __device__ __forceinline__ void grid_corner(const SPX_CP spx_grid_desc& g, int64_t cell, int c, R v[7]);
 template <> __device__ __forceinline__ void grid_corner<double>(const SPX_CP spx_grid_desc& g, int64_t cell, int c, double v[7])
 {
-   const grid_d2* r = reinterpret_cast<const grid_d2*>(g.c64) + (cell * 8 + c) * (SPX_GRID_REC / 2);
+   const int64_t zone = cell + (c & 1) + ((c >> 1) & 1) * (int64_t)g.nz[0] + (c >> 2) * ((int64_t)g.nz[0] * g.nz[1]);
+   const grid_d2* r = reinterpret_cast<const grid_d2*>(g.z64) + zone * (SPX_GRID_REC / 2);
    const grid_d2 a = r[0], b = r[1], d = r[2], e = r[3];
    v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y; v[4] = d.x; v[5] = d.y; v[6] = e.x;
 }
@@ -460,6 +472,21 @@ SPX_HD R kappa_mom_region(const SPX_CP spx_df_t<R>& d, R p, bool outer)
    return lam * (vmag / R(c_code)) * (rig < d.R0 ? R(1) : rsqrt_(x * x * x));
 }
 
+// DiffusionEmpiricalSOQLTandUNLT (src/diffusion_other.cc:1053-1072): kappa = (lam v / 3) f(R) (B0 / |B|) / S
+// with the bent power laws f = cbrt(x (1 + x^4)) (parallel), cbrt(x (1 + x^2)) (perpendicular), x = R / R0.
+// This is the factor before (B0 / |B|), for the parallel component or for the perpendicular one with the
+// mean free path lam_perp (reduced = false) or lam_perp kap_rat_red (reduced = true), in the reference's order.
+template <typename R>
+SPX_HD R kappa_mom_soqlt(const SPX_CP spx_df_t<R>& d, R p, int comp, bool reduced)
+{
+   const R rig = Rigidity(p), vmag = VelP(p);
+   const R x = rig / d.R0;
+   if (comp == 1) return (d.lam_para * vmag / R(3)) * rcbrt(x * (R(1) + sqr(sqr(x))));
+   // lam_perp (Bmix_ind + (1 - Bmix_ind) kap_rat_red) with Bmix_ind 1 or 0 is lam_perp or lam_perp kap_rat_red exactly
+   const R lam = reduced ? d.lam_perp * d.kap_rat_red : d.lam_perp;
+   return (lam * vmag / R(3)) * rcbrt(x * (R(1) + sqr(x)));
+}
+
 // The momentum-only factor of kappa_par: the same at every point of a step's derivative stencil, so
 // it is formed once per step (the same inputs give the same value).
 template <typename R>
@@ -470,6 +497,7 @@ SPX_HD R kappa_mom(const SPX_CP spx_df_t<R>& d, R p)
    case SPX_DF_KINRAD: return d.kap0 * rpow(EnrKin(p) / d.T0, d.pow_law_T);                   // :713-723
    case SPX_DF_RIGBMAG: return (d.lam0 * VelP(p) / R(3)) * rpow(Rigidity(p) / d.R0, d.pow_law_R);   // :804-809
    case SPX_DF_STRAUSS: case SPX_DF_POTGIETER: return kappa_mom_region(d, p, false);
+   case SPX_DF_SOQLT: return kappa_mom_soqlt(d, p, 1, false);
    default: return R(0);
    }
 }

@@ -16,7 +16,7 @@
 enum spx_bg_kind { SPX_BG_NONE = 0, SPX_BG_UNIFORM = 1, SPX_BG_SOLARWIND = 2, SPX_BG_SHOCK = 3, SPX_BG_SMOOTHSHOCK = 4, SPX_BG_GRID = 5 };
 enum spx_df_kind {
    SPX_DF_NONE = 0, SPX_DF_ISO, SPX_DF_PARA, SPX_DF_PERP, SPX_DF_FULL, SPX_DF_FLOWMOM, SPX_DF_KINRAD, SPX_DF_RIGBMAG, SPX_DF_PERPISO,
-   SPX_DF_STRAUSS, SPX_DF_POTGIETER, SPX_DF_QLT, SPX_DF_WNLT, SPX_DF_WNLT_RAMP
+   SPX_DF_STRAUSS, SPX_DF_POTGIETER, SPX_DF_QLT, SPX_DF_WNLT, SPX_DF_WNLT_RAMP, SPX_DF_SOQLT
 };
 enum spx_bnd_class { SPX_BC_TIME = 0, SPX_BC_SPACE = 1, SPX_BC_MOMENTUM = 2 };
 enum spx_bnd_kind { SPX_BK_TIME = 0, SPX_BK_SPHERE = 1, SPX_BK_PLANE = 2, SPX_BK_MOMENTUM = 3 };
@@ -48,10 +48,13 @@ typedef struct {
    // 0 when the derivative points of div(kappa) need no field values: kappa_perp == kappa_par
    // identically, kappa reads neither U nor |B|, and no interpolated |B| can fall below sp_tiny
    int32_t fd_fields, pad_;
-   // device cell records: for each zone, the 8 zone records of the trilinear stencil whose lowest corner it
-   // is, in the interpolation's corner order c = i + 2j + 4k (512 B in double, 256 B in float: one stencil
-   // is 4 or 2 whole 128-B lines instead of four scattered rows of the zone array)
-   const double* c64;
+   // device records.  FP64: the zone array itself (64 B per zone, x fastest; a stencil reads four rows of two
+   // x-neighbours).  FP32: cell records, for each zone the 8 zone records of the trilinear stencil whose
+   // lowest corner it is, in the interpolation's corner order c = i + 2j + 4k (256 B: one stencil is two
+   // whole 128-B lines instead of four scattered rows).  The FP64 kernel keeps the zone array: its cell
+   // records would be 512 B per zone, 8x the grid, and on grids beyond a few GB the scattered gathers then
+   // outrun the address-translation caches (measured: half the rate on the 64^3-block grid).
+   const double* z64;
    const float* c32;
 } spx_grid_desc;
 
@@ -78,7 +81,7 @@ typedef spx_bg_t<double> spx_bg_desc;
 template <typename T>
 struct spx_df_t {
    int32_t kind, stream_dep_idx, pos_dep;
-   int32_t region_idx;                  // Strauss / Potgieter: LISM_idx, the region indicator read (-1: none)
+   int32_t region_idx;                  // Strauss / Potgieter: LISM_idx, the region indicator read (-1: none); SOQLT: Bmix_idx
    T D0, Dperp, Dpara;
    T kap0, U0, pow_law_U, p0, pow_law_p, kap_rat;
    T T0, r0, pow_law_T, pow_law_r, u_up, w_sh, s_sh, dn_up_rat;
@@ -93,6 +96,9 @@ struct spx_df_t {
    T A2T, A2L, ps_plus;
    T l_max_HP, z_nose, z_sheath, k_min_ref, A2A_ref, A2T_ref, A2L_ref, dl_max, dz;
    T hyp_c, hyp_g, hyp_r;
+   // DiffusionEmpiricalSOQLTandUNLT: mean free paths, the perpendicular reduction factor and radius, and the
+   // solar-cycle divisor 1 + solar_cycle_effect cos^2(region / 2), constant on the backgrounds it runs on
+   T lam_para, lam_perp, kap_rat_red, r_lim_red, sc_den;
 };
 typedef spx_df_t<double> spx_df_desc;
 
@@ -119,7 +125,7 @@ typedef struct {
    int32_t spectrum_type;               // DISTRO_KINETIC_ENERGY_POWER_LAW_TYPE
    int32_t max_attempts;
    int64_t max_steps;                   // max_trajectory_steps
-   double dt_scale;                     // Parker transports: factor on the physical time step (1: the reference's)
+   double dt_scale;                     // Parker and guiding-centre transports: factor on the physical time step (1: the reference's)
    spx_bg_desc bg;
    spx_df_desc df;
    spx_bg_t<float> bgf;                 // the same parameters rounded to float (mixed-precision kernel)

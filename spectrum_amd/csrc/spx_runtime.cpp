@@ -158,7 +158,7 @@ hipError_t Pool::init(const spx_plan_desc& s, int prec, uint64_t lanes, int spl,
                            &t0, &x0, &y0, &z0, &m00, &m01, &m02};
    for (auto* b : dv) TRY(b->alloc(n));
    TRY(bdelta.alloc(n * SPX_MAX_BND)); TRY(bdelta_old.alloc(n * SPX_MAX_BND));
-   TRY(traj.alloc(n)); TRY(nn.alloc(n)); TRY(nu.alloc(n)); TRY(steps.alloc(n));
+   TRY(traj.alloc(n)); TRY(nn.alloc(n)); TRY(nu.alloc(n)); TRY(steps.alloc(n)); TRY(kc.alloc(n * SPX_KC));
    TRY(queries.alloc((size_t)cap));
    TRY(queue.alloc((size_t)cap));
    TRY(d_launch.alloc(1));
@@ -174,7 +174,7 @@ hipError_t Pool::init(const spx_plan_desc& s, int prec, uint64_t lanes, int spl,
    host.active = active.p;
    host.t = t.p; host.x = x.p; host.y = y.p; host.z = z.p; host.m0 = m0.p; host.m1 = m1.p; host.m2 = m2.p;
    host.amp = amp.p; host.wgt = wgt.p; host.dt_prev = dt_prev.p; host.src_old = src_old.p; host.ds_prev = ds_prev.p;
-   host.dmax = dmax.p; host.dt_adapt = dt_adapt.p; host.pf = pf.p;
+   host.dmax = dmax.p; host.dt_adapt = dt_adapt.p; host.pf = pf.p; host.kc = kc.p;
    host.bdelta = bdelta.p; host.bdelta_old = bdelta_old.p; host.bcross = bcross.p; host.bstat = bstat.p;
    host.traj = traj.p; host.nn = nn.p; host.nu = nu.p; host.attempt = attempt.p; host.steps = steps.p; host.n_events = n_events.p;
    host.t0 = t0.p; host.x0 = x0.p; host.y0 = y0.p; host.z0 = z0.p; host.m00 = m00.p; host.m01 = m01.p; host.m02 = m02.p;
@@ -306,7 +306,7 @@ void Pool::release()
    DevBuf<double>* dv[] = {&t, &x, &y, &z, &m0, &m1, &m2, &amp, &wgt, &dt_prev, &src_old, &ds_prev, &dmax, &dt_adapt, &pf, &bdelta,
                            &bdelta_old, &t0, &x0, &y0, &z0, &m00, &m01, &m02, &ev_w, &evs};
    for (auto* b : dv) b->release();
-   traj.release(); nn.release(); nu.release(); steps.release(); queries.release(); d_launch.release();
+   traj.release(); nn.release(); nu.release(); steps.release(); kc.release(); queries.release(); d_launch.release();
    for (hipEvent_t e : tev) (void)hipEventDestroy(e);
    tev.clear();
    if (ev0) (void)hipEventDestroy(ev0);

@@ -83,6 +83,7 @@ struct Pool {
    DevBuf<double> t0, x0, y0, z0, m00, m01, m02, ev_w, evs;
    DevBuf<uint64_t> traj;
    DevBuf<uint32_t> nn, nu;
+   DevBuf<float> kc;
    DevBuf<int64_t> steps;
    DevBuf<QueryDesc> queries;
    DevBuf<int32_t> queue;

@@ -27,7 +27,7 @@ constexpr size_t lane_bytes(size_t n)
 {
    return 6 * align256(4 * n) + 2 * align256(4 * n * SPX_MAX_BND) + align256(4 * n * SPX_MAX_DIST) + align256(4 * n * 3) +
           22 * align256(8 * n) + 2 * align256(8 * n * SPX_MAX_BND) + align256(8 * n * SPX_MAX_DIST) + align256(8 * n * 27) +
-          align256(8 * n) + 2 * align256(4 * n) + align256(8 * n);
+          align256(8 * n) + 2 * align256(4 * n) + align256(8 * n) + align256(4 * n * SPX_KC);
 }
 
 constexpr size_t header_bytes()
@@ -105,6 +105,7 @@ size_t carve(unsigned char* base, uint64_t n, StaticRun& R)
    L.bdelta = c.take<double>(n * SPX_MAX_BND); L.bdelta_old = c.take<double>(n * SPX_MAX_BND);
    L.ev_w = c.take<double>(n * SPX_MAX_DIST); L.evs = c.take<double>(n * 27);
    L.traj = c.take<uint64_t>(n); L.nn = c.take<uint32_t>(n); L.nu = c.take<uint32_t>(n); L.steps = c.take<int64_t>(n);
+   L.kc = c.take<float>(n * SPX_KC);
    L.queries = R.query;
    L.queue = R.queue;
    L.n_queue = 1;

@@ -454,18 +454,64 @@ def gc_sector_start():
     return s
 
 
+def soqlt(lam_par, lam_perp, R0, bmix_idx, kr_red, r_lim, sc_idx, sc_eff):
+    return (DC().insert_double(lam_par).insert_double(lam_perp).insert_double(R0).insert_double(5.0e-6 / cf.unit_magnetic)
+            .insert_int(bmix_idx).insert_double(kr_red).insert_double(r_lim).insert_int(sc_idx).insert_double(sc_eff))
+
+
+def parker_soqlt_radial_limit():
+    """Parker (backward) on the SolarWind background with DiffusionEmpiricalSOQLTandUNLT: the bent power laws
+    on both sides of R0, the perpendicular mean free path reduced inside 3 au (Bmix_idx 1: region[1] = -1),
+    the solar-cycle divisor; started at 3 au so that trajectories cross the reduction radius."""
+    s = cf.Setup("parker_soqlt_radial_limit", cf.TRAJ_PARKER, cf.BACKWARD)
+    s.add(cf.BACKGROUND, "BackgroundSolarWind", cf.solarwind())
+    p0 = cf.mom_from_ekin(1.0 * cf.GeV)
+    s.add(cf.DIFFUSION, "DiffusionEmpiricalSOQLTandUNLT", soqlt(0.03 * cf.one_au, 0.001 * cf.one_au, p0 * cf.c_code, 1, 0.3,
+                                                                 3.0 * cf.one_au, 0, 0.25))
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((0.6 * 3.0 * cf.one_au, 0.64 * 3.0 * cf.one_au, 0.48 * 3.0 * cf.one_au)))
+    s.add(cf.INITIAL, "InitialMomentumThickShell", DC().insert_double(0.3 * p0).insert_double(3.0 * p0).insert_bool(True))
+    s.add(cf.BOUNDARY, "BoundarySphereAbsorb", cf.boundary(1, [0], ("vec", (0.0, 0.0, 0.0)), ("double", 4.0 * cf.one_au)))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", -20.0 * cf.one_day)))
+    d = dist((8, 1, 1), (0.1 * p0, -1.0, -1.0), (10.0 * p0, 1.0, 1.0), (1, 0, 0), (1, 0, 0))
+    d.insert_double(1.0).insert_double(0.0).insert_int(1).insert_int(0)
+    s.add(cf.DISTRIBUTION, "DistributionMomentumUniform", d)
+    return s
+
+
+def gc_soqlt_diff():
+    """GuidingDiff in the Parker spiral with DiffusionEmpiricalSOQLTandUNLT: kappa_perp ~ 1/|B|, reduced inside
+    1.25 au (Bmix_idx 1), and its perpendicular drift by DiffusionBase's directional derivative."""
+    s = cf.Setup("gc_soqlt_diff", cf.TRAJ_GUIDING_DIFF, cf.FORWARD)
+    s.add(cf.BACKGROUND, "BackgroundSolarWind", cf.solarwind())
+    p0 = cf.mom_from_ekin(1.0 * cf.GeV)
+    s.add(cf.DIFFUSION, "DiffusionEmpiricalSOQLTandUNLT", soqlt(2.0 * cf.one_au, 0.002 * cf.one_au, p0 * cf.c_code, 1, 0.4,
+                                                                 1.25 * cf.one_au, 1, 0.3))
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((1.2 * cf.one_au, 0.4 * cf.one_au, 0.1 * cf.one_au)))
+    s.add(cf.INITIAL, "InitialMomentumThickShell", DC().insert_double(cf.mom_from_ekin(5.0 * cf.MeV)).insert_double(cf.mom_from_ekin(500.0 * cf.MeV)).insert_bool(True))
+    s.add(cf.BOUNDARY, "BoundarySphereAbsorb", cf.boundary(1, [0], ("vec", (0.0, 0.0, 0.0)), ("double", 1.4 * cf.one_au)))
+    s.add(cf.BOUNDARY, "BoundarySphereAbsorb", cf.boundary(1, [0], ("vec", (0.0, 0.0, 0.0)), ("double", 0.3 * cf.one_au)))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", 0.1 * cf.one_day)))
+    d = dist((16, 16, 0), (1.0 * cf.one_au, -1.0, 0.0), (1.5 * cf.one_au, 1.0, 1.0), (0, 0, 0), (1, 0, 0))
+    d.insert_double(1.0).insert_double(0.0).insert_int(1)
+    s.add(cf.DISTRIBUTION, "DistributionPositionPitchUniform", d)
+    return s
+
+
 ALL = [shock_flowmom_reflect, solarwind_kinrad_reflect_recurrent, smoothshock_perp_momentum, uniform_para_plane_pass,
        lambda: modulation_spectrum_type(0), lambda: modulation_spectrum_type(2), gc_rigidity_reflect, gc_isotropic_only,
        guiding_spiral, guiding_scatt_uniform, guiding_diff_rigidity, parker_strauss_heliopause, parker_potgieter_uniform,
        gc_qlt_spiral, fieldline_spiral, fieldline_backward_shock, lorentz_uniform, lorentz_spiral_backward,
        focused_spiral, focused_uniform_backward, gc_wnlt_scatt_spiral, gc_wnlt_ramp_diffscatt,
-       gc_strauss_diff, focused_smoothshock, uniform_sphere_start, gc_sector_start]
+       gc_strauss_diff, focused_smoothshock, uniform_sphere_start, gc_sector_start, parker_soqlt_radial_limit, gc_soqlt_diff]
 NAMES = ["shock_flowmom_reflect", "solarwind_kinrad_reflect_recurrent", "smoothshock_perp_momentum", "uniform_para_plane_pass",
          "modulation_type0", "modulation_type2", "gc_rigidity_reflect", "gc_isotropic_only",
          "guiding_spiral", "guiding_scatt_uniform", "guiding_diff_rigidity", "parker_strauss_heliopause", "parker_potgieter_uniform",
          "gc_qlt_spiral", "fieldline_spiral", "fieldline_backward_shock", "lorentz_uniform", "lorentz_spiral_backward",
          "focused_spiral", "focused_uniform_backward", "gc_wnlt_scatt_spiral", "gc_wnlt_ramp_diffscatt",
-         "gc_strauss_diff", "focused_smoothshock", "uniform_sphere_start", "gc_sector_start"]
+         "gc_strauss_diff", "focused_smoothshock", "uniform_sphere_start", "gc_sector_start", "parker_soqlt_radial_limit",
+         "gc_soqlt_diff"]
 
 
 # Setups whose endpoints amplify last-bit differences far beyond 1e-10: QLT's mu derivative is

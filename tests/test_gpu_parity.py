@@ -215,3 +215,24 @@ def test_autotune_picks_a_shape_and_keeps_results(device):
     assert ne == 0 and c.sum() == 0            # the probe's events were discarded
     _, rec = sim.run(2000, seed=9)
     np.testing.assert_array_equal(rec, rec_ref)
+
+
+@pytest.mark.parametrize("case", ["parker", "gc"])
+def test_step_scale_fp64_counter(device, oracle_mod, case):
+    """The step-scale option (the physical time step times 1/4, for convergence studies) changes the
+    kernel and the oracle alike: Parker modulation and the guiding centre with pitch-angle scattering."""
+    if case == "parker":
+        s = cf.parker_modulation(n_energies=8, t_limit_years=1.0 / 365.0, time_action=0)
+        n, lanes, spl = 256, 512, 500
+    else:
+        s = cf.gc_uniform(theta0=0.6435011087932844, t_limit=100.0)
+        n, lanes, spl = 256, 512, 700
+    sim = E.Simulation.from_setup(s, precision=E.PREC_FP64, lanes=lanes, steps_per_launch=spl, step_scale=0.25)
+    st, rec = sim.run(n, seed=29)
+    o = oracle_mod.Oracle(s)
+    o.set_option("step_scale", 0.25)
+    _, orec = o.run(n, seed=29, threads=THREADS)
+    check_parity(st, rec, orec, sim, o, n)
+    o1 = oracle_mod.Oracle(s)
+    _, orec1 = o1.run(n, seed=29, threads=THREADS)
+    assert rec["steps"].sum() > 3 * orec1["steps"].sum()     # the smaller step took effect

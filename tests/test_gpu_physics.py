@@ -51,34 +51,56 @@ def test_parker_uniform_walk_msd(prec, flow):
         within(v.mean(), v.std() / math.sqrt(n), 2.0 * 2.0 * T.mean(), f"<dx_{k}^2>")
 
 
+def converged(results, expect, what):
+    """results: {step scale: (estimate, error)} at scales 1, 1/4, 1/16.  The Euler-Maruyama estimate of a
+    pitch-angle process carries a step bias (chiefly where the rejected draws at |mu| = 1 act); if the bias
+    falls as dt^a with a >= 1/2, the bias left at the finest step is at most the change over the last
+    refinement, so |m_16 - exact| <= |m_4 - m_16| + 3.5 sigma."""
+    (m1, e1), (m4, e4), (m16, e16) = results[1.0], results[0.25], results[0.0625]
+    bound = abs(m4 - m16)
+    noise = NSIG * math.sqrt(2.0 * e16 ** 2 + e4 ** 2)
+    print(f"[check] {what}: exact {expect:.6g}; step x1 {m1:.6g} ({(m1 - expect) / e1:+.2f} sigma), x1/4 {m4:.6g} "
+          f"({(m4 - expect) / e4:+.2f} sigma), x1/16 {m16:.6g} ({(m16 - expect) / e16:+.2f} sigma); bias bound {bound:.3g}", flush=True)
+    assert abs(m16 - expect) <= bound + noise, what
+
+
 @pytest.mark.parametrize("prec", PRECS)
 def test_gc_pitch_angle_isotropization(prec):
+    """<P_l(mu)> = P_l(mu0) exp(-l(l+1) D0 t) for D_mumu = D0 (1 - mu^2), 1e6 trajectories per step scale."""
     mu0 = 0.8
     s = cf.gc_uniform(theta0=math.acos(mu0), t_limit=500.0)
     n = 1000000
-    _, st, rec = run(s, n, prec)
-    assert st.failed == 0
-    p = np.hypot(rec["mom"][:, 0], rec["mom"][:, 2])
-    np.testing.assert_allclose(p, s.meta["p"], rtol=1e-12 if prec == E.PREC_FP64 else 1e-6)
-    mu = rec["mom"][:, 2] / p
     D0, T = s.meta["D0"], s.meta["t_limit"]
-    P = {1: mu, 2: 0.5 * (3 * mu**2 - 1), 3: 0.5 * (5 * mu**3 - 3 * mu)}
     P0 = {1: mu0, 2: 0.5 * (3 * mu0**2 - 1), 3: 0.5 * (5 * mu0**3 - 3 * mu0)}
+    res = {l: {} for l in (1, 2, 3)}
+    for scale in (1.0, 0.25, 0.0625):
+        _, st, rec = run(s, n, prec, step_scale=scale)
+        assert st.failed == 0
+        p = np.hypot(rec["mom"][:, 0], rec["mom"][:, 2])
+        np.testing.assert_allclose(p, s.meta["p"], rtol=1e-12 if prec == E.PREC_FP64 else 1e-6)
+        mu = rec["mom"][:, 2] / p
+        P = {1: mu, 2: 0.5 * (3 * mu**2 - 1), 3: 0.5 * (5 * mu**3 - 3 * mu)}
+        for l in (1, 2, 3):
+            res[l][scale] = (P[l].mean(), P[l].std() / math.sqrt(n))
+        if scale == 1.0:   # the perpendicular spread has no step bias: a constant kappa_perp, Gaussian increments
+            r2 = rec["pos"][:, 0] ** 2 + rec["pos"][:, 1] ** 2
+            within(r2.mean(), r2.std() / math.sqrt(n), 4 * s.meta["kperp"] * T, "<x^2+y^2>")
     for l in (1, 2, 3):
-        within(P[l].mean(), P[l].std() / math.sqrt(n), P0[l] * math.exp(-l * (l + 1) * D0 * T), f"<P_{l}(mu)>")
-    r2 = rec["pos"][:, 0] ** 2 + rec["pos"][:, 1] ** 2
-    within(r2.mean(), r2.std() / math.sqrt(n), 4 * s.meta["kperp"] * T, "<x^2+y^2>")
+        converged(res[l], P0[l] * math.exp(-l * (l + 1) * D0 * T), f"<P_{l}(mu)>")
 
 
 @pytest.mark.parametrize("prec", PRECS)
 def test_gc_isotropic_parallel_spread(prec):
     s = cf.gc_uniform(theta0=None, t_limit=2000.0)
-    n = 100000
-    _, st, rec = run(s, n, prec)
+    n = 200000
     D0, T, v = s.meta["D0"], s.meta["t_limit"], s.meta["v"]
-    z2 = rec["pos"][:, 2] ** 2
     expect = (2 * v * v / 3) * (T / (2 * D0) - (1 - math.exp(-2 * D0 * T)) / (4 * D0 * D0))
-    within(z2.mean(), z2.std() / math.sqrt(n), expect, "<z^2>")
+    res = {}
+    for scale in (1.0, 0.25, 0.0625):
+        _, st, rec = run(s, n, prec, records=True, step_scale=scale)
+        z2 = rec["pos"][:, 2] ** 2
+        res[scale] = (z2.mean(), z2.std() / math.sqrt(n))
+    converged(res, expect, "<z^2>")
 
 
 def test_mixed_matches_fp64_modulation_spectrum():

@@ -232,3 +232,28 @@ def test_soqlt_uniform_field_variances(oracle_mod):
     print(f"[soqlt] <dz^2>/2T {z2 / (2 * T):.5g} vs {k1:.5g}; <dr^2>/4T {r2 / (4 * T):.5g} vs {k0:.5g}")
     assert abs(z2 / (2 * T) / k1 - 1.0) < 5.0 * np.sqrt(2.0 / n)
     assert abs(r2 / (4 * T) / k0 - 1.0) < 5.0 * np.sqrt(1.0 / n)
+
+
+def test_soqlt_host_checks():
+    """The engine rejects what the reference leaves undefined: a pitch-angle transport (no component 2), a
+    region indicator the background does not set (region[2] of the solar wind), a background without
+    the indicators it reads."""
+    import spectrum_amd.engine as E
+    s = F.gc_soqlt_diff()
+    s.traj_type = cf.TRAJ_GUIDING_DIFF_SCATT
+    with pytest.raises(E.SpxError, match="GuidingDiff"):
+        E.Simulation.from_setup(s).run(4)
+    p0 = cf.mom_from_ekin(1.0 * cf.GeV)
+    for bmix, sci in ((2, 0), (0, 2), (-1, 0)):
+        s = F.parker_soqlt_radial_limit()
+        s.plugins = [x for x in s.plugins if x[0] != cf.DIFFUSION]
+        s.add(cf.DIFFUSION, "DiffusionEmpiricalSOQLTandUNLT", F.soqlt(0.03 * cf.one_au, 0.001 * cf.one_au, p0 * cf.c_code, bmix, 0.3,
+                                                                   3.0 * cf.one_au, sci, 0.25))
+        with pytest.raises(E.SpxError, match="region indicator"):
+            E.Simulation.from_setup(s).run(4)
+    s = F.parker_soqlt_radial_limit()
+    s.plugins = [x for x in s.plugins if x[0] != cf.BACKGROUND]
+    s.add(cf.BACKGROUND, "BackgroundShock", cf.background_base(DC(), 0.0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 1.0)
+          .insert_vec((1.0, 0.0, 0.0)).insert_double(0.0).insert_double(4.0))
+    with pytest.raises(E.SpxError, match="SolarWind and Uniform"):
+        E.Simulation.from_setup(s).run(4)
