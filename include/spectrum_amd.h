@@ -187,7 +187,7 @@ int spx_main_loop(spx_sim* sim, const spx_rng* rng, spx_run_stats* stats);
    memory -- code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.  The
    same integration as spx_run (counter streams keyed by `seed`, the sim's precision and steps per
    launch), but every piece of device state -- launch block, query entry, plan, control block,
-   histograms and SPX_STATIC_LANES resident slots, 39.4 MiB in all -- lives in one static device array per GPU,
+   histograms and SPX_STATIC_LANES resident slots, 40.7 MiB in all -- lives in one static device array per GPU,
    held until the process ends; no hipMalloc, hipFree or other allocating call is made.  Results go to
    the caller's host arrays, all distributions in order: `counts` (n_bins of each, spx_distro_info),
    `sums` (n_bins x width of each) and `n_events` (one per distribution); any may be NULL.  They equal
