@@ -128,18 +128,19 @@ def grid_lanes(E, args):
 def grid_workloads(E, cf, args, traffic_all):
     """Parker modulation on a gridded solar wind (BackgroundServerCartesian), the reference's
     Cartesian benchmark: its 25^3-block (100^3-zone, 64 MB of FP64 zone records) grid over +-100 au,
-    and a 64^3-block (256^3-zone, 1.07 GB) grid that no cache holds.  Each step interpolates the grid
+    and a 64^3-block (256^3-zone, 1.07 GB of FP64 zone records, 4.3 GB of FP32 cell records) grid that no
+    cache holds.  Each step interpolates the grid
     at the new position (kappa is isotropic here, so the six derivative points of div(kappa) need only
     their positions checked): one gather of 8 zones.  The algorithmic gather is 8 zones x 6 values x
-    8 B (FP64) or 4 B (FP32) per interpolation; the measured HBM traffic per launch comes from the
-    FETCH_SIZE / WRITE_SIZE profile when present."""
+    8 B (FP64) or 4 B (FP32) per interpolation; the HBM bytes per step come from the FETCH_SIZE /
+    WRITE_SIZE profile of the same legs when present, and the rate from this run's kernel time."""
     from spectrum_amd import cartesian as G
     out = {}
     for nb in args.grid_blocks:
         grid = G.parker_spiral(nb=nb)
         setup = cf.parker_modulation_cartesian(grid)
         name = f"parker_modulation_cartesian_{nb}^3_blocks"
-        res = {"zones": 4 * nb, "zone_records_bytes": {"fp64": 64 * (4 * nb) ** 3, "mixed": 32 * (4 * nb) ** 3}}
+        res = {"zones": 4 * nb, "device_grid_bytes": {"fp64": 64 * (4 * nb) ** 3, "mixed": 8 * 32 * (4 * nb) ** 3}}
         for p in (("fp64", "mixed") if args.precision == "both" else (args.precision,)):
             leg = gpu_leg(E, cf, p, args, setup, 10_000_000, autotune=False,
                           lanes=grid_lanes(E, args), steps_per_launch=args.grid_steps_per_launch)
@@ -152,8 +153,8 @@ def grid_workloads(E, cf, args, traffic_all):
                  "gather_GBps": round(gather * steps_per_launch / per_launch_s / 1e9, 1)}
             t = (traffic_all or {}).get(name, {}).get(p)
             if t:
-                r["hbm_bytes_per_launch"] = t["bytes_per_launch"]
-                r["hbm_GBps"] = round(t["bytes_per_launch"] / (t["avg_launch_ms"] * 1e-3) / 1e9, 1)
+                r["hbm_bytes_per_step"] = round(t["bytes_per_step"], 1)
+                r["hbm_GBps"] = round(t["bytes_per_step"] * steps_per_launch / per_launch_s / 1e9, 1)
                 r["hbm_frac"] = round(r["hbm_GBps"] / PEAK_HBM_GBPS, 4)
             res[p] = r
         if "fp64" in res and "mixed" in res:

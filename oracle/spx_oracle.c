@@ -689,10 +689,12 @@ static int get_fields(const bg_t* bg, lane_t* L, double t, vec3 pos, double pmag
    sp->mask = mask;
    sp->dmax = bg_dmax(bg, L, pos);
    if (bg->kind == BG_GRID) {
-      /* BackgroundServer (src/background_server.cc:80-110): |B| is interpolated, not recomputed */
+      /* BackgroundServer (src/background_server.cc:80-110): |B| is interpolated, not recomputed; the grid is
+         static, so its time derivatives are zero */
       if (!bg->grid || grid_fields(bg->grid, pos, sp)) return 1;
       if (sp->Bmag < sp_tiny) return 1;
       sp->bhat = vdiv(sp->B, sp->Bmag);
+      sp->dUdt = V(0, 0, 0); sp->dBdt = V(0, 0, 0); sp->dBmagdt = 0.0;
       return 0;
    }
    bg_evaluate(bg, L, t, pos, mask, sp);
@@ -1601,9 +1603,11 @@ static int fl_advance(const orc_sim* s, lane_t* L, traj_t* T)
    src/background_base.cc:425-456) */
 static int lz_fields(const orc_sim* s, lane_t* L, traj_t* T)
 {
-   if (common_fields(s, L, T, BG_B)) return 1;
+   /* BackgroundShock sets E0 / E1 by side with the mask E (src/background_shock.cc:91-111); the Cartesian
+      server interpolates U and B and forms E from them (grid_fields) */
+   if (common_fields(s, L, T, s->bg.kind == BG_SHOCK ? (BG_B | BG_E) : BG_B)) return 1;
    if (s->bg.kind == BG_UNIFORM) T->sp.E = s->bg.Eu;
-   else T->sp.E = vdiv(vneg(vcross(T->U0, T->sp.B)), c_code);
+   else if (s->bg.kind != BG_SHOCK && s->bg.kind != BG_GRID) T->sp.E = vdiv(vneg(vcross(T->U0, T->sp.B)), c_code);
    return 0;
 }
 

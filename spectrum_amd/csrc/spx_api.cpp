@@ -581,8 +581,7 @@ static int check_ready(spx_sim* s)
 static int check_transport(spx_sim* s)
 {
    if (s->plan.traj_type == SPX_TRAJ_LORENTZ) {
-      if (s->plan.bg.kind != SPX_BG_SOLARWIND && s->plan.bg.kind != SPX_BG_UNIFORM)
-         return fail(s, SPX_ERR_ARG, "full orbit: SolarWind and Uniform backgrounds");
+      if (s->plan.bg.kind == SPX_BG_SMOOTHSHOCK) return fail(s, SPX_ERR_ARG, "full orbit: SolarWind, Uniform, Shock and Cartesian-grid backgrounds");
       const int im = s->plan.im_kind;
       if (im != SPX_IM_FIXED && im != SPX_IM_SHELL && im != SPX_IM_THICKSHELL)
          return fail(s, SPX_ERR_ARG, "full orbit: initial momentum must be Fixed, Shell or ThickShell");
@@ -595,8 +594,7 @@ static int check_transport(spx_sim* s)
       }
    }
    else if (s->plan.traj_type == SPX_TRAJ_FOCUSED) {
-      if (s->plan.bg.kind != SPX_BG_SOLARWIND && s->plan.bg.kind != SPX_BG_UNIFORM && s->plan.bg.kind != SPX_BG_SMOOTHSHOCK)
-         return fail(s, SPX_ERR_ARG, "focused transport: SolarWind, Uniform and SmoothShock backgrounds");
+      // every background kind: SolarWind, Uniform, Shock, SmoothShock and the Cartesian grid
       if (s->plan.src_kind != SPX_SRC_NONE) return fail(s, SPX_ERR_ARG, "focused transport: no source term");
       for (int d = 0; d < s->plan.n_dist; d++)
          if (s->plan.dist[d].kind == SPX_DS_POSPITCH) return fail(s, SPX_ERR_ARG, "focused transport: its momentum is (p, mu, 0); use DistributionMomentumUniform");

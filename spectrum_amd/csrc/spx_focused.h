@@ -8,7 +8,9 @@
 // The fields carry the mask ALL | gradB | gradU | dUdt (src/trajectory_focused.cc:36-37).  The solar wind
 // differentiates numerically (BackgroundBase::NumericalDerivatives, src/background_base.cc:228-312) in the
 // field-aligned frame at r_g = min(r_L(p), dmax), p the momentum magnitude of the evaluation; it is static,
-// so its time derivative is exactly zero, as is every derivative of the uniform background.  The smooth
+// so its time derivative is exactly zero, as is every derivative of the uniform background and of the
+// discontinuous shock (BackgroundShock::EvaluateBackgroundDerivatives, src/background_shock.cc:116-128:
+// U and B jump at the shock and are constant on either side).  The smooth
 // shock's derivatives are analytic (BackgroundSmoothShock::EvaluateBackgroundDerivatives,
 // src/background_smooth_shock.cc:159-187), dU/dt included when the shock moves; its dmax reads the shock
 // coordinate of the previous evaluation (:192-196), which each slot carries.
@@ -54,6 +56,18 @@ template <typename R, int BG>
 __device__ __forceinline__ bool fc_fields(const SPX_CP spx_bg_t<R>& bg, double t, double x, double y, double z, R pm, R& ds_prev, FcFields<R>& F)
 {
    F.dmax = bg_dmax<R, BG>(bg, x, y, z, ds_prev);
+   if constexpr (BG == SPX_BG_GRID) {
+      // the server's interpolated U, B, |B| and its stencil gradients; a static grid has dU/dt = 0
+      v3<R> U, B, E, gU[3], gB[3];
+      double zl_min;
+      if (!grid_eval_full<R, true>(bg.grid, x, y, z, U, B, F.Bmag, E, gU, gB, F.gradBmag, zl_min)) return false;
+      F.dmax = R(fmin((double)F.dmax, zl_min));
+      if (F.Bmag < R(sp_tiny)) return false;
+      F.U = U;
+      F.bhat = B / F.Bmag;
+      fc_contract(F, U, B, gU, gB, mk<R>(R(0), R(0), R(0)));
+      return true;
+   }
    v3<R> U, B;
    R ds = ds_prev;
    bg_eval<R, BG>(bg, t, x, y, z, 3, U, B, ds);
@@ -370,6 +384,10 @@ hipError_t dispatch_focused(const spx_plan_desc& hp, uint64_t n_lanes, const Lau
       return fwd ? launch_fc_bg<R, SPX_BG_UNIFORM, 1>(n_lanes, d_launch, st, mid) : launch_fc_bg<R, SPX_BG_UNIFORM, 0>(n_lanes, d_launch, st, mid);
    case SPX_BG_SMOOTHSHOCK:
       return fwd ? launch_fc_bg<R, SPX_BG_SMOOTHSHOCK, 1>(n_lanes, d_launch, st, mid) : launch_fc_bg<R, SPX_BG_SMOOTHSHOCK, 0>(n_lanes, d_launch, st, mid);
+   case SPX_BG_SHOCK:
+      return fwd ? launch_fc_bg<R, SPX_BG_SHOCK, 1>(n_lanes, d_launch, st, mid) : launch_fc_bg<R, SPX_BG_SHOCK, 0>(n_lanes, d_launch, st, mid);
+   case SPX_BG_GRID:
+      return fwd ? launch_fc_bg<R, SPX_BG_GRID, 1>(n_lanes, d_launch, st, mid) : launch_fc_bg<R, SPX_BG_GRID, 0>(n_lanes, d_launch, st, mid);
    default: return hipErrorInvalidValue;
    }
 }

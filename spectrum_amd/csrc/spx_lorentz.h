@@ -27,6 +27,15 @@ template <typename R, int BG>
 __device__ __forceinline__ bool lz_fields(const SPX_CP spx_bg_t<R>& bg, double t, double x, double y, double z, v3<R> U0, LzFields<R>& F)
 {
    F.dmax = bg_dmax<R, BG>(bg, x, y, z, R(0));
+   if constexpr (BG == SPX_BG_GRID) {
+      // the server interpolates U and B and forms E from them; |B| is interpolated, dmax limited by the zone
+      // (BackgroundServer, src/background_server.cc:80-110; ServerCartesianFront, src/server_cartesian.cc)
+      v3<R> U, gU[3], gB[3], gm;
+      double zl_min;
+      if (!grid_eval_full<R, false>(bg.grid, x, y, z, U, F.B, F.Bmag, F.E, gU, gB, gm, zl_min)) return false;
+      F.dmax = R(fmin((double)F.dmax, zl_min));
+      return F.Bmag >= R(sp_tiny);
+   }
    v3<R> U, B;
    R ds = R(0);
    bg_eval<R, BG>(bg, t, x, y, z, 2, U, B, ds);
@@ -34,6 +43,8 @@ __device__ __forceinline__ bool lz_fields(const SPX_CP spx_bg_t<R>& bg, double t
    F.Bmag = norm(B);
    if (F.Bmag < R(sp_tiny)) return false;
    if (BG == SPX_BG_UNIFORM) F.E = v3of(bg.Eu);                    // src/background_uniform.cc:58-77
+   else if (BG == SPX_BG_SHOCK)                                     // src/background_shock.cc:91-111: E0 / E1 by side
+      F.E = (dot(rel(bg, x, y, z), v3of(bg.n_shock)) - bg.v_shock * R(t) > R(0)) ? v3of(bg.E0) : v3of(bg.E1);
    else F.E = (-cross(U0, B)) / R(c_code);                          // src/background_solarwind.cc:252
    return true;
 }
@@ -73,9 +84,9 @@ __device__ int lz_set_start(PlanPtr P, QueryPtr Q, LaunchPtr L, const spx_rng_de
    pos_sample(P, rng, first, traj, nu, ex, x, y, z);
    double m1[3], m[3];
    lz_mom_sample(P, rng, first, traj, nu, ex, m1);
-   v3<R> U0, B0;
+   v3<R> U0 = mk<R>(R(0), R(0), R(0)), B0;
    R ds = R(0);
-   bg_eval<R, BG>(bg, t, x, y, z, 3, U0, B0, ds);
+   if (BG != SPX_BG_GRID) bg_eval<R, BG>(bg, t, x, y, z, 3, U0, B0, ds);   // the server's E needs no kept flow
    LzFields<R> F;
    if (!lz_fields<R, BG>(bg, t, x, y, z, U0, F)) return 1;
    lz_mom_sample(P, rng, first, traj, nu, ex, m);
@@ -261,6 +272,10 @@ hipError_t dispatch_lorentz(const spx_plan_desc& hp, uint64_t n_lanes, const Lau
       return fwd ? launch_lz_bg<R, SPX_BG_SOLARWIND, 1>(n_lanes, d_launch, st, mid) : launch_lz_bg<R, SPX_BG_SOLARWIND, 0>(n_lanes, d_launch, st, mid);
    case SPX_BG_UNIFORM:
       return fwd ? launch_lz_bg<R, SPX_BG_UNIFORM, 1>(n_lanes, d_launch, st, mid) : launch_lz_bg<R, SPX_BG_UNIFORM, 0>(n_lanes, d_launch, st, mid);
+   case SPX_BG_SHOCK:
+      return fwd ? launch_lz_bg<R, SPX_BG_SHOCK, 1>(n_lanes, d_launch, st, mid) : launch_lz_bg<R, SPX_BG_SHOCK, 0>(n_lanes, d_launch, st, mid);
+   case SPX_BG_GRID:
+      return fwd ? launch_lz_bg<R, SPX_BG_GRID, 1>(n_lanes, d_launch, st, mid) : launch_lz_bg<R, SPX_BG_GRID, 0>(n_lanes, d_launch, st, mid);
    default: return hipErrorInvalidValue;
    }
 }

@@ -608,8 +608,17 @@ __device__ __forceinline__ void parker_step_body(const Launch* __restrict__ Lg)
    }
 }
 
+// SPX_PARKER_WAVES resident waves per SIMD asked of the compiler for the step kernel (0: its own budget).
+#ifndef SPX_PARKER_WAVES
+#define SPX_PARKER_WAVES 0
+#endif
+#if SPX_PARKER_WAVES > 0
+#define SPX_PARKER_ATTR __attribute__((amdgpu_waves_per_eu(SPX_PARKER_WAVES)))
+#else
+#define SPX_PARKER_ATTR
+#endif
 template <typename R, int BG, int FWD, int TT, int DF>
-__global__ __launch_bounds__(BLOCK) void parker_step_kernel(const Launch* __restrict__ Lg)
+__global__ __launch_bounds__(BLOCK) SPX_PARKER_ATTR void parker_step_kernel(const Launch* __restrict__ Lg)
 {
    parker_step_body<R, BG, FWD, TT, DF>(Lg);
 }

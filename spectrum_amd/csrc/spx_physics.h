@@ -455,6 +455,71 @@ __device__ __forceinline__ bool grid_eval_gc(const SPX_CP spx_grid_desc& g, doub
    return true;
 }
 
+// All the fields the server gives for the full orbit (GRADS = false: U, B, the interpolated |B|, E) and the
+// focused transport (GRADS = true: also grad U and grad B row by row, gU[i] = d U / d x_i, and grad |B|),
+// from the stencil's weights and derivative weights (GetVariablesInterp1 / GetGradientsInterp1,
+// src/server_cartesian.cc:538-609, :630-705); E = -(U x B) / c in fluid units before the unit conversion.
+template <typename R, bool GRADS>
+__device__ __forceinline__ bool grid_eval_full(const SPX_CP spx_grid_desc& g, double x, double y, double z, v3<R>& U, v3<R>& B, R& Bmag,
+                                               v3<R>& E, v3<R> (&gU)[3], v3<R> (&gB)[3], v3<R>& gradBmag, double& zl_min)
+{
+   R lo[3], hi[3], dl[3];
+   int64_t zone0;
+   if (!grid_locate<R>(g, x, y, z, lo, hi, dl, zone0, zl_min)) return false;
+   R acc[7], gu[9], gb[9], gm[3];
+#pragma unroll
+   for (int m = 0; m < 7; m++) acc[m] = R(0);
+#pragma unroll
+   for (int m = 0; m < 9; m++) { gu[m] = R(0); gb[m] = R(0); }
+#pragma unroll
+   for (int m = 0; m < 3; m++) gm[m] = R(0);
+#pragma unroll 1
+   for (int r = 0; r < 4; r++) {
+      const int j = r & 1, k = r >> 1;
+      R v0[7], v1[7];
+      grid_corner<R>(g, zone0, 2 * r, v0);
+      grid_corner<R>(g, zone0, 2 * r + 1, v1);
+      const R fy = j ? lo[1] : hi[1], fz = k ? lo[2] : hi[2];
+#pragma unroll
+      for (int i = 0; i < 2; i++) {
+         const R* v = i ? v1 : v0;
+         const R fx = i ? lo[0] : hi[0];
+         const R w = fx * fy * fz;
+#pragma unroll
+         for (int m = 0; m < 7; m++) acc[m] = acc[m] + w * v[m];
+         if (GRADS) {
+            const R sx = i ? R(1) : R(-1), sy_ = j ? R(1) : R(-1), sz_ = k ? R(1) : R(-1);
+            const R d[3] = {sx * (fy * fz) / dl[0], sy_ * (fx * fz) / dl[1], sz_ * (fx * fy) / dl[2]};
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+#pragma unroll
+               for (int c = 0; c < 3; c++) {
+                  gu[3 * a + c] = gu[3 * a + c] + d[a] * v[c];
+                  gb[3 * a + c] = gb[3 * a + c] + d[a] * v[3 + c];
+               }
+               gm[a] = gm[a] + d[a] * v[6];
+            }
+         }
+      }
+   }
+   const R fv = R(g.f_vel), fm = R(g.f_mag), cf = R(C_CGS / 1.0E7);
+   const v3<R> Uf = mk<R>(acc[0], acc[1], acc[2]), Bf = mk<R>(acc[3], acc[4], acc[5]);
+   const v3<R> Ef = (-cross(Uf, Bf)) / cf;
+   U = mk<R>(Uf.x * fv, Uf.y * fv, Uf.z * fv);
+   B = mk<R>(Bf.x * fm, Bf.y * fm, Bf.z * fm);
+   E = mk<R>(Ef.x * fm, Ef.y * fm, Ef.z * fm);
+   Bmag = acc[6] * fm;
+   if (GRADS) {
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+         gU[a] = mk<R>(gu[3 * a] * fv, gu[3 * a + 1] * fv, gu[3 * a + 2] * fv);
+         gB[a] = mk<R>(gb[3 * a] * fm, gb[3 * a + 1] * fm, gb[3 * a + 2] * fm);
+      }
+      gradBmag = mk<R>(gm[0] * fm, gm[1] * fm, gm[2] * fm);
+   }
+   return true;
+}
+
 // ---- diffusion: DiffusionBase::GetComponent (src/diffusion_base.cc:105-121) + models ----------------
 SPX_HD double rcbrt(double x) { return cbrt(x); }
 SPX_HD float rcbrt(float x) { return cbrtf(x); }

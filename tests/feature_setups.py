@@ -499,19 +499,90 @@ def gc_soqlt_diff():
     return s
 
 
+def shock_oblique(u0=(0.002, 0.001, 0.0), B=(3.0, 4.0, 12.0), compression=3.0, v_shock=0.0):
+    """BackgroundShock with its normal along -x through the origin: upstream is x < v_shock t (the reference's
+    (x - r0) . n - v_shock t > 0), the downstream state from the jump conditions (src/background_shock.cc:58-86)."""
+    dc = cf.background_base(DC(), 0.0, (0.0, 0.0, 0.0), u0, B, 1.0)
+    dc.insert_vec((-1.0, 0.0, 0.0)).insert_double(v_shock).insert_double(compression)
+    return dc
+
+
+def lorentz_shock():
+    """TrajectoryLorentz (forward) across a discontinuous oblique shock: E = E0 upstream, E1 downstream (the
+    background's own induced fields, src/background_shock.cc:91-111), B jumping at x = 0; isotropic momenta
+    on a thick shell started just upstream, planes beyond either side and a time limit."""
+    s = cf.Setup("lorentz_shock", cf.TRAJ_LORENTZ, cf.FORWARD)
+    s.add(cf.BACKGROUND, "BackgroundShock", shock_oblique())
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((-0.05, 0.0, 0.0)))
+    s.add(cf.INITIAL, "InitialMomentumThickShell", DC().insert_double(cf.mom_from_ekin(1.0 * cf.MeV)).insert_double(cf.mom_from_ekin(100.0 * cf.MeV)).insert_bool(True))
+    s.add(cf.BOUNDARY, "BoundaryPlaneReflect", cf.boundary(-1, [-1], ("vec", (0.0, 0.0, -3.0)), ("vec", (0.0, 0.0, 1.0))))
+    s.add(cf.BOUNDARY, "BoundaryPlanePass", cf.boundary(-1, [0], ("vec", (0.0, 0.0, 6.0)), ("vec", (0.0, 0.0, 1.0))))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", 40.0)))
+    d = dist((10, 0, 0), (0.0, 0, 0), (44.0, 0, 0))
+    d.insert_double(1.0).insert_double(0.5).insert_int(1)
+    s.add(cf.DISTRIBUTION, "DistributionTimeUniform", d)
+    return s
+
+
+def focused_shock():
+    """TrajectoryFocused (forward) across a moving discontinuous shock: U and B by side, every derivative
+    zero (BackgroundShock::EvaluateBackgroundDerivatives, src/background_shock.cc:116-128); started upstream
+    on a shell, a pass plane downstream, a time limit, binned in time."""
+    s = cf.Setup("focused_shock", cf.TRAJ_FOCUSED, cf.FORWARD)
+    s.add(cf.BACKGROUND, "BackgroundShock", shock_oblique(u0=(0.02, 0.0, 0.0), B=(2.0, 0.0, 1.0), v_shock=-0.005))
+    p0 = cf.mom_from_ekin(10.0 * cf.MeV)
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((-0.2, 0.0, 0.0)))
+    s.add(cf.INITIAL, "InitialMomentumShell", DC().insert_double(p0))
+    s.add(cf.BOUNDARY, "BoundaryPlanePass", cf.boundary(-1, [0], ("vec", (3.0, 0.0, 0.0)), ("vec", (1.0, 0.0, 0.0))))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", 60.0)))
+    d = dist((10, 0, 0), (0.0, 0, 0), (66.0, 0, 0), (0, 0, 0), (1, 0, 0))
+    d.insert_double(1.0).insert_double(0.0).insert_int(1)
+    s.add(cf.DISTRIBUTION, "DistributionTimeUniform", d)
+    return s
+
+
+def small_spiral_grid():
+    """The Parker spiral sampled on 8^3 blocks of 4^3 zones over +-8 au (0.5 au zones)."""
+    from spectrum_amd import cartesian
+    return cartesian.parker_spiral(nb=8, corner_au=8.0)
+
+
+def lorentz_grid():
+    """TrajectoryLorentz (backward) on the gridded solar wind: B and |B| interpolated, E from the interpolated
+    flow and field (the server's own), dmax limited by the zone; the lorentz_spiral_backward problem."""
+    s = lorentz_spiral_backward()
+    s.name, s.grid = "lorentz_grid", small_spiral_grid()
+    s.plugins = [x for x in s.plugins if x[0] != cf.BACKGROUND]
+    s.add(cf.BACKGROUND, "BackgroundServerCartesian", cf.background_base(DC(), 0.0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), cf.one_au))
+    return s
+
+
+def focused_grid():
+    """TrajectoryFocused (forward) on the gridded solar wind: the server's stencil gradients of U and B give
+    the focusing, cooling and drift terms; the focused_spiral problem."""
+    s = focused_spiral()
+    s.name, s.grid = "focused_grid", small_spiral_grid()
+    s.plugins = [x for x in s.plugins if x[0] != cf.BACKGROUND]
+    s.add(cf.BACKGROUND, "BackgroundServerCartesian", cf.background_base(DC(), 0.0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), cf.one_au))
+    return s
+
+
 ALL = [shock_flowmom_reflect, solarwind_kinrad_reflect_recurrent, smoothshock_perp_momentum, uniform_para_plane_pass,
        lambda: modulation_spectrum_type(0), lambda: modulation_spectrum_type(2), gc_rigidity_reflect, gc_isotropic_only,
        guiding_spiral, guiding_scatt_uniform, guiding_diff_rigidity, parker_strauss_heliopause, parker_potgieter_uniform,
        gc_qlt_spiral, fieldline_spiral, fieldline_backward_shock, lorentz_uniform, lorentz_spiral_backward,
        focused_spiral, focused_uniform_backward, gc_wnlt_scatt_spiral, gc_wnlt_ramp_diffscatt,
-       gc_strauss_diff, focused_smoothshock, uniform_sphere_start, gc_sector_start, parker_soqlt_radial_limit, gc_soqlt_diff]
+       gc_strauss_diff, focused_smoothshock, uniform_sphere_start, gc_sector_start, parker_soqlt_radial_limit, gc_soqlt_diff,
+       lorentz_shock, focused_shock, lorentz_grid, focused_grid]
 NAMES = ["shock_flowmom_reflect", "solarwind_kinrad_reflect_recurrent", "smoothshock_perp_momentum", "uniform_para_plane_pass",
          "modulation_type0", "modulation_type2", "gc_rigidity_reflect", "gc_isotropic_only",
          "guiding_spiral", "guiding_scatt_uniform", "guiding_diff_rigidity", "parker_strauss_heliopause", "parker_potgieter_uniform",
          "gc_qlt_spiral", "fieldline_spiral", "fieldline_backward_shock", "lorentz_uniform", "lorentz_spiral_backward",
          "focused_spiral", "focused_uniform_backward", "gc_wnlt_scatt_spiral", "gc_wnlt_ramp_diffscatt",
          "gc_strauss_diff", "focused_smoothshock", "uniform_sphere_start", "gc_sector_start", "parker_soqlt_radial_limit",
-         "gc_soqlt_diff"]
+         "gc_soqlt_diff", "lorentz_shock", "focused_shock", "lorentz_grid", "focused_grid"]
 
 
 # Setups whose endpoints amplify last-bit differences far beyond 1e-10: QLT's mu derivative is
