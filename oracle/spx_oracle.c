@@ -587,7 +587,12 @@ static void smooth_shock_derivatives(const bg_t* bg, lane_t* L, spd_t* sp, uint1
       sp->gradBmag = mvmul(&sp->gradB, sp->bhat);
    }
    if ((mask & BG_dUdt) || (mask & BG_dEdt)) sp->dUdt = vscl(shock_transition_deriv(ds) * bg->v_shock / bg->width_shock, vsub(bg->u1, bg->u0));
-   if (mask & BG_dBdt) { sp->dBdt = vscl(shock_transition_deriv(ds) * bg->v_shock / bg->width_shock, vsub(bg->B1, bg->B0)); }
+   if (mask & BG_dBdt) {
+      sp->dBdt = vscl(shock_transition_deriv(ds) * bg->v_shock / bg->width_shock, vsub(bg->B1, bg->B0));
+      /* the class sets no d|B|/dt (the guiding centre reads it, src/trajectory_guiding.cc:115 and
+         common/spatial_data.hh:340); this is its value, bhat . dB/dt */
+      sp->dBmagdt = vdot(sp->dBdt, sp->bhat);
+   }
 }
 
 /* ServerCartesianFront::GetVariables + GetGradients with SERVER_INTERP_ORDER 1 and no ghost cells

@@ -611,8 +611,8 @@ static int check_transport(spx_sim* s)
    }
    else if (s->plan.traj_type >= SPX_TRAJ_GUIDING_DIFF_SCATT) {
       if (s->plan.time_flow != SPX_FLOW_FORWARD) return fail(s, SPX_ERR_ARG, "guiding-centre transport runs forward in time");
-      if (s->plan.bg.kind != SPX_BG_SOLARWIND && s->plan.bg.kind != SPX_BG_UNIFORM && s->plan.bg.kind != SPX_BG_GRID)
-         return fail(s, SPX_ERR_ARG, "guiding-centre transport supports the static SolarWind, Uniform and Cartesian-grid backgrounds");
+      // every background kind: the static SolarWind, Uniform and Cartesian grid, the discontinuous shock
+      // and the moving smooth shock (with its dB/dt terms)
       if (s->plan.df.kind == SPX_DF_FLOWMOM || s->plan.df.kind == SPX_DF_KINRAD)
          return fail(s, SPX_ERR_ARG, "guiding-centre transport: diffusion model needs grad U, which its field mask does not compute");
       // Strauss / Potgieter set no component 2 (src/diffusion_other.cc:886-888), so with pitch-angle scattering

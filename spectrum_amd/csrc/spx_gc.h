@@ -20,17 +20,24 @@ struct GcFields {
    v3<R> E, B, bhat, gradBmag, curlbhat;
    R Bmag, dmax;
    R rg;               // the solar wind's derivative step r_g (half of SpatialData::_dr)
+   v3<R> dbhatdt;      // time-dependent backgrounds (the moving smooth shock): d bhat / dt
+   R dBmagdt;          // and d|B| / dt = bhat . dB/dt
 };
 
 // CommonFields with mask ALL|gradB|dBdt (src/trajectory_guiding.cc:51): BackgroundBase::GetFields
 // (src/background_base.cc:425-456) with NumericalDerivatives (:228-312) for SolarWind in the
 // field-aligned frame at r_g = min(r_L(pm), dmax).  gradB enters only through curl(bhat) and
 // grad|B| (common/spatial_data.hh:304-341), which are accumulated here row by row in the order
-// of transpose(fa) * G.  Returns false on ExFieldError.
+// of transpose(fa) * G.  The smooth shock's derivatives are analytic (src/background_smooth_shock.cc:159-187)
+// and its dmax reads the shock coordinate of the previous evaluation, ds_prev (set to this one's); the
+// discontinuous shock has none (src/background_shock.cc:116-128).  Returns false on ExFieldError.
 template <typename R, int BG>
-__device__ __forceinline__ bool gc_fields(const SPX_CP spx_bg_t<R>& bg, double t, double x, double y, double z, R pm, GcFields<R>& F)
+__device__ __forceinline__ bool gc_fields(const SPX_CP spx_bg_t<R>& bg, double t, double x, double y, double z, R pm, R& ds_prev,
+                                          GcFields<R>& F)
 {
-   F.dmax = bg_dmax<R, BG>(bg, x, y, z, R(0));
+   F.dmax = bg_dmax<R, BG>(bg, x, y, z, ds_prev);
+   F.dbhatdt = mk<R>(R(0), R(0), R(0));
+   F.dBmagdt = R(0);
    if constexpr (BG == SPX_BG_GRID) {
       // BackgroundServer: interpolated fields and the stencil's derivatives, no finite differences
       v3<R> U;
@@ -46,14 +53,32 @@ __device__ __forceinline__ bool gc_fields(const SPX_CP spx_bg_t<R>& bg, double t
       return true;
    }
    v3<R> U, B;
-   R ds = R(0);
+   R ds = ds_prev;
    bg_eval<R, BG>(bg, t, x, y, z, 3, U, B, ds);
+   if (BG == SPX_BG_SMOOTHSHOCK) ds_prev = ds;
    F.B = B;
    F.Bmag = norm(B);
    if (F.Bmag < R(sp_tiny)) return false;
    F.bhat = B / F.Bmag;
    if (BG == SPX_BG_UNIFORM) F.E = v3of(bg.Eu);                    // src/background_uniform.cc:58-77
-   else F.E = (-cross(U, B)) / R(c_code);                          // src/background_solarwind.cc:250-253
+   else if (BG == SPX_BG_SHOCK)                                     // src/background_shock.cc:91-111: E0 / E1 by side
+      F.E = (dot(rel(bg, x, y, z), v3of(bg.n_shock)) - bg.v_shock * R(t) > R(0)) ? v3of(bg.E0) : v3of(bg.E1);
+   else F.E = (-cross(U, B)) / R(c_code);                          // src/background_solarwind.cc:250-253, smooth_shock.cc:151
+   if constexpr (BG == SPX_BG_SMOOTHSHOCK) {
+      // grad B = n (B0 - B1) f, grad|B| = grad B . bhat, dB/dt = f v_sh (B1 - B0) with f = transition' / width;
+      // d|B|/dt = bhat . dB/dt (the class sets no d|B|/dt; this is its value), d bhat/dt = (dB/dt - d|B|/dt bhat) / |B|
+      const R f = shock_transition_deriv(ds) / bg.width_shock;
+      const v3<R> n = v3of(bg.n_shock), db = v3of(bg.B0) - v3of(bg.B1);
+      const v3<R> g0 = f * (n.x * db), g1 = f * (n.y * db), g2 = f * (n.z * db);
+      F.gradBmag = mk<R>(dot(g0, F.bhat), dot(g1, F.bhat), dot(g2, F.bhat));
+      const v3<R> curlB = mk<R>(g1.z - g2.y, g2.x - g0.z, g0.y - g1.x);
+      F.curlbhat = (curlB - cross(F.gradBmag, F.bhat)) / F.Bmag;
+      const v3<R> dBdt = (shock_transition_deriv(ds) * bg.v_shock / bg.width_shock) * (v3of(bg.B1) - v3of(bg.B0));
+      F.dBmagdt = dot(dBdt, F.bhat);
+      F.dbhatdt = (dBdt - F.dBmagdt * F.bhat) / F.Bmag;
+      F.rg = R(0);
+      return true;
+   }
    if (BG == SPX_BG_SOLARWIND) {
       const R r_g = fmin(LarmorRadius(pm, F.Bmag), F.dmax);
       F.rg = r_g;
@@ -281,11 +306,13 @@ __device__ __forceinline__ void gc_perp(const SPX_CP spx_df_t<R>& df, const GcFi
 
 // TrajectoryGuiding::DriftCoeff / ModifiedFields (src/trajectory_guiding.cc:65-146): E*, B* and the
 // guiding-centre velocity for velocity components (v_perp, 0, v_par).
-template <typename R>
+template <typename R, int BG>
 __device__ __forceinline__ void gc_drift(const GcFields<R>& F, v3<R> vel, R pp, R pl, v3<R>& drift, v3<R>& Es, v3<R>& Bs)
 {
    const R rL = LarmorRadius(pp, F.Bmag), rR = LarmorRadius(pl, F.Bmag);
-   Es = F.E - ((rL * vel.x) * F.gradBmag) / (R(2) * R(c_code));
+   if (BG == SPX_BG_SMOOTHSHOCK)   // the time-dependent term -r_R |B| (d bhat/dt) / c of E*
+      Es = (F.E - ((rR * F.Bmag) * F.dbhatdt) / R(c_code)) - ((rL * vel.x) * F.gradBmag) / (R(2) * R(c_code));
+   else Es = F.E - ((rL * vel.x) * F.gradBmag) / (R(2) * R(c_code));
    Bs = F.B + (rR * F.Bmag) * F.curlbhat;
    drift = (vel.z * Bs + R(c_code) * cross(Es, F.bhat)) / dot(Bs, F.bhat);
 }
@@ -342,13 +369,18 @@ __device__ int gc_set_start(PlanPtr P, QueryPtr Q, LaunchPtr L, const spx_rng_de
    double pp1, pl1, pp, pl;
    gc_mom_sample(P, rng, first, traj, nu, ex, pp1, pl1);
    const double pm1 = sqrt(pp1 * pp1 + pl1 * pl1);
+   R ds = R(0);   // the smooth shock's coordinate at the start stands for the "previous" evaluation's
+   if (BG == SPX_BG_SMOOTHSHOCK) {
+      const v3<R> d = rel(bg, x, y, z);
+      ds = (dot(d, v3of(bg.n_shock)) - bg.v_shock * R(t)) / bg.width_shock;
+   }
    GcFields<R> F;
-   if (!gc_fields<R, BG>(bg, t, x, y, z, R(pm1), F)) return 1;
+   if (!gc_fields<R, BG>(bg, t, x, y, z, R(pm1), ds, F)) return 1;
    gc_mom_sample(P, rng, first, traj, nu, ex, pp, pl);
    if (ex) return 5;
    const uint64_t n = L->n_lanes;
    L->t[slot] = t; L->x[slot] = x; L->y[slot] = y; L->z[slot] = z; L->m0[slot] = pp; L->m1[slot] = 0.0; L->m2[slot] = pl;
-   L->amp[slot] = 0.0; L->wgt[slot] = 0.0; L->dt_prev[slot] = 0.0; L->src_old[slot] = 0.0; L->ds_prev[slot] = 0.0;
+   L->amp[slot] = 0.0; L->wgt[slot] = 0.0; L->dt_prev[slot] = 0.0; L->src_old[slot] = 0.0; L->ds_prev[slot] = (double)ds;
    L->dmax[slot] = (double)F.dmax;
    L->pf[slot] = pm1;
    L->steps[slot] = 0; L->n_events[slot] = 0;   // the draw counters continue across attempts
@@ -385,6 +417,7 @@ struct GcHot {
    double t, x, y, z, pp, pl;                   // time, position, (p_perp, p_par)
    double pf;                                   // |p| of the last field evaluation
    GcFields<R> F;
+   R ds_prev;                                   // BackgroundSmoothShock's hidden coordinate (smooth shock only)
    double bd[SPX_MAX_BND], bdo[SPX_MAX_BND];
    int bst[SPX_MAX_BND];
    NormalStream<R> rs;
@@ -432,7 +465,7 @@ __device__ __forceinline__ int gc_step(PlanPtr P, LaunchPtr L, const spx_rng_des
    R Dmumu = R(0), Vmu = R(0);
    if (pa_step) gc_dmumu<R, XD>(df, pmag, H.F.Bmag, mu, H.x, H.y, H.z, Dmumu, Vmu);
    v3<R> drift, Es, Bs;
-   gc_drift(H.F, vel, pp, pl, drift, Es, Bs);
+   gc_drift<R, BG>(H.F, vel, pp, pl, drift, Es, Bs);
    // PhysicalStep: TrajectoryGuiding's (src/trajectory_guiding.cc:96-100), GuidingDiff's with V_perp and
    // D_perp (src/trajectory_guiding_diff.cc:306-314), GuidingScatt's pitch-angle limits (:255-262)
    R dtp = R(0.5) * H.F.dmax / ((perp_step ? norm(drift + Vperp) : norm(drift)) + R(0.5) * norm(vel));
@@ -466,8 +499,9 @@ __device__ __forceinline__ int gc_step(PlanPtr P, LaunchPtr L, const spx_rng_des
       vel = gc_vel(pp, pl);
    }
    // Slopes with the scattered momentum
-   gc_drift(H.F, vel, pp, pl, drift, Es, Bs);
-   const R slope_pp = R(0.5) * pp / H.F.Bmag * dot(drift, H.F.gradBmag);
+   gc_drift<R, BG>(H.F, vel, pp, pl, drift, Es, Bs);
+   const R slope_pp = (BG == SPX_BG_SMOOTHSHOCK) ? R(0.5) * pp / H.F.Bmag * (H.F.dBmagdt + dot(drift, H.F.gradBmag))
+                                                 : R(0.5) * pp / H.F.Bmag * dot(drift, H.F.gradBmag);
    const R slope_pl = R(sp_charge) * dot(Es, Bs) / dot(Bs, H.F.bhat);
    pmag = rsqrt_(sqr(pp) + sqr(pl));
    v3<R> slope_pos = drift;
@@ -503,7 +537,7 @@ __device__ __forceinline__ int gc_step(PlanPtr P, LaunchPtr L, const spx_rng_des
    bflags = handle_boundaries<true>(P, L, slot, H, C, tid);
    if (!(bflags & 1)) {
       H.pf = sqrt(H.pp * H.pp + H.pl * H.pl);
-      if (!gc_fields<R, BG>(bg, H.t, H.x, H.y, H.z, R(H.pf), H.F)) return 1;
+      if (!gc_fields<R, BG>(bg, H.t, H.x, H.y, H.z, R(H.pf), H.ds_prev, H.F)) return 1;
    }
    H.steps++;
    if (H.steps > P->max_steps) return 4;
@@ -554,8 +588,12 @@ __global__ __launch_bounds__(BLOCK) SPX_GC_ATTR void gc_step_kernel(const Launch
          H.bd[b] = Lp->bdelta[b * n + slot]; H.bdo[b] = Lp->bdelta_old[b * n + slot];
          C.cross[b][tid] = Lp->bcross[b * n + slot]; H.bst[b] = Lp->bstat[b * n + slot];
       }
-      // the fields are a function of (t, x, |p| of their evaluation) and are recomputed bit for bit
-      (void)gc_fields<R, BG>(PlanView<R>::bg(P), H.t, H.x, H.y, H.z, R(H.pf), H.F);
+      // the fields are a function of (t, x, |p| of their evaluation) and are recomputed bit for bit; dmax
+      // (which may depend on the previous evaluation) is restored
+      H.ds_prev = (BG == SPX_BG_SMOOTHSHOCK) ? R(Lp->ds_prev[slot]) : R(0);
+      R ds_tmp = H.ds_prev;
+      (void)gc_fields<R, BG>(PlanView<R>::bg(P), H.t, H.x, H.y, H.z, R(H.pf), ds_tmp, H.F);
+      if (BG == SPX_BG_SMOOTHSHOCK) H.F.dmax = R(Lp->dmax[slot]);
    }
    int st = run ? SLOT_RUN : SLOT_IDLE;
    int exc = 0;
@@ -591,6 +629,7 @@ __global__ __launch_bounds__(BLOCK) SPX_GC_ATTR void gc_step_kernel(const Launch
    Ls->pf[slot] = H.pf;
    Ls->steps[slot] = H.steps; Ls->nn[slot] = H.rs.nn;
    Ls->dmax[slot] = (double)H.F.dmax;
+   if (BG == SPX_BG_SMOOTHSHOCK) Ls->ds_prev[slot] = (double)H.ds_prev;
    Ls->amp[slot] = C.amp[tid]; Ls->wgt[slot] = C.wgt[tid];
 #pragma unroll
    for (int b = 0; b < SPX_MAX_BND; b++) {
@@ -625,6 +664,8 @@ hipError_t dispatch_gc(const spx_plan_desc& hp, uint64_t n_lanes, const Launch* 
    case SPX_BG_UNIFORM:
       return xd ? launch_gc_bg<R, SPX_BG_UNIFORM, true>(n_lanes, d_launch, st, mid) : launch_gc_bg<R, SPX_BG_UNIFORM, false>(n_lanes, d_launch, st, mid);
    case SPX_BG_GRID: return xd ? hipErrorInvalidValue : launch_gc_bg<R, SPX_BG_GRID, false>(n_lanes, d_launch, st, mid);
+   case SPX_BG_SHOCK: return xd ? hipErrorInvalidValue : launch_gc_bg<R, SPX_BG_SHOCK, false>(n_lanes, d_launch, st, mid);
+   case SPX_BG_SMOOTHSHOCK: return xd ? hipErrorInvalidValue : launch_gc_bg<R, SPX_BG_SMOOTHSHOCK, false>(n_lanes, d_launch, st, mid);
    default: return hipErrorInvalidValue;
    }
 }

@@ -569,20 +569,57 @@ def focused_grid():
     return s
 
 
+def gc_smoothshock_diffscatt():
+    """GuidingDiffScatt across a moving oblique smooth shock: analytic grad B and grad|B|, the time-dependent
+    terms of E* (d bhat / dt) and dp_perp/dt (d|B|/dt), dmax from the shock coordinate of the previous
+    evaluation carried per slot; isotropic pitch-angle scattering and perpendicular diffusion."""
+    s = cf.Setup("gc_smoothshock_diffscatt", cf.TRAJ_GUIDING_DIFF_SCATT, cf.FORWARD)
+    w = 0.002 * cf.one_au
+    B = 5.0e-5 / cf.unit_magnetic
+    dc = cf.background_base(DC(), 0.0, (0.0, 0.0, 0.0), (0.012, 0.003, 0.0), (B, 2.0 * B, 0.5 * B), 10.0 * w)
+    dc.insert_vec((-1.0, 0.0, 0.0)).insert_double(-0.004).insert_double(3.0).insert_double(w).insert_double(0.1)
+    s.add(cf.BACKGROUND, "BackgroundSmoothShock", dc)
+    s.add(cf.DIFFUSION, "DiffusionPerpConstantPitchIsotropic", DC().insert_double(2.0e17 / cf.unit_diffusion).insert_double(0.05))
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((0.15 * w, 0.0, 0.0)))
+    s.add(cf.INITIAL, "InitialMomentumShell", DC().insert_double(cf.mom_from_ekin(2.0 * cf.MeV)))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", 30.0)))
+    d = dist((10, 0, 0), (0.0, 0, 0), (60.0, 0, 0), (0, 0, 0), (1, 0, 0))
+    d.insert_double(1.0).insert_double(0.0).insert_int(1)
+    s.add(cf.DISTRIBUTION, "DistributionTimeUniform", d)
+    return s
+
+
+def guiding_shock():
+    """TrajectoryGuiding across the discontinuous oblique shock: E0 / E1 and B by side, no gradients; started
+    upstream on a thick shell (isotropic mu), a time limit."""
+    s = cf.Setup("guiding_shock", cf.TRAJ_GUIDING, cf.FORWARD)
+    s.add(cf.BACKGROUND, "BackgroundShock", shock_oblique(u0=(0.02, 0.004, 0.0), B=(1.0, 2.0, 0.5), v_shock=-0.002))
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((-0.5, 0.0, 0.0)))
+    s.add(cf.INITIAL, "InitialMomentumThickShell", DC().insert_double(cf.mom_from_ekin(1.0 * cf.MeV)).insert_double(cf.mom_from_ekin(20.0 * cf.MeV)).insert_bool(True))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", 60.0)))
+    d = dist((10, 0, 0), (0.0, 0, 0), (120.0, 0, 0), (0, 0, 0), (1, 0, 0))
+    d.insert_double(1.0).insert_double(0.0).insert_int(1)
+    s.add(cf.DISTRIBUTION, "DistributionTimeUniform", d)
+    return s
+
+
 ALL = [shock_flowmom_reflect, solarwind_kinrad_reflect_recurrent, smoothshock_perp_momentum, uniform_para_plane_pass,
        lambda: modulation_spectrum_type(0), lambda: modulation_spectrum_type(2), gc_rigidity_reflect, gc_isotropic_only,
        guiding_spiral, guiding_scatt_uniform, guiding_diff_rigidity, parker_strauss_heliopause, parker_potgieter_uniform,
        gc_qlt_spiral, fieldline_spiral, fieldline_backward_shock, lorentz_uniform, lorentz_spiral_backward,
        focused_spiral, focused_uniform_backward, gc_wnlt_scatt_spiral, gc_wnlt_ramp_diffscatt,
        gc_strauss_diff, focused_smoothshock, uniform_sphere_start, gc_sector_start, parker_soqlt_radial_limit, gc_soqlt_diff,
-       lorentz_shock, focused_shock, lorentz_grid, focused_grid]
+       lorentz_shock, focused_shock, lorentz_grid, focused_grid, gc_smoothshock_diffscatt, guiding_shock]
 NAMES = ["shock_flowmom_reflect", "solarwind_kinrad_reflect_recurrent", "smoothshock_perp_momentum", "uniform_para_plane_pass",
          "modulation_type0", "modulation_type2", "gc_rigidity_reflect", "gc_isotropic_only",
          "guiding_spiral", "guiding_scatt_uniform", "guiding_diff_rigidity", "parker_strauss_heliopause", "parker_potgieter_uniform",
          "gc_qlt_spiral", "fieldline_spiral", "fieldline_backward_shock", "lorentz_uniform", "lorentz_spiral_backward",
          "focused_spiral", "focused_uniform_backward", "gc_wnlt_scatt_spiral", "gc_wnlt_ramp_diffscatt",
          "gc_strauss_diff", "focused_smoothshock", "uniform_sphere_start", "gc_sector_start", "parker_soqlt_radial_limit",
-         "gc_soqlt_diff", "lorentz_shock", "focused_shock", "lorentz_grid", "focused_grid"]
+         "gc_soqlt_diff", "lorentz_shock", "focused_shock", "lorentz_grid", "focused_grid", "gc_smoothshock_diffscatt",
+         "guiding_shock"]
 
 
 # Setups whose endpoints amplify last-bit differences far beyond 1e-10: QLT's mu derivative is

@@ -401,3 +401,83 @@ def test_wnlt_dmumu_by_quadrature(oracle_mod, q):
             got = wnlt.kappa_at(2, (0.0, 0.0, 0.0), p, mu)
             assert got == pytest.approx(d_qlt + extra, rel=1e-9), (T_MeV, mu)
             assert extra > 1e-3 * d_qlt                              # the nonlinear term is not negligible here
+
+
+# --- Guiding centre at a moving smooth shock: the time-dependent terms ----------------------------------
+def test_guiding_centre_at_a_moving_smooth_shock(oracle_mod):
+    """One guiding-centre step inside the transition of a moving smooth shock (src/background_smooth_shock.cc,
+    src/trajectory_guiding.cc:65-122): with dB/dt != 0, E* = E - r_R |B| (d bhat/dt) / c - (r_L v_perp / 2c)
+    grad|B| and dp_perp/dt = (p_perp / 2|B|) (d|B|/dt + v_d . grad|B|); every derivative is taken here by
+    fourth-order differences of the tanh profile written in numpy, in space and in time.  The time-dependent
+    terms are checked to matter: dropping them moves the answer by far more than the tolerance."""
+    n = np.array([-1.0, 0.0, 0.0])
+    u0 = np.array([0.012, 0.003, 0.0])
+    B0 = B1AU * np.array([1.0, 2.0, 0.5])
+    v_sh, r, w = -0.004, 3.0, 0.002 * AU
+    un, bn = u0 @ n, B0 @ n
+    u1 = ((un - v_sh) * n) / r + v_sh * n + (u0 - un * n)
+    B1 = bn * n + r * (B0 - bn * n)
+
+    def fields(x, t):
+        ds = ((x @ n) - v_sh * t) / w
+        a1 = 0.5 * (1.0 + math.tanh(8.0 * ds))
+        U = a1 * u0 + (1.0 - a1) * u1
+        B = a1 * B0 + (1.0 - a1) * B1
+        return U, B, -np.cross(U, B) / C
+
+    x0 = -0.15 * w * n + np.array([0.0, 0.3 * AU, -0.1 * AU])       # ds = 0.15 inside the transition
+    t0 = 0.0
+    p, theta = mom_of_ekin(2.0 * cf.MeV), 0.7
+    dc = cf.background_base(DC(), 0.0, (0.0, 0.0, 0.0), tuple(u0), tuple(B0), 10.0 * w)
+    dc.insert_vec(tuple(n)).insert_double(v_sh).insert_double(r).insert_double(w).insert_double(0.1)
+    s = cf.Setup("gc_smooth_shock_step", cf.TRAJ_GUIDING, cf.FORWARD)
+    s.add(cf.BACKGROUND, "BackgroundSmoothShock", dc)
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(t0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec(tuple(x0)))
+    s.add(cf.INITIAL, "InitialMomentumRing", DC().insert_double(p).insert_double(theta))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", 0.002)))
+    s.add(cf.DISTRIBUTION, "DistributionTimeUniform", time_dist(1.0))
+    rec = one_step(oracle_mod, s, n_normals=1)
+    assert rec["status"][0] == 0 and rec["steps"][0] == 1
+    dt = rec["t"][0] - rec["t0"][0]
+    pperp, ppar = p * math.sin(theta), p * math.cos(theta)
+
+    U, B, E = fields(x0, t0)
+    Bm = np.linalg.norm(B)
+    b = B / Bm
+    h, ht = 1e-4 * w, 1e-4 * w / abs(v_sh)
+    bmag = lambda x: np.array([np.linalg.norm(fields(x, t0)[1])])
+    bhat = lambda x: fields(x, t0)[1] / np.linalg.norm(fields(x, t0)[1])
+    gradB = np.array([d4(bmag, x0, j, h)[0] for j in range(3)])
+    J = np.array([d4(bhat, x0, j, h) for j in range(3)])            # J[j, i] = d b_i / d x_j
+    curlb = np.array([J[1, 2] - J[2, 1], J[2, 0] - J[0, 2], J[0, 1] - J[1, 0]])
+    tb = lambda t: fields(x0, t)[1]
+    dbdt_num = lambda f: (-f(t0 + 2 * ht) + 8 * f(t0 + ht) - 8 * f(t0 - ht) + f(t0 - 2 * ht)) / (12 * ht)
+    dBmagdt = dbdt_num(lambda t: np.linalg.norm(tb(t)))
+    dbhatdt = dbdt_num(lambda t: tb(t) / np.linalg.norm(tb(t)))
+    v = speed(p)
+    vperp, vpar = v * math.sin(theta), v * math.cos(theta)
+    rL, rR = pperp * C / (Q * Bm), ppar * C / (Q * Bm)
+
+    def predict(with_time):
+        Es = E - rL * vperp * gradB / (2.0 * C) - (rR * Bm * dbhatdt / C if with_time else 0.0)
+        Bs = B + rR * Bm * curlb
+        Bspar = Bs @ b
+        vd = (vpar * Bs + C * np.cross(Es, b)) / Bspar
+        return vd, Q * (Es @ Bs) / Bspar, 0.5 * pperp / Bm * ((dBmagdt if with_time else 0.0) + vd @ gradB)
+
+    vd, dppar, dpperp = predict(True)
+    vd0, dppar0, dpperp0 = predict(False)
+    got_vd = (rec["pos"][0] - x0) / dt
+    got_ppar = (rec["mom"][0, 2] - ppar) / dt
+    got_pperp = (rec["mom"][0, 0] - pperp) / dt
+    vperp_part = vd - (vd @ b) * b
+    tol_v = 1e-7 * np.linalg.norm(vperp_part)       # the differences above are 1e-7 relative or better
+    print(f"[gc smooth shock] |v_d - want| / |v_d,perp| {np.linalg.norm(got_vd - vd) / np.linalg.norm(vperp_part):.2e}; time term moves v_d by "
+          f"{np.linalg.norm(vd0 - vd) / np.linalg.norm(vperp_part):.2e}, dp_perp/dt by {abs(dpperp0 - dpperp) / abs(dpperp):.2e}; "
+          f"dp_perp/dt rel err {abs(got_pperp - dpperp) / abs(dpperp):.2e}, dp_par/dt rel err {abs(got_ppar - dppar) / abs(dppar):.2e}")
+    assert np.linalg.norm(got_vd - vd) < tol_v, (got_vd, vd)
+    assert np.linalg.norm(vd0 - vd) > 1000 * tol_v
+    assert got_pperp == pytest.approx(dpperp, rel=1e-9)
+    assert abs(dpperp0 - dpperp) > 1e6 * 1e-9 * abs(dpperp)
+    assert got_ppar == pytest.approx(dppar, rel=1e-9)
