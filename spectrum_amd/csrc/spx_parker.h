@@ -608,17 +608,23 @@ __device__ __forceinline__ void parker_step_body(const Launch* __restrict__ Lg)
    }
 }
 
-// SPX_PARKER_WAVES resident waves per SIMD asked of the compiler for the step kernel (0: its own budget).
-#ifndef SPX_PARKER_WAVES
-#define SPX_PARKER_WAVES 0
-#endif
-#if SPX_PARKER_WAVES > 0
-#define SPX_PARKER_ATTR __attribute__((amdgpu_waves_per_eu(SPX_PARKER_WAVES)))
+// Resident waves per SIMD asked of the compiler for a step kernel (1: its own register budget).  The FP64
+// smooth-shock source kernel (diffusive shock acceleration) runs 7.7% faster at 3 waves (168 VGPRs, 40 B
+// spilled) than at its natural 2 (177 VGPRs); the FP64 modulation kernel loses 3% at 3 (196 B spilled) and
+// keeps its natural 2, as do the smooth-shock source kernels of the other diffusion classes (192-216 B would
+// spill).  SPX_PARKER_WAVES, when defined, asks it of every instantiation.
+template <typename R, int BG, int TT, int DF>
+constexpr int pk_waves()
+{
+#ifdef SPX_PARKER_WAVES
+   return SPX_PARKER_WAVES;
 #else
-#define SPX_PARKER_ATTR
+   return (!is_f32<R>() && BG == SPX_BG_SMOOTHSHOCK && TT == 1 && DF == SPX_DF_FULL) ? 3 : 1;
 #endif
+}
 template <typename R, int BG, int FWD, int TT, int DF>
-__global__ __launch_bounds__(BLOCK) SPX_PARKER_ATTR void parker_step_kernel(const Launch* __restrict__ Lg)
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(pk_waves<R, BG, TT, DF>())))
+void parker_step_kernel(const Launch* __restrict__ Lg)
 {
    parker_step_body<R, BG, FWD, TT, DF>(Lg);
 }

@@ -138,14 +138,14 @@ def test_mixed_matches_fp64_modulation_spectrum():
 def test_mixed_step_arithmetic_bias_on_shared_paths():
     """What the FP32 step arithmetic changes on one path: both kernels replay the same pre-generated N(0,1)
     increments (the counter streams of the two precisions map Philox blocks to normals differently), and
-    over half a day (a few hundred steps) the paths stay close, so the paired difference of the spectrum
+    over 0.1 day (a few hundred steps) the paths stay close, so the paired difference of the spectrum
     weight J(p) at the time limit isolates the FP32 step error from the Monte Carlo noise.  Its mean (the
     systematic part) must be small against the Monte Carlo error of a 1e7-trajectory spectrum."""
-    n_e, per = 64, 250
-    s = cf.parker_modulation(n_energies=n_e, t_limit_years=0.5 / 365.0, inner_au=None, time_action=0)
+    n_e, per = 64, 100
+    s = cf.parker_modulation(n_energies=n_e, t_limit_years=0.1 / 365.0, inner_au=None, time_action=0)
     n = n_e * per
     rng = np.random.default_rng(77)
-    normals = rng.standard_normal((n, 6000))
+    normals = rng.standard_normal((n, 12000))         # 3 draws per step; the longest of 6400 took 3534 steps (oracle)
     uniforms = rng.random((n, 8))
     out = {}
     for prec in PRECS:
@@ -161,11 +161,17 @@ def test_mixed_step_arithmetic_bias_on_shared_paths():
     dpos = np.linalg.norm(m["pos"] - f["pos"], axis=1) / np.linalg.norm(f["pos"], axis=1)
     same_steps = (m["steps"] == f["steps"]).mean()
     bias, bias_err = rel.mean(), rel.std() / math.sqrt(n)
+    # a trajectory whose step count differs took one step-size decision the other way and left the shared
+    # path (a fraction of a percent); the per-energy bound is taken over those that stayed on it
     bins = np.arange(n) % n_e
-    per_bin = max(abs(rel[bins == b].mean()) + NSIG * rel[bins == b].std() / math.sqrt(per) for b in range(n_e))
-    print(f"[check] shared paths over half a day: weight rel. difference median |.| {np.median(np.abs(rel)):.2e}, mean {bias:+.2e} +- {bias_err:.1e}, "
-          f"worst energy |mean| + 3.5 sigma {per_bin:.2e}; position rel. difference median {np.median(dpos):.2e}; same step count {same_steps:.3f}",
-          flush=True)
+    shared = m["steps"] == f["steps"]
+    per_bin = max(abs(rel[(bins == b) & shared].mean()) + NSIG * rel[(bins == b) & shared].std() / math.sqrt(((bins == b) & shared).sum())
+                  for b in range(n_e))
+    print(f"[check] shared paths over 0.1 day: weight rel. difference median |.| {np.median(np.abs(rel)):.2e}, mean {bias:+.2e} +- {bias_err:.1e} "
+          f"(all trajectories); worst energy |mean| + 3.5 sigma {per_bin:.2e} (those with equal step counts); position rel. difference median "
+          f"{np.median(dpos):.2e}; same step count {same_steps:.4f}", flush=True)
     # the smallest relative Monte Carlo error of a whole-trajectory bin at 156250 per energy (the least
     # modulated energy, 10 GeV) is ~4e-5
+    assert same_steps > 0.99
+    assert abs(bias) + NSIG * bias_err < 1e-5
     assert per_bin < 1e-5
