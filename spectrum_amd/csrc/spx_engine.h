@@ -71,49 +71,101 @@ template <typename R>
 struct NormalStream {
    uint64_t traj;
    uint32_t nn;          // normal draws consumed
-   R s0, s1, s2;         // the next normals of the last Philox block, in order (FP64: its sine branch in s0)
-   bool spare_ok;
+   uint32_t ns;          // spares: draws nn .. nn+ns-1 in s0, s1, s2, always the tail of a Philox block
+   R s0, s1, s2;
    bool exhausted;
 };
 
-// Normal draw nn of the trajectory's stream.  FP64: block nn/2, cosine branch for even nn, sine for odd
-// (normal_pair_f64); FP32: block nn/4, the four normals of normal_quad_f32 in order, the ones not yet
-// drawn kept as a queue (popped by moves: a select between the fields would make the compiler index the
-// lane's state in memory).  The spares are valid only within one launch (spare_ok is cleared when a
-// launch loads the slot), so a stream resumed in the middle of a block regenerates it and picks the same value.
+// The stream: FP64 normal k is the cosine (even k) or sine (odd k) branch of Philox block k/2
+// (normal_pair_f64); FP32 normal k is word k%4 of block k/4 (normal_quad_f32).  The spares left by a
+// draw are valid within one launch; a launch that loads a slot whose stream stops inside a block
+// regenerates that block's tail (prime_normals), so the same draws come out whatever the launch shape.
 template <typename R>
-__device__ __forceinline__ R next_normal(const spx_rng_desc& rng, uint64_t first, NormalStream<R>& S)
+__device__ __forceinline__ void prime_normals(const spx_rng_desc& rng, NormalStream<R>& S)
 {
-   const uint32_t k = S.nn++;
-   if (rng.mode == 1) {
-      if ((int64_t)k >= rng.normal_stride) { S.exhausted = true; return R(0); }
-      return R(rng.normals[(S.traj - first) * (uint64_t)rng.normal_stride + k]);
-   }
+   S.ns = 0;
+   S.s0 = S.s1 = S.s2 = R(0);
+   if (rng.mode == 1) return;
    if constexpr (is_f32<R>()) {
-      const uint32_t j = k & 3u;
-      if (j != 0u && S.spare_ok) {
-         const float v = S.s0;
-         S.s0 = S.s1;
-         S.s1 = S.s2;
-         return v;
+      const uint32_t j = S.nn & 3u;
+      if (j != 0u) {
+         float n0, n1, n2, n3;
+         normal_quad_f32(rng.seed, S.traj, S.nn >> 2, n0, n1, n2, n3);
+         S.s0 = j == 1u ? n1 : (j == 2u ? n2 : n3);
+         S.s1 = j == 1u ? n2 : n3;
+         S.s2 = n3;
+         S.ns = 4u - j;
       }
-      float n0, n1, n2, n3;
-      normal_quad_f32(rng.seed, S.traj, k >> 2, n0, n1, n2, n3);
-      S.s0 = j == 0u ? n1 : (j == 1u ? n2 : n3);
-      S.s1 = j == 0u ? n2 : n3;
-      S.s2 = n3;
-      S.spare_ok = true;
-      return j == 0u ? n0 : (j == 1u ? n1 : (j == 2u ? n2 : n3));
+   }
+   else if (S.nn & 1u) {
+      double c, s;
+      normal_pair_f64(rng.seed, S.traj, S.nn >> 1, c, s);
+      S.s0 = s;
+      S.ns = 1u;
+   }
+}
+
+// The next N (1..3) normal draws of the stream, in order.  A lane generates at most one Philox block
+// per call (FP32: four normals) or two (FP64: two each), at one place in the code: the lanes of a wave
+// sit at different offsets in their blocks, and a generator inlined once per draw would run, masked,
+// once per draw for the whole wave.  New normals are appended behind the spares (at most two of them
+// when a block is needed, so six places suffice), and the spares left over end at a block boundary.
+template <typename R, int N>
+__device__ __forceinline__ void next_normals(const spx_rng_desc& rng, uint64_t first, NormalStream<R>& S, R (&out)[N])
+{
+   static_assert(N >= 1 && N <= 3, "one to three draws per call");
+   if (rng.mode == 1) {
+      const uint64_t base = (S.traj - first) * (uint64_t)rng.normal_stride;
+#pragma unroll
+      for (int i = 0; i < N; i++) {
+         const uint32_t k = S.nn + (uint32_t)i;
+         if ((int64_t)k >= rng.normal_stride) { S.exhausted = true; out[i] = R(0); }
+         else out[i] = R(rng.normals[base + k]);
+      }
+      S.nn += N;
+      return;
+   }
+   uint32_t ns = S.ns;
+   R b0 = S.s0, b1 = S.s1, b2 = S.s2, b3 = R(0), b4 = R(0), b5 = R(0);
+   if constexpr (is_f32<R>()) {
+      if (ns < (uint32_t)N) {   // ns is 0, 1 or 2 here
+         float w0, w1, w2, w3;
+         normal_quad_f32(rng.seed, S.traj, (S.nn + ns) >> 2, w0, w1, w2, w3);
+         b5 = w3;
+         b4 = ns == 1u ? w3 : w2;
+         b3 = ns == 0u ? w3 : (ns == 1u ? w2 : w1);
+         b2 = ns == 0u ? w2 : (ns == 1u ? w1 : w0);
+         b1 = ns == 0u ? w1 : (ns == 1u ? w0 : b1);
+         b0 = ns == 0u ? w0 : b0;
+         ns += 4u;
+      }
    }
    else {
-      if ((k & 1u) && S.spare_ok) { S.spare_ok = false; return S.s0; }
-      double c, s;
-      normal_pair_f64(rng.seed, S.traj, k >> 1, c, s);
-      if (k & 1u) return s;
-      S.s0 = s;
-      S.spare_ok = true;
-      return c;
+      // ns is 0 or 1 on entry: the first pair is appended at ns, a second one (N = 3 from ns = 0) at 2
+      if (ns < (uint32_t)N) {
+         double c, s;
+         normal_pair_f64(rng.seed, S.traj, (S.nn + ns) >> 1, c, s);
+         b2 = ns == 1u ? s : b2;
+         b1 = ns == 1u ? c : s;
+         b0 = ns == 1u ? b0 : c;
+         ns += 2u;
+      }
+      if (N == 3 && ns < 3u) {
+         double c, s;
+         normal_pair_f64(rng.seed, S.traj, (S.nn + ns) >> 1, c, s);
+         b2 = c;
+         b3 = s;
+         ns += 2u;
+      }
    }
+   out[0] = b0;
+   if (N > 1) out[1] = b1;
+   if (N > 2) out[2] = b2;
+   S.s0 = N == 1 ? b1 : (N == 2 ? b2 : b3);
+   S.s1 = N == 1 ? b2 : (N == 2 ? b3 : b4);
+   S.s2 = N == 1 ? b3 : (N == 2 ? b4 : b5);
+   S.ns = ns - (uint32_t)N;
+   S.nn += N;
 }
 
 __device__ __forceinline__ double next_uniform(const spx_rng_desc& rng, uint64_t first, uint64_t traj, uint32_t& nu, bool& exhausted)

@@ -480,17 +480,41 @@ __device__ __forceinline__ int gc_step(PlanPtr P, LaunchPtr L, const spx_rng_des
    double dt = fmin(P->dt_scale == 1.0 ? (double)dtp : (double)dtp * P->dt_scale, C.dt_adapt[tid]);   // 1: the reference's step
    dt = time_proximity<1>(P, H, dt, (double)H.F.dmax);
    const R dtr = R(dt);
+   // The step's normal draws in stream order, taken together: the pitch-angle draw, then the two of the
+   // perpendicular displacement (a pitch-angle redraw moves them up and draws one more)
+   R q0 = R(0), q1 = R(0), q2 = R(0);
+   if (pa_step && perp_step) {
+      R nrm[3];
+      next_normals<R, 3>(rng, first, H.rs, nrm);
+      q0 = nrm[0]; q1 = nrm[1]; q2 = nrm[2];
+   }
+   else if (pa_step) {
+      R nrm[1];
+      next_normals<R, 1>(rng, first, H.rs, nrm);
+      q0 = nrm[0];
+   }
+   else if (perp_step) {
+      R nrm[2];
+      next_normals<R, 2>(rng, first, H.rs, nrm);
+      q0 = nrm[0]; q1 = nrm[1];
+   }
+   if (H.rs.exhausted) return 5;
    // EulerPitchAngleScatt: redraw until |mu'| <= 1
    double ppd = H.pp, pld = H.pl;
    if (pa_step) {
       const R sd = rsqrt_(R(2) * Dmumu * dtr);
       R mu_new;
       int guard = 0;
-      do {
-         mu_new = mu + Vmu * dtr + sd * next_normal<R>(rng, first, H.rs);
+      while (true) {
+         mu_new = mu + Vmu * dtr + sd * q0;
+         if (!(fabs(mu_new) > R(1))) break;
+         if (++guard >= 100000) return 3;   // the first 100000 draws rejected
+         R nrm[1];
+         next_normals<R, 1>(rng, first, H.rs, nrm);
          if (H.rs.exhausted) return 5;
-         if (++guard > 100000) return 3;
-      } while (fabs(mu_new) > R(1));
+         if (perp_step) { q0 = q1; q1 = q2; q2 = nrm[0]; }
+         else q0 = nrm[0];
+      }
       // the new momentum components are formed in double from |p| (an FP32 step keeps |p| exact here)
       ppd = pmag_d * sqrt(1.0 - (double)mu_new * (double)mu_new);
       pld = pmag_d * (double)mu_new;
@@ -512,9 +536,7 @@ __device__ __forceinline__ int gc_step(PlanPtr P, LaunchPtr L, const spx_rng_des
       slope_pos = drift + Vperp;
       // EulerPerpDiffSlopes
       const R sdt = rsqrt_(dtr);
-      const R dWx = sdt * next_normal<R>(rng, first, H.rs);
-      const R dWy = sdt * next_normal<R>(rng, first, H.rs);
-      if (H.rs.exhausted) return 5;
+      const R dWx = sdt * (pa_step ? q1 : q0), dWy = sdt * (pa_step ? q2 : q1);
       Dperp = gc_kperp<R, XD>(df, pmag, mu2, H.F.Bmag, H.x, H.y, H.z, outer);
       dr = from_basis(mk<R>(rsqrt_(R(2) * Dperp) * dWx, rsqrt_(R(2) * Dperp) * dWy, R(0)), fa);
    }
@@ -581,7 +603,8 @@ __global__ __launch_bounds__(BLOCK) SPX_GC_ATTR void gc_step_kernel(const Launch
       }
       H.pf = Lp->pf[slot];
       H.steps = (int32_t)Lp->steps[slot];
-      H.rs.traj = Lp->traj[slot]; H.rs.nn = Lp->nn[slot]; H.rs.spare_ok = false; H.rs.exhausted = false; H.rs.s0 = H.rs.s1 = H.rs.s2 = R(0);
+      H.rs.traj = Lp->traj[slot]; H.rs.nn = Lp->nn[slot]; H.rs.exhausted = false;
+      prime_normals<R>(rng, H.rs);
       C.amp[tid] = Lp->amp[slot]; C.wgt[tid] = Lp->wgt[slot]; C.dt_adapt[tid] = Lp->dt_adapt[slot];
 #pragma unroll
       for (int b = 0; b < SPX_MAX_BND; b++) {

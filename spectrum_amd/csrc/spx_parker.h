@@ -472,9 +472,9 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
    fa[0] = second_unit(H.F.bhat);
    fa[1] = cross(fa[2], fa[0]);
    const R sdt = rsqrt_(R(dt));
-   const R dWx = sdt * next_normal<R>(rng, first, H.rs);
-   const R dWy = sdt * next_normal<R>(rng, first, H.rs);
-   const R dWz = sdt * next_normal<R>(rng, first, H.rs);
+   R nrm[3];
+   next_normals<R, 3>(rng, first, H.rs, nrm);
+   const R dWx = sdt * nrm[0], dWy = sdt * nrm[1], dWz = sdt * nrm[2];
    if (H.rs.exhausted) return 5;
    const v3<R> drp = from_basis(mk<R>(rsqrt_(R(2) * K0) * dWx, rsqrt_(R(2) * K0) * dWy, rsqrt_(R(2) * K1) * dWz), fa);
    // RKStep, Euler (src/trajectory_base.cc:362-389), then the stochastic displacement
@@ -545,7 +545,8 @@ __device__ __forceinline__ void parker_step_body(const Launch* __restrict__ Lg)
       }
       H.steps = (int32_t)Lp->steps[slot];
       H.ds_prev = R(Lp->ds_prev[slot]);
-      H.rs.traj = Lp->traj[slot]; H.rs.nn = Lp->nn[slot]; H.rs.spare_ok = false; H.rs.exhausted = false; H.rs.s0 = H.rs.s1 = H.rs.s2 = R(0);
+      H.rs.traj = Lp->traj[slot]; H.rs.nn = Lp->nn[slot]; H.rs.exhausted = false;
+      prime_normals<R>(rng, H.rs);
       C.amp[tid] = Lp->amp[slot]; C.wgt[tid] = Lp->wgt[slot]; C.dt_adapt[tid] = Lp->dt_adapt[slot];
       C.dt_prev[tid] = Lp->dt_prev[slot]; C.src_old[tid] = Lp->src_old[slot];
 #pragma unroll

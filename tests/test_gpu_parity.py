@@ -135,14 +135,22 @@ def test_dsa_smooth_shock_fp64_counter(device, oracle_mod):
     assert (rec["amp"] != 0).any()
 
 
-def test_results_do_not_depend_on_launch_shape(device):
-    """Counter RNG is keyed by trajectory id: lanes, block, steps per launch and graph capture change nothing."""
-    s = cf.parker_modulation(n_energies=8, t_limit_years=2.0 / 365.0, time_action=0)
-    n = 1000
+@pytest.mark.parametrize("prec", [E.PREC_FP64, E.PREC_MIXED])
+@pytest.mark.parametrize("case", ["parker", "gc"])
+def test_results_do_not_depend_on_launch_shape(device, prec, case):
+    """Counter RNG is keyed by trajectory id: lanes, block, steps per launch and graph capture change nothing.
+    Launches of 97 and 7 steps stop streams at every offset inside a Philox block (3 normals per step), so a
+    resumed stream regenerates the block's tail."""
+    if case == "parker":
+        s = cf.parker_modulation(n_energies=8, t_limit_years=2.0 / 365.0, time_action=0)
+        n = 1000
+    else:
+        s = cf.gc_diff_scatt(t_limit_days=0.05)
+        n = 256
     runs = []
     for opts in (dict(lanes=256, steps_per_launch=97, block=64), dict(lanes=4096, steps_per_launch=1000, block=256),
-                 dict(lanes=512, steps_per_launch=200, block=128, graph_launches=4)):
-        sim = E.Simulation.from_setup(s, precision=E.PREC_FP64, **opts)
+                 dict(lanes=512, steps_per_launch=200, block=128, graph_launches=4), dict(lanes=256, steps_per_launch=7, block=256)):
+        sim = E.Simulation.from_setup(s, precision=prec, **opts)
         st, rec = sim.run(n, seed=5)
         runs.append((st, rec, sim.distro(0)))
     for st, rec, (c, sm, ne) in runs[1:]:
@@ -150,6 +158,63 @@ def test_results_do_not_depend_on_launch_shape(device):
         np.testing.assert_array_equal(c, runs[0][2][0])
         np.testing.assert_array_equal(sm, runs[0][2][1])   # exact fixed-point sums: no order dependence
         assert st.steps_done == runs[0][0].steps_done
+
+
+def f32_counter_normals(oracle_mod, seed, traj, count):
+    """Normal draws 0..count-1 of a trajectory's FP32-step counter stream, restated on the host from the
+    stream's definition: draw k is word k % 4 of Philox4x32-10 block k / 4 (key = seed, counter = (k / 4,
+    2 << 28, traj)); Box-Muller on the word pairs (w0, w1) and (w2, w3), cosine branch first, with the radius
+    uniform (float(w) + 1/2) 2^-32 and the angle uniform ((w >> 8) + 1) 2^-24."""
+    out = np.empty(4 * ((count + 3) // 4))
+    key = (seed & 0xFFFFFFFF, seed >> 32)
+    for b in range(len(out) // 4):
+        w = oracle_mod.philox((b & 0xFFFFFFFF, ((b >> 32) & 0x0FFFFFFF) | (2 << 28), traj & 0xFFFFFFFF, traj >> 32), key)
+        for h in range(2):
+            a, c = int(w[2 * h]), int(w[2 * h + 1])
+            u1 = float(np.float32(float(np.float32(a)) * 2.0 ** -32 + 2.0 ** -33))
+            u2 = ((c >> 8) + 1) * 2.0 ** -24
+            r, ang = np.sqrt(-2.0 * np.log(u1)), 2.0 * np.pi * u2
+            out[4 * b + 2 * h], out[4 * b + 2 * h + 1] = r * np.cos(ang), r * np.sin(ang)
+    return out[:count]
+
+
+def counter_uniforms(oracle_mod, seed, traj, count):
+    """Uniform draws 0..count-1 of a trajectory's counter stream: the 53-bit uniform of words (w0, w1) of
+    Philox block k with the uniform tag (1 << 28) in the counter."""
+    key = (seed & 0xFFFFFFFF, seed >> 32)
+    out = []
+    for k in range(count):
+        w = oracle_mod.philox((k, 1 << 28, traj & 0xFFFFFFFF, traj >> 32), key)
+        x = (int(w[1]) << 32) | int(w[0])
+        out.append(((x >> 11) + 1) * 2.0 ** -53)
+    return np.array(out)
+
+
+def test_mixed_counter_stream_follows_its_definition(device, oracle_mod):
+    """The FP32 step's counter normals are the stream restated above, in order: the same run with those
+    draws replayed from the host ends at the same points (within the FP32 step's rounding and its fast
+    log / sincos), while draws taken from the wrong word, block or branch would move every endpoint by the
+    size of its displacement.  Launches of 5 steps resume streams at every offset inside a block."""
+    s = cf.parker_modulation(n_energies=4, t_limit_years=0.5 / 365.0, time_action=0)
+    n, seed = 64, 9
+    sim = E.Simulation.from_setup(s, precision=E.PREC_MIXED, lanes=256, steps_per_launch=5)
+    st, rec = sim.run(n, seed=seed)
+    assert st.done == n and st.failed == 0
+    count = 3 * int(rec["steps"].max()) + 16
+    normals = np.stack([f32_counter_normals(oracle_mod, seed, i, count) for i in range(n)])
+    uniforms = np.stack([counter_uniforms(oracle_mod, seed, i, 8) for i in range(n)])
+    sim2 = E.Simulation.from_setup(s, precision=E.PREC_MIXED, lanes=256, steps_per_launch=1000)
+    st2, rec2 = sim2.run(n, normals=normals, uniforms=uniforms)
+    assert st2.done == n and st2.failed == 0
+    np.testing.assert_array_equal(rec2["status"], rec["status"])
+    np.testing.assert_array_equal(rec2["pos0"], rec["pos0"])
+    same = rec2["steps"] == rec["steps"]
+    disp = np.linalg.norm(rec["pos"] - rec["pos0"], axis=1)
+    d = np.linalg.norm(rec2["pos"] - rec["pos"], axis=1) / disp
+    print(f"[stream] steps equal {int(same.sum())}/{n}; endpoint difference / displacement: median {np.median(d[same]):.2e} "
+          f"max {d[same].max():.2e}; mean displacement {disp.mean() / cf.one_au:.3f} au", flush=True)
+    assert same.mean() >= 0.95
+    assert np.all(d[same] < 1e-3)
 
 
 def test_first_offset_selects_the_same_streams(device):
