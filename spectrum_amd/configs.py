@@ -90,13 +90,16 @@ def background_base(dc: DataContainer, t0=0.0, r0=(0.0, 0.0, 0.0), u0=(0.0, 0.0,
     return dc
 
 
-def solarwind(V_kms=400.0, B_nT_1au=5.0, omega=2.87e-6, r_ref=3.0 * 6.957e10 / unit_length, dmax0=None, dmax_fraction=0.1):
-    """BackgroundSolarWind parameters (src/background_solarwind.hh:87, src/background_solarwind.cc:58-84)."""
+def solarwind(V_kms=400.0, B_nT_1au=5.0, omega=2.87e-6, r_ref=3.0 * 6.957e10 / unit_length, dmax0=None, dmax_fraction=0.1,
+              axis=(0.0, 0.0, 1.0)):
+    """BackgroundSolarWind parameters (src/background_solarwind.hh:87, src/background_solarwind.cc:58-84); the
+    rotation vector is omega along `axis`."""
     umag = V_kms * 1.0e5 / unit_velocity
     BmagE = B_nT_1au * 1.0e-5 / unit_magnetic
     Bref = BmagE * (one_au / r_ref) ** 2
     dc = background_base(DataContainer(), 0.0, (0.0, 0.0, 0.0), (umag, 0.0, 0.0), (Bref, 0.0, 0.0), one_au if dmax0 is None else dmax0)
-    dc.insert_vec((0.0, 0.0, omega * unit_time)).insert_double(r_ref).insert_double(dmax_fraction)
+    na = math.sqrt(sum(float(v) * float(v) for v in axis))
+    dc.insert_vec(tuple(omega * unit_time * float(v) / na for v in axis)).insert_double(r_ref).insert_double(dmax_fraction)
     return dc
 
 
@@ -134,7 +137,7 @@ def spectrum_bins(T1: float, T2: float, n: int):
 
 def parker_modulation(n_energies=16, T1=0.1 * GeV, T2=10.0 * GeV, r_start_au=1.0, r_outer_au=90.0,
                       t_limit_years=50.0, spectrum_type=1, inner_au=0.01, time_action=-1,
-                      name="parker_modulation") -> Setup:
+                      name="parker_modulation", sw=None) -> Setup:
     """Backward Parker modulation in the Parker spiral (the CPU reference problem and its GPU scale-up).
 
     SolarWind V = 400 km/s, |B| = 5 nT at 1 au, Omega = 2.87e-6 rad/s (r_ref = 3 R_sun as in
@@ -142,10 +145,11 @@ def parker_modulation(n_energies=16, T1=0.1 * GeV, T2=10.0 * GeV, r_start_au=1.0
     kappa_perp = 0.02 kappa_par; start at r = 1 au on the ecliptic at n log-spaced kinetic
     energies; absorbing sphere at 90 au weighted by a LIS j ~ T^-2.7 (intensity weighting).
     An inner absorbing sphere and a time limit without binning follow the reference example
-    (``inner_au=None`` drops the sphere; ``time_action=0`` bins trajectories at the time limit too).
+    (``inner_au=None`` drops the sphere; ``time_action=0`` bins trajectories at the time limit too;
+    ``sw`` replaces the solar wind's parameters, e.g. ``solarwind(axis=...)``).
     """
     s = Setup(name, TRAJ_PARKER, BACKWARD, options={"DISTRO_KINETIC_ENERGY_POWER_LAW_TYPE": spectrum_type})
-    s.add(BACKGROUND, "BackgroundSolarWind", solarwind())
+    s.add(BACKGROUND, "BackgroundSolarWind", solarwind() if sw is None else sw)
     lam0 = 3.0 * (1.5e22 / unit_diffusion) / c_code     # kappa_par = lam0 * v / 3 with v = beta c
     dif = DataContainer().insert_double(lam0).insert_double(GV).insert_double(5.0e-5 / unit_magnetic)
     dif.insert_double(0.5).insert_double(0.0).insert_double(0.02)

@@ -535,6 +535,9 @@ static void finalize_plan(spx_sim* s)
    if (g.kind == SPX_BG_SOLARWIND) {
       f.sw_c1 = (float)(g.B0[0] * g.r_ref * g.r_ref);
       f.sw_k = (float)(g.w0 / g.ur0);
+      // a diagonal basis S = diag(s0, s1, s2) (s0 s1 s2 = 1) maps the local spiral w(p) = (p0 + q p1, p1 - q p0, p2)
+      // to S w(S d) = (d0 + s0 s1 q d1, d1 - s0 s1 q d0, d2) in the global frame
+      f.sw_kg = g.ep_diag ? (float)(g.eprime[0][0] * g.eprime[1][1] * g.w0 / g.ur0) : 0.0f;
       // |B| = |B0| r_ref^2 |w| / r^3 with |w| >= r, so |B| >= |B0| r_ref^2 / r^2
       const double r2_ok = std::fabs(g.B0[0] * g.r_ref * g.r_ref) / sp_tiny_h;
       f.sw_r2_ok = r2_ok > (double)FLT_MAX ? INFINITY : (float)r2_ok;

@@ -73,11 +73,9 @@ __device__ __forceinline__ bool parker_fields(const SPX_CP spx_bg_t<R>& bg, doub
       R d0 = R(0), d1 = R(0), d2 = R(0);
       v3<R> dc = mk<R>(R(0), R(0), R(0));
       if constexpr (is_f32<R>()) {
-         dc = rel(bg, x, y, z);   // FP32 stencil points relative to the origin
-         if (bg.ep_diag == 2) {
-            F.divU = (R(0.5) / r_g) * sw_divu_stencil<R>(bg, dc, r_g, fa);
-            return true;
-         }
+         dc = rel(bg, x, y, z);   // FP32 stencil points relative to the origin; the radial wind in any basis
+         F.divU = (R(0.5) / r_g) * sw_divu_stencil<R>(bg, dc, r_g, fa);
+         return true;
       }
 #pragma unroll 1
       for (int k = 0; k < 3; k++) {
@@ -392,7 +390,7 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
          if (loads ? (!grid_eval<R, false>(bg.grid, xf, yf, zf, G) || G.Bmag < R(sp_tiny)) : !grid_inside<R>(bg.grid, xf, yf, zf)) return 1;
       }
    }
-   else if (is_f32<R>() && BG == SPX_BG_SOLARWIND && !pos_dep && K0 != K1 && bg.ep_diag == 2 &&
+   else if (is_f32<R>() && BG == SPX_BG_SOLARWIND && !pos_dep && K0 != K1 && bg.ep_diag != 0 &&
             sw_stencil_clear<R>(bg, rel(bg, H.x, H.y, H.z), delta)) {
       // kappa is the same at every point: the K0 terms cancel, and (K1 - K0) b b is left
       if constexpr (is_f32<R>()) divK = ((K1 - K0) * hdi) * sw_bb_stencil<R>(bg, rel(bg, H.x, H.y, H.z), delta);

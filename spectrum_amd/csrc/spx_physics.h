@@ -170,9 +170,11 @@ SPX_HD void solarwind_eval(const SPX_CP spx_bg_t<R>& b, v3<R> d, int want, v3<R>
    if (want & 2) B = dg ? diag(Bl) : from_basis(Bl, ep);
 }
 
-// The Parker step's two SolarWind stencils in the FP32 step with the identity basis, restated from the
-// closed forms above.  At the points c +- h e (c = x - r0, |e| = 1), |c +- h e|^2 = |c|^2 + h^2 +- 2h e.c,
-// so each point costs one reciprocal square root and no vector products.
+// The Parker step's two SolarWind stencils in the FP32 step, restated from the closed forms above in the
+// global frame.  At the points c +- h e (c = x - r0, |e| = 1), |c +- h e|^2 = |c|^2 + h^2 +- 2h e.c,
+// so each point costs one reciprocal square root and no vector products.  U = ur0 p / |p| does not depend
+// on the basis; the b b stencil needs a diagonal one (the rotation axis along +-z), whose winding sense
+// in the global frame is the sign of eprime[0][0] eprime[1][1] (sw_kg).
 //
 // NumericalDerivatives' trace for div U (src/background_base.cc:228-312 with the basis e_k):
 // sum_k e_k . [U(c + h e_k) - U(c - h e_k)], with e . U(p) = ur0 (e . p) / |p|.
@@ -191,7 +193,7 @@ SPX_HD R sw_divu_stencil(const SPX_CP spx_bg_t<R>& b, v3<R> c, R h, const v3<R> 
 
 // The b b part of div(kappa)'s stencil (TrajectoryParker::DiffusionCoeff, src/trajectory_parker.cc:120-151,
 // with kappa position independent): sum_j [b_j b](c + h e_j) - [b_j b](c - h e_j) over the Cartesian axes.
-// b is the direction of w = (x + q y, y - q x, z), q = (r - r_ref) w0 / ur0: the radial factor of B cancels
+// b is the direction of w = (x + q y, y - q x, z), q = (r - r_ref) sw_kg: the radial factor of B cancels
 // in b_j b = w_j w / |w|^2.
 // True when no point c +- h e_j can meet GetFields' |B| < sp_tiny (src/background_base.cc:444-448): every
 // |c +- h e_j|^2 is at most |c|^2 + h^2 + 2h (|c_x| + |c_y| + |c_z|).
@@ -215,7 +217,7 @@ SPX_HD v3<R> sw_bb_stencil(const SPX_CP spx_bg_t<R>& b, v3<R> c, R h)
          const v3<R> p = mk<R>(j == 0 ? c.x + sh : c.x, j == 1 ? c.y + sh : c.y, j == 2 ? c.z + sh : c.z);
          const R r2 = c2h + (s == 0 ? h2 : -h2) * cj;
          const R ir = frsq(r2), r = r2 * ir;
-         const R q = (r - b.r_ref) * b.sw_k;
+         const R q = (r - b.r_ref) * b.sw_kg;
          const v3<R> w = mk<R>(p.x + q * p.y, p.y - q * p.x, p.z);
          const R wj = j == 0 ? w.x : (j == 1 ? w.y : w.z);
          const R t = wj / norm2(w);

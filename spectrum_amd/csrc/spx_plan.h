@@ -68,6 +68,7 @@ struct spx_bg_t {
    // BackgroundSolarWind (src/background_solarwind.cc:58-84)
    T r_ref, dmax_fraction, eprime[3][3], ur0, w0, hp_rad;
    T sw_c1, sw_k;                      // FP32 step: B0 r_ref^2 and w0 / ur0 of the closed-form spiral
+   T sw_kg;                            // FP32 step: w0 / ur0 times eprime[0][0] eprime[1][1], the winding in the global frame (diagonal bases)
    T sw_r2_ok;                         // FP32 step: |B| >= sp_tiny wherever |x - r0|^2 <= this (|B0| r_ref^2 / sp_tiny)
    // BackgroundShock / BackgroundSmoothShock (src/background_shock.cc:58-86, src/background_smooth_shock.cc:139-146)
    T n_shock[3], v_shock, compression, u1[3], B1[3], E0[3], E1[3], width_shock;

@@ -217,6 +217,38 @@ def test_mixed_counter_stream_follows_its_definition(device, oracle_mod):
     assert np.all(d[same] < 1e-3)
 
 
+@pytest.mark.parametrize("axis", [(0.0, 0.0, 1.0), (0.0, 0.0, -1.0), (0.0, 0.6, 0.8)])
+def test_mixed_drift_follows_fp64_in_every_basis(device, axis):
+    """With every Wiener increment zero (replayed), a backward Parker step moves by dt (div kappa - U) and
+    changes p by the adiabatic term: the drift alone, about 1 au in 0.1 day at 10 GeV.  The FP32 step takes
+    div U from the radial wind's closed-form stencil (any basis) and div kappa from the spiral's closed-form
+    b b stencil when the rotation axis is along +z or -z (the winding sense in the global frame flips with
+    the axis), and from the generic stencil of the rotated field otherwise; the FP64 kernel evaluates the
+    reference's numerical derivatives of the field itself.  Both must end at the same points to FP32
+    accuracy; a wrong winding sense mirrors the azimuthal drift."""
+    s = cf.parker_modulation(n_energies=16, t_limit_years=0.1 / 365.0, time_action=0, sw=cf.solarwind(axis=axis))
+    n = 64
+    normals = np.zeros((n, 3000))
+    uniforms = np.full((n, 8), 0.5)
+    recs = {}
+    for prec in (E.PREC_FP64, E.PREC_MIXED):
+        sim = E.Simulation.from_setup(s, precision=prec, lanes=256, steps_per_launch=500)
+        st, rec = sim.run(n, normals=normals, uniforms=uniforms)
+        assert st.done == n and st.failed == 0
+        recs[prec] = rec
+    a, b = recs[E.PREC_FP64], recs[E.PREC_MIXED]
+    np.testing.assert_array_equal(b["status"], a["status"])
+    same = b["steps"] == a["steps"]
+    disp = np.linalg.norm(a["pos"] - a["pos0"], axis=1)
+    dx = np.linalg.norm(b["pos"] - a["pos"], axis=1) / disp
+    dp = np.abs(b["mom"][:, 0] - a["mom"][:, 0]) / np.abs(a["mom"][:, 0] - a["mom0"][:, 0])
+    print(f"[drift] axis {axis}: steps equal {int(same.sum())}/{n}; displacement {disp.min() / cf.one_au:.3f}-{disp.max() / cf.one_au:.3f} au; "
+          f"FP32 - FP64 endpoint / displacement max {dx[same].max():.2e}; momentum change max {dp[same].max():.2e}", flush=True)
+    assert same.mean() >= 0.9
+    assert np.all(dx[same] < 1e-3)
+    assert np.all(dp[same] < 1e-3)
+
+
 def test_first_offset_selects_the_same_streams(device):
     """Trajectories first..first+n-1 of a larger run are reproduced by a run started at `first`."""
     s = cf.parker_modulation(n_energies=4, t_limit_years=1.0 / 365.0, time_action=0)
