@@ -204,15 +204,15 @@ def dsa_shock(n_bins=128, p_ratio=1.0e4, downstream_kappa_lengths=4.0, t_limit_d
     return s
 
 
-def gc_diff_scatt(n_r=32, n_mu=64, T_MeV=100.0, nu=1.0e-4, kperp_cgs=8.6e19, t_limit_days=30.0, name="gc_diff_scatt") -> Setup:
+def gc_diff_scatt(n_r=32, n_mu=64, T_MeV=100.0, nu=1.0e-4, kperp_cgs=8.6e19, t_limit_days=30.0, name="gc_diff_scatt", sw=None) -> Setup:
     """Guiding centre with perpendicular diffusion and pitch-angle scattering in the Parker spiral.
 
     D_mumu = nu (1 - mu^2) / 2 with nu = 1e-4 1/s; kappa_perp constant (the value is this build's choice:
     0.02 kappa_par of 100 MeV protons in the C1 model); 100 MeV protons with isotropic mu from 1 au;
-    a 30-day time boundary; (r, mu) distribution at expiry.
+    a 30-day time boundary; (r, mu) distribution at expiry.  ``sw`` replaces the solar wind's parameters.
     """
     s = Setup(name, TRAJ_GUIDING_DIFF_SCATT, FORWARD)
-    s.add(BACKGROUND, "BackgroundSolarWind", solarwind())
+    s.add(BACKGROUND, "BackgroundSolarWind", solarwind() if sw is None else sw)
     D0 = 0.5 * nu * unit_time
     s.add(DIFFUSION, "DiffusionPerpConstantPitchIsotropic", DataContainer().insert_double(kperp_cgs / unit_diffusion).insert_double(D0))
     s.add(INITIAL, "InitialTimeFixed", DataContainer().insert_double(0.0))

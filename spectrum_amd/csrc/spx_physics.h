@@ -157,6 +157,18 @@ SPX_HD void solarwind_local(const SPX_CP spx_bg_t<R>& b, v3<R> pp, int want, v3<
 template <typename R>
 SPX_HD void solarwind_eval(const SPX_CP spx_bg_t<R>& b, v3<R> d, int want, v3<R>& U, v3<R>& B)
 {
+   if (is_f32<R>() && b.ep_diag != 0) {
+      // the closed form in the global frame: a diagonal basis S (det 1) turns S w(S d) into
+      // (d0 + s q d1, d1 - s q d0, d2) with s = eprime[0][0] eprime[1][1], folded into sw_kg
+      const R r2 = norm2(d), ir = frsq(r2), r = r2 * ir;
+      if (want & 1) U = (b.ur0 * ir) * d;
+      if (want & 2) {
+         const R Bri = b.sw_c1 * (ir * ir) * ir;
+         const R q = (r - b.r_ref) * b.sw_kg;
+         B = mk<R>(Bri * (d.x + q * d.y), Bri * (d.y - q * d.x), Bri * d.z);
+      }
+      return;
+   }
    if (b.ep_diag == 2) {
       solarwind_local<R>(b, d, want, U, B);
       return;

@@ -218,6 +218,35 @@ def test_mixed_counter_stream_follows_its_definition(device, oracle_mod):
 
 
 @pytest.mark.parametrize("axis", [(0.0, 0.0, 1.0), (0.0, 0.0, -1.0), (0.0, 0.6, 0.8)])
+def test_mixed_guiding_centre_follows_fp64_in_every_basis(device, axis):
+    """The guiding centre in the spiral with every Wiener increment zero (replayed) and the pitch cosines
+    spread over (-1, 1): streaming, the drifts and the mirror force (which turns the inward-moving
+    particles within 0.02 day) are deterministic.  The FP32 step evaluates the spiral in the global
+    frame (diagonal bases) or through the rotation (a tilted axis); both must follow the FP64 kernel."""
+    s = cf.gc_diff_scatt(t_limit_days=0.02, sw=cf.solarwind(axis=axis))
+    n = 64
+    normals = np.zeros((n, 3000))
+    uniforms = np.tile(np.linspace(0.02, 0.98, n)[:, None], (1, 8))
+    recs = {}
+    for prec in (E.PREC_FP64, E.PREC_MIXED):
+        sim = E.Simulation.from_setup(s, precision=prec, lanes=256, steps_per_launch=500)
+        st, rec = sim.run(n, normals=normals, uniforms=uniforms)
+        assert st.done == n and st.failed == 0
+        recs[prec] = rec
+    a, b = recs[E.PREC_FP64], recs[E.PREC_MIXED]
+    np.testing.assert_array_equal(b["status"], a["status"])
+    same = b["steps"] == a["steps"]
+    dx = np.linalg.norm(b["pos"] - a["pos"], axis=1) / cf.one_au
+    dm = np.linalg.norm(b["mom"] - a["mom"], axis=1) / np.linalg.norm(a["mom"], axis=1)
+    turned = np.sign(a["mom"][:, 2]) != np.sign(a["mom0"][:, 2])
+    print(f"[drift-gc] axis {axis}: steps equal {int(same.sum())}/{n}; mirrored {int(turned.sum())}; FP32 - FP64 endpoint max "
+          f"{dx[same].max():.2e} au, momentum {dm[same].max():.2e}", flush=True)
+    assert same.mean() >= 0.9 and turned.any()
+    assert np.all(dx[same] < 1e-4)
+    assert np.all(dm[same] < 1e-4)
+
+
+@pytest.mark.parametrize("axis", [(0.0, 0.0, 1.0), (0.0, 0.0, -1.0), (0.0, 0.6, 0.8)])
 def test_mixed_drift_follows_fp64_in_every_basis(device, axis):
     """With every Wiener increment zero (replayed), a backward Parker step moves by dt (div kappa - U) and
     changes p by the adiabatic term: the drift alone, about 1 au in 0.1 day at 10 GeV.  The FP32 step takes
