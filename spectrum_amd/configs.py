@@ -303,15 +303,21 @@ def parker_modulation_cartesian(grid, n_bins=100, T1_MeV=10.0, T2_MeV=5000.0, in
 
 
 def parker_modulation_radial(n_energies=8, T1_MeV=10.0, T2_MeV=5000.0, inner_au=0.05, outer_au=80.0, t_limit_years=100.0,
-                             spectrum_type=0, name="parker_modulation_radial") -> Setup:
+                             spectrum_type=0, name="parker_modulation_radial", grid=None, grid_dmax_au=1.0) -> Setup:
     """The physics of the reference's Cartesian modulation benchmark (benchmarks/main_test_modulation_cartesian_parker.cc)
     on the analytic solar wind instead of its sampled grid: backward Parker, radial wind V = 400 km/s,
     isotropic kappa = 1.5e22 cm^2/s (T / 1 GeV) (r / 1 au)^2, absorbing spheres at 0.05 au (cold, 0) and
     80 au (hot, J0 [v] (T / 1 GeV)^-1.8 [/ p^2]), from 1 au at n log-spaced kinetic energies, and a time limit
     long enough that no trajectory reaches it (not binned).  With kappa isotropic the magnetic field drops
-    out, so the spectrum at 1 au is the solution of the spherically symmetric Parker equation."""
-    s = Setup(name, TRAJ_PARKER, BACKWARD, options={"DISTRO_KINETIC_ENERGY_POWER_LAW_TYPE": spectrum_type})
-    s.add(BACKGROUND, "BackgroundSolarWind", solarwind())
+    out, so the spectrum at 1 au is the solution of the spherically symmetric Parker equation.  With ``grid``
+    the wind is that grid's (BackgroundServerCartesian, dmax0 = ``grid_dmax_au``), e.g. the spiral sampled at
+    zone centres (cartesian.parker_spiral), whose interpolated wind approaches the radial one as the zones shrink."""
+    s = Setup(name, TRAJ_PARKER, BACKWARD, options={"DISTRO_KINETIC_ENERGY_POWER_LAW_TYPE": spectrum_type}, grid=grid)
+    if grid is None:
+        s.add(BACKGROUND, "BackgroundSolarWind", solarwind())
+    else:
+        s.add(BACKGROUND, "BackgroundServerCartesian", background_base(DataContainer(), 0.0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0),
+                                                                       grid_dmax_au * one_au))
     dif = DataContainer().insert_double(1.5e22 / unit_diffusion).insert_double(GeV).insert_double(one_au)
     dif.insert_double(1.0).insert_double(2.0).insert_double(1.0).insert_int(0)
     dif.insert_double(1.0).insert_double(0.0).insert_double(1.0)
