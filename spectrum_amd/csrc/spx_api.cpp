@@ -519,6 +519,12 @@ static uint64_t auto_lanes()
 static void finalize_plan(spx_sim* s)
 {
    spx_plan_desc& P = s->plan;
+   // concentric spheres in consecutive slots share one |x - origin| per step
+   for (int b = 0; b < P.n_bnd; b++) {
+      const spx_bnd_desc& q = P.bnd[b > 0 ? b - 1 : 0];
+      P.bnd[b].same_origin = b > 0 && P.bnd[b].kind == SPX_BK_SPHERE && q.kind == SPX_BK_SPHERE && q.origin[0] == P.bnd[b].origin[0] &&
+                             q.origin[1] == P.bnd[b].origin[1] && q.origin[2] == P.bnd[b].origin[2];
+   }
    const spx_bg_t<double>& g = P.bg;
    spx_bg_t<float>& f = P.bgf;
    std::memset(&f, 0, sizeof(f));

@@ -204,16 +204,25 @@ __device__ __forceinline__ void pos_sample(PlanPtr P, const spx_rng_desc& rng, u
 // ---------------------------------------------------------------------------------------------
 // With R = float the distance to a sphere or plane is formed in FP32 from the FP64 offset (the FP32
 // step's crossing test); time and momentum boundaries and every normal stay in FP64.
+// |x - origin| of a sphere boundary in the step type (FP32 from the FP64 offset): the distance the sphere's
+// delta subtracts the radius from
+template <typename R>
+__device__ __forceinline__ R sphere_dist(const SPX_CP spx_bnd_desc& b, double x, double y, double z)
+{
+   if constexpr (is_f32<R>()) {
+      const v3<float> d = mk<float>((float)(x - b.origin[0]), (float)(y - b.origin[1]), (float)(z - b.origin[2]));
+      const float d2 = norm2(d);
+      return d2 * frsq(d2);
+   }
+   else return norm(mk<double>(x - b.origin[0], y - b.origin[1], z - b.origin[2]));
+}
+
 template <typename R = double>
 __device__ __forceinline__ double bnd_eval(const SPX_CP spx_bnd_desc& b, int cross_left, bool gc, double t, double x, double y, double z,
                                            double m0, double m1, double m2, double* nrm)
 {
    if constexpr (is_f32<R>()) {
-      if (!nrm && b.kind == SPX_BK_SPHERE) {
-         const v3<float> d = mk<float>((float)(x - b.origin[0]), (float)(y - b.origin[1]), (float)(z - b.origin[2]));
-         const float d2 = norm2(d);
-         return (double)(d2 * frsq(d2) - (float)b.radius);
-      }
+      if (!nrm && b.kind == SPX_BK_SPHERE) return (double)(sphere_dist<float>(b, x, y, z) - (float)b.radius);
       if (!nrm && b.kind == SPX_BK_PLANE) {
          const v3<float> d = mk<float>((float)(x - b.origin[0]), (float)(y - b.origin[1]), (float)(z - b.origin[2]));
          return (double)dot(d, mk<float>((float)b.norm[0], (float)b.norm[1], (float)b.norm[2]));
@@ -462,12 +471,22 @@ template <int FWD, typename HOT>
 __device__ __forceinline__ double time_proximity(PlanPtr P, const HOT& H, double dt, double dmax)
 {
    double dn = FWD ? -sp_large * dmax / c_code : sp_large * dmax / c_code;
+   // time boundaries fill the first n_bt slots; one is the common case, and the loop's guards would
+   // otherwise become selects over every slot
+   const int nbt = P->n_bt;
+   if (nbt == 1) {
+      const double d = H.bd[0];
+      if (FWD) { if ((d <= 0.0) && (d > dn)) dn = d; }
+      else { if ((d >= 0.0) && (d < dn)) dn = d; }
+   }
+   else if (nbt > 1) {
 #pragma unroll
-   for (int b = 0; b < SPX_MAX_BND; b++) {
-      if (b < P->n_bt) {
-         const double d = H.bd[b];
-         if (FWD) { if ((d <= 0.0) && (d > dn)) dn = d; }
-         else { if ((d >= 0.0) && (d < dn)) dn = d; }
+      for (int b = 0; b < SPX_MAX_BND; b++) {
+         if (b < nbt) {
+            const double d = H.bd[b];
+            if (FWD) { if ((d <= 0.0) && (d > dn)) dn = d; }
+            else { if ((d >= 0.0) && (d < dn)) dn = d; }
+         }
       }
    }
    if (FWD) { if (dt >= -dn) dt = fmax(-(1.0 + sp_little) * dn, sp_small * dmax / c_code); }
@@ -485,13 +504,24 @@ template <bool GC, typename HOT>
 __device__ __forceinline__ int handle_boundaries(PlanPtr P, LaunchPtr L, uint64_t slot,
                                                  HOT& H, ColdLDS& C, int tid)
 {
+   typedef typename HOT::real R;
    bool any = false;
+   R sdist = R(0);   // |x - origin| of the last sphere, reused by a concentric sphere in the next slot
 #pragma unroll
    for (int b = 0; b < SPX_MAX_BND; b++) {
       if (b < P->n_bnd) {
-         // ComputeBoundary (src/boundary_base.cc:105-127)
-         const int left = (H.bst[b] & SPX_BND_RECURRENT) ? C.cross[b][tid] : 0;
-         const double d = bnd_eval<typename HOT::real>(P->bnd[b], left, GC, H.t, H.x, H.y, H.z, H.m0(), H.m1(), H.m2(), nullptr);
+         // ComputeBoundary (src/boundary_base.cc:105-127); a slot's recurrent bit is its boundary's (status0)
+         const SPX_CP spx_bnd_desc& B = P->bnd[b];
+         double d;
+         if (B.kind == SPX_BK_SPHERE) {
+            if (!B.same_origin) sdist = sphere_dist<R>(B, H.x, H.y, H.z);
+            if constexpr (is_f32<R>()) d = (double)(sdist - (float)B.radius);
+            else d = sdist - B.radius;
+         }
+         else {
+            const int left = (B.status0 & SPX_BND_RECURRENT) ? C.cross[b][tid] : 0;
+            d = bnd_eval<R>(B, left, GC, H.t, H.x, H.y, H.z, H.m0(), H.m1(), H.m2(), nullptr);
+         }
          const double prod = d * H.bdo[b];
          H.bd[b] = (prod == 0.0) ? -H.bdo[b] : d;
          H.bst[b] = (prod <= 0.0) ? (H.bst[b] | SPX_BND_CROSSED) : (H.bst[b] & ~SPX_BND_CROSSED);

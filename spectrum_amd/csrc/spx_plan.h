@@ -107,7 +107,8 @@ typedef struct {
    int32_t cls, kind;           // spx_bnd_class, spx_bnd_kind
    int32_t status0;             // REFLECT / RECURRENT bits
    int32_t max_crossings;
-   int32_t n_act, pad_;
+   int32_t n_act;
+   int32_t same_origin;         // a sphere whose previous slot is a sphere with the same origin (its |x - origin| is reused)
    int32_t actions[SPX_MAX_ACT];
    double timemark, origin[3], radius, norm[3], momentum;
 } spx_bnd_desc;
