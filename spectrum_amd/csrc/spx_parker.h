@@ -332,6 +332,7 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
    parker_kappa<R, BG, DF>(bg, df, km, km2, km3, H.F.Bmag, H.x, H.y, H.z, H.F.U, K0, K1);
    const R delta = fmin(LarmorRadius(p, H.F.Bmag), H.F.dmax);
    const R hdi = is_f32<R>() ? R(0.5) / delta : R(0);
+   const double ydel = is_f32<R>() ? 0.0 : 1.0 / (double)delta;   // FP64: the stencil's nine quotients by delta share it
    v3<R> divK = mk<R>(R(0), R(0), R(0));
    const bool pos_dep = (DF == SPX_DF_FULL) ? false : (df.pos_dep != 0);
    if (BG == SPX_BG_GRID && (pos_dep || K0 != K1)) {
@@ -368,9 +369,9 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
             divK.z += hdi * ((K0f * (j == 2 ? R(1) : R(0)) + cf * bf.z) - (K0b * (j == 2 ? R(1) : R(0)) + cb * bb.z));
          }
          else {
-            divK.x += R(0.5) * ((K0f * (j == 0 ? R(1) : R(0)) + cf * bf.x) - (K0b * (j == 0 ? R(1) : R(0)) + cb * bb.x)) / delta;
-            divK.y += R(0.5) * ((K0f * (j == 1 ? R(1) : R(0)) + cf * bf.y) - (K0b * (j == 1 ? R(1) : R(0)) + cb * bb.y)) / delta;
-            divK.z += R(0.5) * ((K0f * (j == 2 ? R(1) : R(0)) + cf * bf.z) - (K0b * (j == 2 ? R(1) : R(0)) + cb * bb.z)) / delta;
+            divK.x += div_rn(R(0.5) * ((K0f * (j == 0 ? R(1) : R(0)) + cf * bf.x) - (K0b * (j == 0 ? R(1) : R(0)) + cb * bb.x)), delta, ydel);
+            divK.y += div_rn(R(0.5) * ((K0f * (j == 1 ? R(1) : R(0)) + cf * bf.y) - (K0b * (j == 1 ? R(1) : R(0)) + cb * bb.y)), delta, ydel);
+            divK.z += div_rn(R(0.5) * ((K0f * (j == 2 ? R(1) : R(0)) + cf * bf.z) - (K0b * (j == 2 ? R(1) : R(0)) + cb * bb.z)), delta, ydel);
          }
       }
    }
@@ -438,9 +439,9 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
             divK.z += hdi * ((K0f * (j == 2 ? R(1) : R(0)) + cf * bf.z) - (K0b * (j == 2 ? R(1) : R(0)) + cb * bb.z));
          }
          else {
-            divK.x += R(0.5) * ((K0f * (j == 0 ? R(1) : R(0)) + cf * bf.x) - (K0b * (j == 0 ? R(1) : R(0)) + cb * bb.x)) / delta;
-            divK.y += R(0.5) * ((K0f * (j == 1 ? R(1) : R(0)) + cf * bf.y) - (K0b * (j == 1 ? R(1) : R(0)) + cb * bb.y)) / delta;
-            divK.z += R(0.5) * ((K0f * (j == 2 ? R(1) : R(0)) + cf * bf.z) - (K0b * (j == 2 ? R(1) : R(0)) + cb * bb.z)) / delta;
+            divK.x += div_rn(R(0.5) * ((K0f * (j == 0 ? R(1) : R(0)) + cf * bf.x) - (K0b * (j == 0 ? R(1) : R(0)) + cb * bb.x)), delta, ydel);
+            divK.y += div_rn(R(0.5) * ((K0f * (j == 1 ? R(1) : R(0)) + cf * bf.y) - (K0b * (j == 1 ? R(1) : R(0)) + cb * bb.y)), delta, ydel);
+            divK.z += div_rn(R(0.5) * ((K0f * (j == 2 ? R(1) : R(0)) + cf * bf.z) - (K0b * (j == 2 ? R(1) : R(0)) + cb * bb.z)), delta, ydel);
          }
       }
    }
