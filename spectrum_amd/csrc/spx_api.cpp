@@ -238,6 +238,7 @@ static int add_background(spx_sim* s, const std::string& cls, Reader& r)
             if ((j != k && g.eprime[k][j] != 0.0) || (j == k && std::fabs(g.eprime[k][j]) != 1.0)) g.ep_diag = 0;
       if (g.ep_diag && g.eprime[0][0] == 1.0 && g.eprime[1][1] == 1.0 && g.eprime[2][2] == 1.0) g.ep_diag = 2;   // x1 is exact: skipped
       g.ur0 = std::fabs(g.u0[0]);
+      g.ur0_rcp = 1.0 / g.ur0;
       g.hp_rad = 117.0 * 1.495978707E+13 / 1.0E10;
    }
    else if (cls == "BackgroundShock" || cls == "BackgroundSmoothShock") {   // src/background_shock.cc:58-86
@@ -535,6 +536,7 @@ static void finalize_plan(spx_sim* s)
       for (int j = 0; j < 3; j++) f.eprime[k][j] = (float)g.eprime[k][j];
    }
    f.dmax0 = (float)g.dmax0; f.r_ref = (float)g.r_ref; f.dmax_fraction = (float)g.dmax_fraction; f.ur0 = (float)g.ur0;
+   f.ur0_rcp = (float)g.ur0_rcp;
    f.w0 = (float)g.w0; f.hp_rad = (float)g.hp_rad; f.v_shock = (float)g.v_shock; f.compression = (float)g.compression;
    f.width_shock = (float)g.width_shock;
    f.grid = g.grid;

@@ -470,7 +470,8 @@ struct ColdLDS {
 template <int FWD, typename HOT>
 __device__ __forceinline__ double time_proximity(PlanPtr P, const HOT& H, double dt, double dmax)
 {
-   double dn = FWD ? -sp_large * dmax / c_code : sp_large * dmax / c_code;
+   typedef typename HOT::real R;
+   double dn = FWD ? -divc_d<R>(sp_large * dmax, c_code) : divc_d<R>(sp_large * dmax, c_code);
    // time boundaries fill the first n_bt slots; one is the common case, and the loop's guards would
    // otherwise become selects over every slot
    const int nbt = P->n_bt;
@@ -489,8 +490,8 @@ __device__ __forceinline__ double time_proximity(PlanPtr P, const HOT& H, double
          }
       }
    }
-   if (FWD) { if (dt >= -dn) dt = fmax(-(1.0 + sp_little) * dn, sp_small * dmax / c_code); }
-   else { if (dt >= dn) dt = fmax((1.0 + sp_little) * dn, sp_small * dmax / c_code); }
+   if (FWD) { if (dt >= -dn) dt = fmax(-(1.0 + sp_little) * dn, divc_d<R>(sp_small * dmax, c_code)); }
+   else { if (dt >= dn) dt = fmax((1.0 + sp_little) * dn, divc_d<R>(sp_small * dmax, c_code)); }
    return dt;
 }
 

@@ -305,7 +305,7 @@ __device__ __forceinline__ int fieldline_step(PlanPtr P, LaunchPtr L, uint64_t s
    }
    H.steps++;
    if (H.steps > P->max_steps) return 4;
-   if (dt < sp_tiny * (double)H.F.dmax / c_code) return 2;
+   if (dt < divc_d<R>(sp_tiny * (double)H.F.dmax, c_code)) return 2;
    if (!__builtin_isnormal(dt)) return 3;
    return 0;
 }
@@ -452,7 +452,7 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
       H.ds_prev = (dot(d, v3of(bg.n_shock)) - bg.v_shock * R(H.t)) / bg.width_shock;
    }
    const v3<R> slope_pos = FWD ? H.F.U + divK : H.F.U - divK;
-   const R slope_mom = -p * H.F.divU / R(3);
+   const R slope_mom = divc<R>(-p * H.F.divU, R(3));
    double slope_amp = 0.0;
    if (TT == 1) {   // SourceMomentumInjection::EvaluateSource (src/source_other.cc:113-119) with the previous dt
       const double del = H.p - P->src_p_inj;
@@ -513,7 +513,7 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
    H.steps++;
    // Integrate's safety checks
    if (H.steps > P->max_steps) return 4;
-   if (dt < sp_tiny * (double)H.F.dmax / c_code) return 2;
+   if (dt < divc_d<R>(sp_tiny * (double)H.F.dmax, c_code)) return 2;
    if (!__builtin_isnormal(dt)) return 3;
    return 0;
 }

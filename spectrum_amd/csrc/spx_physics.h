@@ -61,6 +61,20 @@ SPX_HD float rtanh(float a) { return tanhf(a); }
 template <typename R> constexpr bool is_f32() { return std::is_same<R, float>::value; }
 __device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
+// a / c for a constant c: the FP64 kernels round it as the division does, through div_rn with the
+// reciprocal the compiler folds (correctly rounded); the FP32 unit, built with reciprocal math, keeps its
+// product by the reciprocal.  divc for the step type R, divc_d for an FP64 quantity in a kernel of step type R.
+template <typename R> SPX_HD R divc(R a, R c)
+{
+   if constexpr (is_f32<R>()) return a / c;
+   else return div_rn(a, c, 1.0 / c);
+}
+template <typename R> SPX_HD double divc_d(double a, double c)
+{
+   if constexpr (is_f32<R>()) return a / c;
+   else return div_rn(a, c, 1.0 / c);
+}
+
 // Compensated (Kahan) addition of an increment to an FP64 sum, for the FP32-step kernels: c carries what
 // the rounding of s dropped (less than an ulp of s, so FP32 holds it to 24 bits) into the next addition.
 // No reassociation is enabled for the FP32 unit, so (t - s) - y is evaluated as written.
@@ -134,6 +148,7 @@ SPX_HD void solarwind_local(const SPX_CP spx_bg_t<R>& b, v3<R> pp, int want, v3<
       }
       return;
    }
+   else {
    R r = norm(pp);
    if (want & 1) u = b.ur0 * unit(pp);
    if (want & 2) {
@@ -145,8 +160,9 @@ SPX_HD void solarwind_local(const SPX_CP spx_bg_t<R>& b, v3<R> pp, int want, v3<
       if (s < r * R(sp_tiny)) { cosphi = R(0); sinphi = R(0); }
       else { cosphi = pp.x / s; sinphi = pp.y / s; }
       R Br = b.B0[0] * sqr(b.r_ref / r);
-      R Bp = -Br * sintheta * r_mns * b.w0 / b.ur0;
+      R Bp = div_rn(-Br * sintheta * r_mns * b.w0, b.ur0, b.ur0_rcp);   // / ur0, rounded as the division
       Bl = mk<R>(Br * sintheta * cosphi - Bp * sinphi, Br * sintheta * sinphi + Bp * cosphi, Br * costheta);
+   }
    }
 }
 
@@ -574,7 +590,7 @@ SPX_HD R kappa_mom(const SPX_CP spx_df_t<R>& d, R p)
    switch (d.kind) {
    case SPX_DF_FLOWMOM: return rpow(p / d.p0, d.pow_law_p);                                   // :622-627
    case SPX_DF_KINRAD: return d.kap0 * rpow(EnrKin(p) / d.T0, d.pow_law_T);                   // :713-723
-   case SPX_DF_RIGBMAG: return (d.lam0 * VelP(p) / R(3)) * rpow(Rigidity(p) / d.R0, d.pow_law_R);   // :804-809
+   case SPX_DF_RIGBMAG: return divc<R>(d.lam0 * VelP(p), R(3)) * rpow(Rigidity(p) / d.R0, d.pow_law_R);   // :804-809
    case SPX_DF_STRAUSS: case SPX_DF_POTGIETER: return kappa_mom_region(d, p, false);
    case SPX_DF_SOQLT: return kappa_mom_soqlt(d, p, 1, false);
    default: return R(0);

@@ -67,6 +67,7 @@ struct spx_bg_t {
    T u0[3], B0[3], dmax0;
    // BackgroundSolarWind (src/background_solarwind.cc:58-84)
    T r_ref, dmax_fraction, eprime[3][3], ur0, w0, hp_rad;
+   T ur0_rcp;                          // 1 / ur0, correctly rounded (the FP64 step's quotients by ur0)
    T sw_c1, sw_k;                      // FP32 step: B0 r_ref^2 and w0 / ur0 of the closed-form spiral
    T sw_kg;                            // FP32 step: w0 / ur0 times eprime[0][0] eprime[1][1], the winding in the global frame (diagonal bases)
    T sw_r2_ok;                         // FP32 step: |B| >= sp_tiny wherever |x - r0|^2 <= this (|B0| r_ref^2 / sp_tiny)
