@@ -202,8 +202,6 @@ __device__ __forceinline__ void pos_sample(PlanPtr P, const spx_rng_desc& rng, u
 // ---------------------------------------------------------------------------------------------
 // Boundaries (src/boundary_base.cc:105-191, src/boundary_{time,space,momentum}.cc)
 // ---------------------------------------------------------------------------------------------
-// With R = float the distance to a sphere or plane is formed in FP32 from the FP64 offset (the FP32
-// step's crossing test); time and momentum boundaries and every normal stay in FP64.
 // |x - origin| of a sphere boundary in the step type (FP32 from the FP64 offset): the distance the sphere's
 // delta subtracts the radius from
 template <typename R>
@@ -217,6 +215,8 @@ __device__ __forceinline__ R sphere_dist(const SPX_CP spx_bnd_desc& b, double x,
    else return norm(mk<double>(x - b.origin[0], y - b.origin[1], z - b.origin[2]));
 }
 
+// With R = float the distance to a sphere or plane is formed in FP32 from the FP64 offset (the FP32
+// step's crossing test); time and momentum boundaries and every normal stay in FP64.
 template <typename R = double>
 __device__ __forceinline__ double bnd_eval(const SPX_CP spx_bnd_desc& b, int cross_left, bool gc, double t, double x, double y, double z,
                                            double m0, double m1, double m2, double* nrm)
