@@ -353,7 +353,7 @@ def main():
                     help="trajectories per energy are log-uniform in [1e3, 1e5] times this factor (1: the workload's sizes)")
     ap.add_argument("--drain-traj", type=int, default=1_000_000, help="time-to-solution leg: trajectories run to completion (0 = skip)")
     ap.add_argument("--drain-seconds", type=float, default=90.0, help="time-to-solution leg: stop after this long")
-    ap.add_argument("--serve-steps-per-launch", type=int, default=500)
+    ap.add_argument("--serve-steps-per-launch", type=int, default=2000)
     ap.add_argument("--serve-seed", type=int, default=7)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB (3 times) and exit: the reference "
