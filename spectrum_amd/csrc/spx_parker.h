@@ -25,6 +25,7 @@ template <typename R>
 struct PkFields {
    v3<R> U, bhat;
    R Bmag, divU, dmax;
+   v3<R> fa0, fa1;   // SolarWind: the field-aligned frame of bhat (second unit vector, bhat x it), reused by the step
 };
 
 // CommonFields with mask U|B|gradU|gradB for the Parker transport (BackgroundBase::GetFields,
@@ -69,6 +70,8 @@ __device__ __forceinline__ bool parker_fields(const SPX_CP spx_bg_t<R>& bg, doub
       fa[2] = F.bhat;
       fa[0] = second_unit(F.bhat);
       fa[1] = cross(fa[2], fa[0]);
+      F.fa0 = fa[0];
+      F.fa1 = fa[1];
       // trace of transpose(fa) * G (common/matrix.hh:509-519, :186-189), summed per diagonal element
       R d0 = R(0), d1 = R(0), d2 = R(0);
       v3<R> dc = mk<R>(R(0), R(0), R(0));
@@ -465,11 +468,14 @@ __device__ __forceinline__ int parker_step(PlanPtr P, LaunchPtr L, const spx_rng
    dtp = fmin(dtp, R(0.5) * R(3) * R(0.01) / fabs(H.F.divU));
    double dt = fmin(P->dt_scale == 1.0 ? (double)dtp : (double)dtp * P->dt_scale, C.dt_adapt[tid]);   // 1: the reference's step
    dt = time_proximity<FWD>(P, H, dt, (double)H.F.dmax);
-   // EulerDiffSlopes (src/trajectory_parker.cc:159-180)
+   // EulerDiffSlopes (src/trajectory_parker.cc:159-180); the solar wind's fields carry the frame of bhat
    v3<R> fa[3];
    fa[2] = H.F.bhat;
-   fa[0] = second_unit(H.F.bhat);
-   fa[1] = cross(fa[2], fa[0]);
+   if (BG == SPX_BG_SOLARWIND) { fa[0] = H.F.fa0; fa[1] = H.F.fa1; }
+   else {
+      fa[0] = second_unit(H.F.bhat);
+      fa[1] = cross(fa[2], fa[0]);
+   }
    const R sdt = rsqrt_(R(dt));
    R nrm[3];
    next_normals<R, 3>(rng, first, H.rs, nrm);
