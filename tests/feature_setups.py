@@ -57,6 +57,31 @@ def solarwind_kinrad_reflect_recurrent():
     return s
 
 
+def parker_offset_spheres():
+    """Backward Parker in the Parker spiral (the headline's diffusion) with three spheres: an absorbing one of
+    3 au and a reflecting one of 0.6 au around the origin (consecutive slots: the second reuses the first's
+    distance), and an absorbing one of 0.15 au around (0.75, 0.1, 0) au (not concentric with the slot
+    before it); a one-day limit, exit-time and spectrum distributions."""
+    s = cf.Setup("parker_offset_spheres", cf.TRAJ_PARKER, cf.BACKWARD)
+    s.add(cf.BACKGROUND, "BackgroundSolarWind", cf.solarwind())
+    lam0 = 3.0 * (1.5e22 / cf.unit_diffusion) / cf.c_code
+    dif = DC().insert_double(lam0).insert_double(cf.GV).insert_double(5.0e-5 / cf.unit_magnetic)
+    dif.insert_double(0.5).insert_double(0.0).insert_double(0.02)
+    s.add(cf.DIFFUSION, "DiffusionRigidityMagneticFieldPowerLaw", dif)
+    s.add(cf.INITIAL, "InitialTimeFixed", DC().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", DC().insert_vec((cf.one_au, 0.0, 0.0)))
+    s.add(cf.INITIAL, "InitialMomentumThickShell", DC().insert_double(cf.mom_from_ekin(0.05 * cf.GeV)).insert_double(cf.mom_from_ekin(2.0 * cf.GeV)).insert_bool(True))
+    s.add(cf.BOUNDARY, "BoundarySphereAbsorb", cf.boundary(1, [0, 0], ("vec", (0.0, 0.0, 0.0)), ("double", 3.0 * cf.one_au)))
+    s.add(cf.BOUNDARY, "BoundarySphereReflect", cf.boundary(-1, [-1, -1], ("vec", (0.0, 0.0, 0.0)), ("double", 0.6 * cf.one_au)))
+    s.add(cf.BOUNDARY, "BoundarySphereAbsorb", cf.boundary(1, [0, 0], ("vec", (0.75 * cf.one_au, 0.1 * cf.one_au, 0.0)), ("double", 0.15 * cf.one_au)))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0, 0], ("double", -1.0 * cf.one_day)))
+    d = dist((10, 0, 0), (-1.0 * cf.one_day, 0, 0), (0.0, 0, 0))
+    d.insert_double(1.0).insert_double(0.0).insert_int(1)
+    s.add(cf.DISTRIBUTION, "DistributionTimeUniform", d)
+    s.add(cf.DISTRIBUTION, "DistributionPositionCumulativeOrder1", dist((8, 0, 0), (-1.0 * cf.one_day, 0, 0), (0.0, 0, 0)))
+    return s
+
+
 def smoothshock_perp_momentum():
     """Forward ParkerSource at a smooth shock with B tangential; perpendicular-only diffusion;
     momentum pass and injection boundaries; <dx dx> and (p) distributions."""
@@ -611,7 +636,7 @@ ALL = [shock_flowmom_reflect, solarwind_kinrad_reflect_recurrent, smoothshock_pe
        gc_qlt_spiral, fieldline_spiral, fieldline_backward_shock, lorentz_uniform, lorentz_spiral_backward,
        focused_spiral, focused_uniform_backward, gc_wnlt_scatt_spiral, gc_wnlt_ramp_diffscatt,
        gc_strauss_diff, focused_smoothshock, uniform_sphere_start, gc_sector_start, parker_soqlt_radial_limit, gc_soqlt_diff,
-       lorentz_shock, focused_shock, lorentz_grid, focused_grid, gc_smoothshock_diffscatt, guiding_shock]
+       lorentz_shock, focused_shock, lorentz_grid, focused_grid, gc_smoothshock_diffscatt, guiding_shock, parker_offset_spheres]
 NAMES = ["shock_flowmom_reflect", "solarwind_kinrad_reflect_recurrent", "smoothshock_perp_momentum", "uniform_para_plane_pass",
          "modulation_type0", "modulation_type2", "gc_rigidity_reflect", "gc_isotropic_only",
          "guiding_spiral", "guiding_scatt_uniform", "guiding_diff_rigidity", "parker_strauss_heliopause", "parker_potgieter_uniform",
@@ -619,7 +644,7 @@ NAMES = ["shock_flowmom_reflect", "solarwind_kinrad_reflect_recurrent", "smooths
          "focused_spiral", "focused_uniform_backward", "gc_wnlt_scatt_spiral", "gc_wnlt_ramp_diffscatt",
          "gc_strauss_diff", "focused_smoothshock", "uniform_sphere_start", "gc_sector_start", "parker_soqlt_radial_limit",
          "gc_soqlt_diff", "lorentz_shock", "focused_shock", "lorentz_grid", "focused_grid", "gc_smoothshock_diffscatt",
-         "guiding_shock"]
+         "guiding_shock", "parker_offset_spheres"]
 
 
 # Setups whose endpoints amplify last-bit differences far beyond 1e-10: QLT's mu derivative is
