@@ -22,6 +22,7 @@ struct GcFields {
    R rg;               // the solar wind's derivative step r_g (half of SpatialData::_dr)
    v3<R> dbhatdt;      // time-dependent backgrounds (the moving smooth shock): d bhat / dt
    R dBmagdt;          // and d|B| / dt = bhat . dB/dt
+   v3<R> fa0, fa1;     // FP32 step on the solar wind: the field-aligned frame of bhat, reused by the step
 };
 
 // CommonFields with mask ALL|gradB|dBdt (src/trajectory_guiding.cc:51): BackgroundBase::GetFields
@@ -86,6 +87,7 @@ __device__ __forceinline__ bool gc_fields(const SPX_CP spx_bg_t<R>& bg, double t
       fa[2] = F.bhat;
       fa[0] = second_unit(F.bhat);
       fa[1] = cross(fa[2], fa[0]);
+      if constexpr (is_f32<R>()) { F.fa0 = fa[0]; F.fa1 = fa[1]; }
       R g01 = R(0), g02 = R(0), g10 = R(0), g12 = R(0), g20 = R(0), g21 = R(0);
       v3<R> gbm = mk<R>(R(0), R(0), R(0));
       v3<R> dc = mk<R>(R(0), R(0), R(0));
@@ -450,8 +452,11 @@ __device__ __forceinline__ int gc_step(PlanPtr P, LaunchPtr L, const spx_rng_des
    v3<R> vel = gc_vel(pp, pl);
    v3<R> fa[3];
    fa[2] = H.F.bhat;
-   fa[0] = second_unit(H.F.bhat);
-   fa[1] = cross(fa[2], fa[0]);
+   if (is_f32<R>() && BG == SPX_BG_SOLARWIND) { fa[0] = H.F.fa0; fa[1] = H.F.fa1; }   // built by gc_fields from this bhat
+   else {
+      fa[0] = second_unit(H.F.bhat);
+      fa[1] = cross(fa[2], fa[0]);
+   }
    // DiffusionCoeff: D_perp, V_perp, D_mumu and its mu derivative (src/diffusion_other.cc:64-79)
    R pmag = R(pmag_d);
    const R mu = pl / norm(mk<R>(pp, R(0), pl));
