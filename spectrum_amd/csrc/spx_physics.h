@@ -32,7 +32,7 @@ constexpr double sp_small = 1.0E-8;
 constexpr double sp_tiny = 1.0E-15;
 
 // ---- arithmetic helpers: IEEE in double, hardware approximations in float -------------------------
-SPX_HD double rsqrt_(double x) { return sqrt(x); }
+SPX_HD double rsqrt_(double x) { return sqrt_rn(x); }
 SPX_HD float rsqrt_(float x) { return __builtin_sqrtf(x); }
 SPX_HD double rdiv(double a, double b) { return a / b; }
 SPX_HD float rdiv(float a, float b) { return a / b; }

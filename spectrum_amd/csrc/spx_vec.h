@@ -44,7 +44,30 @@ template <> SPX_HD v3<double> operator/(v3<double> b, double a)
 }
 template <typename R> SPX_HD R dot(v3<R> a, v3<R> b) { R s = a.x * b.x; s += a.y * b.y; s += a.z * b.z; return s; }
 template <typename R> SPX_HD R norm2(v3<R> a) { R s = a.x * a.x; s += a.y * a.y; s += a.z * a.z; return s; }
+// sqrt(x) in the FP64 kernels: the correctly rounded sequence the compiler emits for sqrt (reciprocal square
+// root seed, then g = x y, h = y / 2 refined by r = 1/2 - h g and two residual corrections d = x - g g),
+// without its rescaling of arguments below 2^-767, which is the identity for every other argument, and
+// with its pass-through of +-0 and +inf.  For x = 0, +inf or x >= 2^-767: squared lengths, fields and
+// momenta of a step, never that small.  The host keeps sqrt.
+SPX_HD double sqrt_rn(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+   const double y = __builtin_amdgcn_rsq(x);
+   double g = x * y, h = y * 0.5;
+   const double r = __builtin_fma(-h, g, 0.5);
+   g = __builtin_fma(g, r, g);
+   double d = __builtin_fma(-g, g, x);
+   h = __builtin_fma(h, r, h);
+   g = __builtin_fma(d, h, g);
+   d = __builtin_fma(-g, g, x);
+   g = __builtin_fma(d, h, g);
+   return (x == 0.0 || x == __builtin_inf()) ? x : g;
+#else
+   return sqrt(x);
+#endif
+}
 template <typename R> SPX_HD R norm(v3<R> a) { return sqrt(norm2(a)); }
+template <> SPX_HD double norm(v3<double> a) { return sqrt_rn(norm2(a)); }
 template <typename R> SPX_HD v3<R> unit(v3<R> a) { R n = norm(a); return a / n; }
 template <typename R> SPX_HD v3<R> cross(v3<R> a, v3<R> b)
 {
