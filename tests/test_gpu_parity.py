@@ -160,6 +160,51 @@ def test_results_do_not_depend_on_launch_shape(device, prec, case):
         assert st.steps_done == runs[0][0].steps_done
 
 
+def uniform_no_libm_setup():
+    """Parker in a uniform oblique field with a flow, constant kappa_par != kappa_perp, a reflecting sphere,
+    an absorbing plane and a time limit, binned by exit time: every operation of its step is +, -, *, /,
+    sqrt or a comparison (no pow, exp, log or trigonometric function, with the increments replayed)."""
+    s = cf.Setup("uniform_no_libm", cf.TRAJ_PARKER, cf.FORWARD)
+    s.add(cf.BACKGROUND, "BackgroundUniform", cf.background_base(cf.DataContainer(), 0.0, (0.0, 0.0, 0.0), (0.002, 0.001, -0.0005),
+                                                                 (3.0, 4.0, 12.0), 5.0))
+    s.add(cf.DIFFUSION, "DiffusionFullConstant", cf.DataContainer().insert_double(0.3).insert_double(2.5))
+    s.add(cf.INITIAL, "InitialTimeFixed", cf.DataContainer().insert_double(0.0))
+    s.add(cf.INITIAL, "InitialSpaceFixed", cf.DataContainer().insert_vec((0.5, -0.25, 1.0)))
+    s.add(cf.INITIAL, "InitialMomentumFixed", cf.DataContainer().insert_vec((cf.mom_from_ekin(cf.MeV), 0.0, 0.0)))
+    s.add(cf.BOUNDARY, "BoundarySphereReflect", cf.boundary(-1, [-1], ("vec", (0.0, 0.0, 0.0)), ("double", 30.0)))
+    s.add(cf.BOUNDARY, "BoundaryPlaneAbsorb", cf.boundary(1, [0], ("vec", (0.0, 0.0, 20.0)), ("vec", (0.0, 0.6, 0.8))))
+    s.add(cf.BOUNDARY, "BoundaryTimeExpire", cf.boundary(1, [0], ("double", 900.0)))
+    d = cf.distribution_base(cf.DataContainer(), (12, 0, 0), (0.0, 0.0, 0.0), (960.0, 0.0, 0.0), (0, 0, 0), (0, 0, 0))
+    d.insert_double(1.0).insert_double(0.5).insert_int(1)
+    s.add(cf.DISTRIBUTION, "DistributionTimeUniform", d)
+    return s
+
+
+def test_fp64_bit_identical_without_transcendentals(device, oracle_mod):
+    """Where the step uses only +, -, *, /, sqrt and comparisons (uniform_no_libm_setup, increments replayed), the
+    FP64 kernel rounds every operation as the oracle does (the oracle is built without contraction, every
+    operation of the kernel's reference order is correctly rounded): endpoints, steps, bins and histogram sums
+    are identical bit for bit.  This pins the kernel's divisions through shared reciprocals and its square
+    roots to the host's correctly rounded ones over every step of every trajectory."""
+    s = uniform_no_libm_setup()
+    n = 512
+    rng = np.random.default_rng(77)
+    normals = rng.standard_normal((n, 6000))
+    uniforms = rng.random((n, 8))
+    sim = E.Simulation.from_setup(s, precision=E.PREC_FP64, lanes=512, steps_per_launch=300)
+    st, rec = sim.run(n, normals=normals, uniforms=uniforms)
+    o = oracle_mod.Oracle(s)
+    _, orec = o.run(n, normals=normals, uniforms=uniforms, threads=THREADS)
+    assert st.done == n and st.failed == 0
+    print(f"[bitexact] steps {int(rec['steps'].sum())}, statuses {np.unique(rec['status'])}, events {np.bincount(rec['n_events'])}", flush=True)
+    for f in orec.dtype.names:
+        np.testing.assert_array_equal(rec[f], orec[f], err_msg=f)
+    c, sm, ne = sim.distro(0)
+    oc, osm, one = o.distro(0)
+    np.testing.assert_array_equal(c, oc)
+    np.testing.assert_array_equal(sm, osm)
+
+
 def f32_counter_normals(oracle_mod, seed, traj, count):
     """Normal draws 0..count-1 of a trajectory's FP32-step counter stream, restated on the host from the
     stream's definition: draw k is word k % 4 of Philox4x32-10 block k / 4 (key = seed, counter = (k / 4,
