@@ -209,8 +209,7 @@ __device__ __forceinline__ R sphere_dist(const SPX_CP spx_bnd_desc& b, double x,
 {
    if constexpr (is_f32<R>()) {
       const v3<float> d = mk<float>((float)(x - b.origin[0]), (float)(y - b.origin[1]), (float)(z - b.origin[2]));
-      const float d2 = norm2(d);
-      return d2 * frsq(d2);
+      return __builtin_sqrtf(norm2(d));   // one v_sqrt_f32 in the FP32 unit (approximate functions allowed); 0 at the origin
    }
    else return norm(mk<double>(x - b.origin[0], y - b.origin[1], z - b.origin[2]));
 }

@@ -56,7 +56,7 @@ __device__ __forceinline__ bool parker_fields(const SPX_CP spx_bg_t<R>& bg, doub
    if constexpr (is_f32<R>()) {
       const R b2 = norm2(B), ib = frsq(b2);
       F.Bmag = b2 * ib;
-      if (F.Bmag < R(sp_tiny)) return false;
+      if (!(F.Bmag >= R(sp_tiny))) return false;   // a zero field gives 0 * inf = NaN here: invalid, as |B| < tiny
       F.bhat = ib * B;
    }
    else {

@@ -160,13 +160,13 @@ def test_results_do_not_depend_on_launch_shape(device, prec, case):
         assert st.steps_done == runs[0][0].steps_done
 
 
-def uniform_no_libm_setup():
+def uniform_no_libm_setup(B=(3.0, 4.0, 12.0)):
     """Parker in a uniform oblique field with a flow, constant kappa_par != kappa_perp, a reflecting sphere,
     an absorbing plane and a time limit, binned by exit time: every operation of its step is +, -, *, /,
     sqrt or a comparison (no pow, exp, log or trigonometric function, with the increments replayed)."""
     s = cf.Setup("uniform_no_libm", cf.TRAJ_PARKER, cf.FORWARD)
     s.add(cf.BACKGROUND, "BackgroundUniform", cf.background_base(cf.DataContainer(), 0.0, (0.0, 0.0, 0.0), (0.002, 0.001, -0.0005),
-                                                                 (3.0, 4.0, 12.0), 5.0))
+                                                                 B, 5.0))
     s.add(cf.DIFFUSION, "DiffusionFullConstant", cf.DataContainer().insert_double(0.3).insert_double(2.5))
     s.add(cf.INITIAL, "InitialTimeFixed", cf.DataContainer().insert_double(0.0))
     s.add(cf.INITIAL, "InitialSpaceFixed", cf.DataContainer().insert_vec((0.5, -0.25, 1.0)))
@@ -203,6 +203,25 @@ def test_fp64_bit_identical_without_transcendentals(device, oracle_mod):
     oc, osm, one = o.distro(0)
     np.testing.assert_array_equal(c, oc)
     np.testing.assert_array_equal(sm, osm)
+
+
+def test_zero_field_is_a_field_error_in_both_precisions(oracle_mod):
+    """|B| below sp_tiny is an invalid field (src/background_base.cc:447): in a uniform zero field every
+    start attempt is discarded as a field error and each trajectory is abandoned after max_attempts, in the
+    oracle, the FP64 kernel and the mixed one (whose FP32 |B| = b2 rsq(b2) is 0 * inf = NaN there)."""
+    s = uniform_no_libm_setup(B=(0.0, 0.0, 0.0))
+    s.options["max_attempts"] = 3
+    n = 64
+    o = oracle_mod.Oracle(s)
+    _, orec = o.run(n, seed=3, threads=THREADS)
+    assert (orec["status"] != 0).all() and (orec["attempts"] == 3).all()
+    for prec in (E.PREC_FP64, E.PREC_MIXED):
+        sim = E.Simulation.from_setup(s, precision=prec, lanes=256, steps_per_launch=100)
+        st, rec = sim.run(n, seed=3)
+        print(f"[zero field] precision {prec}: discards {st.discards} failed {st.failed}", flush=True)
+        np.testing.assert_array_equal(rec["status"], orec["status"])
+        np.testing.assert_array_equal(rec["attempts"], orec["attempts"])
+        assert st.discards["field"] == o.discards()["field"] == 3 * n
 
 
 def f32_counter_normals(oracle_mod, seed, traj, count):
