@@ -448,7 +448,7 @@ __device__ __forceinline__ int gc_step(PlanPtr P, LaunchPtr L, const spx_rng_des
    const bool pa_step = (tt == SPX_TRAJ_GUIDING_DIFF_SCATT || tt == SPX_TRAJ_GUIDING_SCATT);
    const bool perp_step = (tt == SPX_TRAJ_GUIDING_DIFF_SCATT || tt == SPX_TRAJ_GUIDING_DIFF);
    R pp = R(H.pp), pl = R(H.pl);
-   const double pmag_d = sqrt(H.pp * H.pp + H.pl * H.pl);
+   const double pmag_d = sqrt_d<R>(H.pp * H.pp + H.pl * H.pl);
    v3<R> vel = gc_vel(pp, pl);
    v3<R> fa[3];
    fa[2] = H.F.bhat;
@@ -521,7 +521,7 @@ __device__ __forceinline__ int gc_step(PlanPtr P, LaunchPtr L, const spx_rng_des
          else q0 = nrm[0];
       }
       // the new momentum components are formed in double from |p| (an FP32 step keeps |p| exact here)
-      ppd = pmag_d * sqrt(1.0 - (double)mu_new * (double)mu_new);
+      ppd = pmag_d * sqrt_d<R>(1.0 - (double)mu_new * (double)mu_new);
       pld = pmag_d * (double)mu_new;
       pp = R(ppd);
       pl = R(pld);
@@ -563,7 +563,7 @@ __device__ __forceinline__ int gc_step(PlanPtr P, LaunchPtr L, const spx_rng_des
    }
    bflags = handle_boundaries<true>(P, L, slot, H, C, tid);
    if (!(bflags & 1)) {
-      H.pf = sqrt(H.pp * H.pp + H.pl * H.pl);
+      H.pf = sqrt_d<R>(H.pp * H.pp + H.pl * H.pl);
       if (!gc_fields<R, BG>(bg, H.t, H.x, H.y, H.z, R(H.pf), H.ds_prev, H.F)) return 1;
    }
    H.steps++;

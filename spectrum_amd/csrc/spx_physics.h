@@ -74,6 +74,13 @@ template <typename R> SPX_HD double divc_d(double a, double c)
    if constexpr (is_f32<R>()) return a / c;
    else return div_rn(a, c, 1.0 / c);
 }
+// sqrt of an FP64 quantity in a kernel of step type R: correctly rounded (sqrt_rn) in the FP64 kernels,
+// the unit's own square root in the FP32-step kernels.
+template <typename R> SPX_HD double sqrt_d(double x)
+{
+   if constexpr (is_f32<R>()) return sqrt(x);
+   else return sqrt_rn(x);
+}
 
 // Compensated (Kahan) addition of an increment to an FP64 sum, for the FP32-step kernels: c carries what
 // the rounding of s dropped (less than an ulp of s, so FP32 holds it to 24 bits) into the next addition.
@@ -668,7 +675,7 @@ __device__ __forceinline__ void normal_pair_f64(uint64_t seed, uint64_t traj, ui
    uint32_t o[4];
    philox_block(seed, traj, block, 2u, o);
    const double u1 = u53(o[0], o[1]), u2 = u53(o[2], o[3]);
-   const double r = sqrt(-2.0 * log(u1));
+   const double r = sqrt_rn(-2.0 * log(u1));   // -2 ln(u1) is 0 or above 2^-53
    double sn, cs;
    sincospi(2.0 * u2, &sn, &cs);
    c = r * cs;
