@@ -99,14 +99,18 @@ def test_parker_modulation_fp64_counter(device, oracle_mod, flow_days):
 
 
 def test_parker_modulation_fp64_full_trajectories(device, oracle_mod):
-    """Whole backward trajectories to the 90 au sphere (the reference problem), 48 trajectories."""
+    """Whole backward trajectories to the 90 au sphere (the reference problem), 48 trajectories of ~1.7e5
+    steps.  Over that many steps some trajectories amplify last-bit differences: the oracle's own build
+    with contracted multiply-adds moves them by up to 8e-11, so each trajectory is held to
+    max(1e-10, 100 x its spread) while steps, statuses and bins stay exact."""
     s = cf.parker_modulation(n_energies=16)
     n = 48
     sim = E.Simulation.from_setup(s, precision=E.PREC_FP64, lanes=256, steps_per_launch=4000)
     st, rec = sim.run(n, seed=1)
     o = oracle_mod.Oracle(s)
     _, orec = o.run(n, seed=1, threads=THREADS)
-    check_parity(st, rec, orec, sim, o, n)
+    _, prec = oracle_mod.Oracle(s, oracle_mod.FMA_LIB_PATH).run(n, seed=1, threads=THREADS)
+    check_parity(st, rec, orec, sim, o, n, sens=sensitivity(orec, prec))
 
 
 def test_parker_modulation_fp64_replay(device, oracle_mod):
