@@ -92,9 +92,10 @@ __device__ __forceinline__ bool gc_fields(const SPX_CP spx_bg_t<R>& bg, double t
       v3<R> gbm = mk<R>(R(0), R(0), R(0));
       v3<R> dc = mk<R>(R(0), R(0), R(0));
       if constexpr (is_f32<R>()) dc = rel(bg, x, y, z);   // FP32 stencil points relative to the origin
+      // the frame vectors rotate through e (moves instead of a select of each component per point)
+      v3<R> e = fa[0], e_next = fa[1];
 #pragma unroll 1
       for (int k = 0; k < 3; k++) {
-         const v3<R> e = k == 0 ? fa[0] : (k == 1 ? fa[1] : fa[2]);
          R dr = r_g;
          v3<R> Ud, Bf, Bb;
          R dsd;
@@ -121,6 +122,8 @@ __device__ __forceinline__ bool gc_fields(const SPX_CP spx_bg_t<R>& bg, double t
             g01 += e.x * gB.y; g02 += e.x * gB.z; g10 += e.y * gB.x; g12 += e.y * gB.z; g20 += e.z * gB.x; g21 += e.z * gB.y;
             gbm.x += e.x * gm; gbm.y += e.y * gm; gbm.z += e.z * gm;
          }
+         e = e_next;
+         e_next = fa[2];
       }
       F.gradBmag = gbm;
       const v3<R> curlB = mk<R>(g12 - g21, g20 - g02, g01 - g10);

@@ -80,9 +80,10 @@ __device__ __forceinline__ bool parker_fields(const SPX_CP spx_bg_t<R>& bg, doub
          F.divU = (R(0.5) / r_g) * sw_divu_stencil<R>(bg, dc, r_g, fa);
          return true;
       }
+      // the frame vectors rotate through e (moves instead of a select of each component per point)
+      v3<R> e = fa[0], e_next = fa[1];
 #pragma unroll 1
       for (int k = 0; k < 3; k++) {
-         const v3<R> e = k == 0 ? fa[0] : (k == 1 ? fa[1] : fa[2]);
          R dr = r_g;
          v3<R> Uf, Ub, Bd;
          R dsd;
@@ -101,6 +102,8 @@ __device__ __forceinline__ bool parker_fields(const SPX_CP spx_bg_t<R>& bg, doub
          const v3<R> G = is_f32<R>() ? (R(0.5) / r_g) * (Uf - Ub) : (Uf - Ub) / dr;   // FP32: one reciprocal per point
          if (k == 0) { d0 = e.x * G.x; d1 = e.y * G.y; d2 = e.z * G.z; }
          else { d0 += e.x * G.x; d1 += e.y * G.y; d2 += e.z * G.z; }
+         e = e_next;
+         e_next = fa[2];
       }
       F.divU = d0 + d1 + d2;
    }

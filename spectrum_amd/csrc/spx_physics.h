@@ -161,17 +161,20 @@ SPX_HD void solarwind_local(const SPX_CP spx_bg_t<R>& b, v3<R> pp, int want, v3<
       return;
    }
    else {
+   // the quotients by r (those of unit(pp), cos(theta), r_ref / r) and by s share one correctly rounded
+   // reciprocal each; div_rn rounds every quotient as the division does
    R r = norm(pp);
-   if (want & 1) u = b.ur0 * unit(pp);
+   const R yr = R(1) / r;
+   if (want & 1) u = b.ur0 * mk<R>(div_rn(pp.x, r, yr), div_rn(pp.y, r, yr), div_rn(pp.z, r, yr));
    if (want & 2) {
       R r_mns = r - b.r_ref;
-      R costheta = pp.z / r;
+      R costheta = div_rn(pp.z, r, yr);
       R sintheta = rsqrt_(fmax(R(1) - sqr(costheta), R(0)));
       R s = rsqrt_(sqr(pp.x) + sqr(pp.y));
       R cosphi, sinphi;
       if (s < r * R(sp_tiny)) { cosphi = R(0); sinphi = R(0); }
-      else { cosphi = pp.x / s; sinphi = pp.y / s; }
-      R Br = b.B0[0] * sqr(b.r_ref / r);
+      else { const R ys = R(1) / s; cosphi = div_rn(pp.x, s, ys); sinphi = div_rn(pp.y, s, ys); }
+      R Br = b.B0[0] * sqr(div_rn(b.r_ref, r, yr));
       R Bp = div_rn(-Br * sintheta * r_mns * b.w0, b.ur0, b.ur0_rcp);   // / ur0, rounded as the division
       Bl = mk<R>(Br * sintheta * cosphi - Bp * sinphi, Br * sintheta * sinphi + Bp * cosphi, Br * costheta);
    }
